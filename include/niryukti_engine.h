@@ -1,0 +1,80 @@
+/* MI355X PDHG LP engine: array-level C-ABI.
+ *
+ * niryukti_solve_lp() is the in-memory form of the reference's
+ *   vantage::Result vantage::solve(const Model &, const Options &)   src/solver.cpp:55
+ * for continuous LPs: the model arrives as CSR arrays instead of a Model
+ * object, options and results as plain structs (same names, defaults and
+ * meaning as include/vantage/vantage.hpp:40-123), and errors as a return code
+ * plus niryukti_engine_last_error() instead of exceptions.  Statuses are the
+ * reference's strings: OPTIMAL, INFEASIBLE, UNBOUNDED, ITERATION_LIMIT,
+ * TIME_LIMIT, NUMERICAL_ERROR, INTERRUPTED, UNSUPPORTED.
+ *
+ * The niryukti_instance_* functions expose the seeded synthetic workload
+ * generators (planted optimum known in closed form) used by the benchmark.
+ */
+#ifndef NIRYUKTI_ENGINE_H
+#define NIRYUKTI_ENGINE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int64_t rows, cols;
+    const int64_t *ptr;   /* rows + 1 */
+    const int64_t *index; /* nnz */
+    const double *value;  /* nnz */
+    const double *c, *lb, *ub; /* cols (canonical minimisation costs) */
+    const double *rl, *ru;     /* rows */
+    double offset, sense;      /* reported objective = sense * (c^T x + offset) */
+} niryukti_lp;
+
+typedef struct {
+    const char *device;           /* "gpu" (default) | "hip" | "auto"; "cpu"/"cuda" -> UNSUPPORTED */
+    const char *method;           /* "pdhg" | "halpern" | "rhpdhg" | "r2hpdhg" */
+    const char *scaling;          /* "ruiz" | "combined" */
+    const char *primal_weight;    /* "displacement" | "pid" */
+    const char *matrix_precision; /* "fp64" | "mixed" */
+    const char *gpu_indices;      /* "auto" | "32" | "64" */
+    double tol, time_limit;
+    int64_t iteration_limit;
+    int check_every, scaling_passes, threads;
+    int presolve, restart, adaptive, graphs, verbose;
+    const double *initial_x; /* NULL or cols */
+    const double *initial_y; /* NULL or rows */
+} niryukti_options;
+
+typedef struct {
+    char status[32];
+    char message[256];
+    double objective; /* sense-restored; NaN when not finite */
+    double primal_residual, dual_residual, relative_gap, kkt_error, complementarity, dual_bound;
+    double primal_absolute, dual_absolute;
+    int finite;
+    int64_t iterations, restarts, rejected_steps, weight_updates, host_candidate_checks;
+    int64_t removed_rows, removed_columns;
+    double seconds, preprocess_seconds, transfer_seconds, iteration_seconds, verification_seconds;
+    int index_bits;
+    double *x; /* NULL or cols: receives the primal solution */
+    double *y; /* NULL or rows: receives the dual solution */
+} niryukti_result;
+
+void niryukti_default_options(niryukti_options *o);
+/* 0 on success (any solver status), -1 on invalid input or a device error. */
+int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *o, niryukti_result *r);
+const char *niryukti_engine_last_error(void);
+
+/* Seeded synthetic instances.  config 0 (2,000 x 5,000 equality LP), 1
+ * (1,000,000 x 2,000,000 power-law LP), 4 (block-angular); `scale` divides the
+ * dimensions.  config 2 selects instance `index` of the batched workload. */
+typedef struct niryukti_instance niryukti_instance;
+niryukti_instance *niryukti_instance_create(int config, uint64_t seed, int64_t scale, int64_t index);
+void niryukti_instance_destroy(niryukti_instance *p);
+/* Borrowed views valid until destroy; planted_x / planted_y are the certified optimum. */
+void niryukti_instance_view(const niryukti_instance *p, niryukti_lp *lp, double *planted_objective,
+                            const double **planted_x, const double **planted_y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,303 @@
+"""MI355X-native restarted-PDHG LP engine (Niryukti-compatible surface).
+
+The compute path is the native library ``niryukti_amd/lib/libniryukti.so``:
+hand-written gfx950 HIP kernels behind C-ABIs declared in ``include/``.  This
+module is a thin ctypes binding over those C-ABIs; it never computes anything
+itself and raises if the native library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "load_library", "LP", "solve_lp", "Instance", "Backend", "device_count", "device_name",
+    "ABI_VERSION",
+]
+
+_PKG = Path(__file__).resolve().parent
+LIBRARY_PATH = Path(os.environ.get("NIRYUKTI_LIBRARY", _PKG / "lib" / "libniryukti.so"))
+ABI_VERSION = 1
+
+_f64p = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+
+class _LP(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("cols", C.c_int64), ("ptr", _i64p), ("index", _i64p), ("value", _f64p),
+                ("c", _f64p), ("lb", _f64p), ("ub", _f64p), ("rl", _f64p), ("ru", _f64p),
+                ("offset", C.c_double), ("sense", C.c_double)]
+
+
+class _Options(C.Structure):
+    _fields_ = [("device", C.c_char_p), ("method", C.c_char_p), ("scaling", C.c_char_p),
+                ("primal_weight", C.c_char_p), ("matrix_precision", C.c_char_p), ("gpu_indices", C.c_char_p),
+                ("tol", C.c_double), ("time_limit", C.c_double), ("iteration_limit", C.c_int64),
+                ("check_every", C.c_int), ("scaling_passes", C.c_int), ("threads", C.c_int),
+                ("presolve", C.c_int), ("restart", C.c_int), ("adaptive", C.c_int), ("graphs", C.c_int),
+                ("verbose", C.c_int), ("initial_x", _f64p), ("initial_y", _f64p)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("status", C.c_char * 32), ("message", C.c_char * 256), ("objective", C.c_double),
+                ("primal_residual", C.c_double), ("dual_residual", C.c_double), ("relative_gap", C.c_double),
+                ("kkt_error", C.c_double), ("complementarity", C.c_double), ("dual_bound", C.c_double),
+                ("primal_absolute", C.c_double), ("dual_absolute", C.c_double), ("finite", C.c_int),
+                ("iterations", C.c_int64), ("restarts", C.c_int64), ("rejected_steps", C.c_int64),
+                ("weight_updates", C.c_int64), ("host_candidate_checks", C.c_int64),
+                ("removed_rows", C.c_int64), ("removed_columns", C.c_int64), ("seconds", C.c_double),
+                ("preprocess_seconds", C.c_double), ("transfer_seconds", C.c_double),
+                ("iteration_seconds", C.c_double), ("verification_seconds", C.c_double),
+                ("index_bits", C.c_int), ("x", _f64p), ("y", _f64p)]
+
+
+class _PdhgLP(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("cols", C.c_int64), ("ptr", _i64p), ("index", _i64p), ("value", _f64p),
+                ("c", _f64p), ("lb", _f64p), ("ub", _f64p), ("rl", _f64p), ("ru", _f64p)]
+
+
+class _PdhgConfig(C.Structure):
+    _fields_ = [("adaptive", C.c_int), ("halpern", C.c_int), ("reflection", C.c_double),
+                ("residual_restarts", C.c_int), ("graphs", C.c_int), ("index_bits", C.c_int),
+                ("fp32_values", C.c_int)]
+
+
+class _PdhgStatus(C.Structure):
+    _fields_ = [("accepted_total", C.c_int64), ("rejected_total", C.c_int64), ("restart_requested", C.c_int),
+                ("index_bits", C.c_int), ("factor", C.c_double)]
+
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Loads the native engine; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBRARY_PATH.exists():
+        raise RuntimeError(f"native engine library not found at {LIBRARY_PATH}; build it with "
+                           "`python -m niryukti_amd.build` (the engine has no non-native fallback)")
+    lib = C.CDLL(str(LIBRARY_PATH))
+    lib.niryukti_default_options.argtypes = [C.POINTER(_Options)]
+    lib.niryukti_solve_lp.argtypes = [C.POINTER(_LP), C.POINTER(_Options), C.POINTER(_Result)]
+    lib.niryukti_solve_lp.restype = C.c_int
+    lib.niryukti_engine_last_error.restype = C.c_char_p
+    lib.niryukti_instance_create.argtypes = [C.c_int, C.c_uint64, C.c_int64, C.c_int64]
+    lib.niryukti_instance_create.restype = C.c_void_p
+    lib.niryukti_instance_destroy.argtypes = [C.c_void_p]
+    lib.niryukti_instance_view.argtypes = [C.c_void_p, C.POINTER(_LP), C.POINTER(C.c_double),
+                                           C.POINTER(_f64p), C.POINTER(_f64p)]
+    lib.npdhg_last_error.restype = C.c_char_p
+    lib.npdhg_device_count.restype = C.c_int
+    lib.npdhg_device_name.argtypes = [C.c_int, C.c_char_p, C.c_int]
+    lib.npdhg_create.argtypes = [C.POINTER(_PdhgLP), _f64p, _f64p, C.POINTER(_PdhgConfig), C.POINTER(C.c_void_p)]
+    lib.npdhg_destroy.argtypes = [C.c_void_p]
+    lib.npdhg_advance.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int)]
+    lib.npdhg_status_get.argtypes = [C.c_void_p, C.POINTER(_PdhgStatus)]
+    lib.npdhg_candidates.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p]
+    lib.npdhg_reset.argtypes = [C.c_void_p, _f64p, _f64p]
+    lib.npdhg_reset_candidate.argtypes = [C.c_void_p, C.c_int]
+    lib.npdhg_spmv.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p]
+    _lib = lib
+    return lib
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a: np.ndarray, kind=_f64p):
+    return a.ctypes.data_as(kind)
+
+
+@dataclass
+class LP:
+    """An LP in the reference's canonical form: min c^T x + offset, rl <= A x <= ru, lb <= x <= ub."""
+
+    ptr: np.ndarray
+    index: np.ndarray
+    value: np.ndarray
+    c: np.ndarray
+    lb: np.ndarray
+    ub: np.ndarray
+    rl: np.ndarray
+    ru: np.ndarray
+    offset: float = 0.0
+    sense: float = 1.0
+
+    def __post_init__(self):
+        self.ptr = np.ascontiguousarray(self.ptr, dtype=np.int64)
+        self.index = np.ascontiguousarray(self.index, dtype=np.int64)
+        for name in ("value", "c", "lb", "ub", "rl", "ru"):
+            setattr(self, name, _f64(getattr(self, name)))
+
+    @property
+    def rows(self) -> int:
+        return len(self.rl)
+
+    @property
+    def cols(self) -> int:
+        return len(self.c)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.ptr[-1])
+
+    def _struct(self) -> _LP:
+        return _LP(self.rows, self.cols, _ptr(self.ptr, _i64p), _ptr(self.index, _i64p), _ptr(self.value),
+                   _ptr(self.c), _ptr(self.lb), _ptr(self.ub), _ptr(self.rl), _ptr(self.ru),
+                   float(self.offset), float(self.sense))
+
+    @classmethod
+    def from_dense(cls, A, c, lb, ub, rl, ru, offset=0.0, sense=1.0) -> "LP":
+        A = np.asarray(A, dtype=np.float64)
+        rows, cols = np.nonzero(A)
+        ptr = np.zeros(A.shape[0] + 1, dtype=np.int64)
+        np.add.at(ptr, rows + 1, 1)
+        return cls(np.cumsum(ptr), cols, A[rows, cols], c, lb, ub, rl, ru, offset, sense)
+
+
+_OPTION_STRINGS = ("device", "method", "scaling", "primal_weight", "matrix_precision", "gpu_indices")
+
+
+def solve_lp(lp: LP, **options) -> dict:
+    """Solves `lp` with the GPU engine; keyword options mirror the reference's Options."""
+    lib = load_library()
+    opts = _Options()
+    lib.niryukti_default_options(C.byref(opts))
+    keep = []
+    for key, val in options.items():
+        if key in ("initial_x", "initial_y"):
+            arr = _f64(val)
+            keep.append(arr)
+            setattr(opts, key, _ptr(arr))
+        elif key in _OPTION_STRINGS:
+            b = str(val).encode()
+            keep.append(b)
+            setattr(opts, key, b)
+        elif key in ("presolve", "restart", "adaptive", "graphs", "verbose"):
+            setattr(opts, key, int(bool(val)))
+        elif hasattr(opts, key):
+            setattr(opts, key, val)
+        else:
+            raise ValueError(f"unknown option {key!r}")
+    x = np.zeros(lp.cols)
+    y = np.zeros(lp.rows)
+    res = _Result()
+    res.x = _ptr(x)
+    res.y = _ptr(y)
+    s = lp._struct()
+    rc = lib.niryukti_solve_lp(C.byref(s), C.byref(opts), C.byref(res))
+    if rc != 0:
+        raise RuntimeError(lib.niryukti_engine_last_error().decode())
+    out = {name: getattr(res, name) for name, _ in _Result._fields_ if name not in ("status", "message", "x", "y")}
+    out["status"] = res.status.decode()
+    out["message"] = res.message.decode()
+    out["x"] = x
+    out["y"] = y
+    return out
+
+
+class Instance:
+    """Seeded synthetic LP with a planted optimum, generated natively."""
+
+    def __init__(self, config: int, seed: int = 42, scale: int = 1, index: int = 0):
+        lib = load_library()
+        h = lib.niryukti_instance_create(int(config), int(seed), int(scale), int(index))
+        if not h:
+            raise RuntimeError(lib.niryukti_engine_last_error().decode())
+        self._h = C.c_void_p(h)
+        s = _LP()
+        obj = C.c_double()
+        px, py = _f64p(), _f64p()
+        lib.niryukti_instance_view(self._h, C.byref(s), C.byref(obj), C.byref(px), C.byref(py))
+        m, n = s.rows, s.cols
+        nnz = s.ptr[m] if m >= 0 else 0
+
+        def view(p, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype=dtype)
+            return np.ctypeslib.as_array(p, shape=(count,)).copy()
+
+        self.lp = LP(view(s.ptr, m + 1, np.int64), view(s.index, nnz, np.int64), view(s.value, nnz, np.float64),
+                     view(s.c, n, np.float64), view(s.lb, n, np.float64), view(s.ub, n, np.float64),
+                     view(s.rl, m, np.float64), view(s.ru, m, np.float64))
+        self.planted_objective = obj.value
+        self.planted_x = view(px, n, np.float64)
+        self.planted_y = view(py, m, np.float64)
+        lib.niryukti_instance_destroy(self._h)
+        self._h = None
+
+
+class Backend:
+    """Direct handle on the HIP iteration backend (the npdhg_* seam)."""
+
+    def __init__(self, lp: LP, x0, y0, adaptive=True, halpern=False, reflection=0.0, residual_restarts=False,
+                 graphs=True, index_bits=0, fp32_values=False):
+        self._lib = load_library()
+        self.lp = lp
+        self._s = _PdhgLP(lp.rows, lp.cols, _ptr(lp.ptr, _i64p), _ptr(lp.index, _i64p), _ptr(lp.value),
+                          _ptr(lp.c), _ptr(lp.lb), _ptr(lp.ub), _ptr(lp.rl), _ptr(lp.ru))
+        cfg = _PdhgConfig(int(adaptive), int(halpern), float(reflection), int(residual_restarts), int(graphs),
+                          int(index_bits), int(fp32_values))
+        x0, y0 = _f64(x0), _f64(y0)
+        h = C.c_void_p()
+        self._check(self._lib.npdhg_create(C.byref(self._s), _ptr(x0), _ptr(y0), C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self._lib.npdhg_last_error().decode())
+
+    def advance(self, count: int, tau: float, sigma: float) -> int:
+        acc = C.c_int()
+        self._check(self._lib.npdhg_advance(self._h, int(count), float(tau), float(sigma), C.byref(acc)))
+        return acc.value
+
+    def status(self) -> dict:
+        s = _PdhgStatus()
+        self._check(self._lib.npdhg_status_get(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _PdhgStatus._fields_}
+
+    def candidates(self):
+        x, y = np.zeros(self.lp.cols), np.zeros(self.lp.rows)
+        xa, ya = np.zeros(self.lp.cols), np.zeros(self.lp.rows)
+        self._check(self._lib.npdhg_candidates(self._h, _ptr(x), _ptr(y), _ptr(xa), _ptr(ya)))
+        return x, y, xa, ya
+
+    def reset(self, x, y):
+        x, y = _f64(x), _f64(y)
+        self._check(self._lib.npdhg_reset(self._h, _ptr(x), _ptr(y)))
+
+    def reset_candidate(self, average: bool):
+        self._check(self._lib.npdhg_reset_candidate(self._h, int(bool(average))))
+
+    def spmv(self, x, transpose=False):
+        x = _f64(x)
+        out = np.zeros(self.lp.cols if transpose else self.lp.rows)
+        self._check(self._lib.npdhg_spmv(self._h, int(bool(transpose)), _ptr(x), _ptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.npdhg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def device_count() -> int:
+    return load_library().npdhg_device_count()
+
+
+def device_name(index: int = 0) -> str:
+    buf = C.create_string_buffer(256)
+    load_library().npdhg_device_name(index, buf, 256)
+    return buf.value.decode()
+

@@ -1,0 +1,169 @@
+// Array-level C-ABI of the engine (include/niryukti_engine.h).
+#include "niryukti_engine.h"
+
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+
+#include "instances.hpp"
+#include "lp.hpp"
+
+namespace {
+thread_local std::string g_error;
+
+nyk::Model to_model(const niryukti_lp &lp) {
+    if (lp.rows < 0 || lp.cols < 0 || !lp.ptr) throw std::runtime_error("Invalid LP arrays");
+    nyk::Model m;
+    const int64_t nnz = lp.ptr[lp.rows];
+    if (nnz < 0 || (nnz > 0 && (!lp.index || !lp.value))) throw std::runtime_error("Invalid CSR arrays");
+    if ((lp.cols > 0 && (!lp.c || !lp.lb || !lp.ub)) || (lp.rows > 0 && (!lp.rl || !lp.ru)))
+        throw std::runtime_error("Missing LP vectors");
+    m.A.rows = lp.rows;
+    m.A.cols = lp.cols;
+    m.A.ptr.assign(lp.ptr, lp.ptr + lp.rows + 1);
+    m.A.index.assign(lp.index, lp.index + nnz);
+    m.A.value.assign(lp.value, lp.value + nnz);
+    m.c.assign(lp.c, lp.c + lp.cols);
+    m.lb.assign(lp.lb, lp.lb + lp.cols);
+    m.ub.assign(lp.ub, lp.ub + lp.cols);
+    m.rl.assign(lp.rl, lp.rl + lp.rows);
+    m.ru.assign(lp.ru, lp.ru + lp.rows);
+    m.offset = lp.offset;
+    m.sense = lp.sense == 0 ? 1 : lp.sense;
+    return m;
+}
+
+nyk::Options to_options(const niryukti_options *in, const niryukti_lp &lp) {
+    nyk::Options o;
+    if (!in) return o;
+    auto str = [](const char *s, const char *d) { return std::string(s ? s : d); };
+    o.device = str(in->device, "gpu");
+    o.method = str(in->method, "pdhg");
+    o.scaling = str(in->scaling, "ruiz");
+    o.primal_weight = str(in->primal_weight, "displacement");
+    o.matrix_precision = str(in->matrix_precision, "fp64");
+    o.gpu_indices = str(in->gpu_indices, "auto");
+    o.tol = in->tol;
+    o.time_limit = in->time_limit;
+    o.iteration_limit = in->iteration_limit;
+    o.check_every = in->check_every;
+    o.scaling_passes = in->scaling_passes;
+    o.threads = in->threads;
+    o.presolve = in->presolve != 0;
+    o.restart = in->restart != 0;
+    o.adaptive = in->adaptive != 0;
+    o.graphs = in->graphs != 0;
+    o.verbose = in->verbose != 0;
+    if (in->initial_x) o.initial_x.assign(in->initial_x, in->initial_x + lp.cols);
+    if (in->initial_y) o.initial_y.assign(in->initial_y, in->initial_y + lp.rows);
+    return o;
+}
+}  // namespace
+
+struct niryukti_instance {
+    nyk_gen::Instance p;
+};
+
+extern "C" {
+
+void niryukti_default_options(niryukti_options *o) {
+    std::memset(o, 0, sizeof *o);
+    o->device = "gpu";
+    o->method = "pdhg";
+    o->scaling = "ruiz";
+    o->primal_weight = "displacement";
+    o->matrix_precision = "fp64";
+    o->gpu_indices = "auto";
+    o->tol = 1e-6;
+    o->time_limit = 60;
+    o->iteration_limit = 100000;
+    o->check_every = 100;
+    o->scaling_passes = 5;
+    o->threads = 1;
+    o->presolve = 1;
+    o->restart = 1;
+    o->adaptive = 1;
+    o->graphs = 1;
+}
+
+int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *opts, niryukti_result *out) {
+    try {
+        g_error.clear();
+        if (!lp || !out) throw std::runtime_error("Null argument");
+        auto m = to_model(*lp);
+        auto o = to_options(opts, *lp);
+        auto r = nyk::solve(m, o);
+        std::snprintf(out->status, sizeof out->status, "%s", r.status.c_str());
+        std::snprintf(out->message, sizeof out->message, "%s", r.message.c_str());
+        const auto &a = r.accuracy;
+        out->objective = a.finite ? m.sense * a.objective : NAN;
+        out->primal_residual = a.primal;
+        out->dual_residual = a.dual;
+        out->relative_gap = a.gap;
+        out->kkt_error = a.kkt;
+        out->complementarity = a.complementarity;
+        out->dual_bound = a.lower_bound;
+        out->primal_absolute = a.primal_absolute;
+        out->dual_absolute = a.dual_absolute;
+        out->finite = a.finite ? 1 : 0;
+        out->iterations = r.iterations;
+        out->restarts = r.restarts;
+        out->rejected_steps = r.rejected_steps;
+        out->weight_updates = r.weight_updates;
+        out->host_candidate_checks = r.host_candidate_checks;
+        out->removed_rows = r.removed_rows;
+        out->removed_columns = r.removed_columns;
+        out->seconds = r.seconds;
+        out->preprocess_seconds = r.preprocess_seconds;
+        out->transfer_seconds = r.transfer_seconds;
+        out->iteration_seconds = r.iteration_seconds;
+        out->verification_seconds = r.verification_seconds;
+        out->index_bits = r.gpu_index_bits;
+        if (out->x && r.x.size() == size_t(lp->cols)) std::memcpy(out->x, r.x.data(), r.x.size() * sizeof(double));
+        if (out->y && r.y.size() == size_t(lp->rows)) std::memcpy(out->y, r.y.data(), r.y.size() * sizeof(double));
+        return 0;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+const char *niryukti_engine_last_error(void) { return g_error.c_str(); }
+
+niryukti_instance *niryukti_instance_create(int config, uint64_t seed, int64_t scale, int64_t index) {
+    try {
+        g_error.clear();
+        auto *h = new niryukti_instance;
+        if (config == 2)
+            h->p = nyk_gen::batch_instance(nyk_gen::batch_shapes(int(index) + 1, seed).back());
+        else
+            h->p = nyk_gen::workload(config, seed, scale);
+        return h;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return nullptr;
+    }
+}
+
+void niryukti_instance_destroy(niryukti_instance *p) { delete p; }
+
+void niryukti_instance_view(const niryukti_instance *h, niryukti_lp *lp, double *planted_objective,
+                            const double **planted_x, const double **planted_y) {
+    const auto &p = h->p;
+    lp->rows = p.rows;
+    lp->cols = p.cols;
+    lp->ptr = p.ptr.data();
+    lp->index = p.index.data();
+    lp->value = p.value.data();
+    lp->c = p.c.data();
+    lp->lb = p.lb.data();
+    lp->ub = p.ub.data();
+    lp->rl = p.rl.data();
+    lp->ru = p.ru.data();
+    lp->offset = 0;
+    lp->sense = 1;
+    if (planted_objective) *planted_objective = p.objective;
+    if (planted_x) *planted_x = p.x0.data();
+    if (planted_y) *planted_y = p.y_opt.data();
+}
+}

@@ -1,0 +1,383 @@
+// Restarted-PDHG driver: presolve/scale on the host, iterate on the GPU,
+// certify in the original space on the host.
+//
+// Control flow and every decision rule follow the reference's continuous
+// driver (src/solver.cpp:215-735, PDHG branch): chunked advance of
+// `check_every` accepted steps, candidate check of the current and averaged
+// iterates in original units, best-candidate tracking, OPTIMAL only after a
+// safe re-evaluation, Farkas ray tests after 1000 iterations, the restart rule
+// and the displacement / PID primal-weight update.  Iterations run in the HIP
+// backend behind the npdhg_* C-ABI (include/niryukti_pdhg.h).
+#include "lp.hpp"
+#include "niryukti_pdhg.h"
+#include "parallel.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <csignal>
+#include <cstdio>
+#include <stdexcept>
+
+namespace nyk {
+
+volatile std::sig_atomic_t interrupted = 0;
+
+namespace {
+using Clock = std::chrono::steady_clock;
+double elapsed(Clock::time_point s) { return std::chrono::duration<double>(Clock::now() - s).count(); }
+
+bool anchored(const Options &o) { return o.method == "halpern" || o.method == "rhpdhg" || o.method == "r2hpdhg"; }
+bool stop_requested(const Options &o) {
+    return interrupted || (o.cancellation && o.cancellation->load(std::memory_order_relaxed));
+}
+
+// PID controller on the log primal weight (src/research.cpp:52-70).
+class WeightPid {
+    double integral_ = 0, previous_ = 0;
+    bool initialized_ = false;
+
+  public:
+    double update(double weight, long double dx, long double dy) {
+        if (!(weight > 0) || !std::isfinite(weight) || !(dx > 1e-24L) || !(dy > 1e-24L) || !std::isfinite(dx) ||
+            !std::isfinite(dy))
+            return weight;
+        double error = double(std::log((long double)weight) + .5L * (std::log(dx) - std::log(dy)));
+        error = std::clamp(error, -10., 10.);
+        double next_integral = std::clamp(integral_ + error, -100., 100.);
+        double derivative = initialized_ ? error - previous_ : 0;
+        double correction = std::clamp(.3 * error + .01 * next_integral + .1 * derivative, -std::log(4.), std::log(4.));
+        double logarithm = std::log(weight) - correction;
+        double clipped = std::clamp(logarithm, std::log(1e-4), std::log(1e4));
+        if (clipped == logarithm) integral_ = next_integral;
+        previous_ = error;
+        initialized_ = true;
+        return std::exp(clipped);
+    }
+};
+
+// RAII owner of a device backend.
+class Device {
+  public:
+    Device(const Model &m, const std::vector<double> &x, const std::vector<double> &y, const npdhg_config &cfg) {
+        npdhg_lp lp{m.rows(), m.cols(), m.A.ptr.data(), m.A.index.data(), m.A.value.data(), m.c.data(),
+                    m.lb.data(),  m.ub.data(), m.rl.data(),      m.ru.data()};
+        check(npdhg_create(&lp, x.data(), y.data(), &cfg, &b_));
+    }
+    ~Device() { npdhg_destroy(b_); }
+    Device(const Device &) = delete;
+    Device &operator=(const Device &) = delete;
+    int advance(int count, double tau, double sigma) {
+        int accepted = 0;
+        check(npdhg_advance(b_, count, tau, sigma, &accepted));
+        return accepted;
+    }
+    npdhg_status status() {
+        npdhg_status s{};
+        check(npdhg_status_get(b_, &s));
+        return s;
+    }
+    void candidates(std::vector<double> &x, std::vector<double> &y, std::vector<double> &xa, std::vector<double> &ya) {
+        check(npdhg_candidates(b_, x.data(), y.data(), xa.data(), ya.data()));
+    }
+    void reset_candidate(bool average) { check(npdhg_reset_candidate(b_, average ? 1 : 0)); }
+    void reset(const std::vector<double> &x, const std::vector<double> &y) { check(npdhg_reset(b_, x.data(), y.data())); }
+
+  private:
+    static void check(int rc) {
+        if (rc != 0) throw std::runtime_error(npdhg_last_error());
+    }
+    npdhg_backend *b_ = nullptr;
+};
+
+std::string device_description() {
+    char name[256] = "HIP device";
+    if (npdhg_device_count() > 0) npdhg_device_name(0, name, sizeof name);
+    return name;
+}
+
+Result solve_continuous(const Model &original, const Options &o) {
+    auto start = Clock::now();
+    Result r;
+    r.method_selected = o.method;
+    r.backend = "hip";
+    r.estimated_gpu_bytes = 48. * original.A.nnz() + 240. * (original.rows() + original.cols() + 2);
+    r.device_reason = "Explicit backend request";
+    Verifier verifier(original);
+    r.status = "ITERATION_LIMIT";
+    auto prep = prepare(original, o);
+    r.preprocess_seconds = elapsed(start);
+    if (!prep.failure.empty()) {
+        r.status = prep.failure;
+        r.message = prep.reason;
+        r.infeasibility_ray = prep.failure_ray;
+        if (!r.infeasibility_ray.empty()) r.certificate_margin = verifier.infeasibility_bound(r.infeasibility_ray);
+        r.seconds = elapsed(start);
+        return r;
+    }
+    const Model &m = prep.model;
+    r.removed_columns = original.cols() - m.cols();
+    r.removed_rows = original.rows() - m.rows();
+    std::vector<double> x(m.cols()), y(m.rows());
+    for (size_t j = 0; j < x.size(); ++j) {
+        double initial = o.initial_x.empty() ? 0 : o.initial_x[prep.cols[j]];
+        x[j] = std::clamp(initial / prep.column_scale[j], m.lb[j], m.ub[j]);
+    }
+    if (!o.initial_y.empty())
+        for (size_t i = 0; i < y.size(); ++i) y[i] = o.initial_y[prep.rows[i]] / (prep.row_scale[i] * prep.objective_scale);
+    r.x = prep.restore_x(x);
+    r.y = prep.restore_y(y, original.rl.size());
+    r.accuracy = verifier.evaluate(r.x, r.y);
+    // Feasible start plus an isolated improving column proves unboundedness (solver.cpp:269-288).
+    if (r.accuracy.finite && r.accuracy.primal_absolute == 0) {
+        std::vector<uint8_t> used(m.cols(), 0);
+        for (auto j : m.A.index) used[j] = 1;
+        for (int64_t j = 0; j < m.cols(); ++j)
+            if (!used[j] && ((m.c[j] < 0 && m.ub[j] == kInf) || (m.c[j] > 0 && m.lb[j] == -kInf))) {
+                r.status = "UNBOUNDED";
+                r.message = "Feasible point and isolated improving recession direction: " + m.column_name(j);
+                r.seconds = elapsed(start);
+                return r;
+            }
+    }
+    if (r.accuracy.finite && r.accuracy.kkt <= o.tol) {
+        r.status = "OPTIMAL";
+        r.seconds = elapsed(start);
+        return r;
+    }
+    if (o.time_limit == 0) {
+        r.status = "TIME_LIMIT";
+        r.seconds = elapsed(start);
+        return r;
+    }
+    // ||A||_2 <= sqrt(||A||_1 ||A||_inf) gives the guaranteed step (solver.cpp:299-322).
+    std::vector<double> colsum(m.cols(), 0);
+    double rowmax = 0;
+    for (int64_t i = 0; i < m.rows(); ++i) {
+        double s = 0;
+        for (int64_t k = m.A.ptr[i]; k < m.A.ptr[i + 1]; ++k) {
+            double v = std::abs(m.A.value[k]);
+            s += v;
+            colsum[m.A.index[k]] += v;
+        }
+        rowmax = std::max(rowmax, s);
+    }
+    double colmax = 0;
+    for (double s : colsum) colmax = std::max(colmax, s);
+    const double norm = std::sqrt(rowmax) * std::sqrt(colmax);
+    double step = norm > 0 ? .9 / norm : 1, weight = 1;
+    if ((o.adaptive || anchored(o)) && o.primal_weight != "pid") {  // solver.cpp:323-333
+        long double c2 = 0, b2 = 0;
+        for (double c : m.c) c2 += (long double)c * c;
+        for (int64_t i = 0; i < m.rows(); ++i) {
+            double b = std::clamp(0., m.rl[i], m.ru[i]);
+            b2 += (long double)b * b;
+        }
+        if (c2 > 1e-24 && b2 > 1e-24) weight = std::clamp(std::sqrt(double(c2 / b2)), 1e-4, 1e4);
+    }
+    const bool residual_restarts = o.method == "rhpdhg" || o.method == "r2hpdhg";
+    auto transfer = Clock::now();
+    npdhg_config cfg{};
+    cfg.adaptive = o.adaptive;
+    cfg.halpern = anchored(o);
+    cfg.reflection = o.method == "r2hpdhg" ? 1 : 0;
+    cfg.residual_restarts = residual_restarts && o.restart;
+    cfg.graphs = o.graphs;
+    cfg.index_bits = o.gpu_indices == "64" ? 64 : o.gpu_indices == "32" ? 32 : 0;
+    cfg.fp32_values = o.matrix_precision == "mixed";
+    Device backend(m, x, y, cfg);
+    r.transfer_seconds = elapsed(transfer);
+    r.device_name = device_description();
+    r.gpu_index_bits = backend.status().index_bits;
+    r.graph_execution = o.graphs;
+    r.matrix_precision = o.matrix_precision;
+
+    double restart_kkt = r.accuracy.kkt;
+    int64_t since_restart = 0, previous_rejected = 0;
+    std::vector<double> epochx = x, epochy = y;
+    WeightPid controller;
+    std::vector<double> cx(m.cols()), cy(m.rows()), ax(m.cols()), ay(m.rows());
+    while (r.iterations < o.iteration_limit) {
+        if (stop_requested(o)) {
+            r.status = "INTERRUPTED";
+            break;
+        }
+        if (elapsed(start) >= o.time_limit) {
+            r.status = "TIME_LIMIT";
+            break;
+        }
+        int count = int(std::min<int64_t>(o.check_every, o.iteration_limit - r.iterations));
+        auto tick = Clock::now();
+        int accepted = backend.advance(count, step / weight, step * weight);
+        r.iteration_seconds += elapsed(tick);
+        r.iterations += accepted;
+        since_restart += accepted;
+        auto st = backend.status();
+        r.rejected_steps += st.rejected_total - previous_rejected;
+        previous_rejected = st.rejected_total;
+        const bool restart_requested = st.restart_requested != 0;
+        r.host_candidate_checks++;
+        tick = Clock::now();
+        backend.candidates(cx, cy, ax, ay);
+        std::vector<double> restart_x, restart_y;
+        if (residual_restarts && restart_requested) {
+            restart_x = ax;
+            restart_y = ay;
+        }
+        auto ox = prep.restore_x(cx);
+        auto oy = prep.restore_y(cy, original.rl.size());
+        auto ca = verifier.evaluate(ox, oy, false);
+        auto aox = prep.restore_x(ax);
+        auto aoy = prep.restore_y(ay, original.rl.size());
+        auto aa = verifier.evaluate(aox, aoy, false);
+        const bool avg = aa.kkt < ca.kkt;
+        if (avg) {
+            cx.swap(ax);
+            cy.swap(ay);
+            ox.swap(aox);
+            oy.swap(aoy);
+            ca = aa;
+        }
+        if (ca.finite && ca.kkt < r.accuracy.kkt) {
+            r.x = ox;
+            r.y = oy;
+            r.accuracy = ca;
+        }
+        r.verification_seconds += elapsed(tick);
+        if (o.verbose)
+            std::fprintf(stderr, "iter=%lld objective=%.12g primal=%.3e dual=%.3e gap=%.3e seconds=%.3f\n",
+                         (long long)r.iterations, original.sense * ca.objective, ca.primal, ca.dual, ca.gap,
+                         elapsed(start));
+        if (!ca.finite) {
+            r.status = "NUMERICAL_ERROR";
+            r.message = "Nonfinite iterate or diagnostic";
+            break;
+        }
+        if (ca.kkt <= o.tol) ca = verifier.evaluate(ox, oy);
+        if (ca.kkt <= o.tol) {
+            r.x = std::move(ox);
+            r.y = std::move(oy);
+            r.accuracy = ca;
+            r.status = "OPTIMAL";
+            break;
+        }
+        // Farkas rays from the iterates and the epoch displacement (solver.cpp:599-633).
+        if (r.iterations >= 1000 && ca.primal > o.tol) {
+            auto displacement = cy;
+            for (size_t i = 0; i < displacement.size(); ++i) displacement[i] -= epochy[i];
+            std::vector<std::vector<double>> rays{oy, aoy, prep.restore_y(displacement, original.rl.size())};
+            for (auto &ray : rays) {
+                double norm_y = 0;
+                bool finite = true;
+                for (double v : ray) {
+                    finite = finite && std::isfinite(v);
+                    norm_y = std::max(norm_y, std::abs(v));
+                }
+                if (!finite || !(norm_y > 0)) continue;
+                for (double &v : ray) v /= norm_y;
+                double margin = verifier.infeasibility_bound(ray);
+                if (margin > 0 && std::isfinite(margin)) {
+                    r.status = "INFEASIBLE";
+                    r.message = "Independently verified box/row Farkas certificate";
+                    r.infeasibility_ray = std::move(ray);
+                    r.certificate_margin = margin;
+                    break;
+                }
+            }
+            if (r.status == "INFEASIBLE") break;
+        }
+        const bool restart_due = residual_restarts ? restart_requested
+                                                   : since_restart >= int64_t(o.check_every) * 2 &&
+                                                         (ca.kkt < .5 * restart_kkt ||
+                                                          since_restart >= std::max<int64_t>(2000, r.iterations / 2));
+        if (o.restart && restart_due) {
+            if (residual_restarts) {
+                cx = std::move(restart_x);
+                cy = std::move(restart_y);
+            }
+            if (o.adaptive || anchored(o) || o.primal_weight == "pid") {  // solver.cpp:702-717
+                long double dx = 0, dy = 0;
+                for (size_t j = 0; j < cx.size(); ++j) dx += (long double)(cx[j] - epochx[j]) * (cx[j] - epochx[j]);
+                for (size_t i = 0; i < cy.size(); ++i) dy += (long double)(cy[i] - epochy[i]) * (cy[i] - epochy[i]);
+                if (dx > 1e-24 && dy > 1e-24) {
+                    if (o.primal_weight == "pid")
+                        weight = controller.update(weight, dx, dy);
+                    else
+                        weight = std::sqrt(weight * std::clamp(std::sqrt(double(dy / dx)), 1e-4, 1e4));
+                    r.weight_updates++;
+                }
+            }
+            backend.reset_candidate(residual_restarts || avg);
+            epochx = cx;
+            epochy = cy;
+            restart_kkt = ca.kkt;
+            since_restart = 0;
+            r.restarts++;
+        }
+        cx.resize(m.cols());
+        cy.resize(m.rows());
+        ax.resize(m.cols());
+        ay.resize(m.rows());
+    }
+    auto check = Clock::now();
+    r.accuracy = verifier.evaluate(r.x, r.y);
+    r.verification_seconds += elapsed(check);
+    if (r.status == "OPTIMAL" && (!r.accuracy.finite || r.accuracy.kkt > o.tol)) r.status = "NUMERICAL_ERROR";
+    r.seconds = elapsed(start);
+    return r;
+}
+}  // namespace
+
+// src/solver.cpp:18-54, restricted to the options this engine implements.
+void validate_options(const Options &o) {
+    if (!(o.tol > 0 && o.tol < 1) || !std::isfinite(o.tol) || o.iteration_limit < 0 || o.time_limit < 0 ||
+        std::isnan(o.time_limit) || o.check_every < 1 || o.scaling_passes < 0 || o.scaling_passes > 20 || o.threads < 1)
+        throw std::runtime_error("Invalid solver options");
+    if (o.device != "auto" && o.device != "cpu" && o.device != "cuda" && o.device != "hip" && o.device != "gpu")
+        throw std::runtime_error("Unknown device");
+    if (o.method != "pdhg" && o.method != "auto" && !anchored(o))
+        throw std::runtime_error("Unknown method: use pdhg, halpern, rhpdhg or r2hpdhg");
+    if (o.primal_weight != "displacement" && o.primal_weight != "pid")
+        throw std::runtime_error("Unknown primal weight controller");
+    if (o.scaling != "ruiz" && o.scaling != "combined") throw std::runtime_error("Unknown scaling: use ruiz or combined");
+    if (o.gpu_indices != "auto" && o.gpu_indices != "32" && o.gpu_indices != "64")
+        throw std::runtime_error("GPU indices must be auto, 32 or 64");
+    if (o.matrix_precision != "fp64" && o.matrix_precision != "mixed")
+        throw std::runtime_error("Matrix precision must be fp64 or mixed");
+    if (anchored(o) && o.matrix_precision == "mixed") throw std::runtime_error("Fixed Halpern operators require FP64 matrices");
+}
+
+// src/solver.cpp:55-105 for continuous LPs.
+Result solve(const Model &m, const Options &options) {
+    auto overall = Clock::now();
+    m.validate();
+    validate_options(options);
+    Options o = options;
+    if (o.method == "auto") o.method = "pdhg";
+    if (o.device == "cpu" || o.device == "cuda") {
+        Result r;
+        r.status = "UNSUPPORTED";
+        r.backend = o.device;
+        r.message = "Requested backend not compiled: " + o.device + " (this engine runs PDHG on HIP; use --device gpu)";
+        return r;
+    }
+    if (anchored(o) && o.adaptive) throw std::runtime_error("Experimental Halpern supports continuous LP with --no-adaptive");
+    if (!o.initial_x.empty() && o.initial_x.size() != size_t(m.cols()))
+        throw std::runtime_error("Warm-start primal dimension mismatch");
+    if (!o.initial_y.empty() && o.initial_y.size() != size_t(m.rows()))
+        throw std::runtime_error("Warm-start dual dimension mismatch");
+    for (double v : o.initial_x)
+        if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
+    for (double v : o.initial_y)
+        if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
+    if (npdhg_device_count() < 1) {
+        Result r;
+        r.status = "UNSUPPORTED";
+        r.message = "No HIP device is visible";
+        return r;
+    }
+    auto result = solve_continuous(m, o);
+    result.seconds = elapsed(overall);
+    return result;
+}
+
+}  // namespace nyk

@@ -1,0 +1,122 @@
+// Host-side LP model, options and results of the MI355X PDHG engine.
+//
+// The types keep the reference's public semantics (include/vantage/vantage.hpp:
+// Sparse 17-24, Model 26-39, Options 40-66, Accuracy 78-82, Result 96-123) for
+// the continuous-LP PDHG path, so that results, JSON keys and defaults line up
+// with Niryukti.  Quadratic and integer models are outside this engine.
+#pragma once
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace nyk {
+
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+struct Entry {
+    int64_t row, col;
+    double value;
+};
+
+// Compressed sparse rows with 64-bit offsets and column indices on the host;
+// the device copy narrows to 32-bit indices when they fit.
+struct Csr {
+    int64_t rows = 0, cols = 0;
+    std::vector<int64_t> ptr, index;
+    std::vector<double> value;
+    int64_t nnz() const { return int64_t(value.size()); }
+    // COO -> CSR: rows ascending, columns ascending, duplicates summed in input
+    // order, exact zeros dropped (src/model.cpp:11-38).
+    static Csr from_entries(int64_t rows, int64_t cols, std::vector<Entry> entries);
+    Csr transposed() const;                                   // src/model.cpp:39-58
+    void multiply(const double *x, double *y) const;          // src/model.cpp:59-73
+};
+
+struct Model {
+    std::string name = "model";
+    Csr A;
+    std::vector<double> c, lb, ub, rl, ru;
+    std::vector<std::string> names, row_names;  // may be empty: x<j> / r<i> implied
+    double offset = 0, sense = 1;               // reported objective = sense * canonical
+    int64_t rows() const { return A.rows; }
+    int64_t cols() const { return A.cols; }
+    void validate() const;            // src/model.cpp:74-109 (LP subset)
+    std::string fingerprint() const;  // src/model.cpp:274-309
+    std::string column_name(int64_t j) const { return names.empty() ? "x" + std::to_string(j) : names[j]; }
+    std::string row_name(int64_t i) const { return row_names.empty() ? "r" + std::to_string(i) : row_names[i]; }
+};
+
+struct Options {
+    std::string device = "gpu";
+    std::string method = "pdhg";           // pdhg | halpern | rhpdhg | r2hpdhg
+    std::string scaling = "ruiz";          // ruiz | combined
+    std::string primal_weight = "displacement";  // displacement | pid
+    std::string matrix_precision = "fp64";       // fp64 | mixed (FP32 matrix values)
+    std::string gpu_indices = "auto";            // auto | 32 | 64
+    double tol = 1e-6, time_limit = 60;
+    int64_t iteration_limit = 100000;
+    int check_every = 100, scaling_passes = 5, threads = 1;
+    bool presolve = true, restart = true, adaptive = true, verbose = false;
+    bool graphs = true;      // hipGraph-captured chunks
+    bool device_checks = true;  // original-space KKT evaluated on the GPU between host certifications
+    std::vector<double> initial_x, initial_y;
+    std::shared_ptr<std::atomic<bool>> cancellation;
+};
+
+struct Accuracy {
+    double objective = kInf, primal = kInf, dual = kInf, gap = kInf, kkt = kInf;
+    double primal_absolute = kInf, dual_absolute = kInf, complementarity = kInf, lower_bound = -kInf;
+    bool finite = false;
+};
+
+struct Result {
+    std::string status = "UNKNOWN", message, backend = "hip", device_name;
+    std::vector<double> x, y, infeasibility_ray;
+    double certificate_margin = 0;
+    Accuracy accuracy;
+    double seconds = 0, preprocess_seconds = 0, transfer_seconds = 0, iteration_seconds = 0,
+           verification_seconds = 0;
+    int64_t iterations = 0, restarts = 0, rejected_steps = 0, removed_columns = 0, removed_rows = 0;
+    int64_t weight_updates = 0, host_candidate_checks = 0, device_candidate_checks = 0;
+    int gpu_index_bits = 0;
+    bool graph_execution = false;
+    std::string matrix_precision = "fp64", method_selected, device_reason;
+    double estimated_gpu_bytes = 0;
+};
+
+// Presolved and scaled LP plus the maps back to the original model
+// (src/internal.hpp:16-25).
+struct Prepared {
+    Model model;
+    std::vector<int64_t> cols, rows, lower_rows, upper_rows;
+    std::vector<double> fixed, column_scale, row_scale;
+    double objective_scale = 1;
+    std::string failure, reason;
+    std::vector<double> failure_ray;
+    std::vector<double> restore_x(const std::vector<double> &x) const;                 // preprocess.cpp:121-126
+    std::vector<double> restore_y(const std::vector<double> &y, size_t original_rows) const;  // :127-132
+};
+
+Prepared prepare(const Model &, const Options &);
+
+class Verifier {
+    const Model &model_;
+    Csr transpose_;
+    std::vector<double> activity_, transpose_product_;
+
+  public:
+    explicit Verifier(const Model &model);
+    const Csr &transpose() const { return transpose_; }
+    Accuracy evaluate(const std::vector<double> &x, const std::vector<double> &y, bool safe_bound = true);
+    double infeasibility_bound(const std::vector<double> &y) const;
+};
+Accuracy verify(const Model &, const std::vector<double> &x, const std::vector<double> &y);
+
+void validate_options(const Options &);
+Result solve(const Model &, const Options &);
+
+}  // namespace nyk

@@ -198,13 +198,12 @@ int cmd_prepare() {
     return 0;
 }
 
-// Raw backend trajectory on the prepared (scaled) model: chunks of `count`
-// advance() calls with fixed tau/sigma, candidates recorded after each chunk.
+// Raw backend trajectory on the generated (unscaled) model: chunks of `count`
+// advance() calls with fixed tau/sigma from x0 = clamp(0, lb, ub), y0 = 0,
+// candidates recorded after each chunk.
 int cmd_steps() {
     auto l = load();
-    auto o = options();
-    auto p = prepare(l.model, o);
-    const auto &m = p.model;
+    const auto &m = l.model;
     std::vector<double> x(m.c.size()), y(m.rl.size());
     for (size_t j = 0; j < x.size(); ++j) x[j] = std::clamp(0.0, m.lb[j], m.ub[j]);
     auto b = cpu_backend(m, x, y, flag("--adaptive"), flag("--halpern"), getd("--reflection", 0), false);

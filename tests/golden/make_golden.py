@@ -1,0 +1,74 @@
+"""Writes the reference's own outputs for seeded workloads as golden fixtures.
+
+Runs the reference CPU build (oracle/_ref/ref_driver, built from the reference
+sources by oracle/ref.mk) and stores its results as JSON data under
+tests/golden/.  Run from the repository root after
+`make -f oracle/ref.mk REF=/root/reference`:
+
+    python tests/golden/make_golden.py
+"""
+import json
+import subprocess
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[2]
+DRIVER = ROOT / "oracle" / "_ref" / "ref_driver"
+OUT = ROOT / "tests" / "golden"
+
+SOLVES = [
+    # (config, seed, scale, index, keep_xy)
+    (0, 42, 1, 0, False),
+    (0, 7, 4, 0, True),
+    (0, 3, 20, 0, True),
+    (1, 42, 400, 0, False),
+    (1, 42, 200, 0, False),
+    (2, 3, 1, 0, True),
+    (2, 3, 1, 1, True),
+    (2, 3, 1, 2, False),
+    (2, 42, 1, 0, False),
+    (2, 42, 1, 7, False),
+    (4, 1, 200, 0, True),
+    (4, 42, 200, 0, False),
+]
+
+
+def run(*args):
+    out = subprocess.run([str(DRIVER), *map(str, args)], check=True, capture_output=True, text=True).stdout
+    return json.loads(out)
+
+
+def case_args(config, seed, scale, index):
+    return ["--config", config, "--seed", seed, "--scale", scale, "--index", index, "--threads", 1]
+
+
+def main():
+    solves = []
+    for config, seed, scale, index, keep in SOLVES:
+        r = run("solve", *case_args(config, seed, scale, index), *(["--xy"] if keep else []))
+        r.update(config=config, seed=seed, scale=scale, index=index)
+        solves.append(r)
+        print(config, seed, scale, index, r["status"], r["iterations"], r["objective"])
+    (OUT / "reference_solves.json").write_text(json.dumps(solves, indent=1) + "\n")
+    prepares = []
+    for config, seed, scale, index, scaling in [(0, 3, 20, 0, "ruiz"), (4, 1, 200, 0, "ruiz"), (2, 3, 1, 1, "combined")]:
+        r = run("prepare", *case_args(config, seed, scale, index), "--scaling", scaling)
+        keep = {k: r[k] for k in ("failure", "objective_scale", "offset", "rows", "cols", "col_map", "row_map",
+                                  "column_scale", "row_scale")}
+        keep.update(config=config, seed=seed, scale=scale, index=index, scaling=scaling)
+        prepares.append(keep)
+    (OUT / "reference_prepare.json").write_text(json.dumps(prepares) + "\n")
+    steps = []
+    for config, seed, scale, index, adaptive in [(2, 3, 1, 0, True), (0, 3, 20, 0, True), (0, 3, 20, 0, False)]:
+        args = ["steps", *case_args(config, seed, scale, index), "--count", 10, "--chunks", 3,
+                "--tau", 0.05, "--sigma", 0.05]
+        if adaptive:
+            args.append("--adaptive")
+        r = run(*args)
+        r.update(config=config, seed=seed, scale=scale, index=index, adaptive=adaptive, count=10, tau=0.05,
+                 sigma=0.05)
+        steps.append(r)
+    (OUT / "reference_steps.json").write_text(json.dumps(steps) + "\n")
+
+
+if __name__ == "__main__":
+    main()
