@@ -1,0 +1,66 @@
+/* MI355X batched / served PDHG engine: C-ABI.
+ *
+ * Many independent LPs share one resident device arena and one kernel launch
+ * at a time: every admitted LP owns one workgroup that runs the complete
+ * restarted-PDHG driver of the reference (src/solver.cpp:511-726: chunked
+ * advance, original-space candidate checks of the current and averaged
+ * iterates, best-candidate tracking, restarts and primal-weight updates) on
+ * the device.  The host prepares each LP (presolve and scaling as the
+ * reference's prepare(), src/preprocess.cpp:133), admits it into a free slot
+ * when the arena has room (continuous batching: admission happens between
+ * launches while other LPs keep iterating), and certifies every claimed
+ * optimum with the reference's FP64 verifier and outward-rounded bound
+ * (src/verify.cpp:12-240) before it is reported OPTIMAL.
+ *
+ * There is no counterpart entry point in the reference: its serving path
+ * (python/niryukti/service.py) runs one CLI process per request.  Results are
+ * reported in the same niryukti_result struct as niryukti_solve_lp().
+ */
+#ifndef NIRYUKTI_BATCH_H
+#define NIRYUKTI_BATCH_H
+#include <stdint.h>
+#include "niryukti_engine.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct niryukti_batch niryukti_batch;
+
+typedef struct {
+    int64_t submitted, queued, active, finished;
+    int64_t arena_bytes, arena_used_bytes;
+    int64_t launches, host_certifications, host_farkas_checks;
+    double device_seconds;  /* host wall time spent waiting on kernel launches */
+    double advance_seconds; /* HIP-event time of the PDHG step kernel launches */
+    double check_seconds;   /* HIP-event time of the candidate-check kernel launches */
+    double advance_bytes;   /* algorithmic HBM bytes of all PDHG attempts run by the step kernel */
+    int64_t attempts;       /* PDHG attempts (accepted + rejected) run by the step kernel */
+} niryukti_batch_stats;
+
+/* opts: solver options shared by every LP of the batch (NULL = defaults).
+ * arena_bytes: HBM budget for resident LPs.  max_slots: resident LP cap. */
+int niryukti_batch_create(const niryukti_options *opts, int64_t arena_bytes, int64_t max_slots,
+                          niryukti_batch **out);
+void niryukti_batch_destroy(niryukti_batch *b);
+/* Prepares and queues one LP (arrays are copied); optional warm start x0/y0
+ * in original space (NULL for a cold start).  Returns its ticket. */
+int niryukti_batch_submit(niryukti_batch *b, const niryukti_lp *lp, const double *x0, const double *y0,
+                          int64_t *ticket);
+/* One scheduling round: admit queued LPs into free slots, run one launch of
+ * up to `checks_per_launch` check periods per resident LP, certify or retire
+ * LPs that stopped.  *running receives the number still queued or resident. */
+int niryukti_batch_step(niryukti_batch *b, int checks_per_launch, int64_t *running);
+/* Admits queued LPs into free slots (uploading them to the arena) without
+ * launching; returns the number admitted in *admitted. */
+int niryukti_batch_admit(niryukti_batch *b, int64_t *admitted);
+/* Runs rounds until every submitted LP has finished. */
+int niryukti_batch_run(niryukti_batch *b, int checks_per_launch);
+/* 1 if the ticket finished (result filled), 0 if still running, <0 on error. */
+int niryukti_batch_result(niryukti_batch *b, int64_t ticket, niryukti_result *r);
+int niryukti_batch_stats_get(niryukti_batch *b, niryukti_batch_stats *s);
+const char *niryukti_batch_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

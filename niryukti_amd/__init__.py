@@ -15,7 +15,7 @@ from pathlib import Path
 import numpy as np
 
 __all__ = [
-    "load_library", "LP", "solve_lp", "Instance", "Backend", "device_count", "device_name",
+    "load_library", "LP", "solve_lp", "Instance", "Backend", "Batch", "device_count", "device_name",
     "ABI_VERSION",
 ]
 
@@ -71,6 +71,14 @@ class _PdhgStatus(C.Structure):
                 ("index_bits", C.c_int), ("factor", C.c_double)]
 
 
+class _BatchStats(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("submitted", "queued", "active", "finished", "arena_bytes",
+                                         "arena_used_bytes", "launches", "host_certifications",
+                                         "host_farkas_checks")] + \
+        [(k, C.c_double) for k in ("device_seconds", "advance_seconds", "check_seconds", "advance_bytes")] + \
+        [("attempts", C.c_int64)]
+
+
 _lib = None
 
 
@@ -103,6 +111,15 @@ def load_library() -> C.CDLL:
     lib.npdhg_reset.argtypes = [C.c_void_p, _f64p, _f64p]
     lib.npdhg_reset_candidate.argtypes = [C.c_void_p, C.c_int]
     lib.npdhg_spmv.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p]
+    lib.niryukti_batch_create.argtypes = [C.POINTER(_Options), C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]
+    lib.niryukti_batch_destroy.argtypes = [C.c_void_p]
+    lib.niryukti_batch_submit.argtypes = [C.c_void_p, C.POINTER(_LP), _f64p, _f64p, C.POINTER(C.c_int64)]
+    lib.niryukti_batch_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]
+    lib.niryukti_batch_run.argtypes = [C.c_void_p, C.c_int]
+    lib.niryukti_batch_admit.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    lib.niryukti_batch_result.argtypes = [C.c_void_p, C.c_int64, C.POINTER(_Result)]
+    lib.niryukti_batch_stats_get.argtypes = [C.c_void_p, C.POINTER(_BatchStats)]
+    lib.niryukti_batch_last_error.restype = C.c_char_p
     _lib = lib
     return lib
 
@@ -162,15 +179,9 @@ class LP:
         return cls(np.cumsum(ptr), cols, A[rows, cols], c, lb, ub, rl, ru, offset, sense)
 
 
-_OPTION_STRINGS = ("device", "method", "scaling", "primal_weight", "matrix_precision", "gpu_indices")
-
-
-def solve_lp(lp: LP, **options) -> dict:
-    """Solves `lp` with the GPU engine; keyword options mirror the reference's Options."""
-    lib = load_library()
+def _options_struct(lib, options: dict, keep: list) -> _Options:
     opts = _Options()
     lib.niryukti_default_options(C.byref(opts))
-    keep = []
     for key, val in options.items():
         if key in ("initial_x", "initial_y"):
             arr = _f64(val)
@@ -186,6 +197,26 @@ def solve_lp(lp: LP, **options) -> dict:
             setattr(opts, key, val)
         else:
             raise ValueError(f"unknown option {key!r}")
+    return opts
+
+
+def _result_dict(res: _Result, x, y) -> dict:
+    out = {name: getattr(res, name) for name, _ in _Result._fields_ if name not in ("status", "message", "x", "y")}
+    out["status"] = res.status.decode()
+    out["message"] = res.message.decode()
+    out["x"] = x
+    out["y"] = y
+    return out
+
+
+_OPTION_STRINGS = ("device", "method", "scaling", "primal_weight", "matrix_precision", "gpu_indices")
+
+
+def solve_lp(lp: LP, **options) -> dict:
+    """Solves `lp` with the GPU engine; keyword options mirror the reference's Options."""
+    lib = load_library()
+    keep = []
+    opts = _options_struct(lib, options, keep)
     x = np.zeros(lp.cols)
     y = np.zeros(lp.rows)
     res = _Result()
@@ -195,12 +226,7 @@ def solve_lp(lp: LP, **options) -> dict:
     rc = lib.niryukti_solve_lp(C.byref(s), C.byref(opts), C.byref(res))
     if rc != 0:
         raise RuntimeError(lib.niryukti_engine_last_error().decode())
-    out = {name: getattr(res, name) for name, _ in _Result._fields_ if name not in ("status", "message", "x", "y")}
-    out["status"] = res.status.decode()
-    out["message"] = res.message.decode()
-    out["x"] = x
-    out["y"] = y
-    return out
+    return _result_dict(res, x, y)
 
 
 class Instance:
@@ -286,6 +312,70 @@ class Backend:
     def close(self):
         if getattr(self, "_h", None):
             self._lib.npdhg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Batch:
+    """Batched / served engine (include/niryukti_batch.h): many LPs resident in one
+    HBM arena, one workgroup per LP, continuous admission between launches."""
+
+    def __init__(self, arena_bytes: int = 8 << 30, max_slots: int = 4096, **options):
+        self._lib = load_library()
+        self._keep = []
+        opts = _options_struct(self._lib, options, self._keep)
+        h = C.c_void_p()
+        self._check(self._lib.niryukti_batch_create(C.byref(opts), int(arena_bytes), int(max_slots), C.byref(h)))
+        self._h = h
+        self._shapes = {}
+
+    def _check(self, rc):
+        if rc < 0:
+            raise RuntimeError(self._lib.niryukti_batch_last_error().decode())
+        return rc
+
+    def submit(self, lp: LP, x0=None, y0=None) -> int:
+        s = lp._struct()
+        px = _ptr(_f64(x0)) if x0 is not None else None
+        py = _ptr(_f64(y0)) if y0 is not None else None
+        keep = (x0, y0)
+        t = C.c_int64()
+        self._check(self._lib.niryukti_batch_submit(self._h, C.byref(s), px, py, C.byref(t)))
+        del keep
+        self._shapes[t.value] = (lp.cols, lp.rows)
+        return t.value
+
+    def step(self, checks_per_launch: int = 1) -> int:
+        running = C.c_int64()
+        self._check(self._lib.niryukti_batch_step(self._h, int(checks_per_launch), C.byref(running)))
+        return running.value
+
+    def admit(self) -> int:
+        n = C.c_int64()
+        self._check(self._lib.niryukti_batch_admit(self._h, C.byref(n)))
+        return n.value
+
+    def run(self, checks_per_launch: int = 5) -> None:
+        self._check(self._lib.niryukti_batch_run(self._h, int(checks_per_launch)))
+
+    def result(self, ticket: int):
+        n, m = self._shapes[ticket]
+        x, y = np.zeros(n), np.zeros(m)
+        res = _Result()
+        res.x, res.y = _ptr(x), _ptr(y)
+        done = self._check(self._lib.niryukti_batch_result(self._h, int(ticket), C.byref(res)))
+        return _result_dict(res, x, y) if done else None
+
+    def stats(self) -> dict:
+        st = _BatchStats()
+        self._check(self._lib.niryukti_batch_stats_get(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _BatchStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.niryukti_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

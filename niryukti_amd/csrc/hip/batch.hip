@@ -1,0 +1,1470 @@
+// Batched and served restarted PDHG on MI355X (gfx950): one workgroup per LP.
+//
+// Every resident LP owns one 256-thread workgroup, which runs the reference's continuous driver for that LP entirely on the device
+// (src/solver.cpp:511-726): chunks of `check_every` accepted adaptive PDHG
+// steps (src/cpu.cpp:96-222), activity rebasing, original-space evaluation of
+// the current and averaged candidates (src/verify.cpp:103-240 formulae),
+// best-candidate tracking, the restart rule and the displacement primal-weight
+// update.  Certification stays on the host: a workgroup that finds a candidate
+// with device kkt <= tol parks its LP, and the host re-evaluates it with the
+// reference's FP64 verifier and outward-rounded Lagrangian bound before it is
+// reported OPTIMAL; a park that fails certification resumes the LP exactly
+// where the reference would continue (restart rule with the safe kkt).  The
+// reference's Farkas test (src/solver.cpp:599-633) is preceded on the device by
+// a conservative upper bound of the ray's Lagrangian value; only rays whose
+// bound cannot be shown non-positive go to the host's exact check.
+//
+// Each LP is reduced by its own workgroup in a fixed order, so an LP's result
+// is bit-identical whether it runs alone or among thousands of others.
+//
+// The host side admits LPs into a fixed HBM arena as room frees up
+// (continuous batching between launches) and retires finished ones.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine/lp.hpp"
+#include "engine/parallel.hpp"
+#include "niryukti_batch.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+#define HIP_OK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxRejects = 40;
+
+enum Status : int {
+    kRunning = 0,
+    kParkOptimal = 1,  // candidate with device kkt <= tol: host certifies
+    kParkFarkas = 2,   // a Farkas ray could not be ruled out on the device: host checks
+    kIterationLimit = 3,
+    kNumerical = 4,
+    kExhausted = 5,
+};
+
+// Device pointers and constants of one resident LP.
+struct LpDev {
+    int n, m, n0, m0;
+    const int *ap, *ai, *tp, *ti;  // scaled A (m rows) and A^T (n rows)
+    const double *av, *tv;
+    const double *c, *lb, *ub, *rl, *ru;
+    const int *op, *oi, *otp, *oti;  // original A (m0 rows) and A^T (n0 rows)
+    const double *ov, *otv;
+    const double *c0, *lb0, *ub0, *rl0, *ru0;
+    const int *cols, *lower_rows, *upper_rows;
+    const double *fixed, *dc, *dr;
+    double sobj, offset0, cscale;
+    double *x[2], *xa[2], *dx, *y[2], *ya[2], *ax[2];
+    double *epx, *epy;
+    double *ox[2], *oy[2], *act, *aty, *atyabs;
+    double *bx, *by;
+    const int *ablk, *tblk, *oblk, *otblk;  // row blocks of A, A^T, original A, original A^T
+    int nab, ntb, nob, notb;
+};
+
+struct LpState {
+    double step, weight, factor, mass, restart_kkt, best_kkt, cand_kkt;
+    long long iterations, since_restart, accepted, rejected, restarts, weight_updates, checks, iteration_limit;
+    int cur, status, phase, avg;
+    int consecutive_rejects, farkas_parks;
+};
+
+struct BatchOpts {
+    double tol;
+    int check_every, adaptive, restart, pad;
+};
+
+// ------------------------------------------------------------ reductions --
+
+struct DD {
+    double hi, lo;
+};
+__device__ __forceinline__ void dd_add(DD &a, double b) {
+    const double s = a.hi + b;
+    const double bb = s - a.hi;
+    const double e = (a.hi - (s - bb)) + (b - bb);
+    a.hi = s;
+    a.lo += e;
+}
+__device__ __forceinline__ DD dd_merge(DD a, DD b) {
+    const double s = a.hi + b.hi;
+    const double bb = s - a.hi;
+    double e = (a.hi - (s - bb)) + (b.hi - bb);
+    e += a.lo + b.lo;
+    const double h = s + e;
+    return DD{h, e - (h - s)};
+}
+__device__ __forceinline__ double dd_value(DD a) { return a.hi + a.lo; }
+
+__device__ __forceinline__ double maxd(double a, double b) { return a < b ? b : a; }  // std::max
+__device__ __forceinline__ double mind(double a, double b) { return b < a ? b : a; }  // std::min
+__device__ __forceinline__ double clampd(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+// Reduces NS sums, NM maxima and ND double-double sums over the workgroup in
+// a fixed order; results valid in every thread on return.
+template <int NS, int NM, int ND>
+struct Reducer {
+    double s[NS > 0 ? NS : 1], mx[NM > 0 ? NM : 1];
+    DD d[ND > 0 ? ND : 1];
+};
+
+template <int NS, int NM, int ND>
+__device__ void block_reduce(Reducer<NS, NM, ND> &r, double *lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) r.s[k] += __shfl_xor(r.s[k], off, 64);
+#pragma unroll
+        for (int k = 0; k < NM; ++k) r.mx[k] = maxd(r.mx[k], __shfl_xor(r.mx[k], off, 64));
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            DD o{__shfl_xor(r.d[k].hi, off, 64), __shfl_xor(r.d[k].lo, off, 64)};
+            r.d[k] = dd_merge(r.d[k], o);
+        }
+    }
+    constexpr int W = NS + NM + 2 * ND;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) lds[wave * W + k] = r.s[k];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) lds[wave * W + NS + k] = r.mx[k];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            lds[wave * W + NS + NM + 2 * k] = r.d[k].hi;
+            lds[wave * W + NS + NM + 2 * k + 1] = r.d[k].lo;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NS; ++k) r.s[k] = (lds[k] + lds[W + k]) + (lds[2 * W + k] + lds[3 * W + k]);
+#pragma unroll
+    for (int k = 0; k < NM; ++k)
+        r.mx[k] = maxd(maxd(lds[NS + k], lds[W + NS + k]), maxd(lds[2 * W + NS + k], lds[3 * W + NS + k]));
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+        const int b = NS + NM + 2 * k;
+        DD acc{lds[b], lds[b + 1]};
+        for (int w = 1; w < kWaves; ++w) acc = dd_merge(acc, DD{lds[w * W + b], lds[w * W + b + 1]});
+        r.d[k] = acc;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------ primitives --
+
+// CSR "stream" SpMV over consecutive row blocks of at most kChunk nonzeros:
+// the workgroup computes the block's products with independent, coalesced
+// loads into LDS, then each thread sums whole rows from LDS left to right
+// starting at init(r) (the same order as the reference's sequential row sums,
+// so the row values are bit-identical to the CPU's).  A row longer than
+// kChunk forms its own block and is reduced by the workgroup in a fixed tree.
+// epi(r, sum, abs_sum) consumes each row.
+constexpr int kChunk = 2048;
+
+template <bool ABS, class Init, class Epi>
+__device__ void stream_rows(const int *ptr, const int *idx, const double *val, const int *blk, int nblk,
+                            const double *vec, double *stage, double *red, Init init, Epi epi) {
+    for (int b = 0; b < nblk; ++b) {
+        const int r0 = blk[b], r1 = blk[b + 1];
+        const int k0 = ptr[r0], k1 = ptr[r1];
+        if (k1 - k0 <= kChunk) {
+            for (int k = k0 + threadIdx.x; k < k1; k += kThreads) stage[k - k0] = val[k] * vec[idx[k]];
+            __syncthreads();
+            for (int r = r0 + threadIdx.x; r < r1; r += kThreads) {
+                double acc = init(r), a = 0;
+                const int e = ptr[r + 1] - k0;
+                for (int k = ptr[r] - k0; k < e; ++k) {
+                    acc += stage[k];
+                    if (ABS) a += fabs(stage[k]);
+                }
+                epi(r, acc, a);
+            }
+            __syncthreads();
+        } else {
+            Reducer<2, 0, 0> rr{};
+            for (int k = k0 + threadIdx.x; k < k1; k += kThreads) {
+                const double t = val[k] * vec[idx[k]];
+                rr.s[0] += t;
+                if (ABS) rr.s[1] += fabs(t);
+            }
+            block_reduce(rr, red);
+            if (threadIdx.x == 0) epi(r0, init(r0) + rr.s[0], rr.s[1]);
+            __syncthreads();
+        }
+    }
+}
+
+__device__ void spmv(const int *ptr, const int *idx, const double *val, const int *blk, int nblk, const double *x,
+                     double *out, double *stage, double *red) {
+    stream_rows<false>(ptr, idx, val, blk, nblk, x, stage, red, [](int) { return 0.0; },
+                       [&](int r, double v, double) { out[r] = v; });
+}
+__device__ void copy_vec(double *dst, const double *src, int n) {
+    for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = src[i];
+}
+__device__ void zero_vec(double *dst, int n) {
+    for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = 0;
+}
+
+// Original-space accuracy of one candidate (the non-certifying evaluate of
+// src/verify.cpp:103-240 with compute_safe_bound = false) plus the device
+// pre-filter of the Farkas test on its dual.
+struct Acc {
+    double objective, primal, dual, gap, kkt;
+    int finite;
+    int farkas_possible;  // the dual as a Farkas ray could not be shown non-positive
+};
+
+constexpr double kEps = 2.220446049250313e-16;
+
+// Upper bound of min over the box of g * x for g in [lo, hi] (concave in g).
+__device__ __forceinline__ double phi(double g, double l, double u) {
+    if (g > 0) return isfinite(l) ? g * l : -INFINITY;
+    if (g < 0) return isfinite(u) ? g * u : -INFINITY;
+    return 0;
+}
+
+// Farkas pre-filter over columns for a ray whose A^T ray is in aty/atyabs.
+// Adds into r: s[0] += sum of per-column upper bounds, s[1] += their |.|,
+// s[2] += Lipschitz slack, mx[0] = 1 when a column bound is -inf.
+template <int NS, int NM, int ND>
+__device__ void farkas_columns(const LpDev &L, Reducer<NS, NM, ND> &r) {
+    for (int j = threadIdx.x; j < L.n0; j += kThreads) {
+        const double g = L.aty[j], ga = L.atyabs[j];
+        const int len = L.otp[j + 1] - L.otp[j];
+        const double e = double(len + 4) * 2 * kEps * ga;
+        const double l = L.lb0[j], u = L.ub0[j];
+        double ub = maxd(phi(g - e, l, u), phi(g + e, l, u));
+        if (g - e <= 0 && g + e >= 0) ub = maxd(ub, 0.0);
+        if (ub == -INFINITY) r.mx[0] = 1;
+        else {
+            r.s[0] += ub;
+            r.s[1] += fabs(ub);
+            const double box = maxd(isfinite(l) ? fabs(l) : 0.0, isfinite(u) ? fabs(u) : 0.0);
+            r.s[2] += ga * box;
+        }
+    }
+}
+
+__device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double *ox, double *oy, double *lds,
+                        double *stage) {
+    // restore_x / restore_y (src/preprocess.cpp:121-132)
+    copy_vec(ox, L.fixed, L.n0);
+    zero_vec(oy, L.m0);
+    __syncthreads();
+    for (int j = threadIdx.x; j < L.n; j += kThreads) ox[L.cols[j]] = x[j] * L.dc[j];
+    for (int i = threadIdx.x; i < L.m; i += kThreads) {
+        const int t = y[i] >= 0 ? L.upper_rows[i] : L.lower_rows[i];
+        oy[t] = 0.0 + y[i] * L.dr[i] * L.sobj;
+    }
+    __syncthreads();
+    spmv(L.op, L.oi, L.ov, L.oblk, L.nob, ox, L.act, stage, lds);
+    stream_rows<true>(L.otp, L.oti, L.otv, L.otblk, L.notb, oy, stage, lds, [](int) { return 0.0; },
+                      [&](int j, double v, double a) {
+                          L.aty[j] = v;
+                          L.atyabs[j] = a;
+                      });
+    __syncthreads();
+    // s : objective, Farkas column bound sum, its |.| sum, Lipschitz slack, sum |y b|
+    // mx: nonfinite, Farkas column -inf, primal, primal_abs, dual, dual_abs,
+    //     y != 0 at an infinite row endpoint, infinite Lagrangian box argument
+    // d : support, complement, box_min, sum y b over rows
+    Reducer<5, 8, 4> r{};
+    for (int i = threadIdx.x; i < L.m0; i += kThreads) {
+        const double a = L.act[i], yi = oy[i], lo = L.rl0[i], hi = L.ru0[i];
+        if (!isfinite(a) || !isfinite(yi)) r.mx[0] = 1;
+        const double v = maxd(maxd(0., lo - a), a - hi);
+        if (isfinite(lo)) r.mx[2] = maxd(r.mx[2], maxd(0., lo - a) / (1 + fabs(lo)));
+        if (isfinite(hi)) r.mx[2] = maxd(r.mx[2], maxd(0., a - hi) / (1 + fabs(hi)));
+        r.mx[3] = maxd(r.mx[3], v);
+        if (yi != 0) {
+            const double endpoint = yi > 0 ? hi : lo;
+            if (isfinite(endpoint)) {
+                dd_add(r.d[0], yi * endpoint);
+                dd_add(r.d[1], fabs(yi * (endpoint - a)));
+                dd_add(r.d[3], yi * endpoint);
+                r.s[4] += fabs(yi * endpoint);
+            } else {
+                r.mx[6] = 1;  // no Lagrangian bound; as a Farkas ray its value is -inf
+                r.mx[4] = maxd(r.mx[4], fabs(yi));
+                r.mx[5] = maxd(r.mx[5], fabs(yi));
+            }
+        }
+    }
+    for (int j = threadIdx.x; j < L.n0; j += kThreads) {
+        const double xj = ox[j], l = L.lb0[j], u = L.ub0[j], c = L.c0[j];
+        if (!isfinite(xj)) r.mx[0] = 1;
+        const double v = maxd(maxd(0., l - xj), xj - u);
+        r.mx[2] = maxd(r.mx[2], v / (1 + fabs(xj)));
+        r.mx[3] = maxd(r.mx[3], v);
+        r.s[0] += c * xj;
+        const double linear = c + L.aty[j];
+        if (!isfinite(linear)) r.mx[0] = 1;
+        const double d = clampd(linear, xj - u, xj - l);
+        r.mx[5] = maxd(r.mx[5], fabs(d));
+        r.mx[4] = maxd(r.mx[4], fabs(d) / L.cscale);
+        if (linear > 0 && isfinite(l)) dd_add(r.d[1], fabs(linear * (xj - l)));
+        if (linear < 0 && isfinite(u)) dd_add(r.d[1], fabs(linear * (u - xj)));
+        const double arg = linear > 0 ? l : linear < 0 ? u : clampd(0., l, u);
+        if (!isfinite(arg)) r.mx[7] = 1;  // no finite Lagrangian box bound
+        else dd_add(r.d[2], linear * arg);
+        // Farkas column bound for the candidate's dual as a ray
+        const double g = L.aty[j], ga = L.atyabs[j];
+        const int len = L.otp[j + 1] - L.otp[j];
+        const double e = double(len + 4) * 2 * kEps * ga;
+        double ubd = maxd(phi(g - e, l, u), phi(g + e, l, u));
+        if (g - e <= 0 && g + e >= 0) ubd = maxd(ubd, 0.0);
+        if (ubd == -INFINITY) r.mx[1] = 1;
+        else {
+            r.s[1] += ubd;
+            r.s[2] += fabs(ubd);
+            const double box = maxd(isfinite(l) ? fabs(l) : 0.0, isfinite(u) ? fabs(u) : 0.0);
+            r.s[3] += ga * box;
+        }
+    }
+    block_reduce(r, lds);
+    Acc a{};
+    const double support = dd_value(r.d[0]), complement = dd_value(r.d[1]), box_min = dd_value(r.d[2]);
+    a.objective = L.offset0 + r.s[0];
+    a.primal = r.mx[2];
+    a.dual = r.mx[4];
+    a.gap = complement / (1 + fabs(a.objective - L.offset0));
+    const bool bound_ok = r.mx[6] == 0 && r.mx[7] == 0;
+    if (bound_ok) {
+        const double lower = (L.offset0 - support) + box_min;
+        const double guard = 64 * kEps * (1 + fabs(L.offset0) + fabs(support) + fabs(box_min));
+        const double lb = lower - guard;
+        a.gap = maxd(a.gap, fabs(a.objective - lb) / (1 + fabs(a.objective) + fabs(lb)));
+    }
+    a.kkt = maxd(maxd(a.primal, a.dual), a.gap);
+    a.finite = r.mx[0] == 0 && isfinite(a.objective) && isfinite(a.kkt);
+    if (!a.finite) a.kkt = INFINITY;
+    // Farkas: rows with y != 0 at an infinite endpoint, or a column bound of
+    // -inf, make the ray's value -inf; otherwise U = sum(col bounds) - sum(y b).
+    const bool neg_inf = r.mx[1] != 0 || r.mx[6] != 0;
+    const double value = r.s[1] - dd_value(r.d[3]);
+    const double slack = double(L.n0 + L.m0 + 16) * 4 * kEps * (r.s[2] + r.s[4] + r.s[3]);
+    a.farkas_possible = !(neg_inf || value + slack <= 0);
+    return a;
+}
+
+__device__ void attempt(const LpDev &L, LpState &S, const BatchOpts &o, double tau0, double sigma0, double *lds,
+                        double *stage, int *accept_flag) {
+    const int cur = S.cur;
+    const double f = S.factor, tau = tau0 * f, sigma = sigma0 * f, w = f / (S.mass + f);
+    const double *x = L.x[cur], *xa = L.xa[cur], *y = L.y[cur], *ya = L.ya[cur], *ax = L.ax[cur];
+    double *xn = L.x[cur ^ 1], *xan = L.xa[cur ^ 1], *yn = L.y[cur ^ 1], *yan = L.ya[cur ^ 1], *axn = L.ax[cur ^ 1];
+    Reducer<3, 0, 0> r{};
+    // g = c + A^T y and the projected primal step (src/cpu.cpp:107-132)
+    stream_rows<false>(L.tp, L.ti, L.tv, L.tblk, L.ntb, y, stage, lds, [&](int j) { return L.c[j]; },
+                       [&](int j, double g, double) {
+                           const double xj = x[j];
+                           const double t = clampd(xj - tau * g, L.lb[j], L.ub[j]);
+                           const double d = t - xj;
+                           r.s[0] += d * d;
+                           L.dx[j] = d;
+                           xn[j] = t;
+                           const double v = xa[j];
+                           xan[j] = v + (t - v) * w;
+                       });
+    // delta = A dx and the extrapolated dual step (src/cpu.cpp:136-150)
+    stream_rows<false>(L.ap, L.ai, L.av, L.ablk, L.nab, L.dx, stage, lds, [](int) { return 0.0; },
+                       [&](int i, double delta, double) {
+                           const double a = ax[i], bar = a + 2 * delta, yi = y[i], z = yi + sigma * bar;
+                           const double t = maxd(0., z - sigma * L.ru[i]) + mind(0., z - sigma * L.rl[i]);
+                           const double dy = t - yi;
+                           r.s[1] += (!isfinite(bar) || !isfinite(t)) ? INFINITY : dy * dy;
+                           r.s[2] += dy * delta;
+                           yn[i] = t;
+                           const double v = ya[i];
+                           yan[i] = v + (t - v) * w;
+                           axn[i] = a + delta;
+                       });
+    block_reduce(r, lds);
+    if (threadIdx.x == 0) {
+        // src/cpu.cpp:151-172
+        const double energy = r.s[0] / tau + r.s[1] / sigma, coupling = r.s[2];
+        const double limit = fabs(coupling) > 0 ? energy / (2 * fabs(coupling)) : INFINITY;
+        const bool finite = isfinite(energy) && isfinite(coupling);
+        const bool accept = finite && (!o.adaptive || limit >= 1);
+        if (o.adaptive) {
+            const double k = double(S.accepted + 2);
+            double mult = mind((1 - pow(k, -.3)) * limit, 1 + pow(k, -.6));
+            if (!finite || !isfinite(mult) || mult <= 0) mult = .5;
+            S.factor = clampd(f * mult, 1e-12, 1e12);
+        }
+        if (accept) {
+            S.cur ^= 1;
+            S.accepted += 1;
+            S.mass += f;
+            S.consecutive_rejects = 0;
+        } else {
+            S.rejected += 1;
+            S.consecutive_rejects += 1;
+        }
+        *accept_flag = accept ? 1 : (o.adaptive && S.consecutive_rejects < kMaxRejects ? 0 : -1);
+    }
+    __syncthreads();
+}
+
+// Phase of a resident LP between the two kernels.
+enum Phase : int { kAdvance = 0, kResume = 1, kCheck = 2 };
+
+// Hot loop: `check_every` accepted steps for every running LP whose next
+// action is to advance (src/cpu.cpp:103-220 per step).
+__global__ __launch_bounds__(kThreads) void batch_advance_kernel(const LpDev *lps, LpState *states, const int *slots,
+                                                                 BatchOpts o) {
+    const int s = slots[blockIdx.x];
+    __shared__ LpState S;
+    __shared__ double lds[kWaves * 4];
+    __shared__ double stage[kChunk];
+    __shared__ int flag;
+    if (threadIdx.x == 0) S = states[s];
+    __syncthreads();
+    if (S.status != kRunning || S.phase != kAdvance) return;
+    const LpDev &L = lps[s];
+    const long long remaining = S.iteration_limit - S.iterations;
+    if (remaining <= 0) {
+        if (threadIdx.x == 0) states[s].status = kIterationLimit;
+        return;
+    }
+    const long long count = remaining < o.check_every ? remaining : o.check_every;
+    const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
+    for (long long done = 0; done < count;) {
+        attempt(L, S, o, tau0, sigma0, lds, stage, &flag);
+        if (flag == 1) ++done;
+        else if (flag < 0) {
+            if (threadIdx.x == 0) S.status = kExhausted;
+            break;
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (S.status == kRunning) {
+            S.iterations += count;
+            S.since_restart += count;
+            S.checks += 1;
+            S.phase = kCheck;
+        }
+        states[s] = S;
+    }
+}
+
+// Check period end (src/solver.cpp:550-725): candidate evaluation, best
+// tracking, parking for host certification / Farkas checks, restart rule.
+__global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps, LpState *states, const int *slots,
+                                                               BatchOpts o) {
+    const int s = slots[blockIdx.x];
+    __shared__ LpState S;
+    __shared__ double lds[kWaves * 24];
+    __shared__ double stage[kChunk];
+    if (threadIdx.x == 0) S = states[s];
+    __syncthreads();
+    if (S.status != kRunning || S.phase == kAdvance) return;
+    const LpDev &L = lps[s];
+    const int cur = S.cur;
+    double cand_kkt;
+    int avg;
+    if (S.phase == kCheck) {
+        // candidates(): rebase the activity recurrence (src/cpu.cpp:229-230)
+        spmv(L.ap, L.ai, L.av, L.ablk, L.nab, L.x[cur], L.ax[cur], stage, lds);
+        const Acc cur_acc = evaluate(L, L.x[cur], L.y[cur], L.ox[0], L.oy[0], lds, stage);
+        const Acc avg_acc = evaluate(L, L.xa[cur], L.ya[cur], L.ox[1], L.oy[1], lds, stage);
+        avg = avg_acc.kkt < cur_acc.kkt ? 1 : 0;
+        const Acc ca = avg ? avg_acc : cur_acc;
+        const bool better = ca.finite && ca.kkt < S.best_kkt;
+        if (better) {
+            copy_vec(L.bx, L.ox[avg], L.n0);
+            copy_vec(L.by, L.oy[avg], L.m0);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (better) S.best_kkt = ca.kkt;
+            S.avg = avg;
+            S.cand_kkt = ca.kkt;
+            S.phase = kAdvance;
+            if (!ca.finite) S.status = kNumerical;
+            else if (ca.kkt <= o.tol) S.status = kParkOptimal;
+        }
+        __syncthreads();
+        if (S.status != kRunning) {
+            if (threadIdx.x == 0) states[s] = S;
+            return;
+        }
+        if (S.iterations >= 1000 && ca.primal > o.tol) {
+            bool possible = cur_acc.farkas_possible || avg_acc.farkas_possible;
+            if (!possible) {
+                // displacement ray restore_y(cy - epochy) (src/solver.cpp:603-610)
+                const double *cy = avg ? L.ya[cur] : L.y[cur];
+                zero_vec(L.oy[0], L.m0);
+                __syncthreads();
+                for (int i = threadIdx.x; i < L.m; i += kThreads) {
+                    const double d = cy[i] - L.epy[i];
+                    const int t = d >= 0 ? L.upper_rows[i] : L.lower_rows[i];
+                    L.oy[0][t] = 0.0 + d * L.dr[i] * L.sobj;
+                }
+                __syncthreads();
+                stream_rows<true>(L.otp, L.oti, L.otv, L.otblk, L.notb, L.oy[0], stage, lds, [](int) { return 0.0; },
+                                  [&](int j, double v, double a) {
+                                      L.aty[j] = v;
+                                      L.atyabs[j] = a;
+                                  });
+                __syncthreads();
+                Reducer<4, 2, 1> r{};
+                for (int i = threadIdx.x; i < L.m0; i += kThreads) {
+                    const double yi = L.oy[0][i];
+                    if (!isfinite(yi)) r.mx[1] = 1;
+                    if (yi != 0) {
+                        const double e = yi > 0 ? L.ru0[i] : L.rl0[i];
+                        if (!isfinite(e)) r.mx[0] = 1;
+                        else {
+                            dd_add(r.d[0], yi * e);
+                            r.s[3] += fabs(yi * e);
+                        }
+                    }
+                }
+                farkas_columns(L, r);
+                block_reduce(r, lds);
+                const double value = r.s[0] - dd_value(r.d[0]);
+                const double slack = double(L.n0 + L.m0 + 16) * 4 * kEps * (r.s[1] + r.s[2] + r.s[3]);
+                // all-zero and nonfinite rays are skipped by the reference
+                possible = !(r.mx[0] != 0 || r.mx[1] != 0 || value + slack <= 0);
+            }
+            if (possible) {
+                if (threadIdx.x == 0) {
+                    S.status = kParkFarkas;
+                    S.farkas_parks += 1;
+                    states[s] = S;
+                }
+                return;
+            }
+        }
+        cand_kkt = ca.kkt;
+    } else {
+        // resume after a host check that did not finish the LP
+        cand_kkt = S.cand_kkt;
+        avg = S.avg;
+        __syncthreads();
+        if (threadIdx.x == 0) S.phase = kAdvance;
+    }
+    // restart rule and primal weight (src/solver.cpp:692-725)
+    const bool due = S.since_restart >= 2LL * o.check_every &&
+                     (cand_kkt < .5 * S.restart_kkt ||
+                      S.since_restart >= (S.iterations / 2 > 2000 ? S.iterations / 2 : 2000));
+    if (o.restart && due) {
+        const double *cx = avg ? L.xa[cur] : L.x[cur];
+        const double *cy = avg ? L.ya[cur] : L.y[cur];
+        Reducer<0, 0, 2> r{};
+        for (int j = threadIdx.x; j < L.n; j += kThreads) {
+            const double d = cx[j] - L.epx[j];
+            dd_add(r.d[0], d * d);
+        }
+        for (int i = threadIdx.x; i < L.m; i += kThreads) {
+            const double d = cy[i] - L.epy[i];
+            dd_add(r.d[1], d * d);
+        }
+        block_reduce(r, lds);
+        if (threadIdx.x == 0 && o.adaptive) {
+            const double dx = dd_value(r.d[0]), dy = dd_value(r.d[1]);
+            if (dx > 1e-24 && dy > 1e-24) {
+                S.weight = sqrt(S.weight * clampd(sqrt(dy / dx), 1e-4, 1e4));
+                S.weight_updates += 1;
+            }
+        }
+        if (avg) {
+            copy_vec(L.x[cur], L.xa[cur], L.n);
+            copy_vec(L.y[cur], L.ya[cur], L.m);
+            __syncthreads();
+        }
+        spmv(L.ap, L.ai, L.av, L.ablk, L.nab, L.x[cur], L.ax[cur], stage, lds);
+        copy_vec(L.epx, L.x[cur], L.n);
+        copy_vec(L.epy, L.y[cur], L.m);
+        zero_vec(L.xa[cur], L.n);
+        zero_vec(L.ya[cur], L.m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            S.mass = 0;
+            S.restart_kkt = cand_kkt;
+            S.since_restart = 0;
+            S.restarts += 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) states[s] = S;
+}
+
+// ------------------------------------------------------------------ host --
+
+using Clock = std::chrono::steady_clock;
+
+size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+// First-fit allocator over one device arena.
+class Arena {
+  public:
+    void init(size_t bytes) {
+        HIP_OK(hipMalloc(&base_, bytes));
+        size_ = bytes;
+        free_[0] = bytes;
+    }
+    ~Arena() {
+        if (base_) hipFree(base_);
+    }
+    bool allocate(size_t bytes, size_t &offset) {
+        bytes = align_up(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= bytes) {
+                offset = it->first;
+                size_t rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[offset + bytes] = rest;
+                used_ += bytes;
+                return true;
+            }
+        return false;
+    }
+    void release(size_t offset, size_t bytes) {
+        bytes = align_up(bytes);
+        used_ -= bytes;
+        auto it = free_.emplace(offset, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    char *base() const { return static_cast<char *>(base_); }
+    size_t size() const { return size_; }
+    size_t used() const { return used_; }
+
+  private:
+    void *base_ = nullptr;
+    size_t size_ = 0, used_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+struct Job {
+    int64_t ticket = 0;
+    nyk::Model original;
+    nyk::Prepared prep;
+    std::unique_ptr<nyk::Verifier> verifier;
+    nyk::Result result;
+    bool finished = false;
+    std::vector<double> x, y;  // scaled start point
+    double step = 1, weight = 1;
+    Clock::time_point start;
+    int slot = -1;
+    size_t offset = 0, bytes = 0;
+};
+
+// Splits CSR rows into consecutive blocks of at most kChunk nonzeros; a row
+// longer than kChunk is a block of its own.  Returns block start rows plus
+// the final row count.
+std::vector<int64_t> row_blocks(const std::vector<int64_t> &ptr) {
+    const int64_t rows = int64_t(ptr.size()) - 1;
+    std::vector<int64_t> b{0};
+    for (int64_t r = 0; r < rows;) {
+        const int64_t start = r;
+        if (ptr[r + 1] - ptr[r] > kChunk)
+            ++r;
+        else
+            while (r < rows && ptr[r + 1] - ptr[start] <= kChunk) ++r;
+        b.push_back(r);
+    }
+    return b;
+}
+
+// Offsets of the transpose of a CSR matrix (column counts, prefix-summed).
+std::vector<int64_t> t_ptr(const nyk::Model &m) {
+    std::vector<int64_t> p(m.cols() + 1, 0);
+    for (auto j : m.A.index) p[j + 1]++;
+    for (int64_t j = 0; j < m.cols(); ++j) p[j + 1] += p[j];
+    return p;
+}
+
+// Builds one LP's device image: every array at a 256-byte aligned offset.
+struct Layout {
+    size_t total = 0;
+    template <class T>
+    size_t add(size_t count) {
+        size_t at = total;
+        total = align_up(total + std::max<size_t>(count, 1) * sizeof(T));
+        return at;
+    }
+};
+
+class BatchEngine {
+  public:
+    BatchEngine(const nyk::Options &o, int64_t arena_bytes, int64_t max_slots) : opt_(o) {
+        if (o.method != "pdhg") throw std::runtime_error("The batched engine runs the pdhg method");
+        if (arena_bytes <= 0 || max_slots <= 0) throw std::runtime_error("Invalid arena size or slot count");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count < 1) throw std::runtime_error("No HIP device is visible");
+        max_slots_ = int(max_slots);
+        arena_.init(size_t(arena_bytes));
+        HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        HIP_OK(hipMalloc(&d_desc_, sizeof(LpDev) * max_slots_));
+        HIP_OK(hipMalloc(&d_state_, sizeof(LpState) * max_slots_));
+        HIP_OK(hipMalloc(&d_slots_, sizeof(int) * max_slots_));
+
+        desc_.resize(max_slots_);
+        state_.resize(max_slots_);
+        slot_job_.assign(max_slots_, -1);
+        slot_bytes_.assign(max_slots_, 0);
+        for (int s = max_slots_ - 1; s >= 0; --s) free_slots_.push_back(s);
+    }
+    ~BatchEngine() {
+        if (stream_) hipStreamSynchronize(stream_);
+        hipFree(d_desc_);
+        hipFree(d_state_);
+        hipFree(d_slots_);
+        for (auto &e : ev_) hipEventDestroy(e);
+        if (stream_) hipStreamDestroy(stream_);
+    }
+
+    int64_t submit(const niryukti_lp &lp, const double *x0, const double *y0) {
+        auto job = std::make_unique<Job>();
+        job->ticket = int64_t(jobs_.size());
+        job->start = Clock::now();
+        nyk::Model &m0 = job->original;
+        const int64_t nnz = lp.ptr[lp.rows];
+        m0.A.rows = lp.rows;
+        m0.A.cols = lp.cols;
+        m0.A.ptr.assign(lp.ptr, lp.ptr + lp.rows + 1);
+        m0.A.index.assign(lp.index, lp.index + nnz);
+        m0.A.value.assign(lp.value, lp.value + nnz);
+        m0.c.assign(lp.c, lp.c + lp.cols);
+        m0.lb.assign(lp.lb, lp.lb + lp.cols);
+        m0.ub.assign(lp.ub, lp.ub + lp.cols);
+        m0.rl.assign(lp.rl, lp.rl + lp.rows);
+        m0.ru.assign(lp.ru, lp.ru + lp.rows);
+        m0.offset = lp.offset;
+        m0.sense = lp.sense == 0 ? 1 : lp.sense;
+        m0.validate();
+        if (m0.rows() >= (1LL << 31) || m0.cols() >= (1LL << 31) || nnz >= (1LL << 31))
+            throw std::runtime_error("Batched LPs must have fewer than 2^31 rows, columns and nonzeros");
+        nyk::Options o = opt_;
+        if (x0) o.initial_x.assign(x0, x0 + lp.cols);
+        if (y0) o.initial_y.assign(y0, y0 + lp.rows);
+        for (double v : o.initial_x)
+            if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
+        for (double v : o.initial_y)
+            if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
+        setup(*job, o);
+        jobs_.push_back(std::move(job));
+        Job &j = *jobs_.back();
+        if (!j.finished) queue_.push_back(j.ticket);
+        return j.ticket;
+    }
+
+    int64_t step(int checks) {
+        admit();
+        std::vector<int> active;
+        for (int s = 0; s < max_slots_; ++s)
+            if (slot_job_[s] >= 0 && state_[s].status == kRunning) active.push_back(s);
+        if (!active.empty()) {
+            if (dirty_) {
+                HIP_OK(hipMemcpyAsync(d_desc_, desc_.data(), sizeof(LpDev) * max_slots_, hipMemcpyHostToDevice, stream_));
+                dirty_ = false;
+            }
+            HIP_OK(hipMemcpyAsync(d_state_, state_.data(), sizeof(LpState) * max_slots_, hipMemcpyHostToDevice, stream_));
+            HIP_OK(hipMemcpyAsync(d_slots_, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice, stream_));
+            BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0};
+            auto t0 = Clock::now();
+            const int grid = int(active.size());
+            std::vector<long long> before(active.size());
+            for (size_t k = 0; k < active.size(); ++k)
+                before[k] = state_[active[k]].accepted + state_[active[k]].rejected;
+            while (ev_.size() < size_t(2 * checks + 1)) {
+                hipEvent_t e;
+                HIP_OK(hipEventCreate(&e));
+                ev_.push_back(e);
+            }
+            HIP_OK(hipEventRecord(ev_[0], stream_));
+            for (int c = 0; c < checks; ++c) {
+                batch_advance_kernel<<<grid, kThreads, 0, stream_>>>(d_desc_, d_state_, d_slots_, bo);
+                HIP_OK(hipEventRecord(ev_[2 * c + 1], stream_));
+                batch_check_kernel<<<grid, kThreads, 0, stream_>>>(d_desc_, d_state_, d_slots_, bo);
+                HIP_OK(hipEventRecord(ev_[2 * c + 2], stream_));
+            }
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(state_.data(), d_state_, sizeof(LpState) * max_slots_, hipMemcpyDeviceToHost, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+            float adv_ms = 0, chk_ms = 0;
+            for (int c = 0; c < checks; ++c) {
+                float a = 0, b = 0;
+                HIP_OK(hipEventElapsedTime(&a, ev_[2 * c], ev_[2 * c + 1]));
+                HIP_OK(hipEventElapsedTime(&b, ev_[2 * c + 1], ev_[2 * c + 2]));
+                adv_ms += a;
+                chk_ms += b;
+            }
+            advance_seconds_ += adv_ms * 1e-3;
+            check_seconds_ += chk_ms * 1e-3;
+            for (size_t k = 0; k < active.size(); ++k) {
+                const LpState &st = state_[active[k]];
+                const long long done = st.accepted + st.rejected - before[k];
+                attempts_ += done;
+                advance_bytes_ += double(done) * slot_bytes_[active[k]];
+            }
+            device_seconds_ += std::chrono::duration<double>(Clock::now() - t0).count();
+            ++launches_;
+        }
+        settle(active);
+        int64_t running = int64_t(queue_.size());
+        for (int s = 0; s < max_slots_; ++s) running += slot_job_[s] >= 0;
+        return running;
+    }
+
+    bool result(int64_t ticket, niryukti_result *out) {
+        if (ticket < 0 || ticket >= int64_t(jobs_.size())) throw std::runtime_error("Unknown ticket");
+        Job &j = *jobs_[ticket];
+        if (!j.finished) return false;
+        const auto &r = j.result;
+        const auto &a = r.accuracy;
+        std::snprintf(out->status, sizeof out->status, "%s", r.status.c_str());
+        std::snprintf(out->message, sizeof out->message, "%s", r.message.c_str());
+        out->objective = a.finite ? j.original.sense * a.objective : NAN;
+        out->primal_residual = a.primal;
+        out->dual_residual = a.dual;
+        out->relative_gap = a.gap;
+        out->kkt_error = a.kkt;
+        out->complementarity = a.complementarity;
+        out->dual_bound = a.lower_bound;
+        out->primal_absolute = a.primal_absolute;
+        out->dual_absolute = a.dual_absolute;
+        out->finite = a.finite;
+        out->iterations = r.iterations;
+        out->restarts = r.restarts;
+        out->rejected_steps = r.rejected_steps;
+        out->weight_updates = r.weight_updates;
+        out->host_candidate_checks = r.host_candidate_checks;
+        out->removed_rows = r.removed_rows;
+        out->removed_columns = r.removed_columns;
+        out->seconds = r.seconds;
+        out->preprocess_seconds = r.preprocess_seconds;
+        out->transfer_seconds = r.transfer_seconds;
+        out->iteration_seconds = r.iteration_seconds;
+        out->verification_seconds = r.verification_seconds;
+        out->index_bits = 32;
+        if (out->x && r.x.size() == size_t(j.original.cols())) std::memcpy(out->x, r.x.data(), r.x.size() * 8);
+        if (out->y && r.y.size() == size_t(j.original.rows())) std::memcpy(out->y, r.y.data(), r.y.size() * 8);
+        return true;
+    }
+
+    void stats(niryukti_batch_stats *s) const {
+        s->submitted = int64_t(jobs_.size());
+        s->queued = int64_t(queue_.size());
+        s->active = 0;
+        for (int v : slot_job_) s->active += v >= 0;
+        s->finished = 0;
+        for (const auto &j : jobs_) s->finished += j->finished;
+        s->arena_bytes = int64_t(arena_.size());
+        s->arena_used_bytes = int64_t(arena_.used());
+        s->launches = launches_;
+        s->host_certifications = certifications_;
+        s->host_farkas_checks = farkas_checks_;
+        s->device_seconds = device_seconds_;
+        s->advance_seconds = advance_seconds_;
+        s->check_seconds = check_seconds_;
+        s->advance_bytes = advance_bytes_;
+        s->attempts = attempts_;
+    }
+
+    int64_t admit_only() {
+        const size_t before = queue_.size();
+        admit();
+        return int64_t(before - queue_.size());
+    }
+
+  private:
+    // Host part of src/solver.cpp:215-333 for one LP: prepare, start point,
+    // start-point verification, unboundedness and trivial optimality, steps.
+    void setup(Job &j, const nyk::Options &o) {
+        auto &r = j.result;
+        r.method_selected = "pdhg";
+        r.backend = "hip-batch";
+        j.verifier = std::make_unique<nyk::Verifier>(j.original);
+        r.status = "ITERATION_LIMIT";
+        auto t0 = Clock::now();
+        j.prep = nyk::prepare(j.original, o);
+        r.preprocess_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
+        auto finish = [&](const char *status, std::string message) {
+            r.status = status;
+            r.message = std::move(message);
+            j.finished = true;
+            r.seconds = std::chrono::duration<double>(Clock::now() - j.start).count();
+        };
+        if (!j.prep.failure.empty()) {
+            r.infeasibility_ray = j.prep.failure_ray;
+            if (!r.infeasibility_ray.empty()) r.certificate_margin = j.verifier->infeasibility_bound(r.infeasibility_ray);
+            finish(j.prep.failure.c_str(), j.prep.reason);
+            return;
+        }
+        const nyk::Model &m = j.prep.model;
+        r.removed_columns = j.original.cols() - m.cols();
+        r.removed_rows = j.original.rows() - m.rows();
+        j.x.assign(m.cols(), 0);
+        j.y.assign(m.rows(), 0);
+        for (int64_t c = 0; c < m.cols(); ++c) {
+            double initial = o.initial_x.empty() ? 0 : o.initial_x[j.prep.cols[c]];
+            j.x[c] = std::clamp(initial / j.prep.column_scale[c], m.lb[c], m.ub[c]);
+        }
+        if (!o.initial_y.empty())
+            for (int64_t i = 0; i < m.rows(); ++i)
+                j.y[i] = o.initial_y[j.prep.rows[i]] / (j.prep.row_scale[i] * j.prep.objective_scale);
+        r.x = j.prep.restore_x(j.x);
+        r.y = j.prep.restore_y(j.y, j.original.rl.size());
+        r.accuracy = j.verifier->evaluate(r.x, r.y);
+        if (r.accuracy.finite && r.accuracy.primal_absolute == 0) {
+            std::vector<uint8_t> used(m.cols(), 0);
+            for (auto c : m.A.index) used[c] = 1;
+            for (int64_t c = 0; c < m.cols(); ++c)
+                if (!used[c] && ((m.c[c] < 0 && m.ub[c] == nyk::kInf) || (m.c[c] > 0 && m.lb[c] == -nyk::kInf))) {
+                    finish("UNBOUNDED", "Feasible point and isolated improving recession direction: " + m.column_name(c));
+                    return;
+                }
+        }
+        if (r.accuracy.finite && r.accuracy.kkt <= o.tol) {
+            finish("OPTIMAL", "");
+            return;
+        }
+        if (o.time_limit == 0) {
+            finish("TIME_LIMIT", "");
+            return;
+        }
+        std::vector<double> colsum(m.cols(), 0);
+        double rowmax = 0;
+        for (int64_t i = 0; i < m.rows(); ++i) {
+            double s = 0;
+            for (int64_t k = m.A.ptr[i]; k < m.A.ptr[i + 1]; ++k) {
+                double v = std::abs(m.A.value[k]);
+                s += v;
+                colsum[m.A.index[k]] += v;
+            }
+            rowmax = std::max(rowmax, s);
+        }
+        double colmax = 0;
+        for (double s : colsum) colmax = std::max(colmax, s);
+        const double norm = std::sqrt(rowmax) * std::sqrt(colmax);
+        j.step = norm > 0 ? .9 / norm : 1;
+        j.weight = 1;
+        if (o.adaptive) {
+            long double c2 = 0, b2 = 0;
+            for (double c : m.c) c2 += (long double)c * c;
+            for (int64_t i = 0; i < m.rows(); ++i) {
+                double b = std::clamp(0., m.rl[i], m.ru[i]);
+                b2 += (long double)b * b;
+            }
+            if (c2 > 1e-24 && b2 > 1e-24) j.weight = std::clamp(std::sqrt(double(c2 / b2)), 1e-4, 1e4);
+        }
+    }
+
+    // Algorithmic HBM bytes of one PDHG attempt on the scaled LP: both CSR
+    // streams (8-byte values, 4-byte indices and offsets), each gathered
+    // vector once, and eight FP64 vector streams per side (read x, xa, c, lb,
+    // ub / write x, xa, dx; read y, ya, ax, rl, ru / write y, ya, ax).
+    static double attempt_bytes(const Job &j) {
+        const double n = double(j.prep.model.cols()), m = double(j.prep.model.rows()), nnz = double(j.prep.model.A.nnz());
+        return 2 * nnz * 12 + (n + m + 2) * 4 + m * 8 + n * 8 + 64 * (n + m);
+    }
+
+    // Admits queued LPs while a slot and arena room are free (FIFO).
+    void admit() {
+        while (!queue_.empty() && !free_slots_.empty()) {
+            Job &j = *jobs_[queue_.front()];
+            if (std::chrono::duration<double>(Clock::now() - j.start).count() >= opt_.time_limit) {
+                queue_.pop_front();
+                retire_host(j, "TIME_LIMIT");
+                continue;
+            }
+            std::vector<char> image;
+            LpDev d{};
+            size_t bytes = build_image(j, image, d, nullptr);
+            size_t offset = 0;
+            if (!arena_.allocate(bytes, offset)) {
+                if (arena_.used() == 0)
+                    throw std::runtime_error("LP does not fit the batch arena (" + std::to_string(bytes) + " bytes)");
+                break;
+            }
+            build_image(j, image, d, arena_.base() + offset);
+            HIP_OK(hipMemcpy(arena_.base() + offset, image.data(), bytes, hipMemcpyHostToDevice));
+            const int s = free_slots_.back();
+            free_slots_.pop_back();
+            queue_.pop_front();
+            j.slot = s;
+            j.offset = offset;
+            j.bytes = bytes;
+            slot_job_[s] = int(j.ticket);
+            slot_bytes_[s] = attempt_bytes(j);
+            desc_[s] = d;
+            LpState st{};
+            st.step = j.step;
+            st.weight = j.weight;
+            st.factor = 1;
+            st.restart_kkt = j.result.accuracy.kkt;
+            st.best_kkt = j.result.accuracy.kkt;
+            st.iteration_limit = opt_.iteration_limit;
+            st.status = kRunning;
+            state_[s] = st;
+            dirty_ = true;
+        }
+    }
+
+    // Lays out (and, with a base pointer, fills) the device image of one LP.
+    size_t build_image(const Job &j, std::vector<char> &image, LpDev &d, char *base) {
+        const nyk::Model &m = j.prep.model;
+        const nyk::Model &m0 = j.original;
+        const nyk::Csr &t0 = j.verifier->transpose();
+        const int n = int(m.cols()), mr = int(m.rows()), n0 = int(m0.cols()), mr0 = int(m0.rows());
+        const int64_t nnz = m.A.nnz(), nnz0 = m0.A.nnz();
+        Layout L;
+        size_t o_ap = L.add<int>(mr + 1), o_ai = L.add<int>(nnz), o_av = L.add<double>(nnz);
+        size_t o_tp = L.add<int>(n + 1), o_ti = L.add<int>(nnz), o_tv = L.add<double>(nnz);
+        size_t o_c = L.add<double>(n), o_lb = L.add<double>(n), o_ub = L.add<double>(n);
+        size_t o_rl = L.add<double>(mr), o_ru = L.add<double>(mr);
+        size_t o_op = L.add<int>(mr0 + 1), o_oi = L.add<int>(nnz0), o_ov = L.add<double>(nnz0);
+        size_t o_otp = L.add<int>(n0 + 1), o_oti = L.add<int>(nnz0), o_otv = L.add<double>(nnz0);
+        size_t o_c0 = L.add<double>(n0), o_lb0 = L.add<double>(n0), o_ub0 = L.add<double>(n0);
+        size_t o_rl0 = L.add<double>(mr0), o_ru0 = L.add<double>(mr0);
+        size_t o_cols = L.add<int>(n), o_lr = L.add<int>(mr), o_ur = L.add<int>(mr);
+        size_t o_fixed = L.add<double>(n0), o_dc = L.add<double>(n), o_dr = L.add<double>(mr);
+        size_t o_x[2], o_xa[2], o_y[2], o_ya[2], o_ax[2];
+        for (int h = 0; h < 2; ++h) {
+            o_x[h] = L.add<double>(n);
+            o_xa[h] = L.add<double>(n);
+            o_y[h] = L.add<double>(mr);
+            o_ya[h] = L.add<double>(mr);
+            o_ax[h] = L.add<double>(mr);
+        }
+        size_t o_dx = L.add<double>(n), o_epx = L.add<double>(n), o_epy = L.add<double>(mr);
+        size_t o_ox[2], o_oy[2];
+        for (int h = 0; h < 2; ++h) {
+            o_ox[h] = L.add<double>(n0);
+            o_oy[h] = L.add<double>(mr0);
+        }
+        size_t o_act = L.add<double>(mr0), o_aty = L.add<double>(n0), o_atyabs = L.add<double>(n0);
+        size_t o_bx = L.add<double>(n0), o_by = L.add<double>(mr0);
+        const auto ba = row_blocks(m.A.ptr), bt = row_blocks(t_ptr(m)), bo = row_blocks(m0.A.ptr), bot = row_blocks(t0.ptr);
+        size_t o_ab = L.add<int>(ba.size()), o_tb = L.add<int>(bt.size()), o_ob = L.add<int>(bo.size()),
+               o_otb = L.add<int>(bot.size());
+        if (!base) return L.total;
+        image.assign(L.total, 0);
+        auto put_i = [&](size_t off, const std::vector<int64_t> &v) {
+            int *p = reinterpret_cast<int *>(image.data() + off);
+            for (size_t k = 0; k < v.size(); ++k) p[k] = int(v[k]);
+        };
+        auto put_d = [&](size_t off, const std::vector<double> &v) {
+            if (!v.empty()) std::memcpy(image.data() + off, v.data(), v.size() * 8);
+        };
+        nyk::Csr t = m.A.transposed();
+        put_i(o_ap, m.A.ptr);
+        put_i(o_ai, m.A.index);
+        put_d(o_av, m.A.value);
+        put_i(o_tp, t.ptr);
+        put_i(o_ti, t.index);
+        put_d(o_tv, t.value);
+        put_d(o_c, m.c);
+        put_d(o_lb, m.lb);
+        put_d(o_ub, m.ub);
+        put_d(o_rl, m.rl);
+        put_d(o_ru, m.ru);
+        put_i(o_op, m0.A.ptr);
+        put_i(o_oi, m0.A.index);
+        put_d(o_ov, m0.A.value);
+        put_i(o_otp, t0.ptr);
+        put_i(o_oti, t0.index);
+        put_d(o_otv, t0.value);
+        put_d(o_c0, m0.c);
+        put_d(o_lb0, m0.lb);
+        put_d(o_ub0, m0.ub);
+        put_d(o_rl0, m0.rl);
+        put_d(o_ru0, m0.ru);
+        put_i(o_cols, j.prep.cols);
+        put_i(o_lr, j.prep.lower_rows);
+        put_i(o_ur, j.prep.upper_rows);
+        put_d(o_fixed, j.prep.fixed);
+        put_d(o_dc, j.prep.column_scale);
+        put_d(o_dr, j.prep.row_scale);
+        put_d(o_x[0], j.x);
+        put_d(o_y[0], j.y);
+        put_d(o_epx, j.x);
+        put_d(o_epy, j.y);
+        std::vector<double> ax0(mr, 0);
+        m.A.multiply(j.x.data(), ax0.data());
+        put_d(o_ax[0], ax0);
+        put_d(o_bx, j.result.x);
+        put_d(o_by, j.result.y);
+        auto P = [&](size_t off) { return base + off; };
+        d.n = n;
+        d.m = mr;
+        d.n0 = n0;
+        d.m0 = mr0;
+        d.ap = (const int *)P(o_ap);
+        d.ai = (const int *)P(o_ai);
+        d.av = (const double *)P(o_av);
+        d.tp = (const int *)P(o_tp);
+        d.ti = (const int *)P(o_ti);
+        d.tv = (const double *)P(o_tv);
+        d.c = (const double *)P(o_c);
+        d.lb = (const double *)P(o_lb);
+        d.ub = (const double *)P(o_ub);
+        d.rl = (const double *)P(o_rl);
+        d.ru = (const double *)P(o_ru);
+        d.op = (const int *)P(o_op);
+        d.oi = (const int *)P(o_oi);
+        d.ov = (const double *)P(o_ov);
+        d.otp = (const int *)P(o_otp);
+        d.oti = (const int *)P(o_oti);
+        d.otv = (const double *)P(o_otv);
+        d.c0 = (const double *)P(o_c0);
+        d.lb0 = (const double *)P(o_lb0);
+        d.ub0 = (const double *)P(o_ub0);
+        d.rl0 = (const double *)P(o_rl0);
+        d.ru0 = (const double *)P(o_ru0);
+        d.cols = (const int *)P(o_cols);
+        d.lower_rows = (const int *)P(o_lr);
+        d.upper_rows = (const int *)P(o_ur);
+        d.fixed = (const double *)P(o_fixed);
+        d.dc = (const double *)P(o_dc);
+        d.dr = (const double *)P(o_dr);
+        d.sobj = j.prep.objective_scale;
+        d.offset0 = m0.offset;
+        double cscale = 1;
+        for (double v : m0.c) cscale = std::max(cscale, 1 + std::abs(v));
+        d.cscale = cscale;
+        for (int h = 0; h < 2; ++h) {
+            d.x[h] = (double *)P(o_x[h]);
+            d.xa[h] = (double *)P(o_xa[h]);
+            d.y[h] = (double *)P(o_y[h]);
+            d.ya[h] = (double *)P(o_ya[h]);
+            d.ax[h] = (double *)P(o_ax[h]);
+            d.ox[h] = (double *)P(o_ox[h]);
+            d.oy[h] = (double *)P(o_oy[h]);
+        }
+        d.dx = (double *)P(o_dx);
+        d.epx = (double *)P(o_epx);
+        d.epy = (double *)P(o_epy);
+        d.act = (double *)P(o_act);
+        d.aty = (double *)P(o_aty);
+        d.atyabs = (double *)P(o_atyabs);
+        d.bx = (double *)P(o_bx);
+        d.by = (double *)P(o_by);
+        put_i(o_ab, ba);
+        put_i(o_tb, bt);
+        put_i(o_ob, bo);
+        put_i(o_otb, bot);
+        d.ablk = (const int *)P(o_ab);
+        d.tblk = (const int *)P(o_tb);
+        d.oblk = (const int *)P(o_ob);
+        d.otblk = (const int *)P(o_otb);
+        d.nab = int(ba.size()) - 1;
+        d.ntb = int(bt.size()) - 1;
+        d.nob = int(bo.size()) - 1;
+        d.notb = int(bot.size()) - 1;
+        return L.total;
+    }
+
+    void download(double *host, const double *dev, size_t count) {
+        if (count) HIP_OK(hipMemcpyAsync(host, dev, count * 8, hipMemcpyDeviceToHost, stream_));
+    }
+
+    // Handles every LP whose workgroup stopped: certification, Farkas checks,
+    // limits and errors (src/solver.cpp:583-633, 727-731).
+    void settle(const std::vector<int> &active) {
+        std::vector<int> stopped;
+        for (int s : active)
+            if (state_[s].status != kRunning) stopped.push_back(s);
+        for (int s : active) {
+            Job &j = *jobs_[slot_job_[s]];
+            if (state_[s].status == kRunning &&
+                std::chrono::duration<double>(Clock::now() - j.start).count() >= opt_.time_limit) {
+                state_[s].status = kIterationLimit;  // retired below with TIME_LIMIT
+                j.result.status = "TIME_LIMIT";
+                stopped.push_back(s);
+            }
+        }
+        if (stopped.empty()) return;
+        const size_t ns = stopped.size();
+        std::vector<std::vector<double>> cx(ns), cyo(ns), cy(ns), cya(ns), epy(ns);
+        for (size_t k = 0; k < ns; ++k) {
+            const int s = stopped[k];
+            const LpDev &d = desc_[s];
+            const LpState &st = state_[s];
+            Job &j = *jobs_[slot_job_[s]];
+            if (st.status == kParkOptimal) {
+                // the parked candidate is still in its evaluation buffers
+                cx[k].resize(d.n0);
+                cyo[k].resize(d.m0);
+                download(cx[k].data(), d.ox[st.avg], d.n0);
+                download(cyo[k].data(), d.oy[st.avg], d.m0);
+            } else {
+                j.result.x.resize(d.n0);
+                j.result.y.resize(d.m0);
+                download(j.result.x.data(), d.bx, d.n0);
+                download(j.result.y.data(), d.by, d.m0);
+            }
+            if (st.status == kParkFarkas) {
+                const int c = st.cur;
+                cy[k].resize(d.m);
+                cya[k].resize(d.m);
+                epy[k].resize(d.m);
+                download(cy[k].data(), d.y[c], d.m);
+                download(cya[k].data(), d.ya[c], d.m);
+                download(epy[k].data(), d.epy, d.m);
+            }
+        }
+        HIP_OK(hipStreamSynchronize(stream_));
+        std::vector<int> resume(stopped.size(), 0);
+        std::mutex mu;
+        nyk::parallel_tasks(int64_t(ns), [&](int64_t k) {
+            const int s = stopped[k];
+            Job &j = *jobs_[slot_job_[s]];
+            LpState &st = state_[s];
+            auto &r = j.result;
+            auto t0 = Clock::now();
+            copy_counters(j, st);
+            if (st.status == kParkOptimal) {
+                // src/solver.cpp:588-596: OPTIMAL only after the safe re-evaluation
+                auto safe = j.verifier->evaluate(cx[k], cyo[k]);
+                {
+                    std::lock_guard<std::mutex> lock(mu);
+                    ++certifications_;
+                }
+                if (safe.finite && safe.kkt <= opt_.tol) {
+                    r.x = std::move(cx[k]);
+                    r.y = std::move(cyo[k]);
+                    r.accuracy = safe;
+                    r.status = "OPTIMAL";
+                } else {
+                    st.cand_kkt = safe.kkt;  // the reference continues with the safe accuracy
+                    st.phase = 1;
+                    st.status = kRunning;
+                    resume[k] = 1;
+                }
+            } else if (st.status == kParkFarkas) {
+                {
+                    std::lock_guard<std::mutex> lock(mu);
+                    ++farkas_checks_;
+                }
+                const bool avg = st.avg != 0;
+                auto ysel = avg ? cya[k] : cy[k], yother = avg ? cy[k] : cya[k];
+                auto disp = ysel;
+                for (size_t i = 0; i < disp.size(); ++i) disp[i] -= epy[k][i];
+                std::vector<std::vector<double>> rays{j.prep.restore_y(ysel, j.original.rl.size()),
+                                                      j.prep.restore_y(yother, j.original.rl.size()),
+                                                      j.prep.restore_y(disp, j.original.rl.size())};
+                bool infeasible = false;
+                for (auto &ray : rays) {
+                    double norm_y = 0;
+                    bool finite = true;
+                    for (double v : ray) {
+                        finite = finite && std::isfinite(v);
+                        norm_y = std::max(norm_y, std::abs(v));
+                    }
+                    if (!finite || !(norm_y > 0)) continue;
+                    for (double &v : ray) v /= norm_y;
+                    double margin = j.verifier->infeasibility_bound(ray);
+                    if (margin > 0 && std::isfinite(margin)) {
+                        r.status = "INFEASIBLE";
+                        r.message = "Independently verified box/row Farkas certificate";
+                        r.infeasibility_ray = std::move(ray);
+                        r.certificate_margin = margin;
+                        infeasible = true;
+                        break;
+                    }
+                }
+                if (!infeasible) {
+                    st.phase = 1;
+                    st.status = kRunning;
+                    resume[k] = 1;
+                }
+            } else if (st.status == kNumerical) {
+                r.status = "NUMERICAL_ERROR";
+                r.message = "Nonfinite iterate or diagnostic";
+            } else if (st.status == kExhausted) {
+                r.status = "NUMERICAL_ERROR";
+                r.message = "PDHG backtracking exhausted";
+            } else if (r.status != "TIME_LIMIT") {
+                r.status = "ITERATION_LIMIT";
+            }
+            r.verification_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
+            if (!resume[k]) {
+                // final original-space verification (src/solver.cpp:727-731)
+                if (r.status != "OPTIMAL") r.accuracy = j.verifier->evaluate(r.x, r.y);
+                if (r.status == "OPTIMAL" && (!r.accuracy.finite || r.accuracy.kkt > opt_.tol)) r.status = "NUMERICAL_ERROR";
+            }
+        });
+        for (size_t k = 0; k < ns; ++k)
+            if (!resume[k]) retire(*jobs_[slot_job_[stopped[k]]]);
+    }
+
+    void copy_counters(Job &j, const LpState &st) {
+        auto &r = j.result;
+        r.iterations = st.iterations;
+        r.restarts = st.restarts;
+        r.rejected_steps = st.rejected;
+        r.weight_updates = st.weight_updates;
+        r.host_candidate_checks = st.checks;
+    }
+
+    void retire(Job &j) {
+        arena_.release(j.offset, j.bytes);
+        slot_job_[j.slot] = -1;
+        free_slots_.push_back(j.slot);
+        state_[j.slot] = LpState{};
+        state_[j.slot].status = kIterationLimit;
+        j.slot = -1;
+        j.finished = true;
+        j.result.seconds = std::chrono::duration<double>(Clock::now() - j.start).count();
+    }
+    void retire_host(Job &j, const char *status) {
+        j.result.status = status;
+        j.finished = true;
+        j.result.seconds = std::chrono::duration<double>(Clock::now() - j.start).count();
+    }
+
+    nyk::Options opt_;
+    int max_slots_ = 0;
+    Arena arena_;
+    hipStream_t stream_ = nullptr;
+    LpDev *d_desc_ = nullptr;
+    LpState *d_state_ = nullptr;
+    int *d_slots_ = nullptr;
+    std::vector<LpDev> desc_;
+    std::vector<LpState> state_;
+    std::vector<int> slot_job_, free_slots_;
+    std::vector<double> slot_bytes_;
+    std::vector<hipEvent_t> ev_;
+    double advance_seconds_ = 0, check_seconds_ = 0, advance_bytes_ = 0;
+    int64_t attempts_ = 0;
+    std::vector<std::unique_ptr<Job>> jobs_;
+    std::deque<int64_t> queue_;
+    bool dirty_ = true;
+    int64_t launches_ = 0, certifications_ = 0, farkas_checks_ = 0;
+    double device_seconds_ = 0;
+};
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        g_error.clear();
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+nyk::Options options_from(const niryukti_options *in) {
+    nyk::Options o;
+    if (!in) return o;
+    auto str = [](const char *s, const char *d) { return std::string(s ? s : d); };
+    o.device = str(in->device, "gpu");
+    o.method = str(in->method, "pdhg");
+    o.scaling = str(in->scaling, "ruiz");
+    o.primal_weight = str(in->primal_weight, "displacement");
+    o.matrix_precision = str(in->matrix_precision, "fp64");
+    o.tol = in->tol;
+    o.time_limit = in->time_limit;
+    o.iteration_limit = in->iteration_limit;
+    o.check_every = in->check_every;
+    o.scaling_passes = in->scaling_passes;
+    o.threads = in->threads;
+    o.presolve = in->presolve != 0;
+    o.restart = in->restart != 0;
+    o.adaptive = in->adaptive != 0;
+    return o;
+}
+
+}  // namespace
+
+struct niryukti_batch {
+    BatchEngine engine;
+    niryukti_batch(const nyk::Options &o, int64_t arena, int64_t slots) : engine(o, arena, slots) {}
+};
+
+extern "C" {
+
+int niryukti_batch_create(const niryukti_options *opts, int64_t arena_bytes, int64_t max_slots, niryukti_batch **out) {
+    return guarded([&] {
+        nyk::Options o = options_from(opts);
+        nyk::validate_options(o);
+        if (o.primal_weight != "displacement" || o.matrix_precision != "fp64")
+            throw std::runtime_error("The batched engine supports the displacement weight and FP64 matrices");
+        *out = new niryukti_batch(o, arena_bytes, max_slots);
+    });
+}
+
+void niryukti_batch_destroy(niryukti_batch *b) { delete b; }
+
+int niryukti_batch_submit(niryukti_batch *b, const niryukti_lp *lp, const double *x0, const double *y0, int64_t *ticket) {
+    return guarded([&] {
+        if (!b || !lp) throw std::runtime_error("Null argument");
+        int64_t t = b->engine.submit(*lp, x0, y0);
+        if (ticket) *ticket = t;
+    });
+}
+
+int niryukti_batch_step(niryukti_batch *b, int checks_per_launch, int64_t *running) {
+    return guarded([&] {
+        int64_t r = b->engine.step(std::max(1, checks_per_launch));
+        if (running) *running = r;
+    });
+}
+
+int niryukti_batch_admit(niryukti_batch *b, int64_t *admitted) {
+    return guarded([&] {
+        int64_t a = b->engine.admit_only();
+        if (admitted) *admitted = a;
+    });
+}
+
+int niryukti_batch_run(niryukti_batch *b, int checks_per_launch) {
+    return guarded([&] {
+        while (b->engine.step(std::max(1, checks_per_launch)) > 0) {
+        }
+    });
+}
+
+int niryukti_batch_result(niryukti_batch *b, int64_t ticket, niryukti_result *r) {
+    int done = 0;
+    int rc = guarded([&] { done = b->engine.result(ticket, r) ? 1 : 0; });
+    return rc < 0 ? rc : done;
+}
+
+int niryukti_batch_stats_get(niryukti_batch *b, niryukti_batch_stats *s) {
+    return guarded([&] { b->engine.stats(s); });
+}
+
+const char *niryukti_batch_last_error(void) { return g_error.c_str(); }
+}

@@ -1,0 +1,105 @@
+"""GPU tests of the batched / served engine.
+
+Every batched result must be certified OPTIMAL at rel-KKT <= 1e-6 by the FP64
+verifier, match the planted closed-form optimum and the CPU oracle's objective
+to 1e-6, and be bit-identical whether the LP runs alone or among others
+(each LP is reduced by its own workgroup in a fixed order).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return abs(a - b) / (1 + abs(b))
+
+
+def solve_batch(engine, lps, checks=5, arena=4 << 30, slots=4096, warm=None, **opts):
+    b = engine.Batch(arena_bytes=arena, max_slots=slots, **opts)
+    tickets = []
+    for k, lp in enumerate(lps):
+        x0, y0 = warm[k] if warm else (None, None)
+        tickets.append(b.submit(lp, x0, y0))
+    b.run(checks)
+    out = [b.result(t) for t in tickets]
+    stats = b.stats()
+    b.close()
+    return out, stats
+
+
+def test_batch_certified_against_planted_and_oracle(engine, oracle):
+    insts = [engine.Instance(2, 42, 1, i) for i in range(24)]
+    res, stats = solve_batch(engine, [i.lp for i in insts])
+    assert stats["finished"] == len(insts)
+    for inst, r in zip(insts, res):
+        assert r is not None and r["status"] == "OPTIMAL", r and (r["status"], r["message"])
+        assert r["kkt_error"] <= 1e-6
+        assert rel(r["objective"], inst.planted_objective) <= 1e-6
+        acc = oracle.verify(inst.lp, r["x"], r["y"])
+        assert acc["kkt"] <= 1e-6
+    # the CPU oracle reaches the same optimum on a few of them
+    for inst, r in list(zip(insts, res))[:4]:
+        cpu = oracle.solve(inst.lp)
+        assert cpu["status"] == "OPTIMAL"
+        assert rel(r["objective"], cpu["objective"]) <= 1e-6
+        print(inst.lp.rows, inst.lp.cols, "iterations batch", r["iterations"], "oracle", cpu["iterations"])
+
+
+def test_batch_alone_equals_in_mix(engine):
+    insts = [engine.Instance(2, 7, 1, i) for i in range(40)]
+    mix, _ = solve_batch(engine, [i.lp for i in insts])
+    for k in (0, 13, 39):
+        alone, _ = solve_batch(engine, [insts[k].lp])
+        assert alone[0]["status"] == mix[k]["status"] == "OPTIMAL"
+        assert alone[0]["iterations"] == mix[k]["iterations"]
+        assert np.array_equal(alone[0]["x"], mix[k]["x"]) and np.array_equal(alone[0]["y"], mix[k]["y"])
+
+
+def test_continuous_admission_small_arena(engine):
+    """An arena that holds only a few LPs forces many admission rounds while
+    other LPs keep iterating; results are unchanged."""
+    insts = [engine.Instance(2, 11, 1, i) for i in range(16)]
+    big, _ = solve_batch(engine, [i.lp for i in insts])
+    small, stats = solve_batch(engine, [i.lp for i in insts], checks=1, arena=24 << 20, slots=4)
+    assert stats["launches"] > 4
+    for a, b in zip(big, small):
+        assert a["status"] == b["status"] == "OPTIMAL"
+        assert np.array_equal(a["x"], b["x"]) and a["iterations"] == b["iterations"]
+
+
+def test_batch_matches_single_engine_accuracy(engine):
+    insts = [engine.Instance(0, 42, 4), engine.Instance(4, 42, 200), engine.Instance(1, 42, 400)]
+    res, _ = solve_batch(engine, [i.lp for i in insts])
+    for inst, r in zip(insts, res):
+        single = engine.solve_lp(inst.lp)
+        assert r["status"] == single["status"] == "OPTIMAL"
+        assert rel(r["objective"], single["objective"]) <= 1e-6
+        assert rel(r["objective"], inst.planted_objective) <= 1e-6
+
+
+def test_warm_start_resolve_of_perturbed_rhs(engine, oracle):
+    inst = engine.Instance(2, 5, 1, 2)
+    first, _ = solve_batch(engine, [inst.lp])
+    rng = np.random.default_rng(0)
+    lp = inst.lp
+    b = lp.rl * (1 + rng.uniform(-0.05, 0.05, lp.rows))
+    pert = engine.LP(lp.ptr, lp.index, lp.value, lp.c, lp.lb, lp.ub, b, b)
+    cold, _ = solve_batch(engine, [pert])
+    warm, _ = solve_batch(engine, [pert], warm=[(first[0]["x"], first[0]["y"])])
+    cpu = oracle.solve(pert)
+    for r in (cold[0], warm[0]):
+        assert r["status"] == cpu["status"]
+        if r["status"] == "OPTIMAL":
+            assert rel(r["objective"], cpu["objective"]) <= 1e-6
+    print("cold", cold[0]["iterations"], "warm", warm[0]["iterations"], "cpu", cpu["iterations"])
+
+
+def test_batch_reports_infeasible_and_limits(engine):
+    infeasible = engine.LP(np.array([0, 0]), np.zeros(0), np.zeros(0), [0.0], [0.0], [10.0], [1.0], [np.inf])
+    contradiction = engine.LP.from_dense([[1, 1], [1, 1]], [0, 0], [0, 0], [10, 10], [3, -np.inf], [np.inf, 1])
+    inst = engine.Instance(2, 3, 1, 1)
+    res, _ = solve_batch(engine, [infeasible, contradiction])
+    assert [r["status"] for r in res] == ["INFEASIBLE", "INFEASIBLE"]
+    res, _ = solve_batch(engine, [inst.lp], iteration_limit=200)
+    assert res[0]["status"] == "ITERATION_LIMIT" and res[0]["iterations"] == 200
