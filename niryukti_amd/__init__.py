@@ -76,7 +76,8 @@ class _BatchStats(C.Structure):
                                          "arena_used_bytes", "launches", "host_certifications",
                                          "host_farkas_checks")] + \
         [(k, C.c_double) for k in ("device_seconds", "advance_seconds", "check_seconds", "advance_bytes")] + \
-        [("attempts", C.c_int64)]
+        [("attempts", C.c_int64), ("class_seconds", C.c_double * 4), ("class_bytes", C.c_double * 4),
+         ("class_launches", C.c_int64 * 4)]
 
 
 _lib = None
@@ -371,7 +372,11 @@ class Batch:
     def stats(self) -> dict:
         st = _BatchStats()
         self._check(self._lib.niryukti_batch_stats_get(self._h, C.byref(st)))
-        return {k: getattr(st, k) for k, _ in _BatchStats._fields_}
+        out = {}
+        for k, _ in _BatchStats._fields_:
+            v = getattr(st, k)
+            out[k] = list(v) if k.startswith("class_") else v
+        return out
 
     def close(self):
         if getattr(self, "_h", None):

@@ -25,6 +25,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -51,6 +52,14 @@ thread_local std::string g_error;
     } while (0)
 
 constexpr int kThreads = 256;
+// LDS-resident launch classes: capacity in doubles for y (m) plus dx (n);
+// the last class runs the global-memory stream kernel.
+constexpr int kClasses = 4;
+constexpr int kClassDoubles[kClasses - 1] = {2048, 4096, 8000};
+constexpr int64_t kShortRow = 128;
+// The largest LDS class runs 512-thread workgroups: fewer rows per thread,
+// twice the loads in flight per LP.
+constexpr int kWideThreads = 512;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxRejects = 40;
 
@@ -168,6 +177,58 @@ __device__ void block_reduce(Reducer<NS, NM, ND> &r, double *lds) {
         const int b = NS + NM + 2 * k;
         DD acc{lds[b], lds[b + 1]};
         for (int w = 1; w < kWaves; ++w) acc = dd_merge(acc, DD{lds[w * W + b], lds[w * W + b + 1]});
+        r.d[k] = acc;
+    }
+    __syncthreads();
+}
+
+// block_reduce for a workgroup of NW waves (sums are combined wave 0 first).
+template <int NW, int NS, int NM, int ND>
+__device__ void block_reduce_n(Reducer<NS, NM, ND> &r, double *lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) r.s[k] += __shfl_xor(r.s[k], off, 64);
+#pragma unroll
+        for (int k = 0; k < NM; ++k) r.mx[k] = maxd(r.mx[k], __shfl_xor(r.mx[k], off, 64));
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            DD o{__shfl_xor(r.d[k].hi, off, 64), __shfl_xor(r.d[k].lo, off, 64)};
+            r.d[k] = dd_merge(r.d[k], o);
+        }
+    }
+    constexpr int W = NS + NM + 2 * ND;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) lds[wave * W + k] = r.s[k];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) lds[wave * W + NS + k] = r.mx[k];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            lds[wave * W + NS + NM + 2 * k] = r.d[k].hi;
+            lds[wave * W + NS + NM + 2 * k + 1] = r.d[k].lo;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        double acc = lds[k];
+        for (int w = 1; w < NW; ++w) acc += lds[w * W + k];
+        r.s[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        double acc = lds[NS + k];
+        for (int w = 1; w < NW; ++w) acc = maxd(acc, lds[w * W + NS + k]);
+        r.mx[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+        const int b = NS + NM + 2 * k;
+        DD acc{lds[b], lds[b + 1]};
+        for (int w = 1; w < NW; ++w) acc = dd_merge(acc, DD{lds[w * W + b], lds[w * W + b + 1]});
         r.d[k] = acc;
     }
     __syncthreads();
@@ -371,6 +432,31 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
     return a;
 }
 
+// Accept/reject and adaptive step factor (src/cpu.cpp:151-172).  Thread 0.
+__device__ void decide_batch(LpState &S, const BatchOpts &o, double dx2, double dy2, double coupling, double tau,
+                             double sigma, double f, int *accept_flag) {
+    const double energy = dx2 / tau + dy2 / sigma;
+    const double limit = fabs(coupling) > 0 ? energy / (2 * fabs(coupling)) : INFINITY;
+    const bool finite = isfinite(energy) && isfinite(coupling);
+    const bool accept = finite && (!o.adaptive || limit >= 1);
+    if (o.adaptive) {
+        const double k = double(S.accepted + 2);
+        double mult = mind((1 - pow(k, -.3)) * limit, 1 + pow(k, -.6));
+        if (!finite || !isfinite(mult) || mult <= 0) mult = .5;
+        S.factor = clampd(f * mult, 1e-12, 1e12);
+    }
+    if (accept) {
+        S.cur ^= 1;
+        S.accepted += 1;
+        S.mass += f;
+        S.consecutive_rejects = 0;
+    } else {
+        S.rejected += 1;
+        S.consecutive_rejects += 1;
+    }
+    *accept_flag = accept ? 1 : (o.adaptive && S.consecutive_rejects < kMaxRejects ? 0 : -1);
+}
+
 __device__ void attempt(const LpDev &L, LpState &S, const BatchOpts &o, double tau0, double sigma0, double *lds,
                         double *stage, int *accept_flag) {
     const int cur = S.cur;
@@ -404,29 +490,7 @@ __device__ void attempt(const LpDev &L, LpState &S, const BatchOpts &o, double t
                            axn[i] = a + delta;
                        });
     block_reduce(r, lds);
-    if (threadIdx.x == 0) {
-        // src/cpu.cpp:151-172
-        const double energy = r.s[0] / tau + r.s[1] / sigma, coupling = r.s[2];
-        const double limit = fabs(coupling) > 0 ? energy / (2 * fabs(coupling)) : INFINITY;
-        const bool finite = isfinite(energy) && isfinite(coupling);
-        const bool accept = finite && (!o.adaptive || limit >= 1);
-        if (o.adaptive) {
-            const double k = double(S.accepted + 2);
-            double mult = mind((1 - pow(k, -.3)) * limit, 1 + pow(k, -.6));
-            if (!finite || !isfinite(mult) || mult <= 0) mult = .5;
-            S.factor = clampd(f * mult, 1e-12, 1e12);
-        }
-        if (accept) {
-            S.cur ^= 1;
-            S.accepted += 1;
-            S.mass += f;
-            S.consecutive_rejects = 0;
-        } else {
-            S.rejected += 1;
-            S.consecutive_rejects += 1;
-        }
-        *accept_flag = accept ? 1 : (o.adaptive && S.consecutive_rejects < kMaxRejects ? 0 : -1);
-    }
+    if (threadIdx.x == 0) decide_batch(S, o, r.s[0], r.s[1], r.s[2], tau, sigma, f, accept_flag);
     __syncthreads();
 }
 
@@ -455,6 +519,171 @@ __global__ __launch_bounds__(kThreads) void batch_advance_kernel(const LpDev *lp
     const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
     for (long long done = 0; done < count;) {
         attempt(L, S, o, tau0, sigma0, lds, stage, &flag);
+        if (flag == 1) ++done;
+        else if (flag < 0) {
+            if (threadIdx.x == 0) S.status = kExhausted;
+            break;
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (S.status == kRunning) {
+            S.iterations += count;
+            S.since_restart += count;
+            S.checks += 1;
+            S.phase = kCheck;
+        }
+        states[s] = S;
+    }
+}
+
+// Sequential dot product of one CSR row against an LDS-resident vector, in
+// storage order starting from `init` (the reference's row-sum order), with
+// four entries' loads issued together.
+__device__ __forceinline__ double row_dot(const double *val, const int *idx, int k, int end, const double *vec,
+                                          double acc) {
+    for (; k + 4 <= end; k += 4) {
+        const double v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
+        const int i0 = idx[k], i1 = idx[k + 1], i2 = idx[k + 2], i3 = idx[k + 3];
+        acc += v0 * vec[i0];
+        acc += v1 * vec[i1];
+        acc += v2 * vec[i2];
+        acc += v3 * vec[i3];
+    }
+    for (; k < end; ++k) acc += val[k] * vec[idx[k]];
+    return acc;
+}
+
+// One PDHG attempt for an LP whose dual iterate and primal step fit in LDS:
+// y is staged into LDS once, dx never leaves LDS, and every row is summed by
+// one thread (rows are short in this class).
+// Dot products of NR CSR rows (one per q, `ok[q]` false for absent rows)
+// against an LDS-resident vector.  The rows advance in lock step, two entries
+// per row per step, so 2*NR independent loads are in flight per thread; each
+// row is still summed in storage order starting from acc[q].
+template <int NR>
+__device__ __forceinline__ void rows_dot(const int *ptr, const int *idx, const double *val, const int *r,
+                                         const bool *ok, const double *vec, double *acc) {
+    int k[NR], e[NR], longest = 0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        k[q] = ok[q] ? ptr[r[q]] : 0;
+        e[q] = ok[q] ? ptr[r[q] + 1] : 0;
+        longest = max(longest, e[q] - k[q]);
+    }
+    for (int t = 0; t < longest; t += 2) {
+        double v[NR][2];
+        int ix[NR][2];
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int kk = k[q] + t + u;
+                const bool on = kk < e[q];
+                v[q][u] = on ? val[kk] : 0.0;
+                ix[q][u] = on ? idx[kk] : 0;
+            }
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                if (k[q] + t + u < e[q]) acc[q] += v[q][u] * vec[ix[q][u]];
+    }
+}
+
+// One PDHG attempt for an LP whose dual iterate and primal step fit in LDS:
+// y is staged into LDS once, dx never leaves LDS, and each thread carries NR
+// rows at a time (rows are short in this class).
+template <int NT, int NR>
+__device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, double tau0, double sigma0,
+                            double *red, double *ylds, double *dxlds, int *accept_flag) {
+    const int cur = S.cur;
+    const double f = S.factor, tau = tau0 * f, sigma = sigma0 * f, w = f / (S.mass + f);
+    const double *x = L.x[cur], *xa = L.xa[cur], *y = L.y[cur], *ya = L.ya[cur], *ax = L.ax[cur];
+    double *xn = L.x[cur ^ 1], *xan = L.xa[cur ^ 1], *yn = L.y[cur ^ 1], *yan = L.ya[cur ^ 1], *axn = L.ax[cur ^ 1];
+    for (int i = threadIdx.x; i < L.m; i += NT) ylds[i] = y[i];
+    __syncthreads();
+    Reducer<3, 0, 0> red_acc{};
+    // g = c + A^T y and the projected primal step (src/cpu.cpp:107-132)
+    for (int base = threadIdx.x; base < L.n; base += NR * NT) {
+        int r[NR];
+        bool ok[NR];
+        double g[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            r[q] = base + q * NT;
+            ok[q] = r[q] < L.n;
+            g[q] = ok[q] ? L.c[r[q]] : 0.0;
+        }
+        rows_dot<NR>(L.tp, L.ti, L.tv, r, ok, ylds, g);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (!ok[q]) continue;
+            const int j = r[q];
+            const double xj = x[j], v = xa[j];
+            const double t = clampd(xj - tau * g[q], L.lb[j], L.ub[j]);
+            const double d = t - xj;
+            red_acc.s[0] += d * d;
+            dxlds[j] = d;
+            xn[j] = t;
+            xan[j] = v + (t - v) * w;
+        }
+    }
+    __syncthreads();
+    // delta = A dx and the extrapolated dual step (src/cpu.cpp:136-150)
+    for (int base = threadIdx.x; base < L.m; base += NR * NT) {
+        int r[NR];
+        bool ok[NR];
+        double delta[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            r[q] = base + q * NT;
+            ok[q] = r[q] < L.m;
+            delta[q] = 0.0;
+        }
+        rows_dot<NR>(L.ap, L.ai, L.av, r, ok, dxlds, delta);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (!ok[q]) continue;
+            const int i = r[q];
+            const double a = ax[i], yi = ylds[i], v = ya[i];
+            const double bar = a + 2 * delta[q], z = yi + sigma * bar;
+            const double t = maxd(0., z - sigma * L.ru[i]) + mind(0., z - sigma * L.rl[i]);
+            const double dy = t - yi;
+            red_acc.s[1] += (!isfinite(bar) || !isfinite(t)) ? INFINITY : dy * dy;
+            red_acc.s[2] += dy * delta[q];
+            yn[i] = t;
+            yan[i] = v + (t - v) * w;
+            axn[i] = a + delta[q];
+        }
+    }
+    block_reduce_n<NT / 64>(red_acc, red);
+    if (threadIdx.x == 0) decide_batch(S, o, red_acc.s[0], red_acc.s[1], red_acc.s[2], tau, sigma, f, accept_flag);
+    __syncthreads();
+}
+
+// Hot loop for the LDS-resident class; dynamic LDS holds y (m) and dx (n).
+template <int NT, int NR>
+__global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps, LpState *states, const int *slots,
+                                                               BatchOpts o) {
+    extern __shared__ double dyn[];
+    const int s = slots[blockIdx.x];
+    __shared__ LpState S;
+    __shared__ double red[(NT / 64) * 4];
+    __shared__ int flag;
+    if (threadIdx.x == 0) S = states[s];
+    __syncthreads();
+    if (S.status != kRunning || S.phase != kAdvance) return;
+    const LpDev &L = lps[s];
+    const long long remaining = S.iteration_limit - S.iterations;
+    if (remaining <= 0) {
+        if (threadIdx.x == 0) states[s].status = kIterationLimit;
+        return;
+    }
+    const long long count = remaining < o.check_every ? remaining : o.check_every;
+    const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
+    double *ylds = dyn, *dxlds = dyn + L.m;
+    for (long long done = 0; done < count;) {
+        attempt_lds<NT, NR>(L, S, o, tau0, sigma0, red, ylds, dxlds, &flag);
         if (flag == 1) ++done;
         else if (flag < 0) {
             if (threadIdx.x == 0) S.status = kExhausted;
@@ -686,6 +915,17 @@ struct Job {
     size_t offset = 0, bytes = 0;
 };
 
+// One main stream plus one stream per launch class, created on first use and
+// kept for the life of the process (default priority).
+std::vector<hipStream_t> &stream_pool() {
+    static std::vector<hipStream_t> pool = [] {
+        std::vector<hipStream_t> p(1 + kClasses);
+        for (auto &st : p) HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        return p;
+    }();
+    return pool;
+}
+
 // Splits CSR rows into consecutive blocks of at most kChunk nonzeros; a row
 // longer than kChunk is a block of its own.  Returns block start rows plus
 // the final row count.
@@ -731,15 +971,22 @@ class BatchEngine {
         if (hipGetDeviceCount(&count) != hipSuccess || count < 1) throw std::runtime_error("No HIP device is visible");
         max_slots_ = int(max_slots);
         arena_.init(size_t(arena_bytes));
-        HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        // Streams come from a per-process pool created once, so repeated
+        // engines keep the same stream-to-hardware-queue mapping.
+        auto &pool = stream_pool();
+        stream_ = pool[0];
+        for (int c = 0; c < kClasses; ++c) class_stream_[c] = pool[1 + c % (pool.size() - 1)];
+        if (const char *env = std::getenv("NIRYUKTI_BATCH_ROWS_PER_THREAD")) rows_per_thread_ = std::atoi(env);
         HIP_OK(hipMalloc(&d_desc_, sizeof(LpDev) * max_slots_));
         HIP_OK(hipMalloc(&d_state_, sizeof(LpState) * max_slots_));
-        HIP_OK(hipMalloc(&d_slots_, sizeof(int) * max_slots_));
+        HIP_OK(hipMalloc(&d_slots_, sizeof(int) * max_slots_ * kClasses));
+        HIP_OK(hipEventCreate(&uploaded_));
 
         desc_.resize(max_slots_);
         state_.resize(max_slots_);
         slot_job_.assign(max_slots_, -1);
         slot_bytes_.assign(max_slots_, 0);
+        slot_class_.assign(max_slots_, kClasses - 1);
         for (int s = max_slots_ - 1; s >= 0; --s) free_slots_.push_back(s);
     }
     ~BatchEngine() {
@@ -747,8 +994,11 @@ class BatchEngine {
         hipFree(d_desc_);
         hipFree(d_state_);
         hipFree(d_slots_);
-        for (auto &e : ev_) hipEventDestroy(e);
-        if (stream_) hipStreamDestroy(stream_);
+        for (auto &list : class_events_)
+            for (auto &e : list) hipEventDestroy(e);
+        if (uploaded_) hipEventDestroy(uploaded_);
+        for (auto &cs : class_stream_)
+            if (cs) hipStreamSynchronize(cs);
     }
 
     int64_t submit(const niryukti_lp &lp, const double *x0, const double *y0) {
@@ -788,52 +1038,75 @@ class BatchEngine {
 
     int64_t step(int checks) {
         admit();
+        std::vector<int> by_class[kClasses];
         std::vector<int> active;
         for (int s = 0; s < max_slots_; ++s)
-            if (slot_job_[s] >= 0 && state_[s].status == kRunning) active.push_back(s);
+            if (slot_job_[s] >= 0 && state_[s].status == kRunning) {
+                active.push_back(s);
+                by_class[slot_class_[s]].push_back(s);
+            }
         if (!active.empty()) {
             if (dirty_) {
                 HIP_OK(hipMemcpyAsync(d_desc_, desc_.data(), sizeof(LpDev) * max_slots_, hipMemcpyHostToDevice, stream_));
                 dirty_ = false;
             }
             HIP_OK(hipMemcpyAsync(d_state_, state_.data(), sizeof(LpState) * max_slots_, hipMemcpyHostToDevice, stream_));
-            HIP_OK(hipMemcpyAsync(d_slots_, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice, stream_));
+            for (int c = 0; c < kClasses; ++c)
+                if (!by_class[c].empty())
+                    HIP_OK(hipMemcpyAsync(d_slots_ + size_t(c) * max_slots_, by_class[c].data(),
+                                          sizeof(int) * by_class[c].size(), hipMemcpyHostToDevice, stream_));
+            HIP_OK(hipEventRecord(uploaded_, stream_));
             BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0};
             auto t0 = Clock::now();
-            const int grid = int(active.size());
             std::vector<long long> before(active.size());
             for (size_t k = 0; k < active.size(); ++k)
                 before[k] = state_[active[k]].accepted + state_[active[k]].rejected;
-            while (ev_.size() < size_t(2 * checks + 1)) {
-                hipEvent_t e;
-                HIP_OK(hipEventCreate(&e));
-                ev_.push_back(e);
+            for (int c = 0; c < kClasses; ++c) {
+                if (by_class[c].empty()) continue;
+                auto &ev = class_events_[c];
+                while (ev.size() < size_t(2 * checks + 2)) {
+                    hipEvent_t e;
+                    HIP_OK(hipEventCreate(&e));
+                    ev.push_back(e);
+                }
+                hipStream_t cs = class_stream_[c];
+                HIP_OK(hipStreamWaitEvent(cs, uploaded_, 0));
+                const int grid = int(by_class[c].size());
+                const int *slots = d_slots_ + size_t(c) * max_slots_;
+                HIP_OK(hipEventRecord(ev[0], cs));
+                for (int k = 0; k < checks; ++k) {
+                    if (c < kClasses - 1)
+                        launch_lds(c, grid, cs, slots, bo);
+                    else
+                        batch_advance_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
+                    HIP_OK(hipEventRecord(ev[2 * k + 1], cs));
+                    batch_check_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
+                    HIP_OK(hipEventRecord(ev[2 * k + 2], cs));
+                }
+                HIP_OK(hipGetLastError());
+                HIP_OK(hipStreamWaitEvent(stream_, ev[2 * checks], 0));
             }
-            HIP_OK(hipEventRecord(ev_[0], stream_));
-            for (int c = 0; c < checks; ++c) {
-                batch_advance_kernel<<<grid, kThreads, 0, stream_>>>(d_desc_, d_state_, d_slots_, bo);
-                HIP_OK(hipEventRecord(ev_[2 * c + 1], stream_));
-                batch_check_kernel<<<grid, kThreads, 0, stream_>>>(d_desc_, d_state_, d_slots_, bo);
-                HIP_OK(hipEventRecord(ev_[2 * c + 2], stream_));
-            }
-            HIP_OK(hipGetLastError());
             HIP_OK(hipMemcpyAsync(state_.data(), d_state_, sizeof(LpState) * max_slots_, hipMemcpyDeviceToHost, stream_));
             HIP_OK(hipStreamSynchronize(stream_));
-            float adv_ms = 0, chk_ms = 0;
-            for (int c = 0; c < checks; ++c) {
-                float a = 0, b = 0;
-                HIP_OK(hipEventElapsedTime(&a, ev_[2 * c], ev_[2 * c + 1]));
-                HIP_OK(hipEventElapsedTime(&b, ev_[2 * c + 1], ev_[2 * c + 2]));
-                adv_ms += a;
-                chk_ms += b;
+            for (int c = 0; c < kClasses; ++c) {
+                if (by_class[c].empty()) continue;
+                auto &ev = class_events_[c];
+                for (int k = 0; k < checks; ++k) {
+                    float a = 0, b = 0;
+                    HIP_OK(hipEventElapsedTime(&a, ev[2 * k], ev[2 * k + 1]));
+                    HIP_OK(hipEventElapsedTime(&b, ev[2 * k + 1], ev[2 * k + 2]));
+                    advance_seconds_ += a * 1e-3;
+                    check_seconds_ += b * 1e-3;
+                    class_seconds_[c] += a * 1e-3;
+                    class_launches_[c] += 1;
+                }
             }
-            advance_seconds_ += adv_ms * 1e-3;
-            check_seconds_ += chk_ms * 1e-3;
             for (size_t k = 0; k < active.size(); ++k) {
                 const LpState &st = state_[active[k]];
                 const long long done = st.accepted + st.rejected - before[k];
                 attempts_ += done;
                 advance_bytes_ += double(done) * slot_bytes_[active[k]];
+                class_bytes_[slot_class_[active[k]]] += double(done) * slot_bytes_[active[k]];
             }
             device_seconds_ += std::chrono::duration<double>(Clock::now() - t0).count();
             ++launches_;
@@ -897,6 +1170,11 @@ class BatchEngine {
         s->check_seconds = check_seconds_;
         s->advance_bytes = advance_bytes_;
         s->attempts = attempts_;
+        for (int c = 0; c < kClasses; ++c) {
+            s->class_seconds[c] = class_seconds_[c];
+            s->class_bytes[c] = class_bytes_[c];
+            s->class_launches[c] = class_launches_[c];
+        }
     }
 
     int64_t admit_only() {
@@ -997,6 +1275,39 @@ class BatchEngine {
         return 2 * nnz * 12 + (n + m + 2) * 4 + m * 8 + n * 8 + 64 * (n + m);
     }
 
+    template <int NT, int NR>
+    void launch_lds_t(int c, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
+        batch_advance_lds_kernel<NT, NR><<<grid, NT, size_t(kClassDoubles[c]) * 8, cs>>>(d_desc_, d_state_, slots, bo);
+    }
+    void launch_lds(int c, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
+        const bool wide = c == kClasses - 2;
+        switch (rows_per_thread_) {
+        case 4:
+            wide ? launch_lds_t<kWideThreads, 4>(c, grid, cs, slots, bo) : launch_lds_t<kThreads, 4>(c, grid, cs, slots, bo);
+            break;
+        case 2:
+            wide ? launch_lds_t<kWideThreads, 2>(c, grid, cs, slots, bo) : launch_lds_t<kThreads, 2>(c, grid, cs, slots, bo);
+            break;
+        default:
+            wide ? launch_lds_t<kWideThreads, 1>(c, grid, cs, slots, bo) : launch_lds_t<kThreads, 1>(c, grid, cs, slots, bo);
+        }
+    }
+
+    // Launch class: the smallest LDS capacity that holds y and dx when every
+    // row of A and A^T is short; otherwise the global-memory stream kernel.
+    static int lds_class(const Job &j) {
+        const nyk::Model &m = j.prep.model;
+        int64_t longest = 0;
+        for (int64_t i = 0; i < m.rows(); ++i) longest = std::max(longest, m.A.ptr[i + 1] - m.A.ptr[i]);
+        std::vector<int64_t> col(m.cols(), 0);
+        for (auto c : m.A.index) longest = std::max(longest, ++col[c]);
+        const int64_t need = m.rows() + m.cols();
+        if (longest <= kShortRow)
+            for (int c = 0; c < kClasses - 1; ++c)
+                if (need <= kClassDoubles[c]) return c;
+        return kClasses - 1;
+    }
+
     // Admits queued LPs while a slot and arena room are free (FIFO).
     void admit() {
         while (!queue_.empty() && !free_slots_.empty()) {
@@ -1025,6 +1336,7 @@ class BatchEngine {
             j.bytes = bytes;
             slot_job_[s] = int(j.ticket);
             slot_bytes_[s] = attempt_bytes(j);
+            slot_class_[s] = lds_class(j);
             desc_[s] = d;
             LpState st{};
             st.step = j.step;
@@ -1363,8 +1675,14 @@ class BatchEngine {
     std::vector<LpState> state_;
     std::vector<int> slot_job_, free_slots_;
     std::vector<double> slot_bytes_;
-    std::vector<hipEvent_t> ev_;
+    hipStream_t class_stream_[kClasses] = {};
+    int rows_per_thread_ = 2;  // measured best on gfx950 for the 3-8 nnz/column batch
+    std::vector<hipEvent_t> class_events_[kClasses];
+    hipEvent_t uploaded_ = nullptr;
+    std::vector<int> slot_class_;
     double advance_seconds_ = 0, check_seconds_ = 0, advance_bytes_ = 0;
+    double class_seconds_[kClasses] = {}, class_bytes_[kClasses] = {};
+    int64_t class_launches_[kClasses] = {};
     int64_t attempts_ = 0;
     std::vector<std::unique_ptr<Job>> jobs_;
     std::deque<int64_t> queue_;
