@@ -133,7 +133,19 @@ def main():
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--checks", type=int, default=5, help="check periods per launch")
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        src = torch.ones(args.copy_probe // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)

@@ -557,6 +557,42 @@ __device__ __forceinline__ double row_dot(const double *val, const int *idx, int
 // y is staged into LDS once, dx never leaves LDS, and every row is summed by
 // one thread (rows are short in this class).
 // Dot products of NR CSR rows (one per q, `ok[q]` false for absent rows)
+// against an LDS-resident vector.  The rows advance in lock step, kStep
+// entries per row per step, so up to NR * kStep independent loads are in
+// flight per thread (a row of at most kStep entries costs one round trip);
+// each row is still summed in storage order starting from acc[q].
+constexpr int kStep = 8;
+
+template <int NR>
+__device__ __forceinline__ void rows_dot(const int *k, const int *e, const int *idx, const double *val,
+                                         const double *vec, double *acc) {
+    int longest = 0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) longest = max(longest, e[q] - k[q]);
+    for (int t = 0; t < longest; t += kStep) {
+        double v[NR][kStep];
+        int ix[NR][kStep];
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+#pragma unroll
+            for (int u = 0; u < kStep; ++u) {
+                const int kk = k[q] + t + u;
+                const bool on = kk < e[q];
+                v[q][u] = on ? val[kk] : 0.0;
+                ix[q][u] = on ? idx[kk] : 0;
+            }
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+#pragma unroll
+            for (int u = 0; u < kStep; ++u)
+                if (k[q] + t + u < e[q]) acc[q] += v[q][u] * vec[ix[q][u]];
+    }
+}
+
+// One PDHG attempt for an LP whose dual iterate and primal step fit in LDS:
+// y is staged into LDS once, dx never leaves LDS, and every row is summed by
+// one thread (rows are short in this class).
+// Dot products of NR CSR rows (one per q, `ok[q]` false for absent rows)
 // against an LDS-resident vector.  The rows advance in lock step, two entries
 // per row per step, so 2*NR independent loads are in flight per thread; each
 // row is still summed in storage order starting from acc[q].
@@ -605,22 +641,29 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
     Reducer<3, 0, 0> red_acc{};
     // g = c + A^T y and the projected primal step (src/cpu.cpp:107-132)
     for (int base = threadIdx.x; base < L.n; base += NR * NT) {
-        int r[NR];
+        int r[NR], k[NR], e[NR];
         bool ok[NR];
-        double g[NR];
+        double g[NR], xj_[NR], xa_[NR], lo_[NR], hi_[NR];
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             r[q] = base + q * NT;
             ok[q] = r[q] < L.n;
-            g[q] = ok[q] ? L.c[r[q]] : 0.0;
+            const int j = ok[q] ? r[q] : 0;
+            k[q] = ok[q] ? L.tp[j] : 0;
+            e[q] = ok[q] ? L.tp[j + 1] : 0;
+            g[q] = ok[q] ? L.c[j] : 0.0;
+            xj_[q] = x[j];
+            xa_[q] = xa[j];
+            lo_[q] = L.lb[j];
+            hi_[q] = L.ub[j];
         }
-        rows_dot<NR>(L.tp, L.ti, L.tv, r, ok, ylds, g);
+        rows_dot<NR>(k, e, L.ti, L.tv, ylds, g);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             if (!ok[q]) continue;
             const int j = r[q];
-            const double xj = x[j], v = xa[j];
-            const double t = clampd(xj - tau * g[q], L.lb[j], L.ub[j]);
+            const double xj = xj_[q], v = xa_[q];
+            const double t = clampd(xj - tau * g[q], lo_[q], hi_[q]);
             const double d = t - xj;
             red_acc.s[0] += d * d;
             dxlds[j] = d;
@@ -631,23 +674,30 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
     __syncthreads();
     // delta = A dx and the extrapolated dual step (src/cpu.cpp:136-150)
     for (int base = threadIdx.x; base < L.m; base += NR * NT) {
-        int r[NR];
+        int r[NR], k[NR], e[NR];
         bool ok[NR];
-        double delta[NR];
+        double delta[NR], a_[NR], ya_[NR], lo_[NR], hi_[NR];
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             r[q] = base + q * NT;
             ok[q] = r[q] < L.m;
+            const int i = ok[q] ? r[q] : 0;
+            k[q] = ok[q] ? L.ap[i] : 0;
+            e[q] = ok[q] ? L.ap[i + 1] : 0;
             delta[q] = 0.0;
+            a_[q] = ax[i];
+            ya_[q] = ya[i];
+            lo_[q] = L.rl[i];
+            hi_[q] = L.ru[i];
         }
-        rows_dot<NR>(L.ap, L.ai, L.av, r, ok, dxlds, delta);
+        rows_dot<NR>(k, e, L.ai, L.av, dxlds, delta);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             if (!ok[q]) continue;
             const int i = r[q];
-            const double a = ax[i], yi = ylds[i], v = ya[i];
+            const double a = a_[q], yi = ylds[i], v = ya_[q];
             const double bar = a + 2 * delta[q], z = yi + sigma * bar;
-            const double t = maxd(0., z - sigma * L.ru[i]) + mind(0., z - sigma * L.rl[i]);
+            const double t = maxd(0., z - sigma * hi_[q]) + mind(0., z - sigma * lo_[q]);
             const double dy = t - yi;
             red_acc.s[1] += (!isfinite(bar) || !isfinite(t)) ? INFINITY : dy * dy;
             red_acc.s[2] += dy * delta[q];
