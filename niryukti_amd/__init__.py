@@ -15,13 +15,14 @@ from pathlib import Path
 import numpy as np
 
 __all__ = [
-    "load_library", "LP", "solve_lp", "Instance", "Backend", "Batch", "device_count", "device_name",
-    "ABI_VERSION",
+    "load_library", "LP", "solve_lp", "Instance", "Backend", "Batch", "NativeSession", "device_count",
+    "device_name", "ABI_VERSION", "CLI_PATH",
 ]
 
 _PKG = Path(__file__).resolve().parent
 LIBRARY_PATH = Path(os.environ.get("NIRYUKTI_LIBRARY", _PKG / "lib" / "libniryukti.so"))
 ABI_VERSION = 1
+CLI_PATH = _PKG / "bin" / "niryukti"
 
 _f64p = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
@@ -396,3 +397,5 @@ def device_name(index: int = 0) -> str:
     load_library().npdhg_device_name(index, buf, 256)
     return buf.value.decode()
 
+
+from .session import NativeSession  # noqa: E402  (uses load_library above)

@@ -353,8 +353,19 @@ Result solve(const Model &m, const Options &options) {
     validate_options(options);
     Options o = options;
     if (o.method == "auto") o.method = "pdhg";
+    auto requested = [&](Result &r) {
+        r.method_requested = options.method;
+        r.automatic = options.method == "auto";
+        r.scaling = o.scaling;
+        r.tol = o.tol;
+        r.time_limit = o.time_limit;
+        r.iteration_limit = o.iteration_limit;
+        r.scaling_passes = o.scaling_passes;
+        if (r.method_selected.empty()) r.method_selected = o.method;
+    };
     if (o.device == "cpu" || o.device == "cuda") {
         Result r;
+        requested(r);
         r.status = "UNSUPPORTED";
         r.backend = o.device;
         r.message = "Requested backend not compiled: " + o.device + " (this engine runs PDHG on HIP; use --device gpu)";
@@ -371,11 +382,13 @@ Result solve(const Model &m, const Options &options) {
         if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
     if (npdhg_device_count() < 1) {
         Result r;
+        requested(r);
         r.status = "UNSUPPORTED";
         r.message = "No HIP device is visible";
         return r;
     }
     auto result = solve_continuous(m, o);
+    requested(result);
     result.seconds = elapsed(overall);
     return result;
 }

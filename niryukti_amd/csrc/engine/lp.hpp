@@ -86,6 +86,12 @@ struct Result {
     bool graph_execution = false;
     std::string matrix_precision = "fp64", method_selected, device_reason;
     double estimated_gpu_bytes = 0;
+    // The request as the caller made it (reported under "selection").
+    std::string method_requested, scaling = "ruiz";
+    bool automatic = false;
+    double tol = 1e-6, time_limit = 60;
+    int64_t iteration_limit = 100000;
+    int scaling_passes = 5;
 };
 
 // Presolved and scaled LP plus the maps back to the original model

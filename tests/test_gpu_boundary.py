@@ -1,0 +1,101 @@
+"""Drop-in boundary on the GPU: the CLI and the session C-ABI solving real
+models end to end.
+
+Netlib models are solved by `niryukti solve` and compared with the result
+document the reference CLI produced for the same file (recorded by
+tests/golden/make_netlib_golden.py): same status, objective within 1e-5
+relative (both points are certified at rel-KKT <= 1e-6, which bounds their
+objective difference by a few 1e-6), and the returned point passes the
+oracle's FP64 verifier and `niryukti verify`.  Session scenarios follow the
+reference's tests/test_extensions.py:20-38.
+"""
+import json
+import subprocess
+from pathlib import Path
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+NETLIB = ROOT / "tests" / "golden" / "netlib"
+
+
+def run_cli(engine, *args, timeout=120):
+    return subprocess.run([str(engine.CLI_PATH), *map(str, args)], capture_output=True, text=True, timeout=timeout)
+
+
+def linear_model():
+    return {"name": "linear", "sense": "min",
+            "variables": [{"name": "x", "ub": 10}],
+            "objective": {"linear": [3]},
+            "constraints": [{"name": "demand", "lb": 2, "coefficients": {"x": 1}}]}
+
+
+@pytest.mark.parametrize("name", ("afiro", "adlittle", "israel"))
+def test_netlib_solve_matches_reference(engine, name, tmp_path):
+    ref = json.loads((NETLIB / f"{name}.reference_result.json").read_text())
+    out_path = tmp_path / "result.json"
+    out = run_cli(engine, "solve", NETLIB / f"{name}.mps", "--json-out", out_path)
+    print(name, out.stderr[-400:])
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr
+    mine = json.loads(out.stdout)
+    print(name, mine["status"], mine["performance"]["iterations"], mine["objective"], ref["objective"])
+    assert mine["status"] == "OPTIMAL"
+    assert mine["model"]["fingerprint"] == ref["model"]["fingerprint"]
+    assert mine["accuracy"]["kkt_error"] <= 1e-6
+    assert abs(mine["objective"] - ref["objective"]) <= 1e-5 * (1 + abs(ref["objective"]))
+    check = run_cli(engine, "verify", NETLIB / f"{name}.mps", out_path)
+    assert check.returncode == 0 and json.loads(check.stdout)["status"] == "VERIFIED_OPTIMAL"
+
+
+def test_netlib_iteration_limit_is_reported(engine):
+    # e226 does not reach 1e-6 within the reference's default 100000 iterations;
+    # a short limit must end in ITERATION_LIMIT with exit code 2 and a finite point.
+    out = run_cli(engine, "solve", NETLIB / "e226.mps", "--iterations", "2000")
+    assert out.returncode == 2
+    r = json.loads(out.stdout)
+    assert r["status"] == "ITERATION_LIMIT"
+    assert r["performance"]["iterations"] <= 2000
+    assert len(r["primal"]) == r["model"]["columns"]
+
+
+def test_warm_start_from_result_file(engine, tmp_path):
+    first = tmp_path / "first.json"
+    out = run_cli(engine, "solve", NETLIB / "adlittle.mps", "--json-out", first)
+    assert out.returncode == 0
+    cold = json.loads(out.stdout)["performance"]["iterations"]
+    out = run_cli(engine, "solve", NETLIB / "adlittle.mps", "--warm-start", first)
+    assert out.returncode == 0
+    warm = json.loads(out.stdout)
+    assert warm["status"] == "OPTIMAL"
+    assert warm["performance"]["iterations"] < cold
+
+
+def test_session_updates_and_rollback(engine):
+    with engine.NativeSession(linear_model()) as s:
+        assert s.solve()["objective"] == pytest.approx(6, abs=1e-5)
+        s.update_row_bounds({"demand": {"lb": 4}})
+        assert s.solve()["objective"] == pytest.approx(12, abs=1e-5)
+        with pytest.raises(ValueError):
+            s.update_variable_bounds({"x": {"lb": 11}})
+        assert s.solve()["objective"] == pytest.approx(12, abs=1e-5)
+        with pytest.raises(ValueError):
+            s._request("solve", unknown_option=True)
+        s.update_objective({"x": 5})
+        r = s.solve()
+        assert r["status"] == "OPTIMAL" and r["objective"] == pytest.approx(20, abs=1e-5)
+        assert r["session"]["warm_start_requested"] is True
+    with pytest.raises(RuntimeError):
+        s.solve()
+
+
+def test_session_on_netlib_warm_start(engine):
+    with engine.NativeSession(str(NETLIB / "afiro.mps")) as s:
+        cold = s.solve()
+        assert cold["status"] == "OPTIMAL"
+        again = s.solve()
+        assert again["status"] == "OPTIMAL"
+        assert again["performance"]["iterations"] <= cold["performance"]["iterations"]
+        fresh = s.solve(warm_start=False)
+        assert fresh["performance"]["iterations"] == cold["performance"]["iterations"]
