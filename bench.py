@@ -115,6 +115,105 @@ def cpu_baseline(count, budget_s=20.0):
             "cpu": _cpu_model()}
 
 
+def _csv_rows(directory, suffix):
+    import csv
+    import glob
+
+    rows = []
+    for path in sorted(glob.glob(f"{directory}/**/*{suffix}", recursive=True)):
+        with open(path, newline="") as f:
+            rows += list(csv.DictReader(f))
+    if not rows:
+        raise SystemExit(f"no *{suffix} under {directory} (run rocprofv3 with --output-format csv)")
+    return rows
+
+
+def _col(row, *names):
+    lower = {k.lower(): k for k in row}
+    for n in names:
+        if n.lower() in lower:
+            return row[lower[n.lower()]]
+    raise SystemExit(f"rocprofv3 CSV lacks {names}; columns are {sorted(row)}")
+
+
+def _short(name):
+    return name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").strip()
+
+
+def summarize_trace(directory):
+    """Per (kernel, LDS bytes) dispatch count and mean duration from a
+    `rocprofv3 --kernel-trace --stats --output-format csv` directory, plus the
+    tool's own kernel_stats table."""
+    out = ["# rocprofv3 kernel_stats.csv (durations in ns)"]
+    for r in _csv_rows(directory, "kernel_stats.csv")[:12]:
+        out.append(f"{_short(_col(r, 'Name'))[:60]:60s} calls={_col(r, 'Calls'):>7s} "
+                   f"total_ns={_col(r, 'TotalDurationNs'):>14s} avg_ns={float(_col(r, 'AverageNs')):14.1f} "
+                   f"pct={float(_col(r, 'Percentage')):6.2f}")
+    groups = {}
+    for r in _csv_rows(directory, "kernel_trace.csv"):
+        key = (_short(_col(r, "Kernel_Name")), int(_col(r, "LDS_Block_Size", "Lds_Block_Size", "LDS_Size")))
+        dur = int(_col(r, "End_Timestamp")) - int(_col(r, "Start_Timestamp"))
+        g = groups.setdefault(key, [0, 0])
+        g[0] += 1
+        g[1] += dur
+    out.append("# per kernel and LDS allocation (from kernel_trace.csv)")
+    for (name, lds), (n, tot) in sorted(groups.items(), key=lambda kv: -kv[1][1]):
+        out.append(json.dumps({"kernel": name, "lds_bytes": lds, "dispatches": n,
+                               "mean_ms": round(tot / n / 1e6, 4), "total_s": round(tot / 1e9, 4)}))
+    return "\n".join(out)
+
+
+def _pmc_means(directory, counter):
+    per = {}
+    for r in _csv_rows(directory, "counter_collection.csv"):
+        if _col(r, "Counter_Name") != counter:
+            continue
+        key = (_short(_col(r, "Kernel_Name")), int(_col(r, "LDS_Block_Size", "Lds_Block_Size", "LDS_Size")))
+        per.setdefault(key, {}).setdefault(_col(r, "Dispatch_Id"), 0.0)
+        per[key][_col(r, "Dispatch_Id")] += float(_col(r, "Counter_Value"))
+    return {k: (len(v), sum(v.values()) / len(v)) for k, v in per.items()}
+
+
+def summarize_traffic(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes):
+    """HBM bytes per dispatch from FETCH_SIZE and WRITE_SIZE runs (KiB per
+    dispatch), each scaled by what that counter reads for a plain copy of
+    `copy_bytes` bytes (the largest dispatch of the copy-probe run)."""
+    cf = max(v[1] for v in _pmc_means(copy_fetch_dir, "FETCH_SIZE").values())
+    cw = max(v[1] for v in _pmc_means(copy_write_dir, "WRITE_SIZE").values())
+    fetch_scale = copy_bytes / (cf * 1024.0)
+    write_scale = copy_bytes / (cw * 1024.0)
+    out = [f"# copy probe of {copy_bytes} bytes: FETCH_SIZE {cf:.1f} KiB, WRITE_SIZE {cw:.1f} KiB",
+           f"# scale factors: fetch x{fetch_scale:.4f}, write x{write_scale:.4f}"]
+    fetch, write = _pmc_means(fetch_dir, "FETCH_SIZE"), _pmc_means(write_dir, "WRITE_SIZE")
+    for key in sorted(fetch, key=lambda k: -fetch[k][0] * fetch[k][1]):
+        n, f = fetch[key]
+        w = write.get(key, (0, 0.0))[1]
+        out.append(json.dumps({"kernel": key[0], "lds_bytes": key[1], "dispatches": n,
+                               "fetch_KiB": round(f, 1), "write_KiB": round(w, 1),
+                               "hbm_bytes_per_dispatch": round(f * 1024 * fetch_scale + w * 1024 * write_scale)}))
+    return "\n".join(out)
+
+
+TRAFFIC_PROFILE = ROOT / "profiles" / "batch_advance_hbm_traffic_4096lp.txt"
+
+
+def committed_traffic(kernel, rank):
+    """HBM bytes per launch of `kernel` from the committed FETCH/WRITE summary;
+    `rank` picks among instantiations that share a symbol by LDS size."""
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    entries = []
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith("{"):
+            e = json.loads(line)
+            if e["kernel"].replace(" ", "") == kernel.replace(" ", ""):
+                entries.append(e)
+    entries.sort(key=lambda e: e["lds_bytes"])
+    if rank >= len(entries):
+        return None
+    return entries[rank]["hbm_bytes_per_dispatch"]
+
+
 def _cpu_model():
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -135,7 +234,17 @@ def main():
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
+    ap.add_argument("--summarize-trace", metavar="DIR", help="summarise a rocprofv3 --kernel-trace --stats CSV run")
+    ap.add_argument("--summarize-traffic", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="summarise FETCH_SIZE / WRITE_SIZE CSV runs of the bench and of --copy-probe")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     args = ap.parse_args()
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace))
+        return
+    if args.summarize_traffic:
+        print(summarize_traffic(*args.summarize_traffic, args.copy_bytes))
+        return
     if args.copy_probe:
         import torch
 
@@ -179,8 +288,11 @@ def main():
     dom = max(range(4), key=lambda c: cls_b[c])
     adv_b, adv_s, launches = cls_b[dom], cls_s[dom], cls_n[dom]
     achieved = adv_b / adv_s / 1e9 if adv_s > 0 else 0.0
-    kernel_name = ["batch_advance_lds_kernel<256,2> (LDS 16 KiB class)", "batch_advance_lds_kernel<256,2> (LDS 32 KiB class)",
-                   "batch_advance_lds_kernel<512,2> (LDS 62.5 KiB class)", "batch_advance_kernel (global class)"][dom]
+    rows = int(os.environ.get("NIRYUKTI_BATCH_ROWS_PER_THREAD", "2"))
+    symbol = [f"batch_advance_lds_kernel<256, {rows}>", f"batch_advance_lds_kernel<256, {rows}>",
+              f"batch_advance_lds_kernel<512, {rows}>", "batch_advance_kernel"][dom]
+    kernel_name = symbol + [" (LDS 16 KiB class)", " (LDS 32 KiB class)", " (LDS 62.5 KiB class)", " (global class)"][dom]
+    traffic = committed_traffic(symbol, [0, 1, 0, 0][dom])
     statuses = {}
     for r in last:
         statuses[r["status"]] = statuses.get(r["status"], 0) + 1
@@ -208,7 +320,8 @@ def main():
         "unsolved_indices": [k for k, r in enumerate(last) if r["status"] != "OPTIMAL"][:20],
         "iterations": {"mean": float(its.mean()), "max": int(its.max())},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": TRAFFIC_PROFILE.name if traffic is not None else None,
                      "kernel": kernel_name,
                      "algorithmic_bytes_per_launch": round(adv_b / max(1, launches), 1),
                      "avg_launch_ms": round(1e3 * adv_s / max(1, launches), 3),
