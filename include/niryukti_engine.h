@@ -40,6 +40,7 @@ typedef struct {
     int64_t iteration_limit;
     int check_every, scaling_passes, threads;
     int presolve, restart, adaptive, graphs, verbose;
+    int device_checks;       /* -1 auto (>= 100,000 nonzeros), 0 host checks, 1 device checks */
     const double *initial_x; /* NULL or cols */
     const double *initial_y; /* NULL or rows */
 } niryukti_options;
@@ -51,7 +52,7 @@ typedef struct {
     double primal_residual, dual_residual, relative_gap, kkt_error, complementarity, dual_bound;
     double primal_absolute, dual_absolute;
     int finite;
-    int64_t iterations, restarts, rejected_steps, weight_updates, host_candidate_checks;
+    int64_t iterations, restarts, rejected_steps, weight_updates, host_candidate_checks, device_candidate_checks;
     int64_t removed_rows, removed_columns;
     double seconds, preprocess_seconds, transfer_seconds, iteration_seconds, verification_seconds;
     int index_bits;

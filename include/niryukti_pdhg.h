@@ -78,6 +78,64 @@ int npdhg_reset(npdhg_backend *b, const double *x, const double *y);
  * (src/cuda.cu:892-930). */
 int npdhg_reset_candidate(npdhg_backend *b, int average);
 
+/* ---- Device-side candidate checks ----
+ *
+ * The reference checks both candidates on the host every chunk: restore to the
+ * original space, Verifier::evaluate twice, Farkas rays, epoch displacement
+ * (src/solver.cpp:550-633, 692-716; src/cuda.cu:867-876 copies four vectors
+ * down for it).  These entry points run the same formulas on the device
+ * against a resident copy of the original model, so a check moves a few
+ * scalars instead of four vectors; the host certifies a candidate with its own
+ * FP64 verifier and outward-rounded bound before reporting OPTIMAL. */
+
+/* The original model and the maps of the prepare step (src/internal.hpp:16-25):
+ * restore_x puts x[j] * column_scale[j] at cols_map[j] over `fixed`;
+ * restore_y puts y[i] * row_scale[i] * objective_scale at upper_rows[i] when
+ * y[i] >= 0, else at lower_rows[i]. */
+typedef struct {
+    int64_t rows, cols;                 /* original m0, n0 */
+    const int64_t *ptr, *index;         /* original A, CSR */
+    const double *value;
+    const double *c, *lb, *ub, *rl, *ru; /* canonical (minimisation) data */
+    double offset;
+    const int64_t *cols_map;            /* n entries */
+    const int64_t *lower_rows, *upper_rows; /* m entries */
+    const double *fixed;                /* n0 entries */
+    const double *column_scale, *row_scale; /* n, m entries */
+    double objective_scale;
+} npdhg_original;
+
+/* Verifier::evaluate quantities without the safe bound (src/verify.cpp:103-240),
+ * plus whether the candidate's dual could be a Farkas ray: 0 means a rigorous
+ * upper bound of the ray's Lagrangian value (src/verify.cpp:12-88) is <= 0, so
+ * the host test (src/solver.cpp:599-633) would reject it. */
+typedef struct {
+    double objective, primal, dual, gap, kkt;
+    int finite, farkas_possible;
+} npdhg_accuracy;
+
+/* Uploads the original model; the epoch point becomes the current iterate.
+ * Fails (and device checks stay off) when the row maps are not one-to-one. */
+int npdhg_attach_original(npdhg_backend *b, const npdhg_original *orig);
+/* Evaluates the current (out[0]) and averaged (out[1]) candidates in the
+ * original space, keeps both restored points on the device and rebases the
+ * activity A x as npdhg_candidates does. */
+int npdhg_evaluate(npdhg_backend *b, npdhg_accuracy *current, npdhg_accuracy *average);
+/* Best-so-far tracking: copies the restored candidate of the last evaluate. */
+int npdhg_keep_best(npdhg_backend *b, int average);
+/* Downloads a restored candidate of the last evaluate (which = 0 current,
+ * 1 average) or the best (which = 2); x0 has n0 entries, y0 m0 (either NULL). */
+int npdhg_original_point(npdhg_backend *b, int which, double *x0, double *y0);
+/* Farkas pre-check of the restored epoch displacement of a candidate's dual
+ * (average = 0 current, 1 averaged); with *possible != 0 and ray != NULL, the
+ * restored displacement (m0 entries) is copied out for the host's exact test. */
+int npdhg_farkas_displacement(npdhg_backend *b, int average, int *possible, double *ray);
+/* Squared epoch displacement of a candidate in the scaled space, summed in
+ * double-double (src/solver.cpp:702-716 sums in long double). */
+int npdhg_epoch_displacement(npdhg_backend *b, int average, double *dx2, double *dy2);
+/* Epoch := the current (average = 0) or averaged candidate. */
+int npdhg_set_epoch(npdhg_backend *b, int average);
+
 /* y = A x (or y = A^T x when transpose != 0) on the device with the backend's
  * SpMV kernels; host pointers in and out.  Used by tests of the SpMV alone. */
 int npdhg_spmv(npdhg_backend *b, int transpose, const double *x, double *y);

@@ -40,7 +40,7 @@ class _Options(C.Structure):
                 ("tol", C.c_double), ("time_limit", C.c_double), ("iteration_limit", C.c_int64),
                 ("check_every", C.c_int), ("scaling_passes", C.c_int), ("threads", C.c_int),
                 ("presolve", C.c_int), ("restart", C.c_int), ("adaptive", C.c_int), ("graphs", C.c_int),
-                ("verbose", C.c_int), ("initial_x", _f64p), ("initial_y", _f64p)]
+                ("verbose", C.c_int), ("device_checks", C.c_int), ("initial_x", _f64p), ("initial_y", _f64p)]
 
 
 class _Result(C.Structure):
@@ -50,6 +50,7 @@ class _Result(C.Structure):
                 ("primal_absolute", C.c_double), ("dual_absolute", C.c_double), ("finite", C.c_int),
                 ("iterations", C.c_int64), ("restarts", C.c_int64), ("rejected_steps", C.c_int64),
                 ("weight_updates", C.c_int64), ("host_candidate_checks", C.c_int64),
+                ("device_candidate_checks", C.c_int64),
                 ("removed_rows", C.c_int64), ("removed_columns", C.c_int64), ("seconds", C.c_double),
                 ("preprocess_seconds", C.c_double), ("transfer_seconds", C.c_double),
                 ("iteration_seconds", C.c_double), ("verification_seconds", C.c_double),
@@ -65,6 +66,18 @@ class _PdhgConfig(C.Structure):
     _fields_ = [("adaptive", C.c_int), ("halpern", C.c_int), ("reflection", C.c_double),
                 ("residual_restarts", C.c_int), ("graphs", C.c_int), ("index_bits", C.c_int),
                 ("fp32_values", C.c_int)]
+
+
+class _PdhgOriginal(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("cols", C.c_int64), ("ptr", _i64p), ("index", _i64p), ("value", _f64p),
+                ("c", _f64p), ("lb", _f64p), ("ub", _f64p), ("rl", _f64p), ("ru", _f64p), ("offset", C.c_double),
+                ("cols_map", _i64p), ("lower_rows", _i64p), ("upper_rows", _i64p), ("fixed", _f64p),
+                ("column_scale", _f64p), ("row_scale", _f64p), ("objective_scale", C.c_double)]
+
+
+class _PdhgAccuracy(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("objective", "primal", "dual", "gap", "kkt")] + \
+               [("finite", C.c_int), ("farkas_possible", C.c_int)]
 
 
 class _PdhgStatus(C.Structure):
@@ -304,6 +317,45 @@ class Backend:
 
     def reset_candidate(self, average: bool):
         self._check(self._lib.npdhg_reset_candidate(self._h, int(bool(average))))
+
+    def attach_original(self, lp0: LP, cols_map, lower_rows, upper_rows, fixed, column_scale, row_scale,
+                        objective_scale: float):
+        """Device-side checks against `lp0` with the prepare step's maps."""
+        keep = [np.ascontiguousarray(cols_map, dtype=np.int64), np.ascontiguousarray(lower_rows, dtype=np.int64),
+                np.ascontiguousarray(upper_rows, dtype=np.int64), _f64(fixed), _f64(column_scale), _f64(row_scale)]
+        o = _PdhgOriginal(lp0.rows, lp0.cols, _ptr(lp0.ptr, _i64p), _ptr(lp0.index, _i64p), _ptr(lp0.value),
+                          _ptr(lp0.c), _ptr(lp0.lb), _ptr(lp0.ub), _ptr(lp0.rl), _ptr(lp0.ru), float(lp0.offset),
+                          _ptr(keep[0], _i64p), _ptr(keep[1], _i64p), _ptr(keep[2], _i64p), _ptr(keep[3]),
+                          _ptr(keep[4]), _ptr(keep[5]), float(objective_scale))
+        self._check(self._lib.npdhg_attach_original(self._h, C.byref(o)))
+        self._lp0 = lp0
+
+    def evaluate(self):
+        cur, avg = _PdhgAccuracy(), _PdhgAccuracy()
+        self._check(self._lib.npdhg_evaluate(self._h, C.byref(cur), C.byref(avg)))
+        return tuple({k: getattr(a, k) for k, _ in _PdhgAccuracy._fields_} for a in (cur, avg))
+
+    def keep_best(self, average: bool):
+        self._check(self._lib.npdhg_keep_best(self._h, int(bool(average))))
+
+    def original_point(self, which: int):
+        x, y = np.zeros(self._lp0.cols), np.zeros(self._lp0.rows)
+        self._check(self._lib.npdhg_original_point(self._h, int(which), _ptr(x), _ptr(y)))
+        return x, y
+
+    def farkas_displacement(self, average: bool):
+        possible = C.c_int()
+        ray = np.zeros(self._lp0.rows)
+        self._check(self._lib.npdhg_farkas_displacement(self._h, int(bool(average)), C.byref(possible), _ptr(ray)))
+        return bool(possible.value), ray
+
+    def epoch_displacement(self, average: bool):
+        dx, dy = C.c_double(), C.c_double()
+        self._check(self._lib.npdhg_epoch_displacement(self._h, int(bool(average)), C.byref(dx), C.byref(dy)))
+        return dx.value, dy.value
+
+    def set_epoch(self, average: bool):
+        self._check(self._lib.npdhg_set_epoch(self._h, int(bool(average))))
 
     def spmv(self, x, transpose=False):
         x = _f64(x)

@@ -54,6 +54,7 @@ nyk::Options to_options(const niryukti_options *in, const niryukti_lp &lp) {
     o.adaptive = in->adaptive != 0;
     o.graphs = in->graphs != 0;
     o.verbose = in->verbose != 0;
+    o.device_checks = in->device_checks < 0 ? -1 : in->device_checks > 0 ? 1 : 0;
     if (in->initial_x) o.initial_x.assign(in->initial_x, in->initial_x + lp.cols);
     if (in->initial_y) o.initial_y.assign(in->initial_y, in->initial_y + lp.rows);
     return o;
@@ -84,6 +85,7 @@ void niryukti_default_options(niryukti_options *o) {
     o->restart = 1;
     o->adaptive = 1;
     o->graphs = 1;
+    o->device_checks = -1;
 }
 
 int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *opts, niryukti_result *out) {
@@ -111,6 +113,7 @@ int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *opts, niryu
         out->rejected_steps = r.rejected_steps;
         out->weight_updates = r.weight_updates;
         out->host_candidate_checks = r.host_candidate_checks;
+        out->device_candidate_checks = r.device_candidate_checks;
         out->removed_rows = r.removed_rows;
         out->removed_columns = r.removed_columns;
         out->seconds = r.seconds;

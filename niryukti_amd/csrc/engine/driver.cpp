@@ -82,6 +82,48 @@ class Device {
     void reset_candidate(bool average) { check(npdhg_reset_candidate(b_, average ? 1 : 0)); }
     void reset(const std::vector<double> &x, const std::vector<double> &y) { check(npdhg_reset(b_, x.data(), y.data())); }
 
+    // Device-side checks against a resident copy of the original model; false
+    // when the prepare maps cannot be inverted (the driver then checks on the host).
+    bool attach(const Model &original, const Prepared &prep) {
+        npdhg_original o{};
+        o.rows = original.rows();
+        o.cols = original.cols();
+        o.ptr = original.A.ptr.data();
+        o.index = original.A.index.data();
+        o.value = original.A.value.data();
+        o.c = original.c.data();
+        o.lb = original.lb.data();
+        o.ub = original.ub.data();
+        o.rl = original.rl.data();
+        o.ru = original.ru.data();
+        o.offset = original.offset;
+        o.cols_map = prep.cols.data();
+        o.lower_rows = prep.lower_rows.data();
+        o.upper_rows = prep.upper_rows.data();
+        o.fixed = prep.fixed.data();
+        o.column_scale = prep.column_scale.data();
+        o.row_scale = prep.row_scale.data();
+        o.objective_scale = prep.objective_scale;
+        return npdhg_attach_original(b_, &o) == 0;
+    }
+    void evaluate(npdhg_accuracy &current, npdhg_accuracy &average) { check(npdhg_evaluate(b_, &current, &average)); }
+    void keep_best(bool average) { check(npdhg_keep_best(b_, average ? 1 : 0)); }
+    void original_point(int which, std::vector<double> *x, std::vector<double> *y) {
+        check(npdhg_original_point(b_, which, x ? x->data() : nullptr, y ? y->data() : nullptr));
+    }
+    bool farkas_displacement(bool average, std::vector<double> &ray) {
+        int possible = 0;
+        check(npdhg_farkas_displacement(b_, average ? 1 : 0, &possible, ray.data()));
+        return possible != 0;
+    }
+    void epoch_displacement(bool average, long double &dx, long double &dy) {
+        double a = 0, b = 0;
+        check(npdhg_epoch_displacement(b_, average ? 1 : 0, &a, &b));
+        dx = a;
+        dy = b;
+    }
+    void set_epoch(bool average) { check(npdhg_set_epoch(b_, average ? 1 : 0)); }
+
   private:
     static void check(int rc) {
         if (rc != 0) throw std::runtime_error(npdhg_last_error());
@@ -93,6 +135,157 @@ std::string device_description() {
     char name[256] = "HIP device";
     if (npdhg_device_count() > 0) npdhg_device_name(0, name, sizeof name);
     return name;
+}
+
+constexpr int64_t kDeviceCheckNonzeros = 100000;
+
+Accuracy from_device(const npdhg_accuracy &d) {
+    Accuracy a;
+    a.finite = d.finite != 0;
+    if (!a.finite) return a;
+    a.objective = d.objective;
+    a.primal = d.primal;
+    a.dual = d.dual;
+    a.gap = d.gap;
+    a.kkt = d.kkt;
+    return a;
+}
+
+// Host-side Farkas test of one ray (src/solver.cpp:599-633): normalise by the
+// largest entry, then the exact outward-rounded bound decides.
+bool certify_ray(Verifier &verifier, std::vector<double> ray, Result &r) {
+    double norm_y = 0;
+    for (double v : ray) {
+        if (!std::isfinite(v)) return false;
+        norm_y = std::max(norm_y, std::abs(v));
+    }
+    if (!(norm_y > 0)) return false;
+    for (double &v : ray) v /= norm_y;
+    const double margin = verifier.infeasibility_bound(ray);
+    if (!(margin > 0 && std::isfinite(margin))) return false;
+    r.status = "INFEASIBLE";
+    r.message = "Independently verified box/row Farkas certificate";
+    r.infeasibility_ray = std::move(ray);
+    r.certificate_margin = margin;
+    return true;
+}
+
+// The chunk loop of solve_continuous with the candidate checks on the device:
+// the same decisions in the same order (src/solver.cpp:540-726), fed by
+// npdhg_evaluate's original-space KKT figures.  The host downloads a point
+// only to certify OPTIMAL with its own verifier and safe bound, to test a
+// Farkas ray the device could not rule out, and once at the end for the best
+// candidate.
+void run_device_checks(const Model &original, const Prepared &prep, const Options &o, Verifier &verifier,
+                       Device &backend, Result &r, Clock::time_point start, double step, double weight,
+                       double restart_kkt, WeightPid &controller) {
+    const bool residual_restarts = o.method == "rhpdhg" || o.method == "r2hpdhg";
+    const int64_t n0 = original.cols(), m0 = original.rows();
+    int64_t since_restart = 0, previous_rejected = 0;
+    double best_kkt = r.accuracy.kkt;
+    bool have_best = false;
+    while (r.iterations < o.iteration_limit) {
+        if (stop_requested(o)) {
+            r.status = "INTERRUPTED";
+            break;
+        }
+        if (elapsed(start) >= o.time_limit) {
+            r.status = "TIME_LIMIT";
+            break;
+        }
+        int count = int(std::min<int64_t>(o.check_every, o.iteration_limit - r.iterations));
+        auto tick = Clock::now();
+        int accepted = backend.advance(count, step / weight, step * weight);
+        r.iteration_seconds += elapsed(tick);
+        r.iterations += accepted;
+        since_restart += accepted;
+        auto st = backend.status();
+        r.rejected_steps += st.rejected_total - previous_rejected;
+        previous_rejected = st.rejected_total;
+        const bool restart_requested = st.restart_requested != 0;
+        r.device_candidate_checks++;
+        tick = Clock::now();
+        npdhg_accuracy dc{}, da{};
+        backend.evaluate(dc, da);
+        Accuracy ca = from_device(dc);
+        const Accuracy aa = from_device(da);
+        const bool avg = aa.kkt < ca.kkt;
+        const bool chosen_ray = avg ? da.farkas_possible : dc.farkas_possible;
+        const bool other_ray = avg ? dc.farkas_possible : da.farkas_possible;
+        if (avg) ca = aa;
+        if (ca.finite && ca.kkt < best_kkt) {
+            backend.keep_best(avg);
+            best_kkt = ca.kkt;
+            have_best = true;
+        }
+        r.verification_seconds += elapsed(tick);
+        if (o.verbose)
+            std::fprintf(stderr, "iter=%lld objective=%.12g primal=%.3e dual=%.3e gap=%.3e seconds=%.3f\n",
+                         (long long)r.iterations, original.sense * ca.objective, ca.primal, ca.dual, ca.gap,
+                         elapsed(start));
+        if (!ca.finite) {
+            r.status = "NUMERICAL_ERROR";
+            r.message = "Nonfinite iterate or diagnostic";
+            break;
+        }
+        if (ca.kkt <= o.tol) {
+            tick = Clock::now();
+            r.host_candidate_checks++;
+            std::vector<double> ox(n0), oy(m0);
+            backend.original_point(avg ? 1 : 0, &ox, &oy);
+            auto safe = verifier.evaluate(ox, oy);
+            r.verification_seconds += elapsed(tick);
+            if (safe.kkt <= o.tol) {
+                r.x = std::move(ox);
+                r.y = std::move(oy);
+                r.accuracy = safe;
+                r.status = "OPTIMAL";
+                return;
+            }
+            ca = safe;
+        }
+        if (r.iterations >= 1000 && ca.primal > o.tol) {
+            std::vector<double> ray(m0);
+            bool found = false;
+            if (chosen_ray) {
+                backend.original_point(avg ? 1 : 0, nullptr, &ray);
+                found = certify_ray(verifier, ray, r);
+            }
+            if (!found && other_ray) {
+                backend.original_point(avg ? 0 : 1, nullptr, &ray);
+                found = certify_ray(verifier, ray, r);
+            }
+            if (!found && backend.farkas_displacement(avg, ray)) found = certify_ray(verifier, ray, r);
+            if (found) break;
+        }
+        const bool restart_due = residual_restarts ? restart_requested
+                                                   : since_restart >= int64_t(o.check_every) * 2 &&
+                                                         (ca.kkt < .5 * restart_kkt ||
+                                                          since_restart >= std::max<int64_t>(2000, r.iterations / 2));
+        if (o.restart && restart_due) {
+            const bool to_average = residual_restarts || avg;
+            if (o.adaptive || anchored(o) || o.primal_weight == "pid") {  // solver.cpp:702-717
+                long double dx = 0, dy = 0;
+                backend.epoch_displacement(to_average, dx, dy);
+                if (dx > 1e-24 && dy > 1e-24) {
+                    if (o.primal_weight == "pid")
+                        weight = controller.update(weight, dx, dy);
+                    else
+                        weight = std::sqrt(weight * std::clamp(std::sqrt(double(dy / dx)), 1e-4, 1e4));
+                    r.weight_updates++;
+                }
+            }
+            backend.reset_candidate(to_average);
+            backend.set_epoch(false);
+            restart_kkt = ca.kkt;
+            since_restart = 0;
+            r.restarts++;
+        }
+    }
+    auto check = Clock::now();
+    if (have_best) backend.original_point(2, &r.x, &r.y);
+    r.accuracy = verifier.evaluate(r.x, r.y);
+    r.verification_seconds += elapsed(check);
 }
 
 Result solve_continuous(const Model &original, const Options &o) {
@@ -193,8 +386,14 @@ Result solve_continuous(const Model &original, const Options &o) {
 
     double restart_kkt = r.accuracy.kkt;
     int64_t since_restart = 0, previous_rejected = 0;
-    std::vector<double> epochx = x, epochy = y;
     WeightPid controller;
+    const bool want_device = o.device_checks > 0 || (o.device_checks < 0 && original.A.nnz() >= kDeviceCheckNonzeros);
+    if (want_device && backend.attach(original, prep)) {
+        run_device_checks(original, prep, o, verifier, backend, r, start, step, weight, restart_kkt, controller);
+        r.seconds = elapsed(start);
+        return r;
+    }
+    std::vector<double> epochx = x, epochy = y;
     std::vector<double> cx(m.cols()), cy(m.rows()), ax(m.cols()), ay(m.rows());
     while (r.iterations < o.iteration_limit) {
         if (stop_requested(o)) {

@@ -62,7 +62,9 @@ struct Options {
     int check_every = 100, scaling_passes = 5, threads = 1;
     bool presolve = true, restart = true, adaptive = true, verbose = false;
     bool graphs = true;      // hipGraph-captured chunks
-    bool device_checks = true;  // original-space KKT evaluated on the GPU between host certifications
+    // Candidate checks: -1 on the device for models with >= 100,000 nonzeros,
+    // 0 on the host every chunk, 1 on the device (the host still certifies).
+    int device_checks = -1;
     std::vector<double> initial_x, initial_y;
     std::shared_ptr<std::atomic<bool>> cancellation;
 };

@@ -77,6 +77,23 @@ struct DevCsr {
     long long rows;
 };
 
+// Rows longer than `threshold` nonzeros are not walked by a lane group: they
+// are cut into segments of at most kSegment nonzeros, one workgroup each,
+// placed at the start of the grid.  Every segment block stores its partial sum;
+// the block that completes a row (the last to take the row's ticket) adds the
+// row's partials in segment order and runs the row's epilogue, so the result
+// does not depend on scheduling.  Power-law rows (tens of thousands of
+// nonzeros) then spread over many CUs instead of serialising one lane group.
+struct Segments {
+    const long long *begin, *end;    // nonzero range of segment s
+    const int *owner;                // long-row slot of segment s
+    const int *row, *first, *count;  // per long-row slot: row, first segment, segments
+    double *partial;                 // per segment
+    unsigned *ticket;                // per long-row slot; zero between launches
+    int nseg;
+    long long threshold;
+};
+
 struct PrimalArgs {
     const double *c, *lb, *ub;
     double *x[2], *xa[2];
@@ -134,8 +151,82 @@ __device__ __forceinline__ double load_shared(const double *p) {
     return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Segment `blockIdx.x` of a long row: partial sum (and, with ABS, the sum of
+// absolute products), ticket, and in thread 0 of the block that completes the
+// row, the row index and its totals.
+template <bool ABS, class O, class V>
+__device__ bool segment_row(const DevCsr<O, V> &A, const Segments &sg, const double *v, double *lds, long long &row,
+                            double &total, double &total_abs) {
+    const int s = blockIdx.x;
+    const long long end = sg.end[s];
+    double acc = 0, acca = 0;
+    for (long long k = sg.begin[s] + threadIdx.x; k < end; k += kBlock) {
+        const double p = double(A.val[k]) * v[A.idx[k]];
+        acc += p;
+        if (ABS) acca += fabs(p);
+    }
+    const double part = block_sum(acc, lds);
+    const double parta = ABS ? block_sum(acca, lds) : 0.0;
+    bool done = false;
+    if (threadIdx.x == 0) {
+        store_shared(&sg.partial[2 * s], part);
+        store_shared(&sg.partial[2 * s + 1], parta);
+        const int L = sg.owner[s];
+        const unsigned t = __hip_atomic_fetch_add(&sg.ticket[L], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (t + 1 == unsigned(sg.count[L])) {
+            double sum = 0, suma = 0;
+            for (int q = 0; q < sg.count[L]; ++q) {
+                sum += load_shared(&sg.partial[2 * (sg.first[L] + q)]);
+                suma += load_shared(&sg.partial[2 * (sg.first[L] + q) + 1]);
+            }
+            __hip_atomic_store(&sg.ticket[L], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            row = sg.row[L];
+            total = sum;
+            total_abs = suma;
+            done = true;
+        }
+    }
+    return done;
+}
+
+// Calls row(i, sum_k A[i,k] v[k], sum_k |A[i,k] v[k]|) for every row (the
+// second sum only with ABS, else 0): long rows through their segments, short
+// rows by lane groups of G over the remaining blocks.
+template <int G, bool ABS = false, class O, class V, class Row>
+__device__ __forceinline__ void for_rows(const DevCsr<O, V> &A, const Segments &sg, const double *v, double *lds,
+                                         Row &&row) {
+    if ((int)blockIdx.x < sg.nseg) {
+        long long i = 0;
+        double total = 0, total_abs = 0;
+        if (segment_row<ABS>(A, sg, v, lds, i, total, total_abs)) row(i, total, total_abs);
+        return;
+    }
+    constexpr int rows_per_block = kBlock / G;
+    const int lane = threadIdx.x % G;
+    const long long nb = (long long)gridDim.x - sg.nseg;
+    for (long long base = ((long long)blockIdx.x - sg.nseg) * rows_per_block; base < A.rows;
+         base += nb * rows_per_block) {
+        const long long i = base + threadIdx.x / G;
+        bool valid = i < A.rows;
+        double acc = 0, acca = 0;
+        if (valid) {
+            const O begin = A.ptr[i], end = A.ptr[i + 1];
+            valid = (long long)(end - begin) <= sg.threshold;
+            if (valid)
+                for (O k = begin + lane; k < end; k += G) {
+                    const double p = double(A.val[k]) * v[A.idx[k]];
+                    acc += p;
+                    if (ABS) acca += fabs(p);
+                }
+        }
+        acc = group_sum<G>(acc);
+        if (ABS) acca = group_sum<G>(acca);
+        if (valid && lane == 0) row(i, acc, acca);
+    }
+}
+
 template <int G, class O, class V>
-__global__ __launch_bounds__(kBlock) void primal_kernel(DevCsr<O, V> at, PrimalArgs a) {
+__global__ __launch_bounds__(kBlock) void primal_kernel(DevCsr<O, V> at, Segments sg, PrimalArgs a) {
     const State *st = a.st;
     if (st->stop != kRun) return;
     const int cur = st->cur;
@@ -149,37 +240,24 @@ __global__ __launch_bounds__(kBlock) void primal_kernel(DevCsr<O, V> at, PrimalA
     const double w = f / (st->average_mass + f);
     const double lambda = 1 / (double(st->epoch_steps) + 2);
     const double rho = a.f.reflection;
-    constexpr int rows_per_block = kBlock / G;
-    const int lane = threadIdx.x % G;
     double local = 0;
-    for (long long base = (long long)blockIdx.x * rows_per_block; base < at.rows;
-         base += (long long)gridDim.x * rows_per_block) {
-        const long long j = base + threadIdx.x / G;
-        const bool valid = j < at.rows;
-        double acc = 0;
-        if (valid) {
-            const O end = at.ptr[j + 1];
-            for (O k = at.ptr[j] + lane; k < end; k += G) acc += double(at.val[k]) * y[at.idx[k]];
-        }
-        acc = group_sum<G>(acc);
-        if (valid && lane == 0) {
-            const double g = a.c[j] + acc;
-            const double xj = x[j];
-            const double t = clampd(xj - tau * g, a.lb[j], a.ub[j]);
-            const double d = t - xj;
-            local += d * d;
-            a.dx[j] = d;
-            if (a.f.halpern) {
-                xan[j] = t;
-                xn[j] = lambda * a.anchor_x[j] + (1 - lambda) * ((1 + rho) * t - rho * xj);
-            } else {
-                xn[j] = t;
-                const double v = xa[j];
-                xan[j] = v + (t - v) * w;
-            }
-        }
-    }
     __shared__ double lds[4];
+    for_rows<G>(at, sg, y, lds, [&](long long j, double acc, double) {
+        const double g = a.c[j] + acc;
+        const double xj = x[j];
+        const double t = clampd(xj - tau * g, a.lb[j], a.ub[j]);
+        const double d = t - xj;
+        local += d * d;
+        a.dx[j] = d;
+        if (a.f.halpern) {
+            xan[j] = t;
+            xn[j] = lambda * a.anchor_x[j] + (1 - lambda) * ((1 + rho) * t - rho * xj);
+        } else {
+            xn[j] = t;
+            const double v = xa[j];
+            xan[j] = v + (t - v) * w;
+        }
+    });
     const double s = block_sum(local, lds);
     if (threadIdx.x == 0) store_shared(&a.partial[blockIdx.x], s);
 }
@@ -233,7 +311,7 @@ __device__ void decide(State *st, const Flags &fl, double dx2, double dy2, doubl
 }
 
 template <int G, class O, class V>
-__global__ __launch_bounds__(kBlock) void dual_kernel(DevCsr<O, V> A, DualArgs a) {
+__global__ __launch_bounds__(kBlock) void dual_kernel(DevCsr<O, V> A, Segments sg, DualArgs a) {
     State *st = a.st;
     if (st->stop != kRun) return;
     const int cur = st->cur;
@@ -248,44 +326,30 @@ __global__ __launch_bounds__(kBlock) void dual_kernel(DevCsr<O, V> A, DualArgs a
     const double w = f / (st->average_mass + f);
     const double lambda = 1 / (double(st->epoch_steps) + 2);
     const double rho = a.f.reflection;
-    constexpr int rows_per_block = kBlock / G;
-    const int lane = threadIdx.x % G;
     double sq = 0, cross = 0;
-    for (long long base = (long long)blockIdx.x * rows_per_block; base < A.rows;
-         base += (long long)gridDim.x * rows_per_block) {
-        const long long i = base + threadIdx.x / G;
-        const bool valid = i < A.rows;
-        double acc = 0;
-        if (valid) {
-            const O end = A.ptr[i + 1];
-            for (O k = A.ptr[i] + lane; k < end; k += G) acc += double(A.val[k]) * a.dx[A.idx[k]];
-        }
-        acc = group_sum<G>(acc);
-        if (valid && lane == 0) {
-            const double delta = acc;
-            const double axi = ax[i];
-            const double bar = axi + 2 * delta;
-            const double yi = y[i];
-            const double z = yi + sigma * bar;
-            const double t = maxd(0., z - sigma * a.ru[i]) + mind(0., z - sigma * a.rl[i]);
-            const double dy = t - yi;
-            sq += (!isfinite(bar) || !isfinite(t)) ? INFINITY : dy * dy;
-            cross += dy * delta;
-            const double next_ax = axi + delta;
-            if (a.f.halpern) {
-                yan[i] = t;
-                yn[i] = lambda * a.anchor_y[i] + (1 - lambda) * ((1 + rho) * t - rho * yi);
-                axn[i] = lambda * a.anchor_ax[i] + (1 - lambda) * ((1 + rho) * next_ax - rho * axi);
-            } else {
-                yn[i] = t;
-                const double v = ya[i];
-                yan[i] = v + (t - v) * w;
-                axn[i] = next_ax;
-            }
-        }
-    }
     __shared__ double lds[4];
     __shared__ int last;
+    for_rows<G>(A, sg, a.dx, lds, [&](long long i, double delta, double) {
+        const double axi = ax[i];
+        const double bar = axi + 2 * delta;
+        const double yi = y[i];
+        const double z = yi + sigma * bar;
+        const double t = maxd(0., z - sigma * a.ru[i]) + mind(0., z - sigma * a.rl[i]);
+        const double dy = t - yi;
+        sq += (!isfinite(bar) || !isfinite(t)) ? INFINITY : dy * dy;
+        cross += dy * delta;
+        const double next_ax = axi + delta;
+        if (a.f.halpern) {
+            yan[i] = t;
+            yn[i] = lambda * a.anchor_y[i] + (1 - lambda) * ((1 + rho) * t - rho * yi);
+            axn[i] = lambda * a.anchor_ax[i] + (1 - lambda) * ((1 + rho) * next_ax - rho * axi);
+        } else {
+            yn[i] = t;
+            const double v = ya[i];
+            yan[i] = v + (t - v) * w;
+            axn[i] = next_ax;
+        }
+    });
     const double s1 = block_sum(sq, lds);
     const double s2 = block_sum(cross, lds);
     if (threadIdx.x == 0) {
@@ -312,21 +376,20 @@ __global__ __launch_bounds__(kBlock) void dual_kernel(DevCsr<O, V> A, DualArgs a
 }
 
 template <int G, class O, class V>
-__global__ __launch_bounds__(kBlock) void spmv_kernel(DevCsr<O, V> A, const double *x, double *out) {
-    constexpr int rows_per_block = kBlock / G;
-    const int lane = threadIdx.x % G;
-    for (long long base = (long long)blockIdx.x * rows_per_block; base < A.rows;
-         base += (long long)gridDim.x * rows_per_block) {
-        const long long i = base + threadIdx.x / G;
-        const bool valid = i < A.rows;
-        double acc = 0;
-        if (valid) {
-            const O end = A.ptr[i + 1];
-            for (O k = A.ptr[i] + lane; k < end; k += G) acc += double(A.val[k]) * x[A.idx[k]];
-        }
-        acc = group_sum<G>(acc);
-        if (valid && lane == 0) out[i] = acc;
-    }
+__global__ __launch_bounds__(kBlock) void spmv_kernel(DevCsr<O, V> A, Segments sg, const double *x, double *out) {
+    __shared__ double lds[4];
+    for_rows<G>(A, sg, x, lds, [&](long long i, double acc, double) { out[i] = acc; });
+}
+
+// out = A x and out_abs = |A| |x| (the rounding scale of each entry of out).
+template <int G, class O>
+__global__ __launch_bounds__(kBlock) void spmv_abs_kernel(DevCsr<O, double> A, Segments sg, const double *x,
+                                                          double *out, double *out_abs) {
+    __shared__ double lds[4];
+    for_rows<G, true>(A, sg, x, lds, [&](long long i, double acc, double acca) {
+        out[i] = acc;
+        out_abs[i] = acca;
+    });
 }
 
 __global__ void begin_chunk_kernel(State *st, long long target, double tau0, double sigma0) {
@@ -360,6 +423,299 @@ __global__ void copy_kernel(double *dst, const double *src, long long n) {
         dst[i] = src[i];
 }
 
+// --------------------------------------------------------- device checks --
+//
+// Original-space evaluation of a candidate with the formulas of the
+// reference's verifier (src/verify.cpp:103-240, without the outward-rounded
+// bound, which stays on the host), the Farkas pre-filter of the candidate's
+// dual as a ray, and the epoch displacement.  Long sums are double-double,
+// every reduction has a fixed order.
+
+struct DD {
+    double hi, lo;
+};
+__device__ __forceinline__ void dd_add(DD &a, double b) {
+    const double s = a.hi + b;
+    const double bb = s - a.hi;
+    const double e = (a.hi - (s - bb)) + (b - bb);
+    a.hi = s;
+    a.lo += e;
+}
+__device__ __forceinline__ DD dd_merge(DD a, DD b) {
+    const double s = a.hi + b.hi;
+    const double bb = s - a.hi;
+    double e = (a.hi - (s - bb)) + (b.hi - bb);
+    e += a.lo + b.lo;
+    const double h = s + e;
+    return DD{h, e - (h - s)};
+}
+__device__ __forceinline__ double dd_value(DD a) { return a.hi + a.lo; }
+
+constexpr double kEps = 2.220446049250313e-16;
+
+// Per-thread sums, maxima and double-double sums.
+template <int NS, int NM, int ND>
+struct Terms {
+    static constexpr int kWords = NS + NM + 2 * ND;
+    double s[NS > 0 ? NS : 1];
+    double mx[NM > 0 ? NM : 1];
+    DD d[ND > 0 ? ND : 1];
+
+    __device__ void merge(const Terms &o) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) s[k] += o.s[k];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) mx[k] = maxd(mx[k], o.mx[k]);
+#pragma unroll
+        for (int k = 0; k < ND; ++k) d[k] = dd_merge(d[k], o.d[k]);
+    }
+    __device__ void store(double *p) const {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) store_shared(p + k, s[k]);
+#pragma unroll
+        for (int k = 0; k < NM; ++k) store_shared(p + NS + k, mx[k]);
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            store_shared(p + NS + NM + 2 * k, d[k].hi);
+            store_shared(p + NS + NM + 2 * k + 1, d[k].lo);
+        }
+    }
+    __device__ void load(const double *p) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) s[k] = load_shared(p + k);
+#pragma unroll
+        for (int k = 0; k < NM; ++k) mx[k] = load_shared(p + NS + k);
+#pragma unroll
+        for (int k = 0; k < ND; ++k) d[k] = DD{load_shared(p + NS + NM + 2 * k), load_shared(p + NS + NM + 2 * k + 1)};
+    }
+};
+
+// Reduces over the 256-thread block; the result is valid in thread 0.
+template <int NS, int NM, int ND>
+__device__ void reduce_block(Terms<NS, NM, ND> &t, double *lds) {
+    constexpr int W = Terms<NS, NM, ND>::kWords;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) t.s[k] += __shfl_xor(t.s[k], off, 64);
+#pragma unroll
+        for (int k = 0; k < NM; ++k) t.mx[k] = maxd(t.mx[k], __shfl_xor(t.mx[k], off, 64));
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            const DD o{__shfl_xor(t.d[k].hi, off, 64), __shfl_xor(t.d[k].lo, off, 64)};
+            t.d[k] = dd_merge(t.d[k], o);
+        }
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        double *p = lds + wave * W;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) p[k] = t.s[k];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) p[NS + k] = t.mx[k];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            p[NS + NM + 2 * k] = t.d[k].hi;
+            p[NS + NM + 2 * k + 1] = t.d[k].lo;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) {
+            const double *p = lds + w * W;
+            Terms<NS, NM, ND> o;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) o.s[k] = p[k];
+#pragma unroll
+            for (int k = 0; k < NM; ++k) o.mx[k] = p[NS + k];
+#pragma unroll
+            for (int k = 0; k < ND; ++k) o.d[k] = DD{p[NS + NM + 2 * k], p[NS + NM + 2 * k + 1]};
+            t.merge(o);
+        }
+    }
+    __syncthreads();
+}
+
+// Grid-wide fixed-order reduction: every block stores its terms; the last
+// block to finish merges them in block order.  Returns true in thread 0 of
+// that block, with the grid total in t.
+template <int NS, int NM, int ND>
+__device__ bool reduce_grid(Terms<NS, NM, ND> &t, double *partial, unsigned *ticket, double *lds) {
+    constexpr int W = Terms<NS, NM, ND>::kWords;
+    __shared__ int last;
+    reduce_block(t, lds);
+    if (threadIdx.x == 0) {
+        t.store(partial + (size_t)blockIdx.x * W);
+        const unsigned k = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = k == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return false;
+    Terms<NS, NM, ND> acc{};
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
+        Terms<NS, NM, ND> o;
+        o.load(partial + (size_t)b * W);
+        acc.merge(o);
+    }
+    reduce_block(acc, lds);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t = acc;
+    }
+    return threadIdx.x == 0;
+}
+
+// restore_x (src/preprocess.cpp:121-126) through the inverse column map.
+__global__ void restore_x_kernel(const double *x, const int *colinv, const double *dc, const double *fixed,
+                                 double *ox, long long n0) {
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n0; j += (long long)gridDim.x * blockDim.x) {
+        const int k = colinv[j];
+        ox[j] = k >= 0 ? x[k] * dc[k] : fixed[j];
+    }
+}
+
+// restore_y (src/preprocess.cpp:127-132): each original row receives at most
+// one kept row as its upper and one as its lower representative.
+__global__ void restore_y_kernel(const double *y, const int *upinv, const int *loinv, const double *dr, double sobj,
+                                 double *oy, long long m0) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m0; i += (long long)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        const int u = upinv[i], l = loinv[i];
+        if (u >= 0 && y[u] >= 0) v += y[u] * dr[u] * sobj;
+        if (l >= 0 && !(y[l] >= 0)) v += y[l] * dr[l] * sobj;
+        oy[i] = v;
+    }
+}
+
+__global__ void sub_kernel(const double *a, const double *b, double *out, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = a[i] - b[i];
+}
+
+// Upper bound of min over [l, u] of g * x (concave in g).
+__device__ __forceinline__ double phi(double g, double l, double u) {
+    if (g > 0) return isfinite(l) ? g * l : -INFINITY;
+    if (g < 0) return isfinite(u) ? g * u : -INFINITY;
+    return 0;
+}
+
+struct EvalArgs {
+    long long m0, n0;
+    const double *ox, *oy, *act, *aty, *atyabs;
+    const double *c0, *lb0, *ub0, *rl0, *ru0;
+    const int *collen;
+    double offset0, cscale;
+    double *partial;
+    unsigned *ticket;
+    npdhg_accuracy *out;
+};
+
+// s : objective, Farkas column bound sum, its |.| sum, Lipschitz slack, sum |y b|
+// mx: nonfinite, Farkas column -inf, primal, primal_abs, dual, dual_abs,
+//     y != 0 at an infinite row endpoint, infinite Lagrangian box argument
+// d : support, complement, box_min, sum y b over rows
+using EvalTerms = Terms<5, 8, 4>;
+
+__global__ __launch_bounds__(kBlock) void eval_kernel(EvalArgs a) {
+    EvalTerms r{};
+    const long long stride = (long long)gridDim.x * kBlock;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < a.m0; i += stride) {
+        const double ax = a.act[i], yi = a.oy[i], lo = a.rl0[i], hi = a.ru0[i];
+        if (!isfinite(ax) || !isfinite(yi)) r.mx[0] = 1;
+        const double v = maxd(maxd(0., lo - ax), ax - hi);
+        if (isfinite(lo)) r.mx[2] = maxd(r.mx[2], maxd(0., lo - ax) / (1 + fabs(lo)));
+        if (isfinite(hi)) r.mx[2] = maxd(r.mx[2], maxd(0., ax - hi) / (1 + fabs(hi)));
+        r.mx[3] = maxd(r.mx[3], v);
+        if (yi != 0) {
+            const double endpoint = yi > 0 ? hi : lo;
+            if (isfinite(endpoint)) {
+                dd_add(r.d[0], yi * endpoint);
+                dd_add(r.d[1], fabs(yi * (endpoint - ax)));
+                dd_add(r.d[3], yi * endpoint);
+                r.s[4] += fabs(yi * endpoint);
+            } else {
+                r.mx[6] = 1;
+                r.mx[4] = maxd(r.mx[4], fabs(yi));
+                r.mx[5] = maxd(r.mx[5], fabs(yi));
+            }
+        }
+    }
+    for (long long j = (long long)blockIdx.x * kBlock + threadIdx.x; j < a.n0; j += stride) {
+        const double xj = a.ox[j], l = a.lb0[j], u = a.ub0[j], c = a.c0[j];
+        if (!isfinite(xj)) r.mx[0] = 1;
+        const double v = maxd(maxd(0., l - xj), xj - u);
+        r.mx[2] = maxd(r.mx[2], v / (1 + fabs(xj)));
+        r.mx[3] = maxd(r.mx[3], v);
+        r.s[0] += c * xj;
+        const double linear = c + a.aty[j];
+        if (!isfinite(linear)) r.mx[0] = 1;
+        const double d = clampd(linear, xj - u, xj - l);
+        r.mx[5] = maxd(r.mx[5], fabs(d));
+        r.mx[4] = maxd(r.mx[4], fabs(d) / a.cscale);
+        if (linear > 0 && isfinite(l)) dd_add(r.d[1], fabs(linear * (xj - l)));
+        if (linear < 0 && isfinite(u)) dd_add(r.d[1], fabs(linear * (u - xj)));
+        const double arg = linear > 0 ? l : linear < 0 ? u : clampd(0., l, u);
+        if (!isfinite(arg)) r.mx[7] = 1;
+        else dd_add(r.d[2], linear * arg);
+        // Farkas column bound of the dual as a ray, widened by the rounding of A^T y.
+        const double g = a.aty[j], ga = a.atyabs[j];
+        const double e = double(a.collen[j] + 4) * 2 * kEps * ga;
+        double ubd = maxd(phi(g - e, l, u), phi(g + e, l, u));
+        if (g - e <= 0 && g + e >= 0) ubd = maxd(ubd, 0.0);
+        if (ubd == -INFINITY) r.mx[1] = 1;
+        else {
+            r.s[1] += ubd;
+            r.s[2] += fabs(ubd);
+            const double box = maxd(isfinite(l) ? fabs(l) : 0.0, isfinite(u) ? fabs(u) : 0.0);
+            r.s[3] += ga * box;
+        }
+    }
+    __shared__ double lds[(kBlock / 64) * EvalTerms::kWords];
+    if (!reduce_grid(r, a.partial, a.ticket, lds)) return;
+    npdhg_accuracy out{};
+    const double support = dd_value(r.d[0]), complement = dd_value(r.d[1]), box_min = dd_value(r.d[2]);
+    out.objective = a.offset0 + r.s[0];
+    out.primal = r.mx[2];
+    out.dual = r.mx[4];
+    out.gap = complement / (1 + fabs(out.objective - a.offset0));
+    if (r.mx[6] == 0 && r.mx[7] == 0) {
+        const double lower = (a.offset0 - support) + box_min;
+        const double guard = 64 * kEps * (1 + fabs(a.offset0) + fabs(support) + fabs(box_min));
+        const double lb = lower - guard;
+        out.gap = maxd(out.gap, fabs(out.objective - lb) / (1 + fabs(out.objective) + fabs(lb)));
+    }
+    out.kkt = maxd(maxd(out.primal, out.dual), out.gap);
+    out.finite = r.mx[0] == 0 && isfinite(out.objective) && isfinite(out.kkt);
+    if (!out.finite) out.kkt = INFINITY;
+    const bool neg_inf = r.mx[1] != 0 || r.mx[6] != 0;
+    const double value = r.s[1] - dd_value(r.d[3]);
+    const double slack = double(a.n0 + a.m0 + 16) * 4 * kEps * (r.s[2] + r.s[4] + r.s[3]);
+    out.farkas_possible = !(neg_inf || value + slack <= 0);
+    *a.out = out;
+}
+
+// Squared displacement |x - ex|^2 and |y - ey|^2, products split exactly.
+__global__ __launch_bounds__(kBlock) void displacement_kernel(const double *x, const double *ex, long long n,
+                                                              const double *y, const double *ey, long long m,
+                                                              double *partial, unsigned *ticket, double *out) {
+    Terms<0, 0, 2> r{};
+    const long long stride = (long long)gridDim.x * kBlock;
+    for (long long j = (long long)blockIdx.x * kBlock + threadIdx.x; j < n; j += stride) {
+        const double d = x[j] - ex[j], p = d * d;
+        dd_add(r.d[0], p);
+        r.d[0].lo += fma(d, d, -p);
+    }
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < m; i += stride) {
+        const double d = y[i] - ey[i], p = d * d;
+        dd_add(r.d[1], p);
+        r.d[1].lo += fma(d, d, -p);
+    }
+    __shared__ double lds[(kBlock / 64) * Terms<0, 0, 2>::kWords];
+    if (!reduce_grid(r, partial, ticket, lds)) return;
+    out[0] = dd_value(r.d[0]);
+    out[1] = dd_value(r.d[1]);
+}
+
 // ------------------------------------------------------------------ host --
 
 int pick_group(long long rows, long long nnz) {
@@ -373,6 +729,36 @@ int grid_for(long long rows, int group) {
     long long rpb = kBlock / group;
     long long blocks = (rows + rpb - 1) / rpb;
     return int(std::max<long long>(1, std::min<long long>(blocks, kMaxBlocks)));
+}
+
+constexpr long long kSegment = 2048;  // nonzeros per long-row segment (8 per thread)
+
+// Longest row a lane group of `group` lanes walks itself (32 rounds).
+long long row_threshold(int group) { return std::max<long long>(32LL * group, 128); }
+
+struct HostSegments {
+    std::vector<long long> begin, end;
+    std::vector<int> owner, row, first, count;
+};
+
+HostSegments make_segments(const int64_t *ptr, long long rows, long long threshold) {
+    HostSegments h;
+    for (long long i = 0; i < rows; ++i) {
+        const long long len = ptr[i + 1] - ptr[i];
+        if (len <= threshold) continue;
+        const int slot = int(h.row.size());
+        const long long pieces = (len + kSegment - 1) / kSegment;
+        h.row.push_back(int(i));
+        h.first.push_back(int(h.begin.size()));
+        h.count.push_back(int(pieces));
+        for (long long q = 0; q < pieces; ++q) {
+            h.begin.push_back(ptr[i] + q * kSegment);
+            h.end.push_back(std::min<long long>(ptr[i] + (q + 1) * kSegment, ptr[i + 1]));
+            h.owner.push_back(slot);
+        }
+        if (h.begin.size() > size_t(1) << 30) throw std::runtime_error("Too many long-row segments");
+    }
+    return h;
 }
 
 int elementwise_grid(long long n) { return int(std::max<long long>(1, std::min<long long>((n + 255) / 256, 4096))); }
@@ -432,8 +818,10 @@ class Backend {
             upload_matrix(t.rows, t.cols, t.ptr.data(), t.index.data(), t.value.data(), at_);
             g_a_ = pick_group(m_, nnz_);
             g_at_ = pick_group(n_, nnz_);
-            nb1_ = grid_for(n_, g_at_);
-            nb2_ = grid_for(m_, g_a_);
+            sg_a_ = upload_segments(make_segments(lp.ptr, m_, row_threshold(g_a_)), row_threshold(g_a_));
+            sg_at_ = upload_segments(make_segments(t.ptr.data(), n_, row_threshold(g_at_)), row_threshold(g_at_));
+            nb1_ = sg_at_.nseg + grid_for(n_, g_at_);
+            nb2_ = sg_a_.nseg + grid_for(m_, g_a_);
             c_ = upload(lp.c, n_);
             lb_ = upload(lp.lb, n_);
             ub_ = upload(lp.ub, n_);
@@ -557,6 +945,162 @@ class Backend {
 
     int index_bits() const { return idx64_ ? 64 : 32; }
 
+    void attach_original(const npdhg_original &o) {
+        if (attached_) throw std::runtime_error("Original model already attached");
+        if (o.rows < 0 || o.cols < 0 || o.rows >= (1LL << 31) || o.cols >= (1LL << 31))
+            throw std::runtime_error("Original model dimensions out of range");
+        n0_ = o.cols;
+        m0_ = o.rows;
+        std::vector<int> colinv(n0_, -1), upinv(m0_, -1), loinv(m0_, -1);
+        for (long long j = 0; j < n_; ++j) {
+            const int64_t t = o.cols_map[j];
+            if (t < 0 || t >= n0_ || colinv[t] >= 0) throw std::runtime_error("Column map is not one-to-one");
+            colinv[t] = int(j);
+        }
+        for (long long i = 0; i < m_; ++i) {
+            const int64_t u = o.upper_rows[i], l = o.lower_rows[i];
+            if (u < 0 || u >= m0_ || l < 0 || l >= m0_ || upinv[u] >= 0 || loinv[l] >= 0)
+                throw std::runtime_error("Row maps are not one-to-one");
+            upinv[u] = int(i);
+            loinv[l] = int(i);
+        }
+        const long long nnz0 = o.ptr[o.rows];
+        wide0_ = nnz0 >= (1LL << 31);
+        npdhg_lp view{};
+        view.rows = o.rows;
+        view.cols = o.cols;
+        view.ptr = o.ptr;
+        view.index = o.index;
+        view.value = o.value;
+        upload_matrix(o.rows, o.cols, o.ptr, o.index, o.value, a0_, wide0_, false);
+        HostCsr t = transpose(view);
+        upload_matrix(t.rows, t.cols, t.ptr.data(), t.index.data(), t.value.data(), at0_, wide0_, false);
+        g_a0_ = pick_group(m0_, nnz0);
+        g_at0_ = pick_group(n0_, nnz0);
+        sg_a0_ = upload_segments(make_segments(o.ptr, m0_, row_threshold(g_a0_)), row_threshold(g_a0_));
+        sg_at0_ = upload_segments(make_segments(t.ptr.data(), n0_, row_threshold(g_at0_)), row_threshold(g_at0_));
+        std::vector<int> collen(n0_);
+        for (long long j = 0; j < n0_; ++j) collen[j] = int(t.ptr[j + 1] - t.ptr[j]);
+        collen_ = upload_vector(collen);
+        colinv_ = upload_vector(colinv);
+        upinv_ = upload_vector(upinv);
+        loinv_ = upload_vector(loinv);
+        c0_ = upload(o.c, n0_);
+        lb0_ = upload(o.lb, n0_);
+        ub0_ = upload(o.ub, n0_);
+        rl0_ = upload(o.rl, m0_);
+        ru0_ = upload(o.ru, m0_);
+        fixed_ = upload(o.fixed, n0_);
+        dc_ = upload(o.column_scale, n_);
+        dr_ = upload(o.row_scale, m_);
+        sobj_ = o.objective_scale;
+        offset0_ = o.offset;
+        cscale_ = 1;
+        for (long long j = 0; j < n0_; ++j) cscale_ = std::max(cscale_, 1 + std::abs(o.c[j]));
+        for (int k = 0; k < 2; ++k) {
+            ox_[k] = alloc_d(n0_);
+            oy_[k] = alloc_d(m0_);
+        }
+        bx_ = alloc_d(n0_);
+        by_ = alloc_d(m0_);
+        act_ = alloc_d(m0_);
+        aty_ = alloc_d(n0_);
+        atyabs_ = alloc_d(n0_);
+        oyd_ = alloc_d(m0_);
+        dtmp_ = alloc_d(m_);
+        ex_ = alloc_d(n_);
+        ey_ = alloc_d(m_);
+        eval_blocks_ = int(std::max<long long>(1, std::min<long long>((std::max(n0_, m0_) + kBlock - 1) / kBlock, 1024)));
+        eval_partial_ = alloc_d((long long)eval_blocks_ * EvalTerms::kWords);
+        eval_ticket_ = static_cast<unsigned *>(alloc_bytes(2 * sizeof(unsigned)));
+        HIP_OK(hipMemsetAsync(eval_ticket_, 0, 2 * sizeof(unsigned), stream_));
+        eval_out_ = static_cast<npdhg_accuracy *>(alloc_bytes(3 * sizeof(npdhg_accuracy)));
+        disp_out_ = alloc_d(2);
+        sync_state();
+        copy(ex_, x_[host_.cur], n_);
+        copy(ey_, y_[host_.cur], m_);
+        HIP_OK(hipStreamSynchronize(stream_));
+        attached_ = true;
+    }
+
+    void evaluate(npdhg_accuracy *current, npdhg_accuracy *average) {
+        need_original();
+        sync_state();
+        const int c = host_.cur;
+        eval_point(x_[c], y_[c], ox_[0], oy_[0], eval_out_);
+        eval_point(xa_[c], ya_[c], ox_[1], oy_[1], eval_out_ + 1);
+        spmv(false, x_[c], ax_[c]);  // rebase the activity recurrence, as candidates() does
+        npdhg_accuracy out[2];
+        HIP_OK(hipMemcpyAsync(out, eval_out_, sizeof out, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        if (current) *current = out[0];
+        if (average) *average = out[1];
+        evaluated_ = true;
+    }
+
+    void keep_best(bool average) {
+        need_evaluated();
+        copy(bx_, ox_[average ? 1 : 0], n0_);
+        copy(by_, oy_[average ? 1 : 0], m0_);
+        HIP_OK(hipStreamSynchronize(stream_));
+        best_ = true;
+    }
+
+    void original_point(int which, double *x0, double *y0) {
+        need_evaluated();
+        if (which < 0 || which > 2) throw std::runtime_error("Point must be 0 (current), 1 (average) or 2 (best)");
+        if (which == 2 && !best_) throw std::runtime_error("No best point kept yet");
+        const double *px = which == 2 ? bx_ : ox_[which], *py = which == 2 ? by_ : oy_[which];
+        if (x0) download(x0, px, n0_);
+        if (y0) download(y0, py, m0_);
+    }
+
+    void farkas_displacement(bool average, int *possible, double *ray) {
+        need_evaluated();
+        sync_state();
+        const int c = host_.cur;
+        const double *y = average ? ya_[c] : y_[c];
+        if (m_) {
+            sub_kernel<<<elementwise_grid(m_), 256, 0, stream_>>>(y, ey_, dtmp_, m_);
+            HIP_OK(hipGetLastError());
+        }
+        if (m0_) {
+            restore_y_kernel<<<elementwise_grid(m0_), 256, 0, stream_>>>(dtmp_, upinv_, loinv_, dr_, sobj_, oyd_, m0_);
+            HIP_OK(hipGetLastError());
+        }
+        spmv0(true, oyd_, aty_, atyabs_);
+        launch_eval(ox_[0], oyd_, eval_out_ + 2);
+        npdhg_accuracy out{};
+        HIP_OK(hipMemcpyAsync(&out, eval_out_ + 2, sizeof out, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        *possible = out.farkas_possible;
+        if (out.farkas_possible && ray) download(ray, oyd_, m0_);
+    }
+
+    void epoch_displacement(bool average, double *dx2, double *dy2) {
+        need_original();
+        sync_state();
+        const int c = host_.cur;
+        displacement_kernel<<<eval_blocks_, kBlock, 0, stream_>>>(average ? xa_[c] : x_[c], ex_, n_,
+                                                                  average ? ya_[c] : y_[c], ey_, m_, eval_partial_,
+                                                                  eval_ticket_ + 1, disp_out_);
+        HIP_OK(hipGetLastError());
+        double out[2];
+        HIP_OK(hipMemcpyAsync(out, disp_out_, sizeof out, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        *dx2 = out[0];
+        *dy2 = out[1];
+    }
+
+    void set_epoch(bool average) {
+        need_original();
+        sync_state();
+        const int c = host_.cur;
+        copy(ex_, average ? xa_[c] : x_[c], n_);
+        copy(ey_, average ? ya_[c] : y_[c], m_);
+        HIP_OK(hipStreamSynchronize(stream_));
+    }
+
   private:
     struct DeviceMatrix {
         long long rows = 0, cols = 0, nnz = 0;
@@ -602,6 +1146,10 @@ class Backend {
 
     void upload_matrix(long long rows, long long cols, const int64_t *ptr, const int64_t *index, const double *value,
                        DeviceMatrix &d) {
+        upload_matrix(rows, cols, ptr, index, value, d, idx64_, fp32_);
+    }
+    void upload_matrix(long long rows, long long cols, const int64_t *ptr, const int64_t *index, const double *value,
+                       DeviceMatrix &d, bool wide, bool single) {
         d.rows = rows;
         d.cols = cols;
         d.nnz = ptr[rows];
@@ -610,7 +1158,7 @@ class Backend {
         for (long long i = 0; i < rows; ++i)
             if (ptr[i] > ptr[i + 1] || ptr[i] < 0) throw std::runtime_error("CSR offsets not monotone");
         if (ptr[0] != 0) throw std::runtime_error("CSR offsets must start at 0");
-        if (idx64_) {
+        if (wide) {
             d.ptr = alloc_bytes(size_t(rows + 1) * 8);
             HIP_OK(hipMemcpyAsync(d.ptr, ptr, size_t(rows + 1) * 8, hipMemcpyHostToDevice, stream_));
         } else {
@@ -623,7 +1171,7 @@ class Backend {
             d.idx = static_cast<int *>(alloc_bytes(size_t(d.nnz) * 4));
             if (d.nnz) HIP_OK(hipMemcpy(d.idx, i32.data(), i32.size() * 4, hipMemcpyHostToDevice));
         }
-        if (fp32_) {
+        if (single) {
             std::vector<float> v32(value, value + d.nnz);
             d.val = alloc_bytes(size_t(d.nnz) * 4);
             if (d.nnz) HIP_OK(hipMemcpy(d.val, v32.data(), v32.size() * 4, hipMemcpyHostToDevice));
@@ -631,6 +1179,27 @@ class Backend {
             d.val = alloc_bytes(size_t(d.nnz) * 8);
             if (d.nnz) HIP_OK(hipMemcpyAsync(d.val, value, size_t(d.nnz) * 8, hipMemcpyHostToDevice, stream_));
         }
+    }
+
+    template <class T>
+    T *upload_vector(const std::vector<T> &h) {
+        T *d = static_cast<T *>(alloc_bytes(h.size() * sizeof(T)));
+        if (!h.empty()) HIP_OK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return d;
+    }
+    Segments upload_segments(const HostSegments &h, long long threshold) {
+        Segments sg{};
+        sg.nseg = int(h.begin.size());
+        sg.threshold = threshold;
+        sg.begin = upload_vector(h.begin);
+        sg.end = upload_vector(h.end);
+        sg.owner = upload_vector(h.owner);
+        sg.row = upload_vector(h.row);
+        sg.first = upload_vector(h.first);
+        sg.count = upload_vector(h.count);
+        sg.partial = alloc_d(2 * (long long)h.begin.size());
+        sg.ticket = upload_vector(std::vector<unsigned>(h.row.size(), 0u));
+        return sg;
     }
 
     template <class O, class V>
@@ -683,13 +1252,13 @@ class Backend {
         auto da = dual_args();
         switch (g_at_) {
 #define PRIMAL_CASE(G) \
-    case G: primal_kernel<G, O, V><<<nb1_, kBlock, 0, stream_>>>(at, pa); break;
+    case G: primal_kernel<G, O, V><<<nb1_, kBlock, 0, stream_>>>(at, sg_at_, pa); break;
             PRIMAL_CASE(1) PRIMAL_CASE(2) PRIMAL_CASE(4) PRIMAL_CASE(8) PRIMAL_CASE(16) PRIMAL_CASE(32) PRIMAL_CASE(64)
 #undef PRIMAL_CASE
         }
         switch (g_a_) {
 #define DUAL_CASE(G) \
-    case G: dual_kernel<G, O, V><<<nb2_, kBlock, 0, stream_>>>(a, da); break;
+    case G: dual_kernel<G, O, V><<<nb2_, kBlock, 0, stream_>>>(a, sg_a_, da); break;
             DUAL_CASE(1) DUAL_CASE(2) DUAL_CASE(4) DUAL_CASE(8) DUAL_CASE(16) DUAL_CASE(32) DUAL_CASE(64)
 #undef DUAL_CASE
         }
@@ -708,10 +1277,11 @@ class Backend {
         const DeviceMatrix &d = transpose ? at_ : a_;
         auto mat = view<O, V>(d);
         const int g = transpose ? g_at_ : g_a_;
-        const int grid = grid_for(d.rows, g);
+        const Segments &sg = transpose ? sg_at_ : sg_a_;
+        const int grid = sg.nseg + grid_for(d.rows, g);
         switch (g) {
 #define SPMV_CASE(G) \
-    case G: spmv_kernel<G, O, V><<<grid, kBlock, 0, stream_>>>(mat, x, out); break;
+    case G: spmv_kernel<G, O, V><<<grid, kBlock, 0, stream_>>>(mat, sg, x, out); break;
             SPMV_CASE(1) SPMV_CASE(2) SPMV_CASE(4) SPMV_CASE(8) SPMV_CASE(16) SPMV_CASE(32) SPMV_CASE(64)
 #undef SPMV_CASE
         }
@@ -746,6 +1316,85 @@ class Backend {
         HIP_OK(e);
         graphs_cache_[attempts] = exec;
         return exec;
+    }
+
+    void need_original() const {
+        if (!attached_) throw std::runtime_error("Device checks need npdhg_attach_original first");
+    }
+    void need_evaluated() const {
+        need_original();
+        if (!evaluated_) throw std::runtime_error("No candidate evaluated yet");
+    }
+
+    template <class O>
+    void spmv0_t(bool transpose, const double *x, double *out, double *out_abs) {
+        const DeviceMatrix &d = transpose ? at0_ : a0_;
+        auto mat = view<O, double>(d);
+        const int g = transpose ? g_at0_ : g_a0_;
+        const Segments &sg = transpose ? sg_at0_ : sg_a0_;
+        const int grid = sg.nseg + grid_for(d.rows, g);
+        if (out_abs) {
+            switch (g) {
+#define SPMV_ABS_CASE(G) \
+    case G: spmv_abs_kernel<G, O><<<grid, kBlock, 0, stream_>>>(mat, sg, x, out, out_abs); break;
+                SPMV_ABS_CASE(1) SPMV_ABS_CASE(2) SPMV_ABS_CASE(4) SPMV_ABS_CASE(8) SPMV_ABS_CASE(16)
+                SPMV_ABS_CASE(32) SPMV_ABS_CASE(64)
+#undef SPMV_ABS_CASE
+            }
+        } else {
+            switch (g) {
+#define SPMV0_CASE(G) \
+    case G: spmv_kernel<G, O, double><<<grid, kBlock, 0, stream_>>>(mat, sg, x, out); break;
+                SPMV0_CASE(1) SPMV0_CASE(2) SPMV0_CASE(4) SPMV0_CASE(8) SPMV0_CASE(16) SPMV0_CASE(32) SPMV0_CASE(64)
+#undef SPMV0_CASE
+            }
+        }
+    }
+    // Products with the original model (FP64 values).
+    void spmv0(bool transpose, const double *x, double *out, double *out_abs) {
+        if ((transpose ? n0_ : m0_) == 0) return;
+        if (wide0_) spmv0_t<int64_t>(transpose, x, out, out_abs);
+        else spmv0_t<int32_t>(transpose, x, out, out_abs);
+        HIP_OK(hipGetLastError());
+    }
+
+    void launch_eval(const double *ox, const double *oy, npdhg_accuracy *out) {
+        EvalArgs e{};
+        e.m0 = m0_;
+        e.n0 = n0_;
+        e.ox = ox;
+        e.oy = oy;
+        e.act = act_;
+        e.aty = aty_;
+        e.atyabs = atyabs_;
+        e.c0 = c0_;
+        e.lb0 = lb0_;
+        e.ub0 = ub0_;
+        e.rl0 = rl0_;
+        e.ru0 = ru0_;
+        e.collen = collen_;
+        e.offset0 = offset0_;
+        e.cscale = cscale_;
+        e.partial = eval_partial_;
+        e.ticket = eval_ticket_;
+        e.out = out;
+        eval_kernel<<<eval_blocks_, kBlock, 0, stream_>>>(e);
+        HIP_OK(hipGetLastError());
+    }
+
+    // Restores (x, y) into (ox, oy), forms A0 ox and A0^T oy, evaluates.
+    void eval_point(const double *x, const double *y, double *ox, double *oy, npdhg_accuracy *out) {
+        if (n0_) {
+            restore_x_kernel<<<elementwise_grid(n0_), 256, 0, stream_>>>(x, colinv_, dc_, fixed_, ox, n0_);
+            HIP_OK(hipGetLastError());
+        }
+        if (m0_) {
+            restore_y_kernel<<<elementwise_grid(m0_), 256, 0, stream_>>>(y, upinv_, loinv_, dr_, sobj_, oy, m0_);
+            HIP_OK(hipGetLastError());
+        }
+        spmv0(false, ox, act_, nullptr);
+        spmv0(true, oy, aty_, atyabs_);
+        launch_eval(ox, oy, out);
     }
 
     void sync_state() {
@@ -795,6 +1444,22 @@ class Backend {
     int g_a_ = 1, g_at_ = 1, nb1_ = 1, nb2_ = 1;
     Flags flags_{};
     DeviceMatrix a_, at_;
+    Segments sg_a_{}, sg_at_{};
+    // Original model and device-check buffers (npdhg_attach_original).
+    bool attached_ = false, evaluated_ = false, best_ = false, wide0_ = false;
+    long long n0_ = 0, m0_ = 0;
+    DeviceMatrix a0_, at0_;
+    Segments sg_a0_{}, sg_at0_{};
+    int g_a0_ = 1, g_at0_ = 1, eval_blocks_ = 1;
+    int *colinv_ = nullptr, *upinv_ = nullptr, *loinv_ = nullptr, *collen_ = nullptr;
+    double *c0_ = nullptr, *lb0_ = nullptr, *ub0_ = nullptr, *rl0_ = nullptr, *ru0_ = nullptr, *fixed_ = nullptr;
+    double *dc_ = nullptr, *dr_ = nullptr;
+    double sobj_ = 1, offset0_ = 0, cscale_ = 1;
+    double *ox_[2] = {}, *oy_[2] = {}, *bx_ = nullptr, *by_ = nullptr, *act_ = nullptr, *aty_ = nullptr,
+           *atyabs_ = nullptr, *oyd_ = nullptr, *dtmp_ = nullptr, *ex_ = nullptr, *ey_ = nullptr;
+    double *eval_partial_ = nullptr, *disp_out_ = nullptr;
+    unsigned *eval_ticket_ = nullptr;
+    npdhg_accuracy *eval_out_ = nullptr;
     double *c_ = nullptr, *lb_ = nullptr, *ub_ = nullptr, *rl_ = nullptr, *ru_ = nullptr;
     double *x_[2] = {}, *xa_[2] = {}, *y_[2] = {}, *ya_[2] = {}, *ax_[2] = {};
     double *dx_ = nullptr, *anchor_x_ = nullptr, *anchor_y_ = nullptr, *anchor_ax_ = nullptr;
@@ -879,6 +1544,55 @@ int npdhg_reset(npdhg_backend *b, const double *x, const double *y) {
 
 int npdhg_reset_candidate(npdhg_backend *b, int average) {
     return guarded([&] { b->impl.reset_candidate(average != 0); });
+}
+
+int npdhg_attach_original(npdhg_backend *b, const npdhg_original *orig) {
+    return guarded([&] {
+        if (!b || !orig) throw std::runtime_error("Null argument");
+        b->impl.attach_original(*orig);
+    });
+}
+
+int npdhg_evaluate(npdhg_backend *b, npdhg_accuracy *current, npdhg_accuracy *average) {
+    return guarded([&] {
+        if (!b) throw std::runtime_error("Null backend");
+        b->impl.evaluate(current, average);
+    });
+}
+
+int npdhg_keep_best(npdhg_backend *b, int average) {
+    return guarded([&] {
+        if (!b) throw std::runtime_error("Null backend");
+        b->impl.keep_best(average != 0);
+    });
+}
+
+int npdhg_original_point(npdhg_backend *b, int which, double *x0, double *y0) {
+    return guarded([&] {
+        if (!b) throw std::runtime_error("Null backend");
+        b->impl.original_point(which, x0, y0);
+    });
+}
+
+int npdhg_farkas_displacement(npdhg_backend *b, int average, int *possible, double *ray) {
+    return guarded([&] {
+        if (!b || !possible) throw std::runtime_error("Null argument");
+        b->impl.farkas_displacement(average != 0, possible, ray);
+    });
+}
+
+int npdhg_epoch_displacement(npdhg_backend *b, int average, double *dx2, double *dy2) {
+    return guarded([&] {
+        if (!b || !dx2 || !dy2) throw std::runtime_error("Null argument");
+        b->impl.epoch_displacement(average != 0, dx2, dy2);
+    });
+}
+
+int npdhg_set_epoch(npdhg_backend *b, int average) {
+    return guarded([&] {
+        if (!b) throw std::runtime_error("Null backend");
+        b->impl.set_epoch(average != 0);
+    });
 }
 
 int npdhg_spmv(npdhg_backend *b, int transpose, const double *x, double *y) {

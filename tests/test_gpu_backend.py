@@ -124,7 +124,44 @@ def test_spmv_matches_dense(engine):
     b.close()
 
 
-@pytest.mark.parametrize("case", [(0, 4, 0), (2, 1, 0), (2, 1, 1), (4, 200, 0), (1, 400, 0)])
+def dense_to_csr(A):
+    ptr, idx, val = [0], [], []
+    for row in A:
+        nz = np.flatnonzero(row)
+        idx.extend(nz.tolist())
+        val.extend(row[nz].tolist())
+        ptr.append(len(idx))
+    return np.array(ptr), np.array(idx), np.array(val)
+
+
+def test_spmv_long_rows_and_columns(engine):
+    """Rows and columns far longer than a lane group walks (split into several
+    2,048-nonzero segments, one workgroup each) against a dense product, and
+    bitwise repeatable across calls."""
+    rng = np.random.default_rng(11)
+    rows, cols = 2500, 12000
+    A = np.zeros((rows, cols))
+    for i in range(rows):
+        A[i, rng.choice(cols, 6, replace=False)] = rng.standard_normal(6)
+    A[5, :] = rng.standard_normal(cols)                               # 6 segments
+    A[17, rng.choice(cols, 3000, replace=False)] = rng.standard_normal(3000)  # 2 segments
+    A[2499, rng.choice(cols, 300, replace=False)] = 1.0               # 1 segment
+    A[:, 42] = rng.standard_normal(rows)                              # long column
+    A[rng.random(rows) < 0.3, 7] = -2.0
+    ptr, idx, val = dense_to_csr(A)
+    lp = engine.LP(ptr, idx, val, np.zeros(cols), -np.ones(cols), np.ones(cols), -np.ones(rows), np.ones(rows))
+    for bits in (32, 64):
+        b = engine.Backend(lp, np.zeros(cols), np.zeros(rows), index_bits=bits)
+        x = rng.standard_normal(cols)
+        y = rng.standard_normal(rows)
+        ax, aty = b.spmv(x), b.spmv(y, transpose=True)
+        near(ax, A @ x, 1e-12)
+        near(aty, A.T @ y, 1e-12)
+        assert np.array_equal(ax, b.spmv(x)) and np.array_equal(aty, b.spmv(y, transpose=True))
+        b.close()
+
+
+@pytest.mark.parametrize("case", [(0, 4, 0), (2, 1, 0), (2, 1, 1), (4, 200, 0), (1, 400, 0), (1, 20, 0)])
 def test_adaptive_chunks_track_oracle(engine, oracle, case):
     """Three chunks of 10 adaptive steps on a generated (unscaled) LP."""
     config, scale, index = case
