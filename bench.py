@@ -115,6 +115,82 @@ def cpu_baseline(count, budget_s=20.0):
             "cpu": _cpu_model()}
 
 
+def serve_stream(engine, qps, duration, seed=SEED, warm_fraction=0.2, hbm_budget=64 << 30):
+    """Open-loop serving run: Poisson arrivals at `qps` for `duration` seconds
+    of the serving workload (nonzeros log-uniform over [1e3, 5e6], batch
+    generator), `warm_fraction` of them warm-started re-solves of an earlier,
+    finished request with every row bound scaled by 1 + U(-5%, 5%).
+    Instances are generated before the clock starts."""
+    from niryukti_amd.serve import Server
+
+    rng = np.random.default_rng(seed)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / qps)
+        if t >= duration:
+            break
+        arrivals.append(t)
+    warm_flags = rng.random(len(arrivals)) < warm_fraction
+    fresh = int((~warm_flags).sum()) + 1
+    log(f"serve: {len(arrivals)} arrivals, generating up to {fresh} instances")
+    insts = []
+    for i in range(fresh):
+        insts.append(engine.Instance(3, seed, 1, i))
+        if i % 20 == 19:
+            log(f"serve: generated {i + 1}")
+    reqs, kinds, next_fresh = [], [], 0
+    with Server(hbm_budget=hbm_budget, grid_workers=3) as server:
+        t0 = time.perf_counter()
+        last_log = t0
+        for a, warm in zip(arrivals, warm_flags):
+            if time.perf_counter() - last_log > 5:
+                last_log = time.perf_counter()
+                log(f"serve: t={last_log - t0:.1f}s submitted {len(reqs)}, done {sum(r.done() for r in reqs)}")
+            delay = t0 + a - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            done = [k for k, (r, kind) in enumerate(zip(reqs, kinds)) if kind[0] == "fresh" and r.done()
+                    and not isinstance(r.result, BaseException) and r.result["status"] == "OPTIMAL"]
+            if warm and done:
+                k = done[int(rng.integers(len(done)))]
+                lp = insts[kinds[k][1]].lp
+                f = 1 + rng.uniform(-0.05, 0.05, lp.rows)
+                lp2 = engine.LP(lp.ptr, lp.index, lp.value, lp.c, lp.lb, lp.ub, lp.rl * f, lp.ru * f)
+                reqs.append(server.submit(lp2, reqs[k].result["x"], reqs[k].result["y"]))
+                kinds.append(("warm", kinds[k][1]))
+            else:
+                reqs.append(server.submit(insts[next_fresh].lp))
+                kinds.append(("fresh", next_fresh))
+                next_fresh += 1
+        while not all(r.done() for r in reqs):
+            time.sleep(1.0)
+            if time.perf_counter() - last_log > 5:
+                last_log = time.perf_counter()
+                log(f"serve: t={last_log - t0:.1f}s draining, done {sum(r.done() for r in reqs)}/{len(reqs)}")
+        results = [r.wait() for r in reqs]
+        wall = max(r.finish for r in reqs) - t0
+    lat = np.array([r.latency for r in reqs])
+    ok = np.array([res["status"] == "OPTIMAL" and res["kkt_error"] <= 1e-6 for res in results])
+    planted_ok = sum(1 for res, kind in zip(results, kinds) if kind[0] == "fresh" and res["status"] == "OPTIMAL"
+                     and abs(res["objective"] - insts[kind[1]].planted_objective)
+                     <= 1e-6 * (1 + abs(insts[kind[1]].planted_objective)))
+    statuses, routes = {}, {}
+    for res, r in zip(results, reqs):
+        statuses[res["status"]] = statuses.get(res["status"], 0) + 1
+        routes[r.route] = routes.get(r.route, 0) + 1
+    nnz = np.array([r.lp.nnz for r in reqs])
+    return {"mode": "serve", "qps": qps, "duration_s": duration, "requests": len(reqs),
+            "warm_resolves": int(sum(k[0] == "warm" for k in kinds)), "routes": routes, "statuses": statuses,
+            "certified": int(ok.sum()), "fresh_at_planted_optimum": planted_ok,
+            "solves_per_s": round(float(ok.sum()) / wall, 3), "wall_s": round(wall, 2),
+            "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 1),
+                           "p99": round(1e3 * float(np.percentile(lat, 99)), 1),
+                           "p50_warm": round(1e3 * float(np.percentile(lat[[k[0] == "warm" for k in kinds]], 50)), 1)
+                           if any(k[0] == "warm" for k in kinds) else None},
+            "nonzeros": {"median": int(np.median(nnz)), "mean": int(nnz.mean()), "max": int(nnz.max())},
+            "hbm_budget_bytes": hbm_budget}
+
+
 def _csv_rows(directory, suffix):
     import csv
     import glob
@@ -238,6 +314,9 @@ def main():
     ap.add_argument("--summarize-traffic", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="summarise FETCH_SIZE / WRITE_SIZE CSV runs of the bench and of --copy-probe")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--serve", action="store_true", help="serving stream instead of the batch workload")
+    ap.add_argument("--qps", type=float, default=1.5)
+    ap.add_argument("--duration", type=float, default=40.0)
     args = ap.parse_args()
     if args.summarize_trace:
         print(summarize_trace(args.summarize_trace))
@@ -264,6 +343,9 @@ def main():
     if engine.device_count() < 1:
         raise SystemExit("no HIP device visible")
     log("device:", engine.device_name(0))
+    if args.serve:
+        print(json.dumps(serve_stream(engine, args.qps, args.duration)), flush=True)
+        return
     insts = make_instances(engine, args.count)
     nnz = sum(i.lp.nnz for i in insts)
     log(f"instances: {len(insts)} LPs, {nnz} nonzeros")

@@ -63,6 +63,12 @@ int niryukti_batch_run(niryukti_batch *b, int checks_per_launch);
 /* 1 if the ticket finished (result filled), 0 if still running, <0 on error. */
 int niryukti_batch_result(niryukti_batch *b, int64_t ticket, niryukti_result *r);
 int niryukti_batch_stats_get(niryukti_batch *b, niryukti_batch_stats *s);
+/* Up to `capacity` tickets finished since the last call, in finishing order
+ * (each reported once); *count receives how many were written. */
+int niryukti_batch_finished(niryukti_batch *b, int64_t *tickets, int64_t capacity, int64_t *count);
+/* Frees the host copy of a finished LP once its result has been read; the
+ * ticket is invalid afterwards.  Long-running servers call it per request. */
+int niryukti_batch_release(niryukti_batch *b, int64_t ticket);
 const char *niryukti_batch_last_error(void);
 
 #ifdef __cplusplus

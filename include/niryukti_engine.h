@@ -65,9 +65,22 @@ void niryukti_default_options(niryukti_options *o);
 int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *o, niryukti_result *r);
 const char *niryukti_engine_last_error(void);
 
+/* Models read by the reference's readers (native .json or .mps, src/io.cpp:84-140,
+ * 299-585): exactly one of `path` and `json_text` is given.  NULL on error. */
+typedef struct niryukti_model niryukti_model;
+niryukti_model *niryukti_model_load(const char *path, const char *json_text);
+void niryukti_model_destroy(niryukti_model *m);
+/* Borrowed views valid until destroy. */
+void niryukti_model_view(const niryukti_model *m, niryukti_lp *lp, const char **name, const char **fingerprint);
+/* Result document with the reference's keys (src/io.cpp:902-995) for a result
+ * of this model; the string is valid until the next call on this thread. */
+const char *niryukti_result_json(const niryukti_model *m, const niryukti_result *r, const char *backend);
+
 /* Seeded synthetic instances.  config 0 (2,000 x 5,000 equality LP), 1
  * (1,000,000 x 2,000,000 power-law LP), 4 (block-angular); `scale` divides the
- * dimensions.  config 2 selects instance `index` of the batched workload. */
+ * dimensions.  config 2 selects instance `index` of the batched workload,
+ * config 3 request `index` of the serving stream (nonzeros log-uniform in
+ * [1e3, 5e6], batch generator otherwise). */
 typedef struct niryukti_instance niryukti_instance;
 niryukti_instance *niryukti_instance_create(int config, uint64_t seed, int64_t scale, int64_t index);
 void niryukti_instance_destroy(niryukti_instance *p);

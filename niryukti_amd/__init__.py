@@ -8,6 +8,7 @@ itself and raises if the native library is missing.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import os
 from dataclasses import dataclass
 from pathlib import Path
@@ -15,8 +16,8 @@ from pathlib import Path
 import numpy as np
 
 __all__ = [
-    "load_library", "LP", "solve_lp", "Instance", "Backend", "Batch", "NativeSession", "device_count",
-    "device_name", "ABI_VERSION", "CLI_PATH",
+    "load_library", "LP", "Model", "solve_lp", "Instance", "Backend", "Batch", "NativeSession", "Server",
+    "device_count", "device_name", "ABI_VERSION", "CLI_PATH",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -135,6 +136,14 @@ def load_library() -> C.CDLL:
     lib.niryukti_batch_result.argtypes = [C.c_void_p, C.c_int64, C.POINTER(_Result)]
     lib.niryukti_batch_stats_get.argtypes = [C.c_void_p, C.POINTER(_BatchStats)]
     lib.niryukti_batch_last_error.restype = C.c_char_p
+    lib.niryukti_batch_finished.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]
+    lib.niryukti_batch_release.argtypes = [C.c_void_p, C.c_int64]
+    lib.niryukti_model_load.argtypes = [C.c_char_p, C.c_char_p]
+    lib.niryukti_model_load.restype = C.c_void_p
+    lib.niryukti_model_destroy.argtypes = [C.c_void_p]
+    lib.niryukti_model_view.argtypes = [C.c_void_p, C.POINTER(_LP), C.POINTER(C.c_char_p), C.POINTER(C.c_char_p)]
+    lib.niryukti_result_json.argtypes = [C.c_void_p, C.POINTER(_Result), C.c_char_p]
+    lib.niryukti_result_json.restype = C.c_char_p
     _lib = lib
     return lib
 
@@ -242,6 +251,66 @@ def solve_lp(lp: LP, **options) -> dict:
     if rc != 0:
         raise RuntimeError(lib.niryukti_engine_last_error().decode())
     return _result_dict(res, x, y)
+
+
+class Model:
+    """A model read by the native readers: a .json / .mps path, or native JSON
+    (text or dict).  `lp` holds the canonical arrays; `result_json` renders a
+    result with the reference's document keys."""
+
+    def __init__(self, path=None, *, json_model=None):
+        lib = load_library()
+        if (path is None) == (json_model is None):
+            raise ValueError("give exactly one of path and json_model")
+        text = None
+        if json_model is not None:
+            text = json_model if isinstance(json_model, str) else json.dumps(json_model, allow_nan=False)
+        h = lib.niryukti_model_load(os.fsencode(str(path)) if path is not None else None,
+                                    text.encode() if text is not None else None)
+        if not h:
+            raise ValueError(lib.niryukti_engine_last_error().decode())
+        self._lib = lib
+        self._h = C.c_void_p(h)
+        s = _LP()
+        name, fp = C.c_char_p(), C.c_char_p()
+        lib.niryukti_model_view(self._h, C.byref(s), C.byref(name), C.byref(fp))
+        m, n = s.rows, s.cols
+        nnz = s.ptr[m]
+
+        def view(p, count, dtype):
+            return np.ctypeslib.as_array(p, shape=(count,)).copy() if count else np.zeros(0, dtype=dtype)
+
+        self.lp = LP(view(s.ptr, m + 1, np.int64), view(s.index, nnz, np.int64), view(s.value, nnz, np.float64),
+                     view(s.c, n, np.float64), view(s.lb, n, np.float64), view(s.ub, n, np.float64),
+                     view(s.rl, m, np.float64), view(s.ru, m, np.float64), s.offset, s.sense)
+        self.name = name.value.decode()
+        self.fingerprint = fp.value.decode()
+
+    def result_json(self, result: dict, backend: str = "hip") -> dict:
+        res = _Result()
+        for key, _ in _Result._fields_:
+            if key in ("x", "y"):
+                continue
+            if key in ("status", "message"):
+                setattr(res, key, str(result.get(key, "")).encode()[:250])
+            elif key in result:
+                setattr(res, key, result[key])
+        x = _f64(result["x"]) if result.get("x") is not None else None
+        y = _f64(result["y"]) if result.get("y") is not None else None
+        res.x = _ptr(x) if x is not None and len(x) == self.lp.cols else None
+        res.y = _ptr(y) if y is not None and len(y) == self.lp.rows else None
+        raw = self._lib.niryukti_result_json(self._h, C.byref(res), backend.encode())
+        if raw is None:
+            raise RuntimeError(self._lib.niryukti_engine_last_error().decode())
+        return json.loads(raw)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.niryukti_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Instance:
@@ -422,6 +491,18 @@ class Batch:
         done = self._check(self._lib.niryukti_batch_result(self._h, int(ticket), C.byref(res)))
         return _result_dict(res, x, y) if done else None
 
+    def finished(self, capacity: int = 4096) -> list:
+        """Tickets finished since the last call, in finishing order."""
+        buf = (C.c_int64 * capacity)()
+        n = C.c_int64()
+        self._check(self._lib.niryukti_batch_finished(self._h, buf, int(capacity), C.byref(n)))
+        return list(buf[:n.value])
+
+    def release(self, ticket: int) -> None:
+        """Frees the host copy of a finished LP; the ticket is invalid afterwards."""
+        self._check(self._lib.niryukti_batch_release(self._h, int(ticket)))
+        self._shapes.pop(ticket, None)
+
     def stats(self) -> dict:
         st = _BatchStats()
         self._check(self._lib.niryukti_batch_stats_get(self._h, C.byref(st)))
@@ -451,3 +532,4 @@ def device_name(index: int = 0) -> str:
 
 
 from .session import NativeSession  # noqa: E402  (uses load_library above)
+from .serve import Server  # noqa: E402

@@ -6,6 +6,7 @@
 #include <stdexcept>
 
 #include "instances.hpp"
+#include "io.hpp"
 #include "lp.hpp"
 
 namespace {
@@ -60,6 +61,11 @@ nyk::Options to_options(const niryukti_options *in, const niryukti_lp &lp) {
     return o;
 }
 }  // namespace
+
+struct niryukti_model {
+    nyk::Model m;
+    std::string fingerprint;
+};
 
 struct niryukti_instance {
     nyk_gen::Instance p;
@@ -133,12 +139,100 @@ int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *opts, niryu
 
 const char *niryukti_engine_last_error(void) { return g_error.c_str(); }
 
+niryukti_model *niryukti_model_load(const char *path, const char *json_text) {
+    try {
+        g_error.clear();
+        if ((path == nullptr) == (json_text == nullptr)) throw std::runtime_error("Give exactly one of path and JSON text");
+        auto *h = new niryukti_model;
+        try {
+            h->m = path ? nyk::read_model(path) : nyk::read_json_model(json_text);
+            h->m.validate();
+            h->fingerprint = h->m.fingerprint();
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        return h;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return nullptr;
+    }
+}
+
+void niryukti_model_destroy(niryukti_model *m) { delete m; }
+
+void niryukti_model_view(const niryukti_model *h, niryukti_lp *lp, const char **name, const char **fingerprint) {
+    const nyk::Model &m = h->m;
+    lp->rows = m.rows();
+    lp->cols = m.cols();
+    lp->ptr = m.A.ptr.data();
+    lp->index = m.A.index.data();
+    lp->value = m.A.value.data();
+    lp->c = m.c.data();
+    lp->lb = m.lb.data();
+    lp->ub = m.ub.data();
+    lp->rl = m.rl.data();
+    lp->ru = m.ru.data();
+    lp->offset = m.offset;
+    lp->sense = m.sense;
+    if (name) *name = m.name.c_str();
+    if (fingerprint) *fingerprint = h->fingerprint.c_str();
+}
+
+const char *niryukti_result_json(const niryukti_model *h, const niryukti_result *in, const char *backend) {
+    static thread_local std::string out;
+    try {
+        g_error.clear();
+        if (!h || !in) throw std::runtime_error("Null argument");
+        const nyk::Model &m = h->m;
+        nyk::Result r;
+        r.status = in->status;
+        r.message = in->message;
+        r.backend = backend ? backend : "hip";
+        r.method_selected = r.method_requested = "pdhg";
+        r.accuracy.finite = in->finite != 0;
+        r.accuracy.objective = in->finite ? in->objective / m.sense : nyk::kInf;
+        r.accuracy.primal = in->primal_residual;
+        r.accuracy.dual = in->dual_residual;
+        r.accuracy.gap = in->relative_gap;
+        r.accuracy.kkt = in->kkt_error;
+        r.accuracy.complementarity = in->complementarity;
+        r.accuracy.lower_bound = in->dual_bound;
+        r.accuracy.primal_absolute = in->primal_absolute;
+        r.accuracy.dual_absolute = in->dual_absolute;
+        r.iterations = in->iterations;
+        r.restarts = in->restarts;
+        r.rejected_steps = in->rejected_steps;
+        r.weight_updates = in->weight_updates;
+        r.host_candidate_checks = in->host_candidate_checks;
+        r.device_candidate_checks = in->device_candidate_checks;
+        r.removed_rows = in->removed_rows;
+        r.removed_columns = in->removed_columns;
+        r.seconds = in->seconds;
+        r.preprocess_seconds = in->preprocess_seconds;
+        r.transfer_seconds = in->transfer_seconds;
+        r.iteration_seconds = in->iteration_seconds;
+        r.verification_seconds = in->verification_seconds;
+        r.gpu_index_bits = in->index_bits;
+        r.estimated_gpu_bytes = 48. * m.A.nnz() + 240. * (m.rows() + m.cols() + 2);
+        if (in->x) r.x.assign(in->x, in->x + m.cols());
+        if (in->y) r.y.assign(in->y, in->y + m.rows());
+        out = nyk::result_json(m, r).dump();
+        return out.c_str();
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return nullptr;
+    }
+}
+
 niryukti_instance *niryukti_instance_create(int config, uint64_t seed, int64_t scale, int64_t index) {
     try {
         g_error.clear();
         auto *h = new niryukti_instance;
         if (config == 2)
             h->p = nyk_gen::batch_instance(nyk_gen::batch_shapes(int(index) + 1, seed).back());
+        else if (config == 3)
+            h->p = nyk_gen::batch_instance(nyk_gen::serve_shape(index, seed));
         else
             h->p = nyk_gen::workload(config, seed, scale);
         return h;

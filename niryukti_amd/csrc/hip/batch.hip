@@ -1167,8 +1167,26 @@ class BatchEngine {
         return running;
     }
 
+    // Tickets finished since the last call, in finishing order.
+    int64_t finished(int64_t *tickets, int64_t capacity) {
+        int64_t k = 0;
+        while (k < capacity && !done_.empty()) {
+            tickets[k++] = done_.front();
+            done_.pop_front();
+        }
+        return k;
+    }
+
+    // Frees the host copy of a finished LP (model, prepare maps, result).
+    void release(int64_t ticket) {
+        if (ticket < 0 || ticket >= int64_t(jobs_.size()) || !jobs_[ticket]) throw std::runtime_error("Unknown ticket");
+        if (!jobs_[ticket]->finished) throw std::runtime_error("Ticket has not finished");
+        jobs_[ticket].reset();
+    }
+
     bool result(int64_t ticket, niryukti_result *out) {
         if (ticket < 0 || ticket >= int64_t(jobs_.size())) throw std::runtime_error("Unknown ticket");
+        if (!jobs_[ticket]) throw std::runtime_error("Ticket was released");
         Job &j = *jobs_[ticket];
         if (!j.finished) return false;
         const auto &r = j.result;
@@ -1209,7 +1227,7 @@ class BatchEngine {
         s->active = 0;
         for (int v : slot_job_) s->active += v >= 0;
         s->finished = 0;
-        for (const auto &j : jobs_) s->finished += j->finished;
+        for (const auto &j : jobs_) s->finished += !j || j->finished;
         s->arena_bytes = int64_t(arena_.size());
         s->arena_used_bytes = int64_t(arena_.used());
         s->launches = launches_;
@@ -1249,6 +1267,7 @@ class BatchEngine {
             r.status = status;
             r.message = std::move(message);
             j.finished = true;
+            done_.push_back(j.ticket);
             r.seconds = std::chrono::duration<double>(Clock::now() - j.start).count();
         };
         if (!j.prep.failure.empty()) {
@@ -1706,11 +1725,13 @@ class BatchEngine {
         state_[j.slot].status = kIterationLimit;
         j.slot = -1;
         j.finished = true;
+        done_.push_back(j.ticket);
         j.result.seconds = std::chrono::duration<double>(Clock::now() - j.start).count();
     }
     void retire_host(Job &j, const char *status) {
         j.result.status = status;
         j.finished = true;
+        done_.push_back(j.ticket);
         j.result.seconds = std::chrono::duration<double>(Clock::now() - j.start).count();
     }
 
@@ -1735,7 +1756,7 @@ class BatchEngine {
     int64_t class_launches_[kClasses] = {};
     int64_t attempts_ = 0;
     std::vector<std::unique_ptr<Job>> jobs_;
-    std::deque<int64_t> queue_;
+    std::deque<int64_t> queue_, done_;
     bool dirty_ = true;
     int64_t launches_ = 0, certifications_ = 0, farkas_checks_ = 0;
     double device_seconds_ = 0;
@@ -1828,6 +1849,20 @@ int niryukti_batch_result(niryukti_batch *b, int64_t ticket, niryukti_result *r)
     int done = 0;
     int rc = guarded([&] { done = b->engine.result(ticket, r) ? 1 : 0; });
     return rc < 0 ? rc : done;
+}
+
+int niryukti_batch_finished(niryukti_batch *b, int64_t *tickets, int64_t capacity, int64_t *count) {
+    return guarded([&] {
+        if (!b || !count || (capacity > 0 && !tickets)) throw std::runtime_error("Null argument");
+        *count = b->engine.finished(tickets, capacity);
+    });
+}
+
+int niryukti_batch_release(niryukti_batch *b, int64_t ticket) {
+    return guarded([&] {
+        if (!b) throw std::runtime_error("Null batch");
+        b->engine.release(ticket);
+    });
 }
 
 int niryukti_batch_stats_get(niryukti_batch *b, niryukti_batch_stats *s) {

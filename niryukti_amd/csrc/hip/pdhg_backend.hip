@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -47,6 +48,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxBlocks = 2048;
 constexpr int kSpareAttempts = 4;  // early-out attempts appended to each chunk for rejections
 constexpr int kMaxRejects = 40;    // consecutive rejections before giving up (src/cpu.cpp:119)
+constexpr int kInlineDecideBlocks = 256;  // larger dual grids decide in decide_kernel
 
 enum Stop : int { kRun = 0, kDone = 1, kRestart = 2, kExhausted = 3, kNonfinite = 4 };
 
@@ -101,6 +103,8 @@ struct PrimalArgs {
     const double *y[2];
     const double *anchor_x;
     double *partial;
+    const double *part;  // chunk partials of A^T y, or null
+    int parts;
     State *st;
     Flags f;
 };
@@ -113,6 +117,9 @@ struct DualArgs {
     double *partial;  // 2 per block: |dy|^2, dy.delta
     const double *primal_partial;
     int primal_blocks;
+    const double *part;  // chunk partials of A dx, or null
+    int parts;
+    int inline_decide;   // 1: last block decides; 0: decide_kernel runs next
     State *st;
     Flags f;
 };
@@ -151,13 +158,30 @@ __device__ __forceinline__ double load_shared(const double *p) {
     return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// This thread's fixed-order sum of values[stride * b + offset] over
+// b = threadIdx.x, threadIdx.x + kBlock, ... < count, eight loads in flight.
+__device__ __forceinline__ double gather_sum(const double *values, int stride, int offset, int count) {
+    constexpr int U = 8;
+    double sum = 0;
+    for (int b0 = threadIdx.x; b0 < count; b0 += U * kBlock) {
+        double t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int b = b0 + u * kBlock;
+            t[u] = b < count ? load_shared(&values[(long long)stride * b + offset]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) sum += t[u];
+    }
+    return sum;
+}
+
 // Segment `blockIdx.x` of a long row: partial sum (and, with ABS, the sum of
 // absolute products), ticket, and in thread 0 of the block that completes the
 // row, the row index and its totals.
 template <bool ABS, class O, class V>
 __device__ bool segment_row(const DevCsr<O, V> &A, const Segments &sg, const double *v, double *lds, long long &row,
-                            double &total, double &total_abs) {
-    const int s = blockIdx.x;
+                            double &total, double &total_abs, int s) {
     const long long end = sg.end[s];
     double acc = 0, acca = 0;
     for (long long k = sg.begin[s] + threadIdx.x; k < end; k += kBlock) {
@@ -167,26 +191,28 @@ __device__ bool segment_row(const DevCsr<O, V> &A, const Segments &sg, const dou
     }
     const double part = block_sum(acc, lds);
     const double parta = ABS ? block_sum(acca, lds) : 0.0;
-    bool done = false;
+    __shared__ int completes;  // long-row slot this block completes, else -1
     if (threadIdx.x == 0) {
         store_shared(&sg.partial[2 * s], part);
         store_shared(&sg.partial[2 * s + 1], parta);
         const int L = sg.owner[s];
         const unsigned t = __hip_atomic_fetch_add(&sg.ticket[L], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (t + 1 == unsigned(sg.count[L])) {
-            double sum = 0, suma = 0;
-            for (int q = 0; q < sg.count[L]; ++q) {
-                sum += load_shared(&sg.partial[2 * (sg.first[L] + q)]);
-                suma += load_shared(&sg.partial[2 * (sg.first[L] + q) + 1]);
-            }
-            __hip_atomic_store(&sg.ticket[L], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            row = sg.row[L];
-            total = sum;
-            total_abs = suma;
-            done = true;
-        }
+        completes = t + 1 == unsigned(sg.count[L]) ? L : -1;
     }
-    return done;
+    __syncthreads();
+    const int L = completes;
+    if (L < 0) return false;
+    // The whole block reads the row's segment partials at once (fixed order).
+    const double *first = sg.partial + 2 * (long long)sg.first[L];
+    const double sum = block_sum(gather_sum(first, 2, 0, sg.count[L]), lds);
+    const double suma = ABS ? block_sum(gather_sum(first, 2, 1, sg.count[L]), lds) : 0.0;
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&sg.ticket[L], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        row = sg.row[L];
+        total = sum;
+        total_abs = suma;
+    }
+    return threadIdx.x == 0;
 }
 
 // Calls row(i, sum_k A[i,k] v[k], sum_k |A[i,k] v[k]|) for every row (the
@@ -198,7 +224,7 @@ __device__ __forceinline__ void for_rows(const DevCsr<O, V> &A, const Segments &
     if ((int)blockIdx.x < sg.nseg) {
         long long i = 0;
         double total = 0, total_abs = 0;
-        if (segment_row<ABS>(A, sg, v, lds, i, total, total_abs)) row(i, total, total_abs);
+        if (segment_row<ABS>(A, sg, v, lds, i, total, total_abs, blockIdx.x)) row(i, total, total_abs);
         return;
     }
     constexpr int rows_per_block = kBlock / G;
@@ -225,6 +251,106 @@ __device__ __forceinline__ void for_rows(const DevCsr<O, V> &A, const Segments &
     }
 }
 
+// XCD-aware split of a product y = M v whose gathered vector v is larger than
+// one XCD's L2: the columns of M are cut into P contiguous chunks (P divides
+// the 8 XCDs), chunk k stored as its own CSR over all rows.  Workgroups are
+// dispatched to XCDs round-robin, so workgroup b runs on XCD b % 8 and works
+// on chunk (b % 8) % P only: every XCD gathers from 1/P of v, which stays in
+// its L2, instead of missing to the memory side for almost every gather.
+// Partial row sums land in part[k * rows + i]; the consuming kernel adds the
+// P partials in chunk order (a fixed order).  Rows handled by long-row
+// segments are left out of the chunks.
+struct ChunkedCsr {
+    const int *ptr;     // P blocks of rows + 1 offsets, each relative to its chunk base
+    const int *idx;     // global column indices, chunk after chunk
+    long long base[8];  // start of chunk k in idx / val
+    long long rows;
+    int P;
+};
+
+struct VecPair {
+    const double *p[2];  // the gathered vector by iterate parity (both the same when not double-buffered)
+};
+
+constexpr int kChunkRows = 4;  // rows per thread per pass of partial_kernel
+
+// Partial sums of one column chunk, several rows per thread with the offsets,
+// the first two entries of every row and their gathers issued together (rows
+// average 1-2 nonzeros per chunk, so this is a latency chain, not a stream).
+// Blocks [0, grid_chunks) work on the chunks; the blocks after them run the
+// long-row segments of the whole matrix and write each long row's total into
+// plane P, which is zero for every other row.
+template <class O, class V>
+__global__ __launch_bounds__(kBlock) void partial_kernel(ChunkedCsr C, const V *val, DevCsr<O, V> A, Segments sg,
+                                                         int grid_chunks, const State *st, VecPair vec,
+                                                         double *part) {
+    if (st->stop != kRun) return;
+    const double *v = vec.p[st->cur];
+    if ((int)blockIdx.x >= grid_chunks) {
+        __shared__ double lds[4];
+        long long i = 0;
+        double total = 0, total_abs = 0;
+        if (segment_row<false>(A, sg, v, lds, i, total, total_abs, blockIdx.x - grid_chunks))
+            part[(long long)C.P * C.rows + i] = total;
+        return;
+    }
+    const int xcd = blockIdx.x & 7;
+    const int k = xcd % C.P;
+    const long long q = (long long)(blockIdx.x >> 3) * (8 / C.P) + xcd / C.P;
+    const long long nq = grid_chunks / C.P;
+    const int *ptr = C.ptr + (long long)k * (C.rows + 1);
+    const int *idx = C.idx + C.base[k];
+    const V *vals = val + C.base[k];
+    double *out = part + (long long)k * C.rows;
+    constexpr long long span = (long long)kBlock * kChunkRows;
+    for (long long b = q * span; b < C.rows; b += nq * span) {
+        int beg[kChunkRows], end[kChunkRows];
+        double acc[kChunkRows];
+#pragma unroll
+        for (int r = 0; r < kChunkRows; ++r) {
+            const long long i = b + threadIdx.x + (long long)r * kBlock;
+            beg[r] = i < C.rows ? ptr[i] : 0;
+            end[r] = i < C.rows ? ptr[i + 1] : 0;
+        }
+        int c0[kChunkRows], c1[kChunkRows];
+        double v0[kChunkRows], v1[kChunkRows];
+#pragma unroll
+        for (int r = 0; r < kChunkRows; ++r) {
+            const bool h0 = beg[r] < end[r], h1 = beg[r] + 1 < end[r];
+            c0[r] = h0 ? idx[beg[r]] : 0;
+            v0[r] = h0 ? double(vals[beg[r]]) : 0.0;
+            c1[r] = h1 ? idx[beg[r] + 1] : 0;
+            v1[r] = h1 ? double(vals[beg[r] + 1]) : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < kChunkRows; ++r) {
+            const double g0 = beg[r] < end[r] ? v[c0[r]] : 0.0;
+            const double g1 = beg[r] + 1 < end[r] ? v[c1[r]] : 0.0;
+            acc[r] = 0.0;
+            if (beg[r] < end[r]) acc[r] += v0[r] * g0;
+            if (beg[r] + 1 < end[r]) acc[r] += v1[r] * g1;
+            for (int e = beg[r] + 2; e < end[r]; ++e) acc[r] += double(vals[e]) * v[idx[e]];
+        }
+#pragma unroll
+        for (int r = 0; r < kChunkRows; ++r) {
+            const long long i = b + threadIdx.x + (long long)r * kBlock;
+            if (i < C.rows) out[i] = acc[r];
+        }
+    }
+}
+
+// for_rows with every row summed from the P + 1 planes of partial_kernel
+// (fixed order), one thread per row: a pure stream.
+template <class Row>
+__device__ __forceinline__ void for_rows_partials(long long rows, const double *part, int P, Row &&row) {
+    const long long stride = (long long)gridDim.x * kBlock;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < rows; i += stride) {
+        double acc = 0;
+        for (int k = 0; k <= P; ++k) acc += part[(long long)k * rows + i];
+        row(i, acc, 0.0);
+    }
+}
+
 template <int G, class O, class V>
 __global__ __launch_bounds__(kBlock) void primal_kernel(DevCsr<O, V> at, Segments sg, PrimalArgs a) {
     const State *st = a.st;
@@ -242,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void primal_kernel(DevCsr<O, V> at, Segment
     const double rho = a.f.reflection;
     double local = 0;
     __shared__ double lds[4];
-    for_rows<G>(at, sg, y, lds, [&](long long j, double acc, double) {
+    auto update = [&](long long j, double acc, double) {
         const double g = a.c[j] + acc;
         const double xj = x[j];
         const double t = clampd(xj - tau * g, a.lb[j], a.ub[j]);
@@ -257,7 +383,9 @@ __global__ __launch_bounds__(kBlock) void primal_kernel(DevCsr<O, V> at, Segment
             const double v = xa[j];
             xan[j] = v + (t - v) * w;
         }
-    });
+    };
+    if (a.part) for_rows_partials(at.rows, a.part, a.parts, update);
+    else for_rows<G>(at, sg, y, lds, update);
     const double s = block_sum(local, lds);
     if (threadIdx.x == 0) store_shared(&a.partial[blockIdx.x], s);
 }
@@ -329,7 +457,7 @@ __global__ __launch_bounds__(kBlock) void dual_kernel(DevCsr<O, V> A, Segments s
     double sq = 0, cross = 0;
     __shared__ double lds[4];
     __shared__ int last;
-    for_rows<G>(A, sg, a.dx, lds, [&](long long i, double delta, double) {
+    auto update = [&](long long i, double delta, double) {
         const double axi = ax[i];
         const double bar = axi + 2 * delta;
         const double yi = y[i];
@@ -349,31 +477,47 @@ __global__ __launch_bounds__(kBlock) void dual_kernel(DevCsr<O, V> A, Segments s
             yan[i] = v + (t - v) * w;
             axn[i] = next_ax;
         }
-    });
+    };
+    if (a.part) for_rows_partials(A.rows, a.part, a.parts, update);
+    else for_rows<G>(A, sg, a.dx, lds, update);
     const double s1 = block_sum(sq, lds);
     const double s2 = block_sum(cross, lds);
     if (threadIdx.x == 0) {
         store_shared(&a.partial[2 * blockIdx.x], s1);
         store_shared(&a.partial[2 * blockIdx.x + 1], s2);
+    }
+    // Large grids hand the decision to decide_kernel: one ticket per block on
+    // a single address serialises at the memory side (tens of ns each).
+    if (!a.inline_decide) return;
+    if (threadIdx.x == 0) {
         const unsigned ticket = __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         last = ticket == gridDim.x - 1;
     }
     __syncthreads();
     if (!last) return;
-    double p = 0, d = 0, c = 0;
-    for (int b = threadIdx.x; b < a.primal_blocks; b += kBlock) p += load_shared(&a.primal_partial[b]);
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock) {
-        d += load_shared(&a.partial[2 * b]);
-        c += load_shared(&a.partial[2 * b + 1]);
-    }
-    p = block_sum(p, lds);
-    d = block_sum(d, lds);
-    c = block_sum(c, lds);
+    // Every thread issues its share of the partial loads together (they are
+    // memory-side round trips), then adds them in a fixed order.
+    const double p = block_sum(gather_sum(a.primal_partial, 1, 0, a.primal_blocks), lds);
+    const double d = block_sum(gather_sum(a.partial, 2, 0, (int)gridDim.x), lds);
+    const double c = block_sum(gather_sum(a.partial, 2, 1, (int)gridDim.x), lds);
     if (threadIdx.x == 0) {
         decide(st, a.f, p, d, c);
         __hip_atomic_store(&st->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
+
+// The accept/reject decision of one attempt after a dual_kernel with
+// inline_decide == 0 (same partials, same fixed order).
+__global__ __launch_bounds__(kBlock) void decide_kernel(DualArgs a, int dual_blocks) {
+    State *st = a.st;
+    if (st->stop != kRun) return;
+    __shared__ double lds[4];
+    const double p = block_sum(gather_sum(a.primal_partial, 1, 0, a.primal_blocks), lds);
+    const double d = block_sum(gather_sum(a.partial, 2, 0, dual_blocks), lds);
+    const double c = block_sum(gather_sum(a.partial, 2, 1, dual_blocks), lds);
+    if (threadIdx.x == 0) decide(st, a.f, p, d, c);
+}
+
 
 template <int G, class O, class V>
 __global__ __launch_bounds__(kBlock) void spmv_kernel(DevCsr<O, V> A, Segments sg, const double *x, double *out) {
@@ -761,6 +905,66 @@ HostSegments make_segments(const int64_t *ptr, long long rows, long long thresho
     return h;
 }
 
+// Gathered vectors up to this size stay in one XCD's L2 (4 MB) next to the
+// streamed matrix; larger ones are split into chunks of at most this size.
+constexpr long long kChunkBytes = 2LL << 20;
+
+int chunk_count(long long vector_length) {
+    if (const char *e = std::getenv("NIRYUKTI_XCD_CHUNKS"))
+        if (std::atoi(e) == 0) return 0;
+    int p = 1;
+    while (p < 8 && vector_length * 8 / p > kChunkBytes) p *= 2;
+    return p > 1 ? p : 0;
+}
+
+struct HostChunks {
+    int P = 0;
+    std::vector<int> ptr;  // P * (rows + 1)
+    std::vector<int> idx;
+    std::vector<double> val;
+    long long base[8] = {};
+};
+
+// Column chunks of a CSR matrix for partial_kernel; rows longer than
+// `threshold` (segment rows) are left out.
+HostChunks make_chunks(const int64_t *ptr, const int64_t *index, const double *value, long long rows,
+                       long long cols, long long threshold, int P) {
+    HostChunks h;
+    h.P = P;
+    const long long width = (cols + P - 1) / P;
+    std::vector<long long> count(size_t(P) * size_t(rows + 1), 0);
+    for (long long i = 0; i < rows; ++i) {
+        if (ptr[i + 1] - ptr[i] > threshold) continue;
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e) count[size_t(index[e] / width) * (rows + 1) + i + 1]++;
+    }
+    long long total = 0;
+    h.ptr.resize(count.size());
+    for (int k = 0; k < P; ++k) {
+        h.base[k] = total;
+        long long run = 0;
+        for (long long i = 0; i <= rows; ++i) {
+            run += count[size_t(k) * (rows + 1) + i];
+            if (run >= (1LL << 31)) throw std::runtime_error("Chunk too large for 32-bit offsets");
+            h.ptr[size_t(k) * (rows + 1) + i] = int(run);
+        }
+        total += run;
+    }
+    h.idx.resize(total);
+    h.val.resize(total);
+    std::vector<long long> fill(P);
+    for (int k = 0; k < P; ++k) fill[k] = h.base[k];
+    for (long long i = 0; i < rows; ++i) {
+        if (ptr[i + 1] - ptr[i] > threshold) continue;
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e) {
+            const int k = int(index[e] / width);
+            h.idx[fill[k]] = int(index[e]);
+            h.val[fill[k]] = value[e];
+            ++fill[k];
+        }
+    }
+    return h;
+}
+
 int elementwise_grid(long long n) { return int(std::max<long long>(1, std::min<long long>((n + 255) / 256, 4096))); }
 
 struct HostCsr {
@@ -822,6 +1026,10 @@ class Backend {
             sg_at_ = upload_segments(make_segments(t.ptr.data(), n_, row_threshold(g_at_)), row_threshold(g_at_));
             nb1_ = sg_at_.nseg + grid_for(n_, g_at_);
             nb2_ = sg_a_.nseg + grid_for(m_, g_a_);
+            setup_chunks(lp.ptr, lp.index, lp.value, m_, n_, row_threshold(g_a_), ch_a_, ch_a_val_, g_ch_a_,
+                         nbp_a_, part_a_);
+            setup_chunks(t.ptr.data(), t.index.data(), t.value.data(), n_, m_, row_threshold(g_at_), ch_at_,
+                         ch_at_val_, g_ch_at_, nbp_at_, part_at_);
             c_ = upload(lp.c, n_);
             lb_ = upload(lp.lb, n_);
             ub_ = upload(lp.ub, n_);
@@ -1181,6 +1389,30 @@ class Backend {
         }
     }
 
+    void setup_chunks(const int64_t *ptr, const int64_t *index, const double *value, long long rows, long long cols,
+                      long long threshold, ChunkedCsr &c, void *&val, int &group, int &grid, double *&part) {
+        c = ChunkedCsr{};
+        const int P = chunk_count(cols);
+        if (!P || rows == 0) return;
+        HostChunks h = make_chunks(ptr, index, value, rows, cols, threshold, P);
+        c.P = P;
+        c.rows = rows;
+        for (int k = 0; k < 8; ++k) c.base[k] = h.base[k];
+        c.ptr = upload_vector(h.ptr);
+        c.idx = upload_vector(h.idx);
+        if (fp32_) val = upload_vector(std::vector<float>(h.val.begin(), h.val.end()));
+        else val = upload_vector(h.val);
+        group = 1;
+        const long long rpb = (long long)kBlock * kChunkRows;
+        const long long per_chunk = std::min<long long>((rows + rpb - 1) / rpb, 4096 / P);
+        const long long per_xcd = (per_chunk + (8 / P) - 1) / (8 / P);
+        grid = int(8 * std::max<long long>(1, per_xcd));
+        // P chunk planes plus the long-row plane, which stays zero except at long rows.
+        part = alloc_d((long long)(P + 1) * rows);
+        fill_kernel<<<elementwise_grid((P + 1) * rows), 256, 0, stream_>>>(part, (P + 1) * rows, 0.0);
+        HIP_OK(hipGetLastError());
+    }
+
     template <class T>
     T *upload_vector(const std::vector<T> &h) {
         T *d = static_cast<T *>(alloc_bytes(h.size() * sizeof(T)));
@@ -1220,6 +1452,8 @@ class Backend {
         p.dx = dx_;
         p.anchor_x = anchor_x_;
         p.partial = part1_;
+        p.part = ch_at_.P ? part_at_ : nullptr;
+        p.parts = ch_at_.P;
         p.st = st_;
         p.f = flags_;
         return p;
@@ -1239,6 +1473,9 @@ class Backend {
         d.partial = part2_;
         d.primal_partial = part1_;
         d.primal_blocks = nb1_;
+        d.part = ch_a_.P ? part_a_ : nullptr;
+        d.parts = ch_a_.P;
+        d.inline_decide = nb2_ <= kInlineDecideBlocks ? 1 : 0;
         d.st = st_;
         d.f = flags_;
         return d;
@@ -1250,19 +1487,29 @@ class Backend {
         auto a = view<O, V>(a_);
         auto pa = primal_args();
         auto da = dual_args();
+        if (ch_at_.P) launch_partial(ch_at_, ch_at_val_, at, sg_at_, nbp_at_, VecPair{{y_[0], y_[1]}}, part_at_);
         switch (g_at_) {
 #define PRIMAL_CASE(G) \
     case G: primal_kernel<G, O, V><<<nb1_, kBlock, 0, stream_>>>(at, sg_at_, pa); break;
             PRIMAL_CASE(1) PRIMAL_CASE(2) PRIMAL_CASE(4) PRIMAL_CASE(8) PRIMAL_CASE(16) PRIMAL_CASE(32) PRIMAL_CASE(64)
 #undef PRIMAL_CASE
         }
+        if (ch_a_.P) launch_partial(ch_a_, ch_a_val_, a, sg_a_, nbp_a_, VecPair{{dx_, dx_}}, part_a_);
         switch (g_a_) {
 #define DUAL_CASE(G) \
     case G: dual_kernel<G, O, V><<<nb2_, kBlock, 0, stream_>>>(a, sg_a_, da); break;
             DUAL_CASE(1) DUAL_CASE(2) DUAL_CASE(4) DUAL_CASE(8) DUAL_CASE(16) DUAL_CASE(32) DUAL_CASE(64)
 #undef DUAL_CASE
         }
+        if (!da.inline_decide) decide_kernel<<<1, kBlock, 0, stream_>>>(da, nb2_);
     }
+    template <class O, class V>
+    void launch_partial(const ChunkedCsr &c, const void *val, const DevCsr<O, V> &m, const Segments &sg, int grid,
+                        VecPair vec, double *part) {
+        partial_kernel<O, V><<<grid + sg.nseg, kBlock, 0, stream_>>>(c, static_cast<const V *>(val), m, sg, grid, st_,
+                                                                    vec, part);
+    }
+
     void launch_attempt() {
         if (idx64_) {
             if (fp32_) launch_attempt_t<int64_t, float>(); else launch_attempt_t<int64_t, double>();
@@ -1445,6 +1692,10 @@ class Backend {
     Flags flags_{};
     DeviceMatrix a_, at_;
     Segments sg_a_{}, sg_at_{};
+    ChunkedCsr ch_a_{}, ch_at_{};  // XCD column chunks (P == 0: not used)
+    void *ch_a_val_ = nullptr, *ch_at_val_ = nullptr;
+    int g_ch_a_ = 1, g_ch_at_ = 1, nbp_a_ = 8, nbp_at_ = 8;
+    double *part_a_ = nullptr, *part_at_ = nullptr;
     // Original model and device-check buffers (npdhg_attach_original).
     bool attached_ = false, evaluated_ = false, best_ = false, wide0_ = false;
     long long n0_ = 0, m0_ = 0;

@@ -377,6 +377,18 @@ inline Instance batch_instance(const BatchShape &s) {
     return p;
 }
 
+// Serving stream: request `index` draws its size log-uniformly in nonzeros
+// over [1e3, 5e6] (about 13.75 nonzeros per row with n in [2m, 3m] and 3-8
+// per column) and is otherwise a batch instance.
+inline BatchShape serve_shape(int64_t index, uint64_t seed) {
+    Rng rng(splitmix(seed ^ 0x73 ^ (uint64_t(index) * 0x9E3779B97F4A7C15ULL)));
+    const double target = std::exp(rng.uniform(std::log(1e3), std::log(5e6)));
+    const int64_t m = std::max<int64_t>(10, int64_t(std::llround(target / 13.75)));
+    const int64_t n = 2 * m + int64_t(rng.below(uint64_t(m + 1)));
+    const double upper = rng.uniform(1.0, 50.0);
+    return {m, n, upper, splitmix(seed * 0x100000001b3ULL + uint64_t(index) + 0x5e7)};
+}
+
 // Named workloads.  `scale` divides the row/column counts (1 = full size) so
 // tests can run the same structure small.
 inline Instance workload(int config, uint64_t seed, int64_t scale = 1) {

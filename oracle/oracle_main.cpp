@@ -21,6 +21,8 @@ int main(int argc, char **argv) {
     try {
         if (config == 2)
             p = nyk_gen::batch_instance(nyk_gen::batch_shapes(std::stoi(get("--index", "0")) + 1, seed).back());
+        else if (config == 3)
+            p = nyk_gen::batch_instance(nyk_gen::serve_shape(std::stoll(get("--index", "0")), seed));
         else
             p = nyk_gen::workload(config, seed, std::stoll(get("--scale", "1")));
     } catch (const std::exception &e) {

@@ -91,6 +91,8 @@ Loaded load() {
     nyk_gen::Instance p;
     if (config == 2)
         p = nyk_gen::batch_instance(nyk_gen::batch_shapes(int(getl("--index", 0)) + 1, seed).back());
+    else if (config == 3)
+        p = nyk_gen::batch_instance(nyk_gen::serve_shape(getl("--index", 0), seed));
     else if (config == 1 && flag("--alpha")) {
         int64_t sc = getl("--scale", 1);
         p = nyk_gen::powerlaw_lp(1000000 / sc, 2000000 / sc, seed, getd("--zipf", 1.5),

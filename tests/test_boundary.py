@@ -143,3 +143,34 @@ def test_session_protocol(engine):
     with engine.NativeSession(linear_model()) as s:
         assert s._lib.vantage_request(s._handle, b"not json") is None
         assert lib.niryukti_abi_version() == 1
+
+
+def test_serving_request_validation(engine):
+    """`/v1/solve` body checks of the reference (python/niryukti/service.py:54-69)."""
+    from niryukti_amd.serve import parse_request
+
+    model, extra = parse_request({"model": linear_model(), "options": {"tol": 1e-7, "threads": 4}})
+    assert model.lp.rows == 1 and model.lp.cols == 1 and extra == {"tol": 1e-7}
+    assert parse_request({"model": linear_model()})[1] == {}
+    assert parse_request({"model": linear_model(), "options": {"method": "halpern"}})[1] == {"method": "halpern"}
+    for bad in ({"model": linear_model(), "extra": 1}, {"model": "x"}, {"options": {}},
+                {"model": linear_model(), "options": {"gpu": 1}},
+                {"model": linear_model(), "options": {"time_limit": 0}},
+                {"model": linear_model(), "options": {"time_limit": 7200}},
+                {"model": linear_model(), "options": {"threads": 65}},
+                {"model": {"variables": "bad"}}, [1, 2]):
+        with pytest.raises((ValueError, TypeError)):
+            parse_request(bad)
+
+
+def test_model_handle_and_result_document(engine):
+    m = engine.Model(NETLIB / "afiro.mps")
+    ref = json.loads((NETLIB / "afiro.reference_result.json").read_text())
+    assert m.fingerprint == ref["model"]["fingerprint"]
+    doc = m.result_json({"status": "OPTIMAL", "message": "", "objective": ref["objective"], "finite": 1,
+                         "kkt_error": 0.0, "x": ref["primal"], "y": ref["dual"], "iterations": 7})
+    assert doc["model"]["fingerprint"] == ref["model"]["fingerprint"]
+    assert doc["objective"] == pytest.approx(ref["objective"], rel=1e-12)
+    assert doc["primal"] == ref["primal"] and doc["performance"]["iterations"] == 7
+    with pytest.raises(ValueError):
+        engine.Model(json_model={"variables": [{"name": "x", "type": "integer"}]})

@@ -161,6 +161,32 @@ def test_spmv_long_rows_and_columns(engine):
         b.close()
 
 
+def test_xcd_chunked_products_track_oracle(engine, oracle, monkeypatch):
+    """Config 1 at half size: both gathered vectors exceed 2 MB, so A dx and
+    A^T y run as XCD column chunks plus fixed-order partial sums.  Three
+    adaptive chunks must track the oracle, and the chunked and direct paths
+    must agree to rounding."""
+    inst = engine.Instance(1, 11, 2, 0)
+    lp = inst.lp
+    x0, y0 = np.zeros(lp.cols), np.zeros(lp.rows)
+    cpu = oracle.Backend(lp, x0, y0, adaptive=True)
+    gpu = engine.Backend(lp, x0, y0, adaptive=True)
+    monkeypatch.setenv("NIRYUKTI_XCD_CHUNKS", "0")
+    direct = engine.Backend(lp, x0, y0, adaptive=True)
+    for _ in range(3):
+        assert cpu.advance(10, 0.05, 0.05) == gpu.advance(10, 0.05, 0.05) == direct.advance(10, 0.05, 0.05) == 10
+        x, y, xa, ya = cpu.candidates()
+        for b in (gpu, direct):
+            gx, gy, gxa, gya = b.candidates()
+            assert np.abs(gx - x).max() <= 1e-9 * (1 + np.abs(x).max())
+            assert np.abs(gy - y).max() <= 1e-9 * (1 + np.abs(y).max())
+            assert np.abs(gxa - xa).max() <= 1e-9 * (1 + np.abs(x).max())
+            assert np.abs(gya - ya).max() <= 1e-9 * (1 + np.abs(y).max())
+    assert gpu.status()["rejected_total"] == cpu.rejected() == direct.status()["rejected_total"]
+    gpu.close()
+    direct.close()
+
+
 @pytest.mark.parametrize("case", [(0, 4, 0), (2, 1, 0), (2, 1, 1), (4, 200, 0), (1, 400, 0), (1, 20, 0)])
 def test_adaptive_chunks_track_oracle(engine, oracle, case):
     """Three chunks of 10 adaptive steps on a generated (unscaled) LP."""
