@@ -46,9 +46,10 @@ def make_instances(engine, count):
     return [engine.Instance(2, SEED, 1, i) for i in range(count)]
 
 
-def batch_step(engine, insts, checks):
+def batch_step(engine, insts, checks, handoff=0):
     """One step: untimed upload, timed solve of the whole batch."""
     b = engine.Batch(arena_bytes=96 << 30, max_slots=len(insts), time_limit=1e9)
+    b.set_handoff(handoff, 4)
     tickets = [b.submit(i.lp) for i in insts]
     admitted = b.admit()
     assert admitted == len(insts), "arena could not hold the whole batch"
@@ -307,6 +308,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--checks", type=int, default=5, help="check periods per launch")
+    ap.add_argument("--handoff", type=int, default=0,
+                    help="iterations after which an LP moves to the grid-wide engine (0: never)")
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
@@ -350,11 +353,11 @@ def main():
     nnz = sum(i.lp.nnz for i in insts)
     log(f"instances: {len(insts)} LPs, {nnz} nonzeros")
     for w in range(args.warmup):
-        el, solved, _, _, _ = batch_step(engine, insts, args.checks)
+        el, solved, _, _, _ = batch_step(engine, insts, args.checks, args.handoff)
         log(f"warmup {w}: {el:.3f} s, {solved} solved")
     total_t, total_solved, lat, stats_all, last = 0.0, 0, [], [], None
     for s in range(args.steps):
-        el, solved, results, stats, latencies = batch_step(engine, insts, args.checks)
+        el, solved, results, stats, latencies = batch_step(engine, insts, args.checks, args.handoff)
         log(f"step {s}: {el:.3f} s, {solved}/{len(insts)} solved, stats {stats}")
         total_t += el
         total_solved += solved
@@ -395,12 +398,15 @@ def main():
         "config": {"workload": f"batch of {args.count} independent LPs: m log-uniform [50,2000], n uniform "
                                f"[2m,3m], 3-8 nnz/col, N(0,1), box [0,U], U~U(1,50), planted optimum, seed {SEED}",
                    "lps": args.count, "nonzeros": int(nnz), "tol": 1e-6, "checks_per_launch": args.checks,
+                   "handoff_iterations": args.handoff,
                    "engine": "batched (one workgroup per LP), FP64"},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 1) if lat else None,
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 1) if lat else None},
         "statuses": statuses,
         "unsolved_indices": [k for k, r in enumerate(last) if r["status"] != "OPTIMAL"][:20],
+        "unsolved_messages": sorted({r["message"][:120] for r in last if r["status"] != "OPTIMAL" and r["message"]})[:5],
         "iterations": {"mean": float(its.mean()), "max": int(its.max())},
+        "handoffs": int(stats_all[-1].get("handoffs", 0)),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": TRAFFIC_PROFILE.name if traffic is not None else None,

@@ -40,6 +40,8 @@ typedef struct {
      * number of step-kernel launches.  Classes run on separate streams. */
     double class_seconds[4], class_bytes[4];
     int64_t class_launches[4];
+    int64_t handoffs;         /* LPs moved to the grid-wide engine so far */
+    int64_t handoffs_running; /* of those, still running there */
 } niryukti_batch_stats;
 
 /* opts: solver options shared by every LP of the batch (NULL = defaults).
@@ -63,6 +65,12 @@ int niryukti_batch_run(niryukti_batch *b, int checks_per_launch);
 /* 1 if the ticket finished (result filled), 0 if still running, <0 on error. */
 int niryukti_batch_result(niryukti_batch *b, int64_t ticket, niryukti_result *r);
 int niryukti_batch_stats_get(niryukti_batch *b, niryukti_batch_stats *s);
+/* LPs still running after `iterations` iterations (a multiple of check_every;
+ * 0, the default, disables it) leave the batch and continue on the grid-wide
+ * single-LP engine, warm started from their current iterate, on
+ * `workers` host threads (default 4).  The move depends only on the LP's own
+ * iteration count, so results stay the same alone and in any mix. */
+int niryukti_batch_set_handoff(niryukti_batch *b, int64_t iterations, int workers);
 /* Up to `capacity` tickets finished since the last call, in finishing order
  * (each reported once); *count receives how many were written. */
 int niryukti_batch_finished(niryukti_batch *b, int64_t *tickets, int64_t capacity, int64_t *count);

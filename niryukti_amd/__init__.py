@@ -92,7 +92,7 @@ class _BatchStats(C.Structure):
                                          "host_farkas_checks")] + \
         [(k, C.c_double) for k in ("device_seconds", "advance_seconds", "check_seconds", "advance_bytes")] + \
         [("attempts", C.c_int64), ("class_seconds", C.c_double * 4), ("class_bytes", C.c_double * 4),
-         ("class_launches", C.c_int64 * 4)]
+         ("class_launches", C.c_int64 * 4), ("handoffs", C.c_int64), ("handoffs_running", C.c_int64)]
 
 
 _lib = None
@@ -138,6 +138,7 @@ def load_library() -> C.CDLL:
     lib.niryukti_batch_last_error.restype = C.c_char_p
     lib.niryukti_batch_finished.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]
     lib.niryukti_batch_release.argtypes = [C.c_void_p, C.c_int64]
+    lib.niryukti_batch_set_handoff.argtypes = [C.c_void_p, C.c_int64, C.c_int]
     lib.niryukti_model_load.argtypes = [C.c_char_p, C.c_char_p]
     lib.niryukti_model_load.restype = C.c_void_p
     lib.niryukti_model_destroy.argtypes = [C.c_void_p]
@@ -490,6 +491,11 @@ class Batch:
         res.x, res.y = _ptr(x), _ptr(y)
         done = self._check(self._lib.niryukti_batch_result(self._h, int(ticket), C.byref(res)))
         return _result_dict(res, x, y) if done else None
+
+    def set_handoff(self, iterations: int, workers: int = 4) -> None:
+        """LPs still running after `iterations` iterations continue on the
+        grid-wide engine (0 disables); see include/niryukti_batch.h."""
+        self._check(self._lib.niryukti_batch_set_handoff(self._h, int(iterations), int(workers)))
 
     def finished(self, capacity: int = 4096) -> list:
         """Tickets finished since the last call, in finishing order."""

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -70,6 +71,7 @@ enum Status : int {
     kIterationLimit = 3,
     kNumerical = 4,
     kExhausted = 5,
+    kHandoff = 6,      // reached the handoff iteration count: continues on the grid-wide engine
 };
 
 // Device pointers and constants of one resident LP.
@@ -102,6 +104,7 @@ struct LpState {
 struct BatchOpts {
     double tol;
     int check_every, adaptive, restart, pad;
+    long long handoff;  // iterations after which an LP leaves the batch (0: never)
 };
 
 // ------------------------------------------------------------ reductions --
@@ -891,6 +894,9 @@ __global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps,
         }
     }
     __syncthreads();
+    // A long-running LP leaves the batch at a check, so the point where it
+    // moves depends only on its own iteration count.
+    if (threadIdx.x == 0 && o.handoff > 0 && S.status == kRunning && S.iterations >= o.handoff) S.status = kHandoff;
     if (threadIdx.x == 0) states[s] = S;
 }
 
@@ -953,6 +959,7 @@ class Arena {
 
 struct Job {
     int64_t ticket = 0;
+    std::vector<double> handoff_x, handoff_y;  // original-space warm start (current iterate) for the grid engine
     nyk::Model original;
     nyk::Prepared prep;
     std::unique_ptr<nyk::Verifier> verifier;
@@ -1040,6 +1047,13 @@ class BatchEngine {
         for (int s = max_slots_ - 1; s >= 0; --s) free_slots_.push_back(s);
     }
     ~BatchEngine() {
+        {
+            std::lock_guard<std::mutex> lock(handoff_mu_);
+            stopping_ = true;
+            handoff_queue_.clear();
+        }
+        handoff_cv_.notify_all();
+        for (auto &w : workers_) w.join();
         if (stream_) hipStreamSynchronize(stream_);
         hipFree(d_desc_);
         hipFree(d_state_);
@@ -1106,7 +1120,7 @@ class BatchEngine {
                     HIP_OK(hipMemcpyAsync(d_slots_ + size_t(c) * max_slots_, by_class[c].data(),
                                           sizeof(int) * by_class[c].size(), hipMemcpyHostToDevice, stream_));
             HIP_OK(hipEventRecord(uploaded_, stream_));
-            BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0};
+            BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0, handoff_};
             auto t0 = Clock::now();
             std::vector<long long> before(active.size());
             for (size_t k = 0; k < active.size(); ++k)
@@ -1162,9 +1176,24 @@ class BatchEngine {
             ++launches_;
         }
         settle(active);
-        int64_t running = int64_t(queue_.size());
+        drain_handoffs(active.empty());
+        int64_t running = int64_t(queue_.size()) + handoffs_outstanding_;
         for (int s = 0; s < max_slots_; ++s) running += slot_job_[s] >= 0;
         return running;
+    }
+
+    // Handoff of long-running LPs to the grid-wide single-LP engine: after
+    // `iterations` iterations in the batch an LP continues on its own, warm
+    // started from its current iterate, on `workers` host threads
+    // (each solve has its own HIP stream).  One workgroup walks a small LP
+    // through a chain of dependent memory round trips per step; the grid
+    // engine spreads the same step over the whole GPU, so the slow tail of a
+    // batch finishes several times sooner.  0 iterations disables it.
+    void set_handoff(int64_t iterations, int workers) {
+        if (iterations < 0 || workers < 1 || workers > 16) throw std::runtime_error("Invalid handoff settings");
+        if (handoffs_outstanding_) throw std::runtime_error("Handoff settings cannot change while LPs are handed off");
+        handoff_ = iterations;
+        handoff_workers_ = workers;
     }
 
     // Tickets finished since the last call, in finishing order.
@@ -1233,6 +1262,8 @@ class BatchEngine {
         s->launches = launches_;
         s->host_certifications = certifications_;
         s->host_farkas_checks = farkas_checks_;
+        s->handoffs = handoffs_total_;
+        s->handoffs_running = handoffs_outstanding_;
         s->device_seconds = device_seconds_;
         s->advance_seconds = advance_seconds_;
         s->check_seconds = check_seconds_;
@@ -1396,7 +1427,8 @@ class BatchEngine {
                 break;
             }
             build_image(j, image, d, arena_.base() + offset);
-            HIP_OK(hipMemcpy(arena_.base() + offset, image.data(), bytes, hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpyAsync(arena_.base() + offset, image.data(), bytes, hipMemcpyHostToDevice, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));  // `image` is a temporary
             const int s = free_slots_.back();
             free_slots_.pop_back();
             queue_.pop_front();
@@ -1601,7 +1633,13 @@ class BatchEngine {
             const LpDev &d = desc_[s];
             const LpState &st = state_[s];
             Job &j = *jobs_[slot_job_[s]];
-            if (st.status == kParkOptimal) {
+            if (st.status == kHandoff) {
+                // the current (scaled) iterate; restored on the host below
+                cx[k].resize(d.n);
+                cyo[k].resize(d.m);
+                download(cx[k].data(), d.x[st.cur], d.n);
+                download(cyo[k].data(), d.y[st.cur], d.m);
+            } else if (st.status == kParkOptimal) {
                 // the parked candidate is still in its evaluation buffers
                 cx[k].resize(d.n0);
                 cyo[k].resize(d.m0);
@@ -1633,7 +1671,11 @@ class BatchEngine {
             auto &r = j.result;
             auto t0 = Clock::now();
             copy_counters(j, st);
-            if (st.status == kParkOptimal) {
+            if (st.status == kHandoff) {
+                j.handoff_x = j.prep.restore_x(cx[k]);
+                j.handoff_y = j.prep.restore_y(cyo[k], j.original.rl.size());
+                resume[k] = 2;
+            } else if (st.status == kParkOptimal) {
                 // src/solver.cpp:588-596: OPTIMAL only after the safe re-evaluation
                 auto safe = j.verifier->evaluate(cx[k], cyo[k]);
                 {
@@ -1704,8 +1746,112 @@ class BatchEngine {
                 if (r.status == "OPTIMAL" && (!r.accuracy.finite || r.accuracy.kkt > opt_.tol)) r.status = "NUMERICAL_ERROR";
             }
         });
-        for (size_t k = 0; k < ns; ++k)
-            if (!resume[k]) retire(*jobs_[slot_job_[stopped[k]]]);
+        for (size_t k = 0; k < ns; ++k) {
+            if (resume[k] == 2) hand_off(*jobs_[slot_job_[stopped[k]]]);
+            else if (!resume[k]) retire(*jobs_[slot_job_[stopped[k]]]);
+        }
+    }
+
+    // Frees the LP's slot and arena space and queues it for a grid worker.
+    void hand_off(Job &j) {
+        arena_.release(j.offset, j.bytes);
+        slot_job_[j.slot] = -1;
+        free_slots_.push_back(j.slot);
+        state_[j.slot] = LpState{};
+        state_[j.slot].status = kIterationLimit;
+        j.slot = -1;
+        start_workers();
+        {
+            std::lock_guard<std::mutex> lock(handoff_mu_);
+            handoff_queue_.push_back(&j);
+        }
+        ++handoffs_outstanding_;
+        ++handoffs_total_;
+        handoff_cv_.notify_one();
+    }
+
+    void start_workers() {
+        if (!workers_.empty()) return;
+        int device = 0;
+        HIP_OK(hipGetDevice(&device));
+        for (int w = 0; w < handoff_workers_; ++w)
+            workers_.emplace_back([this, device] {
+                (void)hipSetDevice(device);
+                for (;;) {
+                    Job *j = nullptr;  // jobs_ may grow meanwhile: work on the Job itself
+                    {
+                        std::unique_lock<std::mutex> lock(handoff_mu_);
+                        handoff_cv_.wait(lock, [&] { return stopping_ || !handoff_queue_.empty(); });
+                        if (handoff_queue_.empty()) return;
+                        j = handoff_queue_.front();
+                        handoff_queue_.pop_front();
+                    }
+                    run_handed_off(*j);
+                    {
+                        std::lock_guard<std::mutex> lock(handoff_mu_);
+                        handoff_done_.push_back(j->ticket);
+                    }
+                    done_cv_.notify_all();
+                }
+            });
+    }
+
+    // Continues an LP on the grid-wide engine with the iteration and time
+    // budget it has left; counters add up over both phases.
+    void run_handed_off(Job &j) {
+        auto &r = j.result;
+        nyk::Options o = opt_;
+        o.initial_x = std::move(j.handoff_x);
+        o.initial_y = std::move(j.handoff_y);
+        o.iteration_limit = std::max<int64_t>(0, opt_.iteration_limit - r.iterations);
+        o.time_limit = std::max(0.0, opt_.time_limit - std::chrono::duration<double>(Clock::now() - j.start).count());
+        o.device = "gpu";
+        try {
+            nyk::Result g = nyk::solve(j.original, o);
+            const int64_t it = r.iterations, rs = r.restarts, rj = r.rejected_steps, wu = r.weight_updates;
+            const int64_t hc = r.host_candidate_checks;
+            const double pre = r.preprocess_seconds, ver = r.verification_seconds;
+            r.status = g.status;
+            r.message = g.message;
+            r.x = std::move(g.x);
+            r.y = std::move(g.y);
+            r.accuracy = g.accuracy;
+            r.infeasibility_ray = std::move(g.infeasibility_ray);
+            r.certificate_margin = g.certificate_margin;
+            r.iterations = it + g.iterations;
+            r.restarts = rs + g.restarts;
+            r.rejected_steps = rj + g.rejected_steps;
+            r.weight_updates = wu + g.weight_updates;
+            r.host_candidate_checks = hc + g.host_candidate_checks;
+            r.device_candidate_checks = g.device_candidate_checks;
+            r.preprocess_seconds = pre + g.preprocess_seconds;
+            r.verification_seconds = ver + g.verification_seconds;
+            r.iteration_seconds += g.iteration_seconds;
+            r.backend = "hip-batch+hip";
+        } catch (const std::exception &e) {
+            r.status = "NUMERICAL_ERROR";
+            r.message = std::string("Grid-engine continuation failed: ") + e.what();
+        }
+    }
+
+    // Moves finished handoffs to the finished list (main thread only); waits
+    // briefly for one when nothing else is running.
+    void drain_handoffs(bool idle) {
+        if (!handoffs_outstanding_) return;
+        std::deque<int64_t> done;
+        {
+            std::unique_lock<std::mutex> lock(handoff_mu_);
+            if (idle && handoff_done_.empty() && queue_.empty())
+                done_cv_.wait_for(lock, std::chrono::milliseconds(2), [&] { return !handoff_done_.empty(); });
+            done.swap(handoff_done_);
+        }
+        for (int64_t t : done) {
+            Job &j = *jobs_[t];
+            j.finished = true;
+            done_.push_back(t);
+            j.result.seconds = std::chrono::duration<double>(Clock::now() - j.start).count();
+            --handoffs_outstanding_;
+        }
     }
 
     void copy_counters(Job &j, const LpState &st) {
@@ -1757,6 +1903,16 @@ class BatchEngine {
     int64_t attempts_ = 0;
     std::vector<std::unique_ptr<Job>> jobs_;
     std::deque<int64_t> queue_, done_;
+    // handoff to the grid-wide engine
+    int64_t handoff_ = 0;
+    int handoff_workers_ = 4;
+    int64_t handoffs_outstanding_ = 0, handoffs_total_ = 0;
+    std::vector<std::thread> workers_;
+    std::mutex handoff_mu_;
+    std::condition_variable handoff_cv_, done_cv_;
+    std::deque<Job *> handoff_queue_;
+    std::deque<int64_t> handoff_done_;
+    bool stopping_ = false;
     bool dirty_ = true;
     int64_t launches_ = 0, certifications_ = 0, farkas_checks_ = 0;
     double device_seconds_ = 0;
@@ -1849,6 +2005,13 @@ int niryukti_batch_result(niryukti_batch *b, int64_t ticket, niryukti_result *r)
     int done = 0;
     int rc = guarded([&] { done = b->engine.result(ticket, r) ? 1 : 0; });
     return rc < 0 ? rc : done;
+}
+
+int niryukti_batch_set_handoff(niryukti_batch *b, int64_t iterations, int workers) {
+    return guarded([&] {
+        if (!b) throw std::runtime_error("Null batch");
+        b->engine.set_handoff(iterations, workers);
+    });
 }
 
 int niryukti_batch_finished(niryukti_batch *b, int64_t *tickets, int64_t capacity, int64_t *count) {

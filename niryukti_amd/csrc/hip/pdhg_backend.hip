@@ -1323,6 +1323,13 @@ class Backend {
         allocations_.push_back(p);
         return p;
     }
+    // Host-to-device copy ordered on this backend's stream, complete on
+    // return (the source may be a temporary).  Nothing here touches the
+    // legacy default stream, which other threads' graph captures forbid.
+    void copy_in(void *dst, const void *src, size_t bytes) {
+        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+    }
     double *alloc_d(long long n) { return static_cast<double *>(alloc_bytes(size_t(n) * sizeof(double))); }
     void free_one(void *p) {
         auto it = std::find(allocations_.begin(), allocations_.end(), p);
@@ -1372,17 +1379,17 @@ class Backend {
         } else {
             std::vector<int32_t> p32(ptr, ptr + rows + 1);
             d.ptr = alloc_bytes(size_t(rows + 1) * 4);
-            HIP_OK(hipMemcpy(d.ptr, p32.data(), p32.size() * 4, hipMemcpyHostToDevice));
+            copy_in(d.ptr, p32.data(), p32.size() * 4);
         }
         {
             std::vector<int32_t> i32(index, index + d.nnz);
             d.idx = static_cast<int *>(alloc_bytes(size_t(d.nnz) * 4));
-            if (d.nnz) HIP_OK(hipMemcpy(d.idx, i32.data(), i32.size() * 4, hipMemcpyHostToDevice));
+            if (d.nnz) copy_in(d.idx, i32.data(), i32.size() * 4);
         }
         if (single) {
             std::vector<float> v32(value, value + d.nnz);
             d.val = alloc_bytes(size_t(d.nnz) * 4);
-            if (d.nnz) HIP_OK(hipMemcpy(d.val, v32.data(), v32.size() * 4, hipMemcpyHostToDevice));
+            if (d.nnz) copy_in(d.val, v32.data(), v32.size() * 4);
         } else {
             d.val = alloc_bytes(size_t(d.nnz) * 8);
             if (d.nnz) HIP_OK(hipMemcpyAsync(d.val, value, size_t(d.nnz) * 8, hipMemcpyHostToDevice, stream_));
@@ -1416,7 +1423,7 @@ class Backend {
     template <class T>
     T *upload_vector(const std::vector<T> &h) {
         T *d = static_cast<T *>(alloc_bytes(h.size() * sizeof(T)));
-        if (!h.empty()) HIP_OK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        if (!h.empty()) copy_in(d, h.data(), h.size() * sizeof(T));
         return d;
     }
     Segments upload_segments(const HostSegments &h, long long threshold) {
@@ -1547,7 +1554,9 @@ class Backend {
         auto it = graphs_cache_.find(attempts);
         if (it != graphs_cache_.end()) return it->second;
         hipGraph_t graph = nullptr;
-        HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        // Relaxed: other threads (other solves, the batched engine) keep
+        // running their own streams while this one captures.
+        HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
         try {
             for (int i = 0; i < attempts; ++i) launch_attempt();
         } catch (...) {

@@ -103,3 +103,34 @@ def test_batch_reports_infeasible_and_limits(engine):
     assert [r["status"] for r in res] == ["INFEASIBLE", "INFEASIBLE"]
     res, _ = solve_batch(engine, [inst.lp], iteration_limit=200)
     assert res[0]["status"] == "ITERATION_LIMIT" and res[0]["iterations"] == 200
+
+
+def test_handoff_to_grid_engine(engine, oracle):
+    """LPs still running after the handoff count continue on the grid-wide
+    engine from their last checked candidate.  Results must be certified, at
+    the planted optimum, and the same alone and in the mix, since the move
+    depends only on the LP's own iteration count."""
+    insts = [engine.Instance(2, 42, 1, i) for i in range(10)]
+
+    def run(lps, checks):
+        b = engine.Batch(arena_bytes=4 << 30, max_slots=64)
+        b.set_handoff(300, 2)
+        tickets = [b.submit(lp) for lp in lps]
+        b.run(checks)
+        out = [b.result(t) for t in tickets]
+        stats = b.stats()
+        b.close()
+        return out, stats
+
+    mix, stats = run([i.lp for i in insts], 2)
+    assert stats["handoffs"] > 0 and stats["handoffs_running"] == 0
+    moved = [k for k, r in enumerate(mix) if r["iterations"] > 300]
+    assert len(moved) == stats["handoffs"]
+    for k, inst in enumerate(insts):
+        r = mix[k]
+        assert r["status"] == "OPTIMAL" and r["kkt_error"] <= 1e-6, (k, r["status"])
+        assert rel(r["objective"], inst.planted_objective) <= 1e-6
+        assert oracle.verify(inst.lp, r["x"], r["y"])["kkt"] <= 1e-6
+    for k in moved[:2]:
+        alone, _ = run([insts[k].lp], 5)
+        assert np.array_equal(alone[0]["x"], mix[k]["x"]) and alone[0]["iterations"] == mix[k]["iterations"]

@@ -94,3 +94,26 @@ def test_analytical_lps(engine):
     assert engine.solve_lp(m, iteration_limit=0, time_limit=0)["status"] == "TIME_LIMIT"
     contradiction = engine.LP.from_dense([[1, 1], [1, 1]], [0, 0], [0, 0], [10, 10], [3, -np.inf], [np.inf, 1])
     assert engine.solve_lp(contradiction)["status"] == "INFEASIBLE"
+
+
+def test_concurrent_solves_from_threads(engine):
+    """Single-LP solves on several host threads at once (each captures its own
+    hipGraph on its own stream) while a batch runs on the main thread: every
+    result must equal the same solve run alone."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    insts = [engine.Instance(2, 42, 1, i) for i in (421, 542, 7, 11, 19, 23)]
+    alone = [engine.solve_lp(i.lp) for i in insts]
+    batch_lps = [engine.Instance(2, 42, 1, i).lp for i in range(40, 56)]
+    with ThreadPoolExecutor(4) as pool:
+        futures = [pool.submit(engine.solve_lp, i.lp) for i in insts]
+        b = engine.Batch(arena_bytes=1 << 30, max_slots=64)
+        tickets = [b.submit(lp) for lp in batch_lps]
+        b.run(1)
+        batch = [b.result(t) for t in tickets]
+        b.close()
+        results = [f.result() for f in futures]
+    for a, r in zip(alone, results):
+        assert r["status"] == a["status"] == "OPTIMAL", r["message"]
+        assert np.array_equal(a["x"], r["x"]) and a["iterations"] == r["iterations"]
+    assert all(r["status"] == "OPTIMAL" for r in batch)
