@@ -192,6 +192,44 @@ def serve_stream(engine, qps, duration, seed=SEED, warm_fraction=0.2, hbm_budget
             "hbm_budget_bytes": hbm_budget}
 
 
+def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16):
+    """One LP of config 0 / 1 / 4 on the grid-wide single-LP engine for a fixed
+    iteration budget, beside the reference's CPU PDHG on the same instance and
+    budget (oracle/_ref/ref_driver, `cpu_threads` OpenMP threads)."""
+    inst = engine.Instance(config, seed, scale)
+    lp = inst.lp
+    m, n, nnz = lp.rows, lp.cols, lp.nnz
+    log(f"single: config {config} scale {scale}: {m} x {n}, {nnz} nonzeros")
+    r = engine.solve_lp(lp, iteration_limit=iterations, time_limit=1e9, tol=tol)
+    attempts = r["iterations"] + r["rejected_steps"]
+    s_p = 4 if nnz < 2 ** 31 else 8
+    # algorithmic bytes of one attempt (FP64 values, int32 indices), original sizes
+    b_attempt = 2 * nnz * 12 + (n + m + 2) * s_p + m * 8 + n * 8 + 64 * (n + m)
+    achieved = b_attempt * attempts / max(r["iteration_seconds"], 1e-12) / 1e9
+    out = {"mode": "single", "config": config, "scale": scale, "rows": m, "cols": n, "nonzeros": nnz,
+           "tol": tol, "status": r["status"], "iterations": r["iterations"], "attempts": attempts,
+           "kkt": r["kkt_error"], "objective": r["objective"], "planted_objective": inst.planted_objective,
+           "seconds": round(r["seconds"], 3), "iteration_seconds": round(r["iteration_seconds"], 3),
+           "preprocess_seconds": round(r["preprocess_seconds"], 3),
+           "verification_seconds": round(r["verification_seconds"], 3),
+           "ms_per_attempt": round(1e3 * r["iteration_seconds"] / max(attempts, 1), 4),
+           "algorithmic_bytes_per_attempt": b_attempt,
+           "achieved_GBs": round(achieved, 1), "hbm_frac": round(achieved / HBM_PEAK_GBS, 4)}
+    ref = ROOT / "oracle" / "_ref" / "ref_driver"
+    if cpu_threads and ref.exists():
+        cmd = [str(ref), "solve", "--config", str(config), "--seed", str(seed), "--scale", str(scale),
+               "--iterations", str(iterations), "--threads", str(cpu_threads), "--tol", str(tol)]
+        c = json.loads(subprocess.run(cmd, capture_output=True, text=True, check=True).stdout)
+        out["cpu_baseline"] = {"kind": "reference", "threads": cpu_threads, "status": c["status"],
+                               "iterations": c["iterations"], "kkt": c["kkt"],
+                               "seconds": round(c["seconds"], 3), "iteration_seconds": round(c["iteration_seconds"], 3),
+                               "ms_per_iteration": round(1e3 * c["iteration_seconds"] / max(c["iterations"], 1), 3)}
+        out["speedup_per_iteration"] = round(out["cpu_baseline"]["ms_per_iteration"] /
+                                             max(1e3 * r["iteration_seconds"] / max(r["iterations"], 1), 1e-9), 1)
+        out["speedup_wall"] = round(c["seconds"] / max(r["seconds"], 1e-9), 1)
+    return out
+
+
 def _csv_rows(directory, suffix):
     import csv
     import glob
@@ -318,6 +356,12 @@ def main():
                     help="summarise FETCH_SIZE / WRITE_SIZE CSV runs of the bench and of --copy-probe")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--serve", action="store_true", help="serving stream instead of the batch workload")
+    ap.add_argument("--single", type=int, default=None, metavar="CONFIG",
+                    help="one LP of config 0, 1 or 4 on the single-LP engine beside the reference CPU")
+    ap.add_argument("--scale", type=int, default=1)
+    ap.add_argument("--iterations", type=int, default=1000)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--qps", type=float, default=1.5)
     ap.add_argument("--duration", type=float, default=40.0)
     args = ap.parse_args()
@@ -348,6 +392,10 @@ def main():
     log("device:", engine.device_name(0))
     if args.serve:
         print(json.dumps(serve_stream(engine, args.qps, args.duration)), flush=True)
+        return
+    if args.single is not None:
+        print(json.dumps(single_lp(engine, args.single, args.scale, args.iterations, args.tol,
+                                   cpu_threads=args.cpu_threads)), flush=True)
         return
     insts = make_instances(engine, args.count)
     nnz = sum(i.lp.nnz for i in insts)

@@ -61,6 +61,10 @@ constexpr int64_t kShortRow = 128;
 // The largest LDS class runs 512-thread workgroups: fewer rows per thread,
 // twice the loads in flight per LP.
 constexpr int kWideThreads = 512;
+// Launches with at most kTailLps LPs in a class (one per CU or fewer) run
+// kTailThreads-thread workgroups with one row per thread.
+constexpr int kTailLps = 256;
+constexpr int kTailThreads = 1024;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxRejects = 40;
 
@@ -629,21 +633,46 @@ __device__ __forceinline__ void rows_dot(const int *ptr, const int *idx, const d
     }
 }
 
+// Sums over rows are taken per tile of 64 consecutive rows (one wave, fixed
+// butterfly) and the tiles are then added in tile order, so an attempt's
+// energy and coupling do not depend on the workgroup size or rows per thread:
+// an LP can change launch shape (the tail mode below) and still follow
+// exactly the trajectory it follows alone.
+constexpr int kMaxTiles = 128;  // 64-row tiles: m, n <= 8000 in the LDS classes
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum of tiles[0..count) in tile order, in a fixed tree over one wave; the
+// result is valid in every lane of wave 0.
+__device__ __forceinline__ double tile_total(const double *tiles, int count) {
+    const int lane = threadIdx.x & 63;
+    double acc = 0;
+    for (int t = lane; t < count; t += 64) acc += tiles[t];
+    return wave_sum(acc);
+}
+
 // One PDHG attempt for an LP whose dual iterate and primal step fit in LDS:
 // y is staged into LDS once, dx never leaves LDS, and each thread carries NR
 // rows at a time (rows are short in this class).
 template <int NT, int NR>
 __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, double tau0, double sigma0,
-                            double *red, double *ylds, double *dxlds, int *accept_flag) {
+                            double *tiles, double *ylds, double *dxlds, int *accept_flag) {
     const int cur = S.cur;
     const double f = S.factor, tau = tau0 * f, sigma = sigma0 * f, w = f / (S.mass + f);
     const double *x = L.x[cur], *xa = L.xa[cur], *y = L.y[cur], *ya = L.ya[cur], *ax = L.ax[cur];
     double *xn = L.x[cur ^ 1], *xan = L.xa[cur ^ 1], *yn = L.y[cur ^ 1], *yan = L.ya[cur ^ 1], *axn = L.ax[cur ^ 1];
     for (int i = threadIdx.x; i < L.m; i += NT) ylds[i] = y[i];
     __syncthreads();
-    Reducer<3, 0, 0> red_acc{};
+    double *tile_dx = tiles, *tile_dy = tiles + kMaxTiles, *tile_cr = tiles + 2 * kMaxTiles;
+    const int wave = threadIdx.x >> 6;
     // g = c + A^T y and the projected primal step (src/cpu.cpp:107-132)
-    for (int base = threadIdx.x; base < L.n; base += NR * NT) {
+    // Passes are uniform over the workgroup (the tile sums need whole waves).
+    for (int b0 = 0; b0 < L.n; b0 += NR * NT) {
+        const int base = b0 + int(threadIdx.x);
         int r[NR], k[NR], e[NR];
         bool ok[NR];
         double g[NR], xj_[NR], xa_[NR], lo_[NR], hi_[NR];
@@ -663,20 +692,26 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
         rows_dot<NR>(k, e, L.ti, L.tv, ylds, g);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-            if (!ok[q]) continue;
-            const int j = r[q];
-            const double xj = xj_[q], v = xa_[q];
-            const double t = clampd(xj - tau * g[q], lo_[q], hi_[q]);
-            const double d = t - xj;
-            red_acc.s[0] += d * d;
-            dxlds[j] = d;
-            xn[j] = t;
-            xan[j] = v + (t - v) * w;
+            double d2 = 0;
+            if (ok[q]) {
+                const int j = r[q];
+                const double xj = xj_[q], v = xa_[q];
+                const double t = clampd(xj - tau * g[q], lo_[q], hi_[q]);
+                const double d = t - xj;
+                d2 = d * d;
+                dxlds[j] = d;
+                xn[j] = t;
+                xan[j] = v + (t - v) * w;
+            }
+            const int tile = (b0 + q * NT) / 64 + wave;
+            d2 = wave_sum(d2);
+            if ((threadIdx.x & 63) == 0 && tile * 64 < L.n) tile_dx[tile] = d2;
         }
     }
     __syncthreads();
     // delta = A dx and the extrapolated dual step (src/cpu.cpp:136-150)
-    for (int base = threadIdx.x; base < L.m; base += NR * NT) {
+    for (int b0 = 0; b0 < L.m; b0 += NR * NT) {
+        const int base = b0 + int(threadIdx.x);
         int r[NR], k[NR], e[NR];
         bool ok[NR];
         double delta[NR], a_[NR], ya_[NR], lo_[NR], hi_[NR];
@@ -696,21 +731,35 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
         rows_dot<NR>(k, e, L.ai, L.av, dxlds, delta);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-            if (!ok[q]) continue;
-            const int i = r[q];
-            const double a = a_[q], yi = ylds[i], v = ya_[q];
-            const double bar = a + 2 * delta[q], z = yi + sigma * bar;
-            const double t = maxd(0., z - sigma * hi_[q]) + mind(0., z - sigma * lo_[q]);
-            const double dy = t - yi;
-            red_acc.s[1] += (!isfinite(bar) || !isfinite(t)) ? INFINITY : dy * dy;
-            red_acc.s[2] += dy * delta[q];
-            yn[i] = t;
-            yan[i] = v + (t - v) * w;
-            axn[i] = a + delta[q];
+            double sq = 0, cr = 0;
+            if (ok[q]) {
+                const int i = r[q];
+                const double a = a_[q], yi = ylds[i], v = ya_[q];
+                const double bar = a + 2 * delta[q], z = yi + sigma * bar;
+                const double t = maxd(0., z - sigma * hi_[q]) + mind(0., z - sigma * lo_[q]);
+                const double dy = t - yi;
+                sq = (!isfinite(bar) || !isfinite(t)) ? INFINITY : dy * dy;
+                cr = dy * delta[q];
+                yn[i] = t;
+                yan[i] = v + (t - v) * w;
+                axn[i] = a + delta[q];
+            }
+            const int tile = (b0 + q * NT) / 64 + wave;
+            sq = wave_sum(sq);
+            cr = wave_sum(cr);
+            if ((threadIdx.x & 63) == 0 && tile * 64 < L.m) {
+                tile_dy[tile] = sq;
+                tile_cr[tile] = cr;
+            }
         }
     }
-    block_reduce_n<NT / 64>(red_acc, red);
-    if (threadIdx.x == 0) decide_batch(S, o, red_acc.s[0], red_acc.s[1], red_acc.s[2], tau, sigma, f, accept_flag);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const double dx2 = tile_total(tile_dx, (L.n + 63) / 64);
+        const double dy2 = tile_total(tile_dy, (L.m + 63) / 64);
+        const double cross = tile_total(tile_cr, (L.m + 63) / 64);
+        if (threadIdx.x == 0) decide_batch(S, o, dx2, dy2, cross, tau, sigma, f, accept_flag);
+    }
     __syncthreads();
 }
 
@@ -721,7 +770,7 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     extern __shared__ double dyn[];
     const int s = slots[blockIdx.x];
     __shared__ LpState S;
-    __shared__ double red[(NT / 64) * 4];
+    __shared__ double tiles[3 * kMaxTiles];
     __shared__ int flag;
     if (threadIdx.x == 0) S = states[s];
     __syncthreads();
@@ -736,7 +785,7 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
     double *ylds = dyn, *dxlds = dyn + L.m;
     for (long long done = 0; done < count;) {
-        attempt_lds<NT, NR>(L, S, o, tau0, sigma0, red, ylds, dxlds, &flag);
+        attempt_lds<NT, NR>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, &flag);
         if (flag == 1) ++done;
         else if (flag < 0) {
             if (threadIdx.x == 0) S.status = kExhausted;
@@ -1380,6 +1429,12 @@ class BatchEngine {
         batch_advance_lds_kernel<NT, NR><<<grid, NT, size_t(kClassDoubles[c]) * 8, cs>>>(d_desc_, d_state_, slots, bo);
     }
     void launch_lds(int c, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
+        if (grid <= kTailLps) {
+            // Few LPs left in this class: the GPU is mostly idle, so each LP
+            // gets a 1024-thread workgroup and needs fewer passes per step.
+            launch_lds_t<kTailThreads, 1>(c, grid, cs, slots, bo);
+            return;
+        }
         const bool wide = c == kClasses - 2;
         switch (rows_per_thread_) {
         case 4:

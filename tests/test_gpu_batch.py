@@ -134,3 +134,22 @@ def test_handoff_to_grid_engine(engine, oracle):
     for k in moved[:2]:
         alone, _ = run([insts[k].lp], 5)
         assert np.array_equal(alone[0]["x"], mix[k]["x"]) and alone[0]["iterations"] == mix[k]["iterations"]
+
+
+def test_launch_shape_does_not_change_results(engine):
+    """A crowded class runs 256/512-thread workgroups with several rows per
+    thread; once few LPs are left it switches to 1024-thread workgroups.  The
+    per-tile sums keep every step's arithmetic independent of that shape, so an
+    LP that changes shape mid-solve matches the same LP run alone throughout in
+    the tail shape, and the rows-per-thread setting changes nothing."""
+    insts = [engine.Instance(2, 5, 1, i) for i in range(900)]
+    mix, stats = solve_batch(engine, [i.lp for i in insts], checks=1)
+    assert stats["launches"] > 2
+    # the smallest class starts with well over 256 LPs
+    assert sum(i.lp.rows + i.lp.cols <= 2048 for i in insts) > 400
+    picks = [0, 101, 257, 511, 899]
+    alone, _ = solve_batch(engine, [insts[k].lp for k in picks])
+    for k, a in zip(picks, alone):
+        assert a["status"] == mix[k]["status"] == "OPTIMAL"
+        assert a["iterations"] == mix[k]["iterations"]
+        assert np.array_equal(a["x"], mix[k]["x"]) and np.array_equal(a["y"], mix[k]["y"])
