@@ -50,13 +50,19 @@ def batch_step(engine, insts, checks, handoff=0):
     """One step: untimed upload, timed solve of the whole batch."""
     b = engine.Batch(arena_bytes=96 << 30, max_slots=len(insts), time_limit=1e9)
     b.set_handoff(handoff, 4)
-    tickets = [b.submit(i.lp) for i in insts]
+    tickets, submitted = [], []
+    for i in insts:
+        tickets.append(b.submit(i.lp))
+        submitted.append(time.perf_counter())
     admitted = b.admit()
     assert admitted == len(insts), "arena could not hold the whole batch"
     t0 = time.perf_counter()
     b.run(checks)
     elapsed = time.perf_counter() - t0
     results = [b.result(t) for t in tickets]
+    # an LP's `seconds` run from its submission; latency counts from the timed start
+    for r, ts in zip(results, submitted):
+        r["latency"] = r["seconds"] - (t0 - ts)
     stats = b.stats()
     b.close()
     solved, latencies = 0, []
@@ -65,7 +71,7 @@ def batch_step(engine, insts, checks, handoff=0):
             abs(r["objective"] - inst.planted_objective) <= 1e-6 * (1 + abs(inst.planted_objective))
         solved += ok
         if ok:
-            latencies.append(r["seconds"])
+            latencies.append(r["latency"])
     return elapsed, solved, results, stats, latencies
 
 
@@ -192,10 +198,11 @@ def serve_stream(engine, qps, duration, seed=SEED, warm_fraction=0.2, hbm_budget
             "hbm_budget_bytes": hbm_budget}
 
 
-def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16):
+def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16, cpu_iterations=None):
     """One LP of config 0 / 1 / 4 on the grid-wide single-LP engine for a fixed
-    iteration budget, beside the reference's CPU PDHG on the same instance and
-    budget (oracle/_ref/ref_driver, `cpu_threads` OpenMP threads)."""
+    iteration budget, beside the reference's CPU PDHG on the same instance
+    (oracle/_ref/ref_driver, `cpu_threads` OpenMP threads, `cpu_iterations`
+    iterations, by default the same budget)."""
     inst = engine.Instance(config, seed, scale)
     lp = inst.lp
     m, n, nnz = lp.rows, lp.cols, lp.nnz
@@ -218,7 +225,7 @@ def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16)
     ref = ROOT / "oracle" / "_ref" / "ref_driver"
     if cpu_threads and ref.exists():
         cmd = [str(ref), "solve", "--config", str(config), "--seed", str(seed), "--scale", str(scale),
-               "--iterations", str(iterations), "--threads", str(cpu_threads), "--tol", str(tol)]
+               "--iterations", str(cpu_iterations or iterations), "--threads", str(cpu_threads), "--tol", str(tol)]
         c = json.loads(subprocess.run(cmd, capture_output=True, text=True, check=True).stdout)
         out["cpu_baseline"] = {"kind": "reference", "threads": cpu_threads, "status": c["status"],
                                "iterations": c["iterations"], "kkt": c["kkt"],
@@ -362,6 +369,7 @@ def main():
     ap.add_argument("--iterations", type=int, default=1000)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-iterations", type=int, default=None)
     ap.add_argument("--qps", type=float, default=1.5)
     ap.add_argument("--duration", type=float, default=40.0)
     args = ap.parse_args()
@@ -395,7 +403,7 @@ def main():
         return
     if args.single is not None:
         print(json.dumps(single_lp(engine, args.single, args.scale, args.iterations, args.tol,
-                                   cpu_threads=args.cpu_threads)), flush=True)
+                                   cpu_threads=args.cpu_threads, cpu_iterations=args.cpu_iterations)), flush=True)
         return
     insts = make_instances(engine, args.count)
     nnz = sum(i.lp.nnz for i in insts)
@@ -424,7 +432,8 @@ def main():
     rows = int(os.environ.get("NIRYUKTI_BATCH_ROWS_PER_THREAD", "2"))
     symbol = [f"batch_advance_lds_kernel<256, {rows}>", f"batch_advance_lds_kernel<256, {rows}>",
               f"batch_advance_lds_kernel<512, {rows}>", "batch_advance_kernel"][dom]
-    kernel_name = symbol + [" (LDS 16 KiB class)", " (LDS 32 KiB class)", " (LDS 62.5 KiB class)", " (global class)"][dom]
+    kernel_name = symbol + [" (class m+n <= 2048, LDS)", " (class m+n <= 4096, LDS)", " (class m+n <= 8000, LDS)",
+                            " (global class)"][dom]
     traffic = committed_traffic(symbol, [0, 1, 0, 0][dom])
     statuses = {}
     for r in last:

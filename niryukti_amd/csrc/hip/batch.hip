@@ -57,6 +57,8 @@ constexpr int kThreads = 256;
 // the last class runs the global-memory stream kernel.
 constexpr int kClasses = 4;
 constexpr int kClassDoubles[kClasses - 1] = {2048, 4096, 8000};
+// Upper bound of the LDS step kernel's static allocation (state, tile sums, flag).
+constexpr int kStaticLds = 4096;
 constexpr int64_t kShortRow = 128;
 // The largest LDS class runs 512-thread workgroups: fewer rows per thread,
 // twice the loads in flight per LP.
@@ -560,12 +562,9 @@ __device__ __forceinline__ double row_dot(const double *val, const int *idx, int
     return acc;
 }
 
-// One PDHG attempt for an LP whose dual iterate and primal step fit in LDS:
-// y is staged into LDS once, dx never leaves LDS, and every row is summed by
-// one thread (rows are short in this class).
-// Dot products of NR CSR rows (one per q, `ok[q]` false for absent rows)
-// against an LDS-resident vector.  The rows advance in lock step, kStep
-// entries per row per step, so up to NR * kStep independent loads are in
+// Dot products of NR CSR rows (entries [k[q], e[q]); absent rows have
+// k == e) against an LDS-resident vector.  The rows advance in lock step,
+// kStep entries per row per step, so up to NR * kStep independent loads are in
 // flight per thread (a row of at most kStep entries costs one round trip);
 // each row is still summed in storage order starting from acc[q].
 constexpr int kStep = 8;
@@ -596,43 +595,6 @@ __device__ __forceinline__ void rows_dot(const int *k, const int *e, const int *
     }
 }
 
-// One PDHG attempt for an LP whose dual iterate and primal step fit in LDS:
-// y is staged into LDS once, dx never leaves LDS, and every row is summed by
-// one thread (rows are short in this class).
-// Dot products of NR CSR rows (one per q, `ok[q]` false for absent rows)
-// against an LDS-resident vector.  The rows advance in lock step, two entries
-// per row per step, so 2*NR independent loads are in flight per thread; each
-// row is still summed in storage order starting from acc[q].
-template <int NR>
-__device__ __forceinline__ void rows_dot(const int *ptr, const int *idx, const double *val, const int *r,
-                                         const bool *ok, const double *vec, double *acc) {
-    int k[NR], e[NR], longest = 0;
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
-        k[q] = ok[q] ? ptr[r[q]] : 0;
-        e[q] = ok[q] ? ptr[r[q] + 1] : 0;
-        longest = max(longest, e[q] - k[q]);
-    }
-    for (int t = 0; t < longest; t += 2) {
-        double v[NR][2];
-        int ix[NR][2];
-#pragma unroll
-        for (int q = 0; q < NR; ++q)
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const int kk = k[q] + t + u;
-                const bool on = kk < e[q];
-                v[q][u] = on ? val[kk] : 0.0;
-                ix[q][u] = on ? idx[kk] : 0;
-            }
-#pragma unroll
-        for (int q = 0; q < NR; ++q)
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-                if (k[q] + t + u < e[q]) acc[q] += v[q][u] * vec[ix[q][u]];
-    }
-}
-
 // Sums over rows are taken per tile of 64 consecutive rows (one wave, fixed
 // butterfly) and the tiles are then added in tile order, so an attempt's
 // energy and coupling do not depend on the workgroup size or rows per thread:
@@ -656,11 +618,14 @@ __device__ __forceinline__ double tile_total(const double *tiles, int count) {
 }
 
 // One PDHG attempt for an LP whose dual iterate and primal step fit in LDS:
-// y is staged into LDS once, dx never leaves LDS, and each thread carries NR
-// rows at a time (rows are short in this class).
+// y is staged into LDS once per attempt, dx never leaves LDS, the CSR offsets
+// of A and A^T sit in LDS for the whole launch (so a row's entries are one
+// memory round trip away, not two), and each thread carries NR rows at a
+// time (rows are short in this class).
 template <int NT, int NR>
 __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, double tau0, double sigma0,
-                            double *tiles, double *ylds, double *dxlds, int *accept_flag) {
+                            double *tiles, double *ylds, double *dxlds, const int *tplds, const int *aplds,
+                            int *accept_flag) {
     const int cur = S.cur;
     const double f = S.factor, tau = tau0 * f, sigma = sigma0 * f, w = f / (S.mass + f);
     const double *x = L.x[cur], *xa = L.xa[cur], *y = L.y[cur], *ya = L.ya[cur], *ax = L.ax[cur];
@@ -681,8 +646,8 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             r[q] = base + q * NT;
             ok[q] = r[q] < L.n;
             const int j = ok[q] ? r[q] : 0;
-            k[q] = ok[q] ? L.tp[j] : 0;
-            e[q] = ok[q] ? L.tp[j + 1] : 0;
+            k[q] = ok[q] ? tplds[j] : 0;
+            e[q] = ok[q] ? tplds[j + 1] : 0;
             g[q] = ok[q] ? L.c[j] : 0.0;
             xj_[q] = x[j];
             xa_[q] = xa[j];
@@ -720,8 +685,8 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             r[q] = base + q * NT;
             ok[q] = r[q] < L.m;
             const int i = ok[q] ? r[q] : 0;
-            k[q] = ok[q] ? L.ap[i] : 0;
-            e[q] = ok[q] ? L.ap[i + 1] : 0;
+            k[q] = ok[q] ? aplds[i] : 0;
+            e[q] = ok[q] ? aplds[i + 1] : 0;
             delta[q] = 0.0;
             a_[q] = ax[i];
             ya_[q] = ya[i];
@@ -784,8 +749,12 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     const long long count = remaining < o.check_every ? remaining : o.check_every;
     const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
     double *ylds = dyn, *dxlds = dyn + L.m;
+    int *tplds = reinterpret_cast<int *>(dyn + L.m + L.n), *aplds = tplds + L.n + 1;
+    for (int j = threadIdx.x; j <= L.n; j += NT) tplds[j] = L.tp[j];
+    for (int i = threadIdx.x; i <= L.m; i += NT) aplds[i] = L.ap[i];
+    __syncthreads();
     for (long long done = 0; done < count;) {
-        attempt_lds<NT, NR>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, &flag);
+        attempt_lds<NT, NR>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, &flag);
         if (flag == 1) ++done;
         else if (flag < 0) {
             if (threadIdx.x == 0) S.status = kExhausted;
@@ -1083,6 +1052,15 @@ class BatchEngine {
         stream_ = pool[0];
         for (int c = 0; c < kClasses; ++c) class_stream_[c] = pool[1 + c % (pool.size() - 1)];
         if (const char *env = std::getenv("NIRYUKTI_BATCH_ROWS_PER_THREAD")) rows_per_thread_ = std::atoi(env);
+        // Class capacities (m + n) bounded by what one workgroup may allocate
+        // next to the kernel's static LDS.
+        int device = 0;
+        HIP_OK(hipGetDevice(&device));
+        hipDeviceProp_t prop;
+        HIP_OK(hipGetDeviceProperties(&prop, device));
+        const long long room = (long long)prop.sharedMemPerBlock - kStaticLds - 16;
+        for (int c = 0; c < kClasses - 1; ++c)
+            class_cap_[c] = int(std::max<long long>(0, std::min<long long>(kClassDoubles[c], room / 12)));
         HIP_OK(hipMalloc(&d_desc_, sizeof(LpDev) * max_slots_));
         HIP_OK(hipMalloc(&d_state_, sizeof(LpState) * max_slots_));
         HIP_OK(hipMalloc(&d_slots_, sizeof(int) * max_slots_ * kClasses));
@@ -1426,7 +1404,7 @@ class BatchEngine {
 
     template <int NT, int NR>
     void launch_lds_t(int c, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
-        batch_advance_lds_kernel<NT, NR><<<grid, NT, size_t(kClassDoubles[c]) * 8, cs>>>(d_desc_, d_state_, slots, bo);
+        batch_advance_lds_kernel<NT, NR><<<grid, NT, lds_bytes(class_cap_[c]), cs>>>(d_desc_, d_state_, slots, bo);
     }
     void launch_lds(int c, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
         if (grid <= kTailLps) {
@@ -1450,7 +1428,11 @@ class BatchEngine {
 
     // Launch class: the smallest LDS capacity that holds y and dx when every
     // row of A and A^T is short; otherwise the global-memory stream kernel.
-    static int lds_class(const Job &j) {
+    // Dynamic LDS of an LP with m + n <= cap: y and dx (doubles) and the two
+    // offset arrays (n + 1 and m + 1 ints).
+    static size_t lds_bytes(int cap) { return size_t(cap) * 12 + 16; }
+
+    int lds_class(const Job &j) const {
         const nyk::Model &m = j.prep.model;
         int64_t longest = 0;
         for (int64_t i = 0; i < m.rows(); ++i) longest = std::max(longest, m.A.ptr[i + 1] - m.A.ptr[i]);
@@ -1459,7 +1441,7 @@ class BatchEngine {
         const int64_t need = m.rows() + m.cols();
         if (longest <= kShortRow)
             for (int c = 0; c < kClasses - 1; ++c)
-                if (need <= kClassDoubles[c]) return c;
+                if (need <= class_cap_[c]) return c;
         return kClasses - 1;
     }
 
@@ -1949,6 +1931,7 @@ class BatchEngine {
     std::vector<double> slot_bytes_;
     hipStream_t class_stream_[kClasses] = {};
     int rows_per_thread_ = 2;  // measured best on gfx950 for the 3-8 nnz/column batch
+    int class_cap_[kClasses - 1] = {};
     std::vector<hipEvent_t> class_events_[kClasses];
     hipEvent_t uploaded_ = nullptr;
     std::vector<int> slot_class_;
