@@ -423,18 +423,19 @@ def main():
     value = total_solved / total_t
     # Dominant kernel: the launch class (one step-kernel instantiation) that
     # moved the most algorithmic bytes; its rate = bytes / its event time.
-    cls_b = [sum(st["class_bytes"][c] for st in stats_all) for c in range(4)]
-    cls_s = [sum(st["class_seconds"][c] for st in stats_all) for c in range(4)]
-    cls_n = [sum(st["class_launches"][c] for st in stats_all) for c in range(4)]
-    dom = max(range(4), key=lambda c: cls_b[c])
+    groups = len(stats_all[0]["class_bytes"])
+    cls_b = [sum(st["class_bytes"][c] for st in stats_all) for c in range(groups)]
+    cls_s = [sum(st["class_seconds"][c] for st in stats_all) for c in range(groups)]
+    cls_n = [sum(st["class_launches"][c] for st in stats_all) for c in range(groups)]
+    dom = max(range(groups), key=lambda c: cls_b[c])
     adv_b, adv_s, launches = cls_b[dom], cls_s[dom], cls_n[dom]
     achieved = adv_b / adv_s / 1e9 if adv_s > 0 else 0.0
     rows = int(os.environ.get("NIRYUKTI_BATCH_ROWS_PER_THREAD", "2"))
     symbol = [f"batch_advance_lds_kernel<256, {rows}>", f"batch_advance_lds_kernel<256, {rows}>",
-              f"batch_advance_lds_kernel<512, {rows}>", "batch_advance_kernel"][dom]
+              f"batch_advance_lds_kernel<512, {rows}>", "batch_advance_kernel", "batch_advance_lds_kernel<1024, 1>"][dom]
     kernel_name = symbol + [" (class m+n <= 2048, LDS)", " (class m+n <= 4096, LDS)", " (class m+n <= 8000, LDS)",
-                            " (global class)"][dom]
-    traffic = committed_traffic(symbol, [0, 1, 0, 0][dom])
+                            " (global class)", " (tail group)"][dom]
+    traffic = committed_traffic(symbol, [0, 1, 0, 0, 0][dom])
     statuses = {}
     for r in last:
         statuses[r["status"]] = statuses.get(r["status"], 0) + 1

@@ -35,12 +35,13 @@ typedef struct {
     double check_seconds;   /* HIP-event time of the candidate-check kernel launches */
     double advance_bytes;   /* algorithmic HBM bytes of all PDHG attempts run by the step kernel */
     int64_t attempts;       /* PDHG attempts (accepted + rejected) run by the step kernel */
-    /* Per launch class (LPs with m + n <= 2048 / 4096 / 8000 whose y, dx and
-     * CSR offsets live in LDS, then the global-memory class): step-kernel
-     * event time, algorithmic bytes and number of step-kernel launches.
-     * Classes run on separate streams. */
-    double class_seconds[4], class_bytes[4];
-    int64_t class_launches[4];
+    /* Per launch group: the classes of LPs with m + n <= 2048 / 4096 / 8000
+     * whose y, dx and CSR offsets live in LDS, the global-memory class, and
+     * the tail group (LDS classes with at most 256 LPs left, merged into one
+     * 1024-thread launch): step-kernel event time, algorithmic bytes and
+     * number of step-kernel launches.  Groups run on separate streams. */
+    double class_seconds[5], class_bytes[5];
+    int64_t class_launches[5];
     int64_t handoffs;         /* LPs moved to the grid-wide engine so far */
     int64_t handoffs_running; /* of those, still running there */
 } niryukti_batch_stats;

@@ -91,8 +91,8 @@ class _BatchStats(C.Structure):
                                          "arena_used_bytes", "launches", "host_certifications",
                                          "host_farkas_checks")] + \
         [(k, C.c_double) for k in ("device_seconds", "advance_seconds", "check_seconds", "advance_bytes")] + \
-        [("attempts", C.c_int64), ("class_seconds", C.c_double * 4), ("class_bytes", C.c_double * 4),
-         ("class_launches", C.c_int64 * 4), ("handoffs", C.c_int64), ("handoffs_running", C.c_int64)]
+        [("attempts", C.c_int64), ("class_seconds", C.c_double * 5), ("class_bytes", C.c_double * 5),
+         ("class_launches", C.c_int64 * 5), ("handoffs", C.c_int64), ("handoffs_running", C.c_int64)]
 
 
 _lib = None

@@ -63,8 +63,8 @@ constexpr int64_t kShortRow = 128;
 // The largest LDS class runs 512-thread workgroups: fewer rows per thread,
 // twice the loads in flight per LP.
 constexpr int kWideThreads = 512;
-// Launches with at most kTailLps LPs in a class (one per CU or fewer) run
-// kTailThreads-thread workgroups with one row per thread.
+// LDS classes with at most kTailLps LPs left (one per CU or fewer) are merged
+// into one tail launch of kTailThreads-thread workgroups, one row per thread.
 constexpr int kTailLps = 256;
 constexpr int kTailThreads = 1024;
 constexpr int kWaves = kThreads / 64;
@@ -994,7 +994,7 @@ struct Job {
 // kept for the life of the process (default priority).
 std::vector<hipStream_t> &stream_pool() {
     static std::vector<hipStream_t> pool = [] {
-        std::vector<hipStream_t> p(1 + kClasses);
+        std::vector<hipStream_t> p(2 + kClasses);
         for (auto &st : p) HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         return p;
     }();
@@ -1050,7 +1050,7 @@ class BatchEngine {
         // engines keep the same stream-to-hardware-queue mapping.
         auto &pool = stream_pool();
         stream_ = pool[0];
-        for (int c = 0; c < kClasses; ++c) class_stream_[c] = pool[1 + c % (pool.size() - 1)];
+        for (int c = 0; c <= kClasses; ++c) class_stream_[c] = pool[1 + c];
         if (const char *env = std::getenv("NIRYUKTI_BATCH_ROWS_PER_THREAD")) rows_per_thread_ = std::atoi(env);
         // Class capacities (m + n) bounded by what one workgroup may allocate
         // next to the kernel's static LDS.
@@ -1063,7 +1063,7 @@ class BatchEngine {
             class_cap_[c] = int(std::max<long long>(0, std::min<long long>(kClassDoubles[c], room / 12)));
         HIP_OK(hipMalloc(&d_desc_, sizeof(LpDev) * max_slots_));
         HIP_OK(hipMalloc(&d_state_, sizeof(LpState) * max_slots_));
-        HIP_OK(hipMalloc(&d_slots_, sizeof(int) * max_slots_ * kClasses));
+        HIP_OK(hipMalloc(&d_slots_, sizeof(int) * max_slots_ * (kClasses + 1)));
         HIP_OK(hipEventCreate(&uploaded_));
 
         desc_.resize(max_slots_);
@@ -1071,6 +1071,7 @@ class BatchEngine {
         slot_job_.assign(max_slots_, -1);
         slot_bytes_.assign(max_slots_, 0);
         slot_class_.assign(max_slots_, kClasses - 1);
+        slot_group_.assign(max_slots_, kClasses - 1);
         for (int s = max_slots_ - 1; s >= 0; --s) free_slots_.push_back(s);
     }
     ~BatchEngine() {
@@ -1142,32 +1143,54 @@ class BatchEngine {
                 dirty_ = false;
             }
             HIP_OK(hipMemcpyAsync(d_state_, state_.data(), sizeof(LpState) * max_slots_, hipMemcpyHostToDevice, stream_));
-            for (int c = 0; c < kClasses; ++c)
-                if (!by_class[c].empty())
-                    HIP_OK(hipMemcpyAsync(d_slots_ + size_t(c) * max_slots_, by_class[c].data(),
-                                          sizeof(int) * by_class[c].size(), hipMemcpyHostToDevice, stream_));
+            // Launch groups: each crowded LDS class, the global class, and one
+            // tail group that gathers every LDS class with few LPs left, so a
+            // handful of slow LPs share one launch per check period instead
+            // of one per class (launches of different classes can land on the
+            // same hardware queue and then run one after another).
+            struct Group {
+                int kind;  // 0..kClasses-1: that class; kClasses: the tail group
+                int cap;
+                std::vector<int> slots;
+            };
+            std::vector<Group> groups;
+            Group tail{kClasses, 0, {}};
+            for (int c = 0; c < kClasses; ++c) {
+                if (by_class[c].empty()) continue;
+                if (c < kClasses - 1 && int(by_class[c].size()) <= kTailLps) {
+                    tail.cap = std::max(tail.cap, class_cap_[c]);
+                    tail.slots.insert(tail.slots.end(), by_class[c].begin(), by_class[c].end());
+                } else {
+                    groups.push_back(Group{c, c < kClasses - 1 ? class_cap_[c] : 0, by_class[c]});
+                }
+            }
+            if (!tail.slots.empty()) groups.push_back(std::move(tail));
+            for (const Group &g : groups)
+                HIP_OK(hipMemcpyAsync(d_slots_ + size_t(g.kind) * max_slots_, g.slots.data(), sizeof(int) * g.slots.size(),
+                                      hipMemcpyHostToDevice, stream_));
             HIP_OK(hipEventRecord(uploaded_, stream_));
             BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0, handoff_};
             auto t0 = Clock::now();
             std::vector<long long> before(active.size());
             for (size_t k = 0; k < active.size(); ++k)
                 before[k] = state_[active[k]].accepted + state_[active[k]].rejected;
-            for (int c = 0; c < kClasses; ++c) {
-                if (by_class[c].empty()) continue;
-                auto &ev = class_events_[c];
+            for (const Group &g : groups) {
+                auto &ev = class_events_[g.kind];
                 while (ev.size() < size_t(2 * checks + 2)) {
                     hipEvent_t e;
                     HIP_OK(hipEventCreate(&e));
                     ev.push_back(e);
                 }
-                hipStream_t cs = class_stream_[c];
+                hipStream_t cs = class_stream_[g.kind];
                 HIP_OK(hipStreamWaitEvent(cs, uploaded_, 0));
-                const int grid = int(by_class[c].size());
-                const int *slots = d_slots_ + size_t(c) * max_slots_;
+                const int grid = int(g.slots.size());
+                const int *slots = d_slots_ + size_t(g.kind) * max_slots_;
                 HIP_OK(hipEventRecord(ev[0], cs));
                 for (int k = 0; k < checks; ++k) {
-                    if (c < kClasses - 1)
-                        launch_lds(c, grid, cs, slots, bo);
+                    if (g.kind == kClasses)
+                        launch_lds_t<kTailThreads, 1>(g.cap, grid, cs, slots, bo);
+                    else if (g.kind < kClasses - 1)
+                        launch_lds(g.cap, g.kind == kClasses - 2, grid, cs, slots, bo);
                     else
                         batch_advance_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
                     HIP_OK(hipEventRecord(ev[2 * k + 1], cs));
@@ -1179,25 +1202,25 @@ class BatchEngine {
             }
             HIP_OK(hipMemcpyAsync(state_.data(), d_state_, sizeof(LpState) * max_slots_, hipMemcpyDeviceToHost, stream_));
             HIP_OK(hipStreamSynchronize(stream_));
-            for (int c = 0; c < kClasses; ++c) {
-                if (by_class[c].empty()) continue;
-                auto &ev = class_events_[c];
+            for (const Group &g : groups) {
+                auto &ev = class_events_[g.kind];
                 for (int k = 0; k < checks; ++k) {
                     float a = 0, b = 0;
                     HIP_OK(hipEventElapsedTime(&a, ev[2 * k], ev[2 * k + 1]));
                     HIP_OK(hipEventElapsedTime(&b, ev[2 * k + 1], ev[2 * k + 2]));
                     advance_seconds_ += a * 1e-3;
                     check_seconds_ += b * 1e-3;
-                    class_seconds_[c] += a * 1e-3;
-                    class_launches_[c] += 1;
+                    class_seconds_[g.kind] += a * 1e-3;
+                    class_launches_[g.kind] += 1;
                 }
+                for (int s : g.slots) slot_group_[s] = g.kind;
             }
             for (size_t k = 0; k < active.size(); ++k) {
                 const LpState &st = state_[active[k]];
                 const long long done = st.accepted + st.rejected - before[k];
                 attempts_ += done;
                 advance_bytes_ += double(done) * slot_bytes_[active[k]];
-                class_bytes_[slot_class_[active[k]]] += double(done) * slot_bytes_[active[k]];
+                class_bytes_[slot_group_[active[k]]] += double(done) * slot_bytes_[active[k]];
             }
             device_seconds_ += std::chrono::duration<double>(Clock::now() - t0).count();
             ++launches_;
@@ -1296,7 +1319,7 @@ class BatchEngine {
         s->check_seconds = check_seconds_;
         s->advance_bytes = advance_bytes_;
         s->attempts = attempts_;
-        for (int c = 0; c < kClasses; ++c) {
+        for (int c = 0; c <= kClasses; ++c) {
             s->class_seconds[c] = class_seconds_[c];
             s->class_bytes[c] = class_bytes_[c];
             s->class_launches[c] = class_launches_[c];
@@ -1403,31 +1426,24 @@ class BatchEngine {
     }
 
     template <int NT, int NR>
-    void launch_lds_t(int c, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
-        batch_advance_lds_kernel<NT, NR><<<grid, NT, lds_bytes(class_cap_[c]), cs>>>(d_desc_, d_state_, slots, bo);
+    void launch_lds_t(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
+        batch_advance_lds_kernel<NT, NR><<<grid, NT, lds_bytes(cap), cs>>>(d_desc_, d_state_, slots, bo);
     }
-    void launch_lds(int c, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
-        if (grid <= kTailLps) {
-            // Few LPs left in this class: the GPU is mostly idle, so each LP
-            // gets a 1024-thread workgroup and needs fewer passes per step.
-            launch_lds_t<kTailThreads, 1>(c, grid, cs, slots, bo);
-            return;
-        }
-        const bool wide = c == kClasses - 2;
+    // A crowded LDS class: 256-thread workgroups (512 for the largest class)
+    // with rows_per_thread_ rows per thread.
+    void launch_lds(int cap, bool wide, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
         switch (rows_per_thread_) {
         case 4:
-            wide ? launch_lds_t<kWideThreads, 4>(c, grid, cs, slots, bo) : launch_lds_t<kThreads, 4>(c, grid, cs, slots, bo);
+            wide ? launch_lds_t<kWideThreads, 4>(cap, grid, cs, slots, bo) : launch_lds_t<kThreads, 4>(cap, grid, cs, slots, bo);
             break;
         case 2:
-            wide ? launch_lds_t<kWideThreads, 2>(c, grid, cs, slots, bo) : launch_lds_t<kThreads, 2>(c, grid, cs, slots, bo);
+            wide ? launch_lds_t<kWideThreads, 2>(cap, grid, cs, slots, bo) : launch_lds_t<kThreads, 2>(cap, grid, cs, slots, bo);
             break;
         default:
-            wide ? launch_lds_t<kWideThreads, 1>(c, grid, cs, slots, bo) : launch_lds_t<kThreads, 1>(c, grid, cs, slots, bo);
+            wide ? launch_lds_t<kWideThreads, 1>(cap, grid, cs, slots, bo) : launch_lds_t<kThreads, 1>(cap, grid, cs, slots, bo);
         }
     }
 
-    // Launch class: the smallest LDS capacity that holds y and dx when every
-    // row of A and A^T is short; otherwise the global-memory stream kernel.
     // Dynamic LDS of an LP with m + n <= cap: y and dx (doubles) and the two
     // offset arrays (n + 1 and m + 1 ints).
     static size_t lds_bytes(int cap) { return size_t(cap) * 12 + 16; }
@@ -1929,15 +1945,15 @@ class BatchEngine {
     std::vector<LpState> state_;
     std::vector<int> slot_job_, free_slots_;
     std::vector<double> slot_bytes_;
-    hipStream_t class_stream_[kClasses] = {};
+    hipStream_t class_stream_[kClasses + 1] = {};  // per class, then the tail group
     int rows_per_thread_ = 2;  // measured best on gfx950 for the 3-8 nnz/column batch
     int class_cap_[kClasses - 1] = {};
-    std::vector<hipEvent_t> class_events_[kClasses];
+    std::vector<hipEvent_t> class_events_[kClasses + 1];
     hipEvent_t uploaded_ = nullptr;
-    std::vector<int> slot_class_;
+    std::vector<int> slot_class_, slot_group_;
     double advance_seconds_ = 0, check_seconds_ = 0, advance_bytes_ = 0;
-    double class_seconds_[kClasses] = {}, class_bytes_[kClasses] = {};
-    int64_t class_launches_[kClasses] = {};
+    double class_seconds_[kClasses + 1] = {}, class_bytes_[kClasses + 1] = {};  // per class, then the tail group
+    int64_t class_launches_[kClasses + 1] = {};
     int64_t attempts_ = 0;
     std::vector<std::unique_ptr<Job>> jobs_;
     std::deque<int64_t> queue_, done_;
