@@ -97,6 +97,9 @@ struct LpDev {
     double *ox[2], *oy[2], *act, *aty, *atyabs;
     double *bx, *by;
     const int *ablk, *tblk, *oblk, *otblk;  // row blocks of A, A^T, original A, original A^T
+    // A and A^T in 64-row slices for the LDS step kernel (see rows_dot_sell)
+    const int *asb, *asi, *tsb, *tsi;
+    const double *asv, *tsv;
     int nab, ntb, nob, notb;
 };
 
@@ -595,6 +598,63 @@ __device__ __forceinline__ void rows_dot(const int *k, const int *e, const int *
     }
 }
 
+// LpDev's pointers are loaded from memory, so the compiler cannot tell they
+// point to global memory and emits flat accesses; a flat load also counts
+// against the LDS wait counter, which makes every LDS wait drain all global
+// loads in flight.  The step kernel casts them back to global pointers.
+#define GLOBAL_AS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const GLOBAL_AS T *gin(const T *p) {
+    return (const GLOBAL_AS T *)p;
+}
+template <class T>
+__device__ __forceinline__ GLOBAL_AS T *gout(T *p) {
+    return (GLOBAL_AS T *)p;
+}
+
+// Dot products like rows_dot, read from the sliced layout: the rows of A (or
+// A^T) are cut into slices of 64 consecutive rows, one per wave, and a slice
+// stores its entries entry-major (entry u of its 64 rows are 64 consecutive
+// values, padded to the slice's longest row).  A wave's load of entry u is
+// then one contiguous 512-byte request instead of 64 scattered ones: the step
+// kernel is bound by the per-CU rate of memory requests, not by bytes.  Each
+// row is still summed in storage order, so the sums equal rows_dot's.
+template <int NR>
+__device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const GLOBAL_AS int *sb,
+                                              const GLOBAL_AS int *si, const GLOBAL_AS double *sv, int slices, int b0,
+                                              int nt, const double *vec, double *acc) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int base[NR], len[NR], width = 0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        const int s = (b0 + q * nt) / 64 + wave;  // the same for the whole wave
+        base[q] = 0;
+        len[q] = e[q] - k[q];
+        if (s < slices) {
+            const int b = sb[s];
+            base[q] = b + lane;
+            width = max(width, (sb[s + 1] - b) / 64);
+        }
+    }
+    for (int t = 0; t < width; t += kStep) {
+        double v[NR][kStep];
+        int ix[NR][kStep];
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+#pragma unroll
+            for (int u = 0; u < kStep; ++u) {
+                const bool on = t + u < len[q];
+                v[q][u] = on ? sv[base[q] + (t + u) * 64] : 0.0;
+                ix[q][u] = on ? si[base[q] + (t + u) * 64] : 0;
+            }
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+#pragma unroll
+            for (int u = 0; u < kStep; ++u)
+                if (t + u < len[q]) acc[q] += v[q][u] * vec[ix[q][u]];
+    }
+}
+
 // Sums over rows are taken per tile of 64 consecutive rows (one wave, fixed
 // butterfly) and the tiles are then added in tile order, so an attempt's
 // energy and coupling do not depend on the workgroup size or rows per thread:
@@ -628,8 +688,11 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
                             int *accept_flag) {
     const int cur = S.cur;
     const double f = S.factor, tau = tau0 * f, sigma = sigma0 * f, w = f / (S.mass + f);
-    const double *x = L.x[cur], *xa = L.xa[cur], *y = L.y[cur], *ya = L.ya[cur], *ax = L.ax[cur];
-    double *xn = L.x[cur ^ 1], *xan = L.xa[cur ^ 1], *yn = L.y[cur ^ 1], *yan = L.ya[cur ^ 1], *axn = L.ax[cur ^ 1];
+    const GLOBAL_AS double *x = gin(L.x[cur]), *xa = gin(L.xa[cur]), *y = gin(L.y[cur]), *ya = gin(L.ya[cur]),
+                           *ax = gin(L.ax[cur]);
+    GLOBAL_AS double *xn = gout(L.x[cur ^ 1]), *xan = gout(L.xa[cur ^ 1]), *yn = gout(L.y[cur ^ 1]),
+                     *yan = gout(L.ya[cur ^ 1]), *axn = gout(L.ax[cur ^ 1]);
+    const GLOBAL_AS double *c = gin(L.c), *lb = gin(L.lb), *ub = gin(L.ub), *rl = gin(L.rl), *ru = gin(L.ru);
     for (int i = threadIdx.x; i < L.m; i += NT) ylds[i] = y[i];
     __syncthreads();
     double *tile_dx = tiles, *tile_dy = tiles + kMaxTiles, *tile_cr = tiles + 2 * kMaxTiles;
@@ -648,13 +711,13 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             const int j = ok[q] ? r[q] : 0;
             k[q] = ok[q] ? tplds[j] : 0;
             e[q] = ok[q] ? tplds[j + 1] : 0;
-            g[q] = ok[q] ? L.c[j] : 0.0;
+            g[q] = ok[q] ? c[j] : 0.0;
             xj_[q] = x[j];
             xa_[q] = xa[j];
-            lo_[q] = L.lb[j];
-            hi_[q] = L.ub[j];
+            lo_[q] = lb[j];
+            hi_[q] = ub[j];
         }
-        rows_dot<NR>(k, e, L.ti, L.tv, ylds, g);
+        rows_dot_sell<NR>(k, e, gin(L.tsb), gin(L.tsi), gin(L.tsv), (L.n + 63) / 64, b0, NT, ylds, g);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             double d2 = 0;
@@ -690,10 +753,10 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             delta[q] = 0.0;
             a_[q] = ax[i];
             ya_[q] = ya[i];
-            lo_[q] = L.rl[i];
-            hi_[q] = L.ru[i];
+            lo_[q] = rl[i];
+            hi_[q] = ru[i];
         }
-        rows_dot<NR>(k, e, L.ai, L.av, dxlds, delta);
+        rows_dot_sell<NR>(k, e, gin(L.asb), gin(L.asi), gin(L.asv), (L.m + 63) / 64, b0, NT, dxlds, delta);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             double sq = 0, cr = 0;
@@ -750,8 +813,9 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
     double *ylds = dyn, *dxlds = dyn + L.m;
     int *tplds = reinterpret_cast<int *>(dyn + L.m + L.n), *aplds = tplds + L.n + 1;
-    for (int j = threadIdx.x; j <= L.n; j += NT) tplds[j] = L.tp[j];
-    for (int i = threadIdx.x; i <= L.m; i += NT) aplds[i] = L.ap[i];
+    const GLOBAL_AS int *tp = gin(L.tp), *ap = gin(L.ap);
+    for (int j = threadIdx.x; j <= L.n; j += NT) tplds[j] = tp[j];
+    for (int i = threadIdx.x; i <= L.m; i += NT) aplds[i] = ap[i];
     __syncthreads();
     for (long long done = 0; done < count;) {
         attempt_lds<NT, NR>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, &flag);
@@ -1024,6 +1088,33 @@ std::vector<int64_t> t_ptr(const nyk::Model &m) {
     for (auto j : m.A.index) p[j + 1]++;
     for (int64_t j = 0; j < m.cols(); ++j) p[j + 1] += p[j];
     return p;
+}
+
+// A CSR matrix in 64-row slices (rows_dot_sell): slice s holds its rows'
+// entries entry-major from base[s], padded to the slice's longest row.
+struct Sliced {
+    std::vector<int64_t> base, index;
+    std::vector<double> value;
+};
+
+Sliced slice_rows(const std::vector<int64_t> &ptr, const std::vector<int64_t> &index, const std::vector<double> &value) {
+    const int64_t rows = int64_t(ptr.size()) - 1, slices = (rows + 63) / 64;
+    Sliced z;
+    z.base.assign(slices + 1, 0);
+    for (int64_t sl = 0; sl < slices; ++sl) {
+        int64_t width = 0;
+        for (int64_t r = sl * 64; r < std::min(rows, sl * 64 + 64); ++r) width = std::max(width, ptr[r + 1] - ptr[r]);
+        z.base[sl + 1] = z.base[sl] + width * 64;
+    }
+    z.index.assign(z.base[slices], 0);
+    z.value.assign(z.base[slices], 0.0);
+    for (int64_t r = 0; r < rows; ++r)
+        for (int64_t u = 0; u < ptr[r + 1] - ptr[r]; ++u) {
+            const int64_t at = z.base[r / 64] + u * 64 + r % 64;
+            z.index[at] = index[ptr[r] + u];
+            z.value[at] = value[ptr[r] + u];
+        }
+    return z;
 }
 
 // Builds one LP's device image: every array at a 256-byte aligned offset.
@@ -1542,6 +1633,10 @@ class BatchEngine {
         const auto ba = row_blocks(m.A.ptr), bt = row_blocks(t_ptr(m)), bo = row_blocks(m0.A.ptr), bot = row_blocks(t0.ptr);
         size_t o_ab = L.add<int>(ba.size()), o_tb = L.add<int>(bt.size()), o_ob = L.add<int>(bo.size()),
                o_otb = L.add<int>(bot.size());
+        nyk::Csr t = m.A.transposed();
+        const Sliced sa = slice_rows(m.A.ptr, m.A.index, m.A.value), st = slice_rows(t.ptr, t.index, t.value);
+        size_t o_asb = L.add<int>(sa.base.size()), o_asi = L.add<int>(sa.index.size()), o_asv = L.add<double>(sa.value.size());
+        size_t o_tsb = L.add<int>(st.base.size()), o_tsi = L.add<int>(st.index.size()), o_tsv = L.add<double>(st.value.size());
         if (!base) return L.total;
         image.assign(L.total, 0);
         auto put_i = [&](size_t off, const std::vector<int64_t> &v) {
@@ -1551,7 +1646,6 @@ class BatchEngine {
         auto put_d = [&](size_t off, const std::vector<double> &v) {
             if (!v.empty()) std::memcpy(image.data() + off, v.data(), v.size() * 8);
         };
-        nyk::Csr t = m.A.transposed();
         put_i(o_ap, m.A.ptr);
         put_i(o_ai, m.A.index);
         put_d(o_av, m.A.value);
@@ -1652,6 +1746,18 @@ class BatchEngine {
         d.tblk = (const int *)P(o_tb);
         d.oblk = (const int *)P(o_ob);
         d.otblk = (const int *)P(o_otb);
+        put_i(o_asb, sa.base);
+        put_i(o_asi, sa.index);
+        put_d(o_asv, sa.value);
+        put_i(o_tsb, st.base);
+        put_i(o_tsi, st.index);
+        put_d(o_tsv, st.value);
+        d.asb = (const int *)P(o_asb);
+        d.asi = (const int *)P(o_asi);
+        d.asv = (const double *)P(o_asv);
+        d.tsb = (const int *)P(o_tsb);
+        d.tsi = (const int *)P(o_tsi);
+        d.tsv = (const double *)P(o_tsv);
         d.nab = int(ba.size()) - 1;
         d.ntb = int(bt.size()) - 1;
         d.nob = int(bo.size()) - 1;
