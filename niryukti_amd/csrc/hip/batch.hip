@@ -255,11 +255,27 @@ __device__ void block_reduce_n(Reducer<NS, NM, ND> &r, double *lds) {
 // so the row values are bit-identical to the CPU's).  A row longer than
 // kChunk forms its own block and is reduced by the workgroup in a fixed tree.
 // epi(r, sum, abs_sum) consumes each row.
+// LpDev's pointers are loaded from memory, so the compiler cannot tell they
+// point to global memory and emits flat accesses; a flat load also counts
+// against the LDS wait counter, which makes every LDS wait drain all global
+// loads in flight.  Device code casts them back to global pointers.
+#define GLOBAL_AS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const GLOBAL_AS T *gin(const T *p) {
+    return (const GLOBAL_AS T *)p;
+}
+template <class T>
+__device__ __forceinline__ GLOBAL_AS T *gout(T *p) {
+    return (GLOBAL_AS T *)p;
+}
+
 constexpr int kChunk = 2048;
 
 template <bool ABS, class Init, class Epi>
-__device__ void stream_rows(const int *ptr, const int *idx, const double *val, const int *blk, int nblk,
-                            const double *vec, double *stage, double *red, Init init, Epi epi) {
+__device__ void stream_rows(const int *ptr_, const int *idx_, const double *val_, const int *blk_, int nblk,
+                            const double *vec_, double *stage, double *red, Init init, Epi epi) {
+    const GLOBAL_AS int *ptr = gin(ptr_), *idx = gin(idx_), *blk = gin(blk_);
+    const GLOBAL_AS double *val = gin(val_), *vec = gin(vec_);
     for (int b = 0; b < nblk; ++b) {
         const int r0 = blk[b], r1 = blk[b + 1];
         const int k0 = ptr[r0], k1 = ptr[r1];
@@ -296,10 +312,13 @@ __device__ void spmv(const int *ptr, const int *idx, const double *val, const in
                        [&](int r, double v, double) { out[r] = v; });
 }
 __device__ void copy_vec(double *dst, const double *src, int n) {
-    for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = src[i];
+    GLOBAL_AS double *d = gout(dst);
+    const GLOBAL_AS double *v = gin(src);
+    for (int i = threadIdx.x; i < n; i += kThreads) d[i] = v[i];
 }
 __device__ void zero_vec(double *dst, int n) {
-    for (int i = threadIdx.x; i < n; i += kThreads) dst[i] = 0;
+    GLOBAL_AS double *d = gout(dst);
+    for (int i = threadIdx.x; i < n; i += kThreads) d[i] = 0;
 }
 
 // Original-space accuracy of one candidate (the non-certifying evaluate of
@@ -326,10 +345,10 @@ __device__ __forceinline__ double phi(double g, double l, double u) {
 template <int NS, int NM, int ND>
 __device__ void farkas_columns(const LpDev &L, Reducer<NS, NM, ND> &r) {
     for (int j = threadIdx.x; j < L.n0; j += kThreads) {
-        const double g = L.aty[j], ga = L.atyabs[j];
-        const int len = L.otp[j + 1] - L.otp[j];
+        const double g = gin(L.aty)[j], ga = gin(L.atyabs)[j];
+        const int len = gin(L.otp)[j + 1] - gin(L.otp)[j];
         const double e = double(len + 4) * 2 * kEps * ga;
-        const double l = L.lb0[j], u = L.ub0[j];
+        const double l = gin(L.lb0)[j], u = gin(L.ub0)[j];
         double ub = maxd(phi(g - e, l, u), phi(g + e, l, u));
         if (g - e <= 0 && g + e >= 0) ub = maxd(ub, 0.0);
         if (ub == -INFINITY) r.mx[0] = 1;
@@ -348,17 +367,17 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
     copy_vec(ox, L.fixed, L.n0);
     zero_vec(oy, L.m0);
     __syncthreads();
-    for (int j = threadIdx.x; j < L.n; j += kThreads) ox[L.cols[j]] = x[j] * L.dc[j];
+    for (int j = threadIdx.x; j < L.n; j += kThreads) ox[gin(L.cols)[j]] = x[j] * gin(L.dc)[j];
     for (int i = threadIdx.x; i < L.m; i += kThreads) {
-        const int t = y[i] >= 0 ? L.upper_rows[i] : L.lower_rows[i];
-        oy[t] = 0.0 + y[i] * L.dr[i] * L.sobj;
+        const int t = y[i] >= 0 ? gin(L.upper_rows)[i] : gin(L.lower_rows)[i];
+        oy[t] = 0.0 + y[i] * gin(L.dr)[i] * L.sobj;
     }
     __syncthreads();
     spmv(L.op, L.oi, L.ov, L.oblk, L.nob, ox, L.act, stage, lds);
     stream_rows<true>(L.otp, L.oti, L.otv, L.otblk, L.notb, oy, stage, lds, [](int) { return 0.0; },
                       [&](int j, double v, double a) {
-                          L.aty[j] = v;
-                          L.atyabs[j] = a;
+                          gout(L.aty)[j] = v;
+                          gout(L.atyabs)[j] = a;
                       });
     __syncthreads();
     // s : objective, Farkas column bound sum, its |.| sum, Lipschitz slack, sum |y b|
@@ -367,7 +386,7 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
     // d : support, complement, box_min, sum y b over rows
     Reducer<5, 8, 4> r{};
     for (int i = threadIdx.x; i < L.m0; i += kThreads) {
-        const double a = L.act[i], yi = oy[i], lo = L.rl0[i], hi = L.ru0[i];
+        const double a = gin(L.act)[i], yi = oy[i], lo = gin(L.rl0)[i], hi = gin(L.ru0)[i];
         if (!isfinite(a) || !isfinite(yi)) r.mx[0] = 1;
         const double v = maxd(maxd(0., lo - a), a - hi);
         if (isfinite(lo)) r.mx[2] = maxd(r.mx[2], maxd(0., lo - a) / (1 + fabs(lo)));
@@ -388,13 +407,13 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
         }
     }
     for (int j = threadIdx.x; j < L.n0; j += kThreads) {
-        const double xj = ox[j], l = L.lb0[j], u = L.ub0[j], c = L.c0[j];
+        const double xj = ox[j], l = gin(L.lb0)[j], u = gin(L.ub0)[j], c = gin(L.c0)[j];
         if (!isfinite(xj)) r.mx[0] = 1;
         const double v = maxd(maxd(0., l - xj), xj - u);
         r.mx[2] = maxd(r.mx[2], v / (1 + fabs(xj)));
         r.mx[3] = maxd(r.mx[3], v);
         r.s[0] += c * xj;
-        const double linear = c + L.aty[j];
+        const double linear = c + gin(L.aty)[j];
         if (!isfinite(linear)) r.mx[0] = 1;
         const double d = clampd(linear, xj - u, xj - l);
         r.mx[5] = maxd(r.mx[5], fabs(d));
@@ -405,8 +424,8 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
         if (!isfinite(arg)) r.mx[7] = 1;  // no finite Lagrangian box bound
         else dd_add(r.d[2], linear * arg);
         // Farkas column bound for the candidate's dual as a ray
-        const double g = L.aty[j], ga = L.atyabs[j];
-        const int len = L.otp[j + 1] - L.otp[j];
+        const double g = gin(L.aty)[j], ga = gin(L.atyabs)[j];
+        const int len = gin(L.otp)[j + 1] - gin(L.otp)[j];
         const double e = double(len + 4) * 2 * kEps * ga;
         double ubd = maxd(phi(g - e, l, u), phi(g + e, l, u));
         if (g - e <= 0 && g + e >= 0) ubd = maxd(ubd, 0.0);
@@ -477,10 +496,10 @@ __device__ void attempt(const LpDev &L, LpState &S, const BatchOpts &o, double t
     double *xn = L.x[cur ^ 1], *xan = L.xa[cur ^ 1], *yn = L.y[cur ^ 1], *yan = L.ya[cur ^ 1], *axn = L.ax[cur ^ 1];
     Reducer<3, 0, 0> r{};
     // g = c + A^T y and the projected primal step (src/cpu.cpp:107-132)
-    stream_rows<false>(L.tp, L.ti, L.tv, L.tblk, L.ntb, y, stage, lds, [&](int j) { return L.c[j]; },
+    stream_rows<false>(L.tp, L.ti, L.tv, L.tblk, L.ntb, y, stage, lds, [&](int j) { return gin(L.c)[j]; },
                        [&](int j, double g, double) {
                            const double xj = x[j];
-                           const double t = clampd(xj - tau * g, L.lb[j], L.ub[j]);
+                           const double t = clampd(xj - tau * g, gin(L.lb)[j], gin(L.ub)[j]);
                            const double d = t - xj;
                            r.s[0] += d * d;
                            L.dx[j] = d;
@@ -492,7 +511,7 @@ __device__ void attempt(const LpDev &L, LpState &S, const BatchOpts &o, double t
     stream_rows<false>(L.ap, L.ai, L.av, L.ablk, L.nab, L.dx, stage, lds, [](int) { return 0.0; },
                        [&](int i, double delta, double) {
                            const double a = ax[i], bar = a + 2 * delta, yi = y[i], z = yi + sigma * bar;
-                           const double t = maxd(0., z - sigma * L.ru[i]) + mind(0., z - sigma * L.rl[i]);
+                           const double t = maxd(0., z - sigma * gin(L.ru)[i]) + mind(0., z - sigma * gin(L.rl)[i]);
                            const double dy = t - yi;
                            r.s[1] += (!isfinite(bar) || !isfinite(t)) ? INFINITY : dy * dy;
                            r.s[2] += dy * delta;
@@ -596,20 +615,6 @@ __device__ __forceinline__ void rows_dot(const int *k, const int *e, const int *
             for (int u = 0; u < kStep; ++u)
                 if (k[q] + t + u < e[q]) acc[q] += v[q][u] * vec[ix[q][u]];
     }
-}
-
-// LpDev's pointers are loaded from memory, so the compiler cannot tell they
-// point to global memory and emits flat accesses; a flat load also counts
-// against the LDS wait counter, which makes every LDS wait drain all global
-// loads in flight.  The step kernel casts them back to global pointers.
-#define GLOBAL_AS __attribute__((address_space(1)))
-template <class T>
-__device__ __forceinline__ const GLOBAL_AS T *gin(const T *p) {
-    return (const GLOBAL_AS T *)p;
-}
-template <class T>
-__device__ __forceinline__ GLOBAL_AS T *gout(T *p) {
-    return (GLOBAL_AS T *)p;
 }
 
 // Dot products like rows_dot, read from the sliced layout: the rows of A (or
@@ -885,15 +890,15 @@ __global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps,
                 zero_vec(L.oy[0], L.m0);
                 __syncthreads();
                 for (int i = threadIdx.x; i < L.m; i += kThreads) {
-                    const double d = cy[i] - L.epy[i];
-                    const int t = d >= 0 ? L.upper_rows[i] : L.lower_rows[i];
-                    L.oy[0][t] = 0.0 + d * L.dr[i] * L.sobj;
+                    const double d = cy[i] - gin(L.epy)[i];
+                    const int t = d >= 0 ? gin(L.upper_rows)[i] : gin(L.lower_rows)[i];
+                    gout(L.oy[0])[t] = 0.0 + d * gin(L.dr)[i] * L.sobj;
                 }
                 __syncthreads();
                 stream_rows<true>(L.otp, L.oti, L.otv, L.otblk, L.notb, L.oy[0], stage, lds, [](int) { return 0.0; },
                                   [&](int j, double v, double a) {
-                                      L.aty[j] = v;
-                                      L.atyabs[j] = a;
+                                      gout(L.aty)[j] = v;
+                                      gout(L.atyabs)[j] = a;
                                   });
                 __syncthreads();
                 Reducer<4, 2, 1> r{};
@@ -901,7 +906,7 @@ __global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps,
                     const double yi = L.oy[0][i];
                     if (!isfinite(yi)) r.mx[1] = 1;
                     if (yi != 0) {
-                        const double e = yi > 0 ? L.ru0[i] : L.rl0[i];
+                        const double e = yi > 0 ? gin(L.ru0)[i] : gin(L.rl0)[i];
                         if (!isfinite(e)) r.mx[0] = 1;
                         else {
                             dd_add(r.d[0], yi * e);
@@ -942,11 +947,11 @@ __global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps,
         const double *cy = avg ? L.ya[cur] : L.y[cur];
         Reducer<0, 0, 2> r{};
         for (int j = threadIdx.x; j < L.n; j += kThreads) {
-            const double d = cx[j] - L.epx[j];
+            const double d = cx[j] - gin(L.epx)[j];
             dd_add(r.d[0], d * d);
         }
         for (int i = threadIdx.x; i < L.m; i += kThreads) {
-            const double d = cy[i] - L.epy[i];
+            const double d = cy[i] - gin(L.epy)[i];
             dd_add(r.d[1], d * d);
         }
         block_reduce(r, lds);
@@ -1143,6 +1148,7 @@ class BatchEngine {
         stream_ = pool[0];
         for (int c = 0; c <= kClasses; ++c) class_stream_[c] = pool[1 + c];
         if (const char *env = std::getenv("NIRYUKTI_BATCH_ROWS_PER_THREAD")) rows_per_thread_ = std::atoi(env);
+        if (const char *env = std::getenv("NIRYUKTI_BATCH_TAIL_ROWS")) tail_rows_ = std::atoi(env);
         // Class capacities (m + n) bounded by what one workgroup may allocate
         // next to the kernel's static LDS.
         int device = 0;
@@ -1279,7 +1285,7 @@ class BatchEngine {
                 HIP_OK(hipEventRecord(ev[0], cs));
                 for (int k = 0; k < checks; ++k) {
                     if (g.kind == kClasses)
-                        launch_lds_t<kTailThreads, 1>(g.cap, grid, cs, slots, bo);
+                        launch_tail(g.cap, grid, cs, slots, bo);
                     else if (g.kind < kClasses - 1)
                         launch_lds(g.cap, g.kind == kClasses - 2, grid, cs, slots, bo);
                     else
@@ -1519,6 +1525,13 @@ class BatchEngine {
     template <int NT, int NR>
     void launch_lds_t(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
         batch_advance_lds_kernel<NT, NR><<<grid, NT, lds_bytes(cap), cs>>>(d_desc_, d_state_, slots, bo);
+    }
+    void launch_tail(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
+        switch (tail_rows_) {
+        case 4: launch_lds_t<kTailThreads, 4>(cap, grid, cs, slots, bo); break;
+        case 2: launch_lds_t<kTailThreads, 2>(cap, grid, cs, slots, bo); break;
+        default: launch_lds_t<kTailThreads, 1>(cap, grid, cs, slots, bo);
+        }
     }
     // A crowded LDS class: 256-thread workgroups (512 for the largest class)
     // with rows_per_thread_ rows per thread.
@@ -2052,6 +2065,7 @@ class BatchEngine {
     std::vector<int> slot_job_, free_slots_;
     std::vector<double> slot_bytes_;
     hipStream_t class_stream_[kClasses + 1] = {};  // per class, then the tail group
+    int tail_rows_ = 1;
     int rows_per_thread_ = 2;  // measured best on gfx950 for the 3-8 nnz/column batch
     int class_cap_[kClasses - 1] = {};
     std::vector<hipEvent_t> class_events_[kClasses + 1];
