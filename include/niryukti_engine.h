@@ -72,6 +72,9 @@ niryukti_model *niryukti_model_load(const char *path, const char *json_text);
 void niryukti_model_destroy(niryukti_model *m);
 /* Borrowed views valid until destroy. */
 void niryukti_model_view(const niryukti_model *m, niryukti_lp *lp, const char **name, const char **fingerprint);
+/* Writes the model as .mps or native .json by the path's extension, in the
+ * reference's `convert` layout (src/io.cpp:765-798, 815-901).  0 or -1. */
+int niryukti_model_write(const niryukti_model *m, const char *path);
 /* Result document with the reference's keys (src/io.cpp:902-995) for a result
  * of this model; the string is valid until the next call on this thread. */
 const char *niryukti_result_json(const niryukti_model *m, const niryukti_result *r, const char *backend);

@@ -143,6 +143,8 @@ def load_library() -> C.CDLL:
     lib.niryukti_model_load.restype = C.c_void_p
     lib.niryukti_model_destroy.argtypes = [C.c_void_p]
     lib.niryukti_model_view.argtypes = [C.c_void_p, C.POINTER(_LP), C.POINTER(C.c_char_p), C.POINTER(C.c_char_p)]
+    lib.niryukti_model_write.argtypes = [C.c_void_p, C.c_char_p]
+    lib.niryukti_model_write.restype = C.c_int
     lib.niryukti_result_json.argtypes = [C.c_void_p, C.POINTER(_Result), C.c_char_p]
     lib.niryukti_result_json.restype = C.c_char_p
     _lib = lib
@@ -286,6 +288,11 @@ class Model:
                      view(s.rl, m, np.float64), view(s.ru, m, np.float64), s.offset, s.sense)
         self.name = name.value.decode()
         self.fingerprint = fp.value.decode()
+
+    def write(self, path) -> None:
+        """Writes .mps or native .json by extension (the reference's `convert`)."""
+        if self._lib.niryukti_model_write(self._h, os.fsencode(str(path))) != 0:
+            raise ValueError(self._lib.niryukti_engine_last_error().decode())
 
     def result_json(self, result: dict, backend: str = "hip") -> dict:
         res = _Result()

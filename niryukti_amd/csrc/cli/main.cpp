@@ -27,6 +27,7 @@ void usage() {
                  "usage: niryukti solve MODEL [options]\n"
                  "       niryukti verify MODEL RESULT.json\n"
                  "       niryukti inspect MODEL\n"
+                 "       niryukti convert MODEL OUTPUT.mps|OUTPUT.json\n"
                  "       niryukti session MODEL        (line-JSON requests on stdin)\n"
                  "       niryukti devices\n"
                  "solve options: --device gpu|hip|auto --method pdhg|halpern|rhpdhg|r2hpdhg --tol X\n"
@@ -190,6 +191,11 @@ int main(int argc, char **argv) {
         if (cmd == "verify") {
             if (argc != 4) throw std::runtime_error("verify MODEL SOLUTION");
             return verify_command(m, argv[3]);
+        }
+        if (cmd == "convert") {
+            if (argc != 4) throw std::runtime_error("convert INPUT OUTPUT");
+            nyk::write_model(m, argv[3]);
+            return 0;
         }
         if (cmd == "solve") return solve_command(argc, argv, m);
         throw std::runtime_error("Unknown command: " + cmd);

@@ -161,6 +161,18 @@ niryukti_model *niryukti_model_load(const char *path, const char *json_text) {
 
 void niryukti_model_destroy(niryukti_model *m) { delete m; }
 
+int niryukti_model_write(const niryukti_model *m, const char *path) {
+    try {
+        g_error.clear();
+        if (!m || !path) throw std::runtime_error("model and path are required");
+        nyk::write_model(m->m, path);
+        return 0;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
 void niryukti_model_view(const niryukti_model *h, niryukti_lp *lp, const char **name, const char **fingerprint) {
     const nyk::Model &m = h->m;
     lp->rows = m.rows();

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <fstream>
 #include <set>
 #include <sstream>
@@ -355,6 +356,123 @@ Model read_mps_model(const std::string &text) {
     m.A = Csr::from_entries(int64_t(rows.size()), int64_t(vars.size()), std::move(e));
     canonicalize(m);
     return m;
+}
+
+namespace {
+
+std::string number17(double v) {
+    char buf[40];
+    std::snprintf(buf, sizeof buf, "%.17g", v);
+    return buf;
+}
+
+// Native JSON document of a model (src/io.cpp:765-798).  The reference's JSON
+// objects keep their keys in sorted order, so the keys are written sorted here
+// too (a constraint's coefficients by variable name); infinite bounds become
+// null, which the reader maps back to the infinite default.
+std::string json_document(const Model &m) {
+    const int64_t n = m.cols(), rows = m.rows();
+    std::vector<double> c(m.c);
+    for (auto &v : c) v *= m.sense;
+    Json doc = Json::object();
+    Json cons = Json::array();
+    std::vector<std::pair<std::string, double>> coef;
+    for (int64_t i = 0; i < rows; ++i) {
+        coef.clear();
+        for (int64_t k = m.A.ptr[i]; k < m.A.ptr[i + 1]; ++k) coef.emplace_back(m.column_name(m.A.index[k]), m.A.value[k]);
+        std::sort(coef.begin(), coef.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        Json row = Json::object();
+        Json a = Json::object();
+        for (const auto &[name, v] : coef) a[name] = v;
+        row["coefficients"] = std::move(a);
+        row["lb"] = m.rl[i];
+        row["name"] = m.row_name(i);
+        row["ub"] = m.ru[i];
+        cons.push_back(std::move(row));
+    }
+    doc["constraints"] = std::move(cons);
+    doc["name"] = m.name;
+    Json obj = Json::object();
+    obj["linear"] = Json(c);
+    obj["offset"] = m.offset * m.sense;
+    obj["quadratic_diagonal"] = Json(std::vector<double>(size_t(n), 0.0));
+    doc["objective"] = std::move(obj);
+    doc["sense"] = m.sense == 1 ? "min" : "max";
+    Json vars = Json::array();
+    for (int64_t j = 0; j < n; ++j) {
+        Json v = Json::object();
+        v["lb"] = m.lb[j];
+        v["name"] = m.column_name(j);
+        v["type"] = "continuous";
+        v["ub"] = m.ub[j];
+        vars.push_back(std::move(v));
+    }
+    doc["variables"] = std::move(vars);
+    return doc.dump(2) + "\n";
+}
+
+// Free-format MPS in the layout of the reference's writer (src/io.cpp:815-901):
+// OBJSENSE section, one RHS entry per row, RANGES for two-sided rows, FX/LO/MI/UP
+// bounds, numbers at 17 significant digits.
+std::string mps_document(const Model &m) {
+    const int64_t n = m.cols(), rows = m.rows();
+    for (int64_t j = 0; j < n; ++j)
+        if (m.column_name(j).find_first_of(" \t\r\n") != std::string::npos)
+            throw std::runtime_error("MPS export with whitespace in names requires fixed-format output; use JSON");
+    for (int64_t i = 0; i < rows; ++i)
+        if (m.row_name(i).find_first_of(" \t\r\n") != std::string::npos)
+            throw std::runtime_error("MPS export with whitespace in names requires fixed-format output; use JSON");
+    std::string out = "NAME          " + m.name + "\nOBJSENSE\n " + (m.sense == 1 ? "MIN" : "MAX") + "\nROWS\n N  OBJ\n";
+    std::vector<double> rhs(static_cast<size_t>(rows));
+    for (int64_t i = 0; i < rows; ++i) {
+        const bool lo = std::isfinite(m.rl[i]), hi = std::isfinite(m.ru[i]);
+        if (!lo && !hi) throw std::runtime_error("Cannot serialize fully free row to supported MPS subset");
+        const char t = m.rl[i] == m.ru[i] ? 'E' : hi ? 'L' : 'G';
+        rhs[i] = t == 'G' ? m.rl[i] : m.ru[i];
+        out += std::string(" ") + t + "  " + m.row_name(i) + "\n";
+    }
+    out += "COLUMNS\n";
+    const Csr at = m.A.transposed();
+    for (int64_t j = 0; j < n; ++j) {
+        const std::string name = m.column_name(j);
+        out += " " + name + " OBJ " + number17(m.c[j] * m.sense) + "\n";
+        for (int64_t k = at.ptr[j]; k < at.ptr[j + 1]; ++k)
+            out += " " + name + " " + m.row_name(at.index[k]) + " " + number17(at.value[k]) + "\n";
+    }
+    out += "RHS\n RHS1 OBJ " + number17(-m.offset * m.sense) + "\n";
+    for (int64_t i = 0; i < rows; ++i) out += " RHS1 " + m.row_name(i) + " " + number17(rhs[i]) + "\n";
+    bool ranges = false;
+    for (int64_t i = 0; i < rows; ++i)
+        if (std::isfinite(m.rl[i]) && std::isfinite(m.ru[i]) && m.rl[i] != m.ru[i]) {
+            if (!ranges) out += "RANGES\n";
+            ranges = true;
+            out += " RNG1 " + m.row_name(i) + " " + number17(m.ru[i] - m.rl[i]) + "\n";
+        }
+    out += "BOUNDS\n";
+    for (int64_t j = 0; j < n; ++j) {
+        const std::string name = m.column_name(j);
+        if (m.lb[j] == m.ub[j]) {
+            out += " FX BND " + name + " " + number17(m.lb[j]) + "\n";
+            continue;
+        }
+        out += std::isfinite(m.lb[j]) ? " LO BND " + name + " " + number17(m.lb[j]) + "\n" : " MI BND " + name + "\n";
+        if (std::isfinite(m.ub[j])) out += " UP BND " + name + " " + number17(m.ub[j]) + "\n";
+    }
+    return out + "ENDATA\n";
+}
+
+}  // namespace
+
+void write_model(const Model &m, const std::string &path) {
+    m.validate();
+    const auto dot = path.find_last_of('.');
+    const std::string ext = dot == std::string::npos ? "" : upper(path.substr(dot + 1));
+    if (ext != "JSON" && ext != "MPS") throw std::runtime_error("Convert output must be .json or .mps");
+    const std::string text = ext == "JSON" ? json_document(m) : mps_document(m);
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("Cannot write " + path);
+    f << text;
+    if (!f) throw std::runtime_error("Cannot write " + path);
 }
 
 Model read_model(const std::string &path) {

@@ -14,6 +14,10 @@ Model read_model(const std::string &path);
 Model read_json_model(const std::string &text);
 Model read_mps_model(const std::string &text);
 
+// Writes .json (native format, src/io.cpp:765-798) or .mps (src/io.cpp:815-901)
+// by extension: the reference's `convert INPUT OUTPUT`.
+void write_model(const Model &, const std::string &path);
+
 // Result document with the reference's keys (src/io.cpp:902-995).
 Json result_json(const Model &, const Result &);
 

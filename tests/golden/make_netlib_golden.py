@@ -2,7 +2,8 @@
 
 Runs the reference build (oracle/_ref/niryukti_ref, from oracle/ref.mk) with
 `solve MODEL --method pdhg --device cpu` and `inspect MODEL`, and stores the
-result documents under tests/golden/netlib/ as data.  Run from the repository
+result documents under tests/golden/netlib/ as data; also keeps the reference's
+`convert afiro.mps afiro.reference_convert.mps` output.  Run from the repository
 root after `make -f oracle/ref.mk REF=/root/reference`.
 """
 import json
@@ -24,3 +25,5 @@ for name in ("afiro", "adlittle", "israel", "e226"):
     doc["reference_inspect"] = inspect
     (OUT / f"{name}.reference_result.json").write_text(json.dumps(doc) + "\n")
     print(name, doc["status"], doc["objective"], doc["performance"]["iterations"], inspect["fingerprint"])
+
+subprocess.run([str(CLI), "convert", str(OUT / "afiro.mps"), str(OUT / "afiro.reference_convert.mps")], check=True)

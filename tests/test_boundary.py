@@ -174,3 +174,82 @@ def test_model_handle_and_result_document(engine):
     assert doc["primal"] == ref["primal"] and doc["performance"]["iterations"] == 7
     with pytest.raises(ValueError):
         engine.Model(json_model={"variables": [{"name": "x", "type": "integer"}]})
+
+
+REF_CLI = ROOT / "oracle" / "_ref" / "niryukti_ref"
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_convert_round_trips_to_reference_fingerprint(engine, tmp_path, name):
+    """`convert` (app/main.cpp:423-428, src/io.cpp:815-901): the written MPS and
+    JSON read back to the model the reference fingerprinted."""
+    ref = json.loads((NETLIB / f"{name}.reference_result.json").read_text())["reference_inspect"]
+    for ext in ("mps", "json"):
+        out = tmp_path / f"{name}.{ext}"
+        assert run_cli(engine, "convert", NETLIB / f"{name}.mps", out).returncode == 0
+        back = json.loads(run_cli(engine, "inspect", out).stdout)
+        assert back["fingerprint"] == ref["fingerprint"], ext
+        if REF_CLI.exists():
+            theirs = tmp_path / f"{name}.ref.{ext}"
+            subprocess.run([str(REF_CLI), "convert", str(NETLIB / f"{name}.mps"), str(theirs)], check=True)
+            if ext == "mps":
+                assert out.read_bytes() == theirs.read_bytes()
+            else:
+                a, b = json.loads(out.read_text()), json.loads(theirs.read_text())
+                assert a == b and list(a) == list(b)
+
+
+def test_convert_mps_layout_matches_reference_file(engine, tmp_path):
+    out = tmp_path / "afiro.mps"
+    assert run_cli(engine, "convert", NETLIB / "afiro.mps", out).returncode == 0
+    assert out.read_bytes() == (NETLIB / "afiro.reference_convert.mps").read_bytes()
+
+
+def test_convert_sections_and_errors(engine, tmp_path):
+    # max sense, objective offset, a ranged row, >= and = rows, free, fixed and
+    # one-sided columns: every MPS section the writer has
+    model = {"name": "sections", "sense": "max",
+             "variables": [{"name": "a", "lb": -2, "ub": 4}, {"name": "b", "lb": None, "ub": 3},
+                           {"name": "c", "lb": 1.5, "ub": 1.5}, {"name": "d", "lb": None}],
+             "objective": {"linear": [1, -2, 0.25, 0.1], "offset": 7},
+             "constraints": [{"name": "range", "lb": -1, "ub": 5, "coefficients": {"a": 1, "b": 2}},
+                             {"name": "ge", "lb": 0.5, "coefficients": {"b": 1, "d": -1}},
+                             {"name": "eq", "lb": 2, "ub": 2, "coefficients": {"a": 1, "c": 1, "d": 3}},
+                             {"name": "le", "ub": 9, "coefficients": {"d": 1}}]}
+    src = tmp_path / "sections.json"
+    src.write_text(json.dumps(model))
+    fp = json.loads(run_cli(engine, "inspect", src).stdout)["fingerprint"]
+    for ext in ("mps", "json"):
+        out = tmp_path / f"out.{ext}"
+        assert run_cli(engine, "convert", src, out).returncode == 0
+        assert json.loads(run_cli(engine, "inspect", out).stdout)["fingerprint"] == fp
+        if REF_CLI.exists():
+            theirs = tmp_path / f"ref.{ext}"
+            subprocess.run([str(REF_CLI), "convert", str(src), str(theirs)], check=True)
+            assert out.read_bytes() == theirs.read_bytes() if ext == "mps" else \
+                json.loads(out.read_text()) == json.loads(theirs.read_text())
+    text = (tmp_path / "out.mps").read_text()
+    for section in ("OBJSENSE\n MAX", "RANGES\n RNG1 range 6", " MI BND b", " FX BND c 1.5", " MI BND d", "RHS1 OBJ -7"):  # MPS objective RHS is -offset
+        assert section in text, section
+    bad = run_cli(engine, "convert", src, tmp_path / "out.lp")
+    assert bad.returncode == 1 and "Convert output must be .json or .mps" in bad.stderr
+    assert run_cli(engine, "convert", src).returncode == 1
+    free_row = dict(model, constraints=[{"name": "free", "coefficients": {"a": 1}}])
+    src.write_text(json.dumps(free_row))
+    assert run_cli(engine, "convert", src, tmp_path / "f.mps").returncode == 1
+    assert run_cli(engine, "convert", src, tmp_path / "f.json").returncode == 0
+    spaced = dict(model, variables=[dict(v, name=v["name"] + " x") for v in model["variables"]],
+                  constraints=[])
+    src.write_text(json.dumps(spaced))
+    out = run_cli(engine, "convert", src, tmp_path / "s.mps")
+    assert out.returncode == 1 and "whitespace" in out.stderr
+
+
+def test_model_handle_writes_reference_layout(engine, tmp_path):
+    m = engine.Model(NETLIB / "afiro.mps")
+    m.write(tmp_path / "afiro.mps")
+    assert (tmp_path / "afiro.mps").read_bytes() == (NETLIB / "afiro.reference_convert.mps").read_bytes()
+    m.write(tmp_path / "afiro.json")
+    assert engine.Model(tmp_path / "afiro.json").fingerprint == m.fingerprint
+    with pytest.raises(ValueError, match="Convert output"):
+        m.write(tmp_path / "afiro.lp")
