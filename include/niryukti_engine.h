@@ -41,6 +41,7 @@ typedef struct {
     int check_every, scaling_passes, threads;
     int presolve, restart, adaptive, graphs, verbose;
     int device_checks;       /* -1 auto (>= 100,000 nonzeros), 0 host checks, 1 device checks */
+    int device_prepare;      /* Ruiz scaling + transpose: -1 auto (>= 100,000 nonzeros, ruiz), 0 host, 1 device */
     const double *initial_x; /* NULL or cols */
     const double *initial_y; /* NULL or rows */
 } niryukti_options;
@@ -56,6 +57,7 @@ typedef struct {
     int64_t removed_rows, removed_columns;
     double seconds, preprocess_seconds, transfer_seconds, iteration_seconds, verification_seconds;
     int index_bits;
+    int device_prepare; /* 1 when scaling and transpose ran on the device */
     double *x; /* NULL or cols: receives the primal solution */
     double *y; /* NULL or rows: receives the dual solution */
 } niryukti_result;

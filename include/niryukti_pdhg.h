@@ -57,6 +57,12 @@ int npdhg_device_name(int device, char *name, int len);
  * Replaces cuda_backend(...) (src/cuda.cu:1288-1303). */
 int npdhg_create(const npdhg_lp *lp, const double *x0, const double *y0, const npdhg_config *cfg,
                  npdhg_backend **out);
+/* The same, with the explicit transpose of lp's matrix supplied by the caller
+ * (cols + 1 offsets, nnz row indices and values in row order per column, as
+ * Sparse::transpose builds it, src/model.cpp:39-58), e.g. from
+ * npdhg_scale_lp (include/niryukti_prepare.h), instead of building it here. */
+int npdhg_create_transposed(const npdhg_lp *lp, const int64_t *t_ptr, const int64_t *t_index, const double *t_value,
+                            const double *x0, const double *y0, const npdhg_config *cfg, npdhg_backend **out);
 void npdhg_destroy(npdhg_backend *b);
 
 /* Performs `count` accepted PDHG steps with base steps tau = primal_step * factor,

@@ -41,7 +41,8 @@ class _Options(C.Structure):
                 ("tol", C.c_double), ("time_limit", C.c_double), ("iteration_limit", C.c_int64),
                 ("check_every", C.c_int), ("scaling_passes", C.c_int), ("threads", C.c_int),
                 ("presolve", C.c_int), ("restart", C.c_int), ("adaptive", C.c_int), ("graphs", C.c_int),
-                ("verbose", C.c_int), ("device_checks", C.c_int), ("initial_x", _f64p), ("initial_y", _f64p)]
+                ("verbose", C.c_int), ("device_checks", C.c_int), ("device_prepare", C.c_int), ("initial_x", _f64p),
+                ("initial_y", _f64p)]
 
 
 class _Result(C.Structure):
@@ -55,7 +56,7 @@ class _Result(C.Structure):
                 ("removed_rows", C.c_int64), ("removed_columns", C.c_int64), ("seconds", C.c_double),
                 ("preprocess_seconds", C.c_double), ("transfer_seconds", C.c_double),
                 ("iteration_seconds", C.c_double), ("verification_seconds", C.c_double),
-                ("index_bits", C.c_int), ("x", _f64p), ("y", _f64p)]
+                ("index_bits", C.c_int), ("device_prepare", C.c_int), ("x", _f64p), ("y", _f64p)]
 
 
 class _PdhgLP(C.Structure):
@@ -84,6 +85,13 @@ class _PdhgAccuracy(C.Structure):
 class _PdhgStatus(C.Structure):
     _fields_ = [("accepted_total", C.c_int64), ("rejected_total", C.c_int64), ("restart_requested", C.c_int),
                 ("index_bits", C.c_int), ("factor", C.c_double)]
+
+
+class _Scaled(C.Structure):
+    _fields_ = [("value", _f64p), ("c", _f64p), ("lb", _f64p), ("ub", _f64p), ("rl", _f64p), ("ru", _f64p),
+                ("row_scale", _f64p), ("column_scale", _f64p), ("objective_scale", C.c_double),
+                ("t_ptr", _i64p), ("t_index", _i64p), ("t_value", _f64p), ("device_seconds", C.c_double),
+                ("seconds", C.c_double)]
 
 
 class _BatchStats(C.Structure):
@@ -127,6 +135,9 @@ def load_library() -> C.CDLL:
     lib.npdhg_reset.argtypes = [C.c_void_p, _f64p, _f64p]
     lib.npdhg_reset_candidate.argtypes = [C.c_void_p, C.c_int]
     lib.npdhg_spmv.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p]
+    lib.npdhg_scale_lp.argtypes = [C.POINTER(_PdhgLP), C.c_int, C.POINTER(_Scaled)]
+    lib.npdhg_scale_lp.restype = C.c_int
+    lib.npdhg_scale_last_error.restype = C.c_char_p
     lib.niryukti_batch_create.argtypes = [C.POINTER(_Options), C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]
     lib.niryukti_batch_destroy.argtypes = [C.c_void_p]
     lib.niryukti_batch_submit.argtypes = [C.c_void_p, C.POINTER(_LP), _f64p, _f64p, C.POINTER(C.c_int64)]
@@ -254,6 +265,26 @@ def solve_lp(lp: LP, **options) -> dict:
     if rc != 0:
         raise RuntimeError(lib.niryukti_engine_last_error().decode())
     return _result_dict(res, x, y)
+
+
+def scale_lp(lp: LP, passes: int = 5) -> dict:
+    """Ruiz passes, objective scale and the explicit transpose of a presolved
+    LP on the device (include/niryukti_prepare.h).  Returns the scaled arrays,
+    the factors and A^T (t_ptr, t_index, t_value)."""
+    lib = load_library()
+    m, n, nnz = lp.rows, lp.cols, lp.nnz
+    s = _PdhgLP(m, n, _ptr(lp.ptr, _i64p), _ptr(lp.index, _i64p), _ptr(lp.value), _ptr(lp.c), _ptr(lp.lb),
+                _ptr(lp.ub), _ptr(lp.rl), _ptr(lp.ru))
+    out = {k: np.zeros(size) for k, size in (("value", nnz), ("c", n), ("lb", n), ("ub", n), ("rl", m), ("ru", m),
+                                             ("row_scale", m), ("column_scale", n), ("t_value", nnz))}
+    out["t_ptr"] = np.zeros(n + 1, np.int64)
+    out["t_index"] = np.zeros(nnz, np.int64)
+    r = _Scaled(*(_ptr(out[k]) for k in ("value", "c", "lb", "ub", "rl", "ru", "row_scale", "column_scale")), 1.0,
+                _ptr(out["t_ptr"], _i64p), _ptr(out["t_index"], _i64p), _ptr(out["t_value"]), 0.0, 0.0)
+    if lib.npdhg_scale_lp(C.byref(s), int(passes), C.byref(r)) != 0:
+        raise RuntimeError(lib.npdhg_scale_last_error().decode())
+    out.update(objective_scale=r.objective_scale, device_seconds=r.device_seconds, seconds=r.seconds)
+    return out
 
 
 class Model:

@@ -33,7 +33,8 @@ void usage() {
                  "solve options: --device gpu|hip|auto --method pdhg|halpern|rhpdhg|r2hpdhg --tol X\n"
                  "       --time-limit S --iterations N --threads T --check-every K --scaling-passes P\n"
                  "       --scaling ruiz|combined --primal-weight displacement|pid --gpu-indices auto|32|64\n"
-                 "       --matrix-precision fp64|mixed --no-presolve --no-restart --no-adaptive --verbose\n"
+                 "       --matrix-precision fp64|mixed --device-prepare auto|on|off --no-presolve --no-restart\n"
+                 "       --no-adaptive --verbose\n"
                  "       --warm-start RESULT.json --allow-model-change --json-out FILE (--solution-out)\n"
                  "       --cuda-graphs and --gpu-monitor are accepted: graphs and device checks are always on\n"
                  "models: .mps or native .json\n";
@@ -62,6 +63,11 @@ int solve_command(int argc, char **argv, const nyk::Model &m) {
         else if (a == "--primal-weight") o.primal_weight = val();
         else if (a == "--gpu-indices") o.gpu_indices = val();
         else if (a == "--matrix-precision") o.matrix_precision = val();
+        else if (a == "--device-prepare") {
+            const std::string v = val();
+            if (v != "auto" && v != "on" && v != "off") throw std::runtime_error("--device-prepare auto|on|off");
+            o.device_prepare = v == "auto" ? -1 : v == "on" ? 1 : 0;
+        }
         else if (a == "--json-out" || a == "--solution-out") output = val();
         else if (a == "--warm-start") warm = val();
         else if (a == "--allow-model-change") allow_model_change = true;

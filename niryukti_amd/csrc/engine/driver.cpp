@@ -10,6 +10,7 @@
 // backend behind the npdhg_* C-ABI (include/niryukti_pdhg.h).
 #include "lp.hpp"
 #include "niryukti_pdhg.h"
+#include "niryukti_prepare.h"
 #include "parallel.hpp"
 
 #include <algorithm>
@@ -58,10 +59,16 @@ class WeightPid {
 // RAII owner of a device backend.
 class Device {
   public:
-    Device(const Model &m, const std::vector<double> &x, const std::vector<double> &y, const npdhg_config &cfg) {
+    // `at`, when not empty, is the transpose of m.A (from the device prepare).
+    Device(const Model &m, const std::vector<double> &x, const std::vector<double> &y, const npdhg_config &cfg,
+           const Csr &at) {
         npdhg_lp lp{m.rows(), m.cols(), m.A.ptr.data(), m.A.index.data(), m.A.value.data(), m.c.data(),
                     m.lb.data(),  m.ub.data(), m.rl.data(),      m.ru.data()};
-        check(npdhg_create(&lp, x.data(), y.data(), &cfg, &b_));
+        if (at.ptr.empty())
+            check(npdhg_create(&lp, x.data(), y.data(), &cfg, &b_));
+        else
+            check(npdhg_create_transposed(&lp, at.ptr.data(), at.index.data(), at.value.data(), x.data(), y.data(),
+                                          &cfg, &b_));
     }
     ~Device() { npdhg_destroy(b_); }
     Device(const Device &) = delete;
@@ -138,6 +145,7 @@ std::string device_description() {
 }
 
 constexpr int64_t kDeviceCheckNonzeros = 100000;
+constexpr int64_t kDevicePrepareNonzeros = 100000;
 
 Accuracy from_device(const npdhg_accuracy &d) {
     Accuracy a;
@@ -288,6 +296,55 @@ void run_device_checks(const Model &original, const Prepared &prep, const Option
     r.verification_seconds += elapsed(check);
 }
 
+// Ruiz passes, objective scale and transpose on the device for a prepared
+// (presolved, unscaled) LP: fills the scaled model and factors into `p`
+// exactly as the host passes would (src/preprocess.cpp:306-345) and returns
+// the transpose of the scaled matrix.
+Csr scale_on_device(Prepared &p, int passes) {
+    Model &m = p.model;
+    npdhg_lp lp{m.rows(), m.cols(), m.A.ptr.data(), m.A.index.data(), m.A.value.data(), m.c.data(),
+                m.lb.data(),  m.ub.data(), m.rl.data(),      m.ru.data()};
+    Csr at;
+    at.rows = m.cols();
+    at.cols = m.rows();
+    at.ptr.resize(size_t(m.cols() + 1));
+    at.index.resize(m.A.value.size());
+    at.value.resize(m.A.value.size());
+    std::vector<double> value(m.A.value.size()), c(m.c.size()), lb(m.c.size()), ub(m.c.size()), rl(m.rl.size()),
+        ru(m.rl.size());
+    npdhg_scaled out{};
+    out.value = value.data();
+    out.c = c.data();
+    out.lb = lb.data();
+    out.ub = ub.data();
+    out.rl = rl.data();
+    out.ru = ru.data();
+    out.row_scale = p.row_scale.data();
+    out.column_scale = p.column_scale.data();
+    out.t_ptr = at.ptr.data();
+    out.t_index = at.index.data();
+    out.t_value = at.value.data();
+    if (npdhg_scale_lp(&lp, passes, &out) != 0)
+        throw std::runtime_error(std::string("device prepare: ") + npdhg_scale_last_error());
+    m.A.value = std::move(value);
+    m.c = std::move(c);
+    m.lb = std::move(lb);
+    m.ub = std::move(ub);
+    m.rl = std::move(rl);
+    m.ru = std::move(ru);
+    if (passes > 0) {
+        p.objective_scale = out.objective_scale;
+        m.offset /= p.objective_scale;
+    }
+    m.validate();
+    return at;
+}
+
+bool wants_device_prepare(const Model &original, const Options &o) {
+    if (o.scaling != "ruiz" || original.A.nnz() >= (int64_t(1) << 31)) return false;
+    return o.device_prepare > 0 || (o.device_prepare < 0 && original.A.nnz() >= kDevicePrepareNonzeros);
+}
+
 Result solve_continuous(const Model &original, const Options &o) {
     auto start = Clock::now();
     Result r;
@@ -297,7 +354,15 @@ Result solve_continuous(const Model &original, const Options &o) {
     r.device_reason = "Explicit backend request";
     Verifier verifier(original);
     r.status = "ITERATION_LIMIT";
-    auto prep = prepare(original, o);
+    const bool device_prepare = wants_device_prepare(original, o);
+    Options host_part = o;
+    if (device_prepare) host_part.scaling_passes = 0;  // presolve only; the passes run on the device
+    auto prep = prepare(original, host_part);
+    Csr transposed;
+    if (device_prepare && prep.failure.empty()) {
+        transposed = scale_on_device(prep, o.scaling_passes);
+        r.device_prepare = true;
+    }
     r.preprocess_seconds = elapsed(start);
     if (!prep.failure.empty()) {
         r.status = prep.failure;
@@ -377,7 +442,8 @@ Result solve_continuous(const Model &original, const Options &o) {
     cfg.graphs = o.graphs;
     cfg.index_bits = o.gpu_indices == "64" ? 64 : o.gpu_indices == "32" ? 32 : 0;
     cfg.fp32_values = o.matrix_precision == "mixed";
-    Device backend(m, x, y, cfg);
+    Device backend(m, x, y, cfg, transposed);
+    transposed = Csr{};
     r.transfer_seconds = elapsed(transfer);
     r.device_name = device_description();
     r.gpu_index_bits = backend.status().index_bits;

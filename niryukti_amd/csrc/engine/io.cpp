@@ -568,6 +568,7 @@ Json result_json(const Model &m, const Result &r) {
     gx["index_bits"] = r.gpu_index_bits;
     gx["cuda_graphs"] = r.graph_execution;
     gx["matrix_precision"] = r.matrix_precision;
+    gx["device_prepare"] = r.device_prepare;
     j["gpu_execution"] = std::move(gx);
     if (!r.infeasibility_ray.empty()) {
         Json cert = Json::object();

@@ -65,6 +65,10 @@ struct Options {
     // Candidate checks: -1 on the device for models with >= 100,000 nonzeros,
     // 0 on the host every chunk, 1 on the device (the host still certifies).
     int device_checks = -1;
+    // Ruiz scaling, objective scaling and the transpose: -1 on the device for
+    // models with >= 100,000 nonzeros and "ruiz" scaling, 0 on the host, 1 on
+    // the device (presolve always runs on the host).
+    int device_prepare = -1;
     std::vector<double> initial_x, initial_y;
     std::shared_ptr<std::atomic<bool>> cancellation;
 };
@@ -86,6 +90,7 @@ struct Result {
     int64_t weight_updates = 0, host_candidate_checks = 0, device_candidate_checks = 0;
     int gpu_index_bits = 0;
     bool graph_execution = false;
+    bool device_prepare = false;  // scaling and transpose ran on the device
     std::string matrix_precision = "fp64", method_selected, device_reason;
     double estimated_gpu_bytes = 0;
     // The request as the caller made it (reported under "selection").

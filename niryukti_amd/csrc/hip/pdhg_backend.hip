@@ -995,7 +995,10 @@ HostCsr transpose(const npdhg_lp &lp) {
 
 class Backend {
   public:
-    Backend(const npdhg_lp &lp, const double *x0, const double *y0, const npdhg_config &cfg) {
+    // `at`, when given, is the explicit transpose of lp's matrix (computed by
+    // the caller, e.g. on the device); otherwise it is built here.
+    Backend(const npdhg_lp &lp, const double *x0, const double *y0, const npdhg_config &cfg,
+            const npdhg_lp *at = nullptr) {
         n_ = lp.cols;
         m_ = lp.rows;
         if (n_ < 0 || m_ < 0) throw std::runtime_error("Negative LP dimension");
@@ -1018,17 +1021,31 @@ class Backend {
         HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         try {
             upload_matrix(lp.rows, lp.cols, lp.ptr, lp.index, lp.value, a_);
-            HostCsr t = transpose(lp);
-            upload_matrix(t.rows, t.cols, t.ptr.data(), t.index.data(), t.value.data(), at_);
+            HostCsr built;
+            const int64_t *tp, *ti;
+            const double *tv;
+            if (at) {
+                if (at->rows != n_ || at->cols != m_ || !at->ptr || at->ptr[n_] != nnz_ || (nnz_ && (!at->index || !at->value)))
+                    throw std::runtime_error("Transpose does not match the LP");
+                tp = at->ptr;
+                ti = at->index;
+                tv = at->value;
+            } else {
+                built = transpose(lp);
+                tp = built.ptr.data();
+                ti = built.index.data();
+                tv = built.value.data();
+            }
+            upload_matrix(n_, m_, tp, ti, tv, at_);
             g_a_ = pick_group(m_, nnz_);
             g_at_ = pick_group(n_, nnz_);
             sg_a_ = upload_segments(make_segments(lp.ptr, m_, row_threshold(g_a_)), row_threshold(g_a_));
-            sg_at_ = upload_segments(make_segments(t.ptr.data(), n_, row_threshold(g_at_)), row_threshold(g_at_));
+            sg_at_ = upload_segments(make_segments(tp, n_, row_threshold(g_at_)), row_threshold(g_at_));
             nb1_ = sg_at_.nseg + grid_for(n_, g_at_);
             nb2_ = sg_a_.nseg + grid_for(m_, g_a_);
             setup_chunks(lp.ptr, lp.index, lp.value, m_, n_, row_threshold(g_a_), ch_a_, ch_a_val_, g_ch_a_,
                          nbp_a_, part_a_);
-            setup_chunks(t.ptr.data(), t.index.data(), t.value.data(), n_, m_, row_threshold(g_at_), ch_at_,
+            setup_chunks(tp, ti, tv, n_, m_, row_threshold(g_at_), ch_at_,
                          ch_at_val_, g_ch_at_, nbp_at_, part_at_);
             c_ = upload(lp.c, n_);
             lb_ = upload(lp.lb, n_);
@@ -1749,8 +1766,9 @@ int guarded(F &&f) {
 
 struct npdhg_backend {
     Backend impl;
-    npdhg_backend(const npdhg_lp &lp, const double *x0, const double *y0, const npdhg_config &cfg)
-        : impl(lp, x0, y0, cfg) {}
+    npdhg_backend(const npdhg_lp &lp, const double *x0, const double *y0, const npdhg_config &cfg,
+                  const npdhg_lp *at = nullptr)
+        : impl(lp, x0, y0, cfg, at) {}
 };
 
 extern "C" {
@@ -1777,6 +1795,20 @@ int npdhg_create(const npdhg_lp *lp, const double *x0, const double *y0, const n
     return guarded([&] {
         if (!lp || !x0 || !y0 || !cfg || !out) throw std::runtime_error("Null argument");
         *out = new npdhg_backend(*lp, x0, y0, *cfg);
+    });
+}
+
+int npdhg_create_transposed(const npdhg_lp *lp, const int64_t *t_ptr, const int64_t *t_index, const double *t_value,
+                            const double *x0, const double *y0, const npdhg_config *cfg, npdhg_backend **out) {
+    return guarded([&] {
+        if (!lp || !t_ptr || !x0 || !y0 || !cfg || !out) throw std::runtime_error("Null argument");
+        npdhg_lp at{};
+        at.rows = lp->cols;
+        at.cols = lp->rows;
+        at.ptr = t_ptr;
+        at.index = t_index;
+        at.value = t_value;
+        *out = new npdhg_backend(*lp, x0, y0, *cfg, &at);
     });
 }
 

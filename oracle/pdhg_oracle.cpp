@@ -1159,6 +1159,34 @@ int oracle_prepare(const oracle_lp *lp, const oracle_options *opts, int64_t *out
     }
 }
 
+int oracle_prepared_model(const oracle_lp *lp, const oracle_options *opts, int64_t *out_rows, int64_t *out_cols,
+                          int64_t *ptr, int64_t *index, double *value, double *c, double *lb, double *ub,
+                          double *rl, double *ru, double *offset, char *failure, int failure_len) {
+    try {
+        auto m = oracle::to_model(lp);
+        auto o = oracle::to_options(lp, opts);
+        auto p = oracle::prepare(m, o);
+        oracle::set_error(failure, failure_len, p.failure.c_str());
+        if (!p.failure.empty()) return 1;
+        const auto &q = p.model;
+        *out_rows = q.A.rows;
+        *out_cols = q.A.cols;
+        std::copy(q.A.ptr.begin(), q.A.ptr.end(), ptr);
+        std::copy(q.A.index.begin(), q.A.index.end(), index);
+        std::copy(q.A.value.begin(), q.A.value.end(), value);
+        std::copy(q.c.begin(), q.c.end(), c);
+        std::copy(q.lb.begin(), q.lb.end(), lb);
+        std::copy(q.ub.begin(), q.ub.end(), ub);
+        std::copy(q.rl.begin(), q.rl.end(), rl);
+        std::copy(q.ru.begin(), q.ru.end(), ru);
+        *offset = q.offset;
+        return 0;
+    } catch (const std::exception &e) {
+        oracle::set_error(failure, failure_len, e.what());
+        return -1;
+    }
+}
+
 void *oracle_backend_create(const oracle_lp *lp, const double *x0, const double *y0, int adaptive, int halpern,
                             double reflection, int residual_restarts, char *err, int errlen) {
     try {

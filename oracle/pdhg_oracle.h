@@ -64,6 +64,12 @@ int oracle_prepare(const oracle_lp *lp, const oracle_options *o, int64_t *out_ro
                    double *column_scale, double *row_scale, double *objective_scale, int64_t *col_map,
                    int64_t *row_map, char *failure, int failure_len);
 
+/* The prepared (presolved, scaled) model itself.  Buffers sized by the
+ * ORIGINAL dimensions: ptr rows+1, index/value nnz, c/lb/ub cols, rl/ru rows. */
+int oracle_prepared_model(const oracle_lp *lp, const oracle_options *o, int64_t *out_rows, int64_t *out_cols,
+                          int64_t *ptr, int64_t *index, double *value, double *c, double *lb, double *ub,
+                          double *rl, double *ru, double *offset, char *failure, int failure_len);
+
 /* Raw iteration backend on an (already prepared) LP. */
 void *oracle_backend_create(const oracle_lp *lp, const double *x0, const double *y0, int adaptive,
                             int halpern, double reflection, int residual_restarts, char *err, int errlen);

@@ -61,6 +61,9 @@ def lib() -> C.CDLL:
         l.oracle_infeasibility_bound.restype = C.c_double
         l.oracle_prepare.argtypes = [C.POINTER(_LP), C.POINTER(_Options), _i64p, _i64p, _f64p, _f64p, _f64p,
                                      _i64p, _i64p, C.c_char_p, C.c_int]
+        l.oracle_prepared_model.argtypes = [C.POINTER(_LP), C.POINTER(_Options), _i64p, _i64p, _i64p, _i64p,
+                                            _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, C.POINTER(C.c_double),
+                                            C.c_char_p, C.c_int]
         l.oracle_backend_create.argtypes = [C.POINTER(_LP), _f64p, _f64p, C.c_int, C.c_int, C.c_double, C.c_int,
                                             C.c_char_p, C.c_int]
         l.oracle_backend_create.restype = C.c_void_p
@@ -169,6 +172,43 @@ def prepare(lp, **kw) -> dict:
     return {"failure": fail.value.decode(), "rows": rows.value, "cols": cols.value,
             "column_scale": cs[:cols.value], "row_scale": rs[:rows.value], "objective_scale": osc.value,
             "col_map": cm[:cols.value], "row_map": rm[:rows.value]}
+
+
+def prepared_model(lp, **kw) -> dict:
+    """The prepared LP itself (canonical arrays after presolve and scaling)."""
+    l = lib()
+    h = _Held(lp)
+    o = _Options()
+    l.oracle_default_options(C.byref(o))
+    keep = []
+    for k, v in kw.items():
+        if k == "scaling":
+            b = str(v).encode()
+            keep.append(b)
+            o.scaling = b
+        elif k == "presolve":
+            o.presolve = int(bool(v))
+        else:
+            setattr(o, k, v)
+    m, n, nnz = h.s.rows, h.s.cols, int(np.asarray(lp.ptr)[-1])
+    rows, cols, offset = C.c_int64(), C.c_int64(), C.c_double()
+    ptr, index = np.zeros(m + 1, np.int64), np.zeros(max(nnz, 1), np.int64)
+    value = np.zeros(max(nnz, 1))
+    c, lb, ub, rl, ru = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(m)
+    fail = C.create_string_buffer(256)
+    f = lambda a: a.ctypes.data_as(_f64p)
+    rc = l.oracle_prepared_model(C.byref(h.s), C.byref(o), C.byref(rows), C.byref(cols),
+                                 ptr.ctypes.data_as(_i64p), index.ctypes.data_as(_i64p), f(value), f(c), f(lb), f(ub),
+                                 f(rl), f(ru), C.byref(offset), fail, 256)
+    if rc < 0:
+        raise RuntimeError(fail.value.decode())
+    if rc > 0:
+        return {"failure": fail.value.decode()}
+    mr, nc = rows.value, cols.value
+    k = int(ptr[mr])
+    return {"failure": "", "rows": mr, "cols": nc, "ptr": ptr[:mr + 1].copy(), "index": index[:k].copy(),
+            "value": value[:k].copy(), "c": c[:nc].copy(), "lb": lb[:nc].copy(), "ub": ub[:nc].copy(),
+            "rl": rl[:mr].copy(), "ru": ru[:mr].copy(), "offset": offset.value}
 
 
 class Backend:

@@ -5,11 +5,15 @@ sources by oracle/ref.mk) and stores its results as JSON data under
 tests/golden/.  Run from the repository root after
 `make -f oracle/ref.mk REF=/root/reference`:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [solves] [prepare] [steps]   (default: all)
 """
+import hashlib
 import json
 import subprocess
+import sys
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parents[2]
 DRIVER = ROOT / "oracle" / "_ref" / "ref_driver"
@@ -41,7 +45,45 @@ def case_args(config, seed, scale, index):
     return ["--config", config, "--seed", seed, "--scale", scale, "--index", index, "--threads", 1]
 
 
+def array_sha256(values, dtype):
+    """sha256 of the little-endian bytes of an array printed by the driver
+    (infinities arrive as "inf" / "-inf" strings)."""
+    a = np.asarray([float(v) if isinstance(v, str) else v for v in values], dtype=dtype)
+    return hashlib.sha256(a.astype(a.dtype.newbyteorder("<")).tobytes()).hexdigest()
+
+
+PREPARES = [(0, 3, 20, 0, "ruiz"), (4, 1, 200, 0, "ruiz"), (2, 3, 1, 1, "combined"), (1, 42, 50, 0, "ruiz")]
+
+
+def prepares():
+    out = []
+    for config, seed, scale, index, scaling in PREPARES:
+        r = run("prepare", *case_args(config, seed, scale, index), "--scaling", scaling)
+        keep = {k: r[k] for k in ("failure", "objective_scale", "offset", "rows", "cols")}
+        if r["cols"] <= 5000:
+            keep.update({k: r[k] for k in ("col_map", "row_map", "column_scale", "row_scale")})
+        # the whole prepared model, as hashes of its arrays
+        keep["model_sha256"] = {k: array_sha256(r[k], np.int64 if k in ("ptr", "index", "col_map", "row_map")
+                                                else np.float64)
+                                for k in ("ptr", "index", "value", "c", "lb", "ub", "rl", "ru", "column_scale",
+                                          "row_scale", "col_map", "row_map")}
+        keep.update(config=config, seed=seed, scale=scale, index=index, scaling=scaling)
+        out.append(keep)
+        print("prepare", config, seed, scale, index, scaling, r["rows"], r["cols"])
+    (OUT / "reference_prepare.json").write_text(json.dumps(out) + "\n")
+
+
 def main():
+    parts = set(sys.argv[1:]) or {"solves", "prepare", "steps"}
+    if "solves" in parts:
+        solves()
+    if "prepare" in parts:
+        prepares()
+    if "steps" in parts:
+        steps()
+
+
+def solves():
     solves = []
     for config, seed, scale, index, keep in SOLVES:
         r = run("solve", *case_args(config, seed, scale, index), *(["--xy"] if keep else []))
@@ -49,14 +91,9 @@ def main():
         solves.append(r)
         print(config, seed, scale, index, r["status"], r["iterations"], r["objective"])
     (OUT / "reference_solves.json").write_text(json.dumps(solves, indent=1) + "\n")
-    prepares = []
-    for config, seed, scale, index, scaling in [(0, 3, 20, 0, "ruiz"), (4, 1, 200, 0, "ruiz"), (2, 3, 1, 1, "combined")]:
-        r = run("prepare", *case_args(config, seed, scale, index), "--scaling", scaling)
-        keep = {k: r[k] for k in ("failure", "objective_scale", "offset", "rows", "cols", "col_map", "row_map",
-                                  "column_scale", "row_scale")}
-        keep.update(config=config, seed=seed, scale=scale, index=index, scaling=scaling)
-        prepares.append(keep)
-    (OUT / "reference_prepare.json").write_text(json.dumps(prepares) + "\n")
+
+
+def steps():
     steps = []
     for config, seed, scale, index, adaptive in [(2, 3, 1, 0, True), (0, 3, 20, 0, True), (0, 3, 20, 0, False)]:
         args = ["steps", *case_args(config, seed, scale, index), "--count", 10, "--chunks", 3,

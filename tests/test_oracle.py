@@ -3,6 +3,7 @@ written by tests/golden/make_golden.py from the reference build, and the live
 reference build when it is present), against closed forms from the
 reference's tests, and against an independent LP solver (HiGHS via scipy).
 """
+import hashlib
 import json
 import subprocess
 from pathlib import Path
@@ -46,9 +47,18 @@ def test_oracle_reproduces_reference_scaling(engine, oracle, case):
     assert p["failure"] == case["failure"]
     assert p["rows"] == case["rows"] and p["cols"] == case["cols"]
     assert p["objective_scale"] == case["objective_scale"]
-    assert np.array_equal(p["column_scale"], case["column_scale"])
-    assert np.array_equal(p["row_scale"], case["row_scale"])
-    assert np.array_equal(p["col_map"], case["col_map"]) and np.array_equal(p["row_map"], case["row_map"])
+    if "column_scale" in case:
+        assert np.array_equal(p["column_scale"], case["column_scale"])
+        assert np.array_equal(p["row_scale"], case["row_scale"])
+        assert np.array_equal(p["col_map"], case["col_map"]) and np.array_equal(p["row_map"], case["row_map"])
+    # the whole scaled model, bit for bit, through hashes of the reference's arrays
+    q = oracle.prepared_model(inst.lp, scaling=case["scaling"])
+    assert q["offset"] == case["offset"]
+    arrays = dict(q, column_scale=p["column_scale"], row_scale=p["row_scale"], col_map=p["col_map"],
+                  row_map=p["row_map"])
+    for key, digest in case["model_sha256"].items():
+        a = np.asarray(arrays[key])
+        assert hashlib.sha256(a.astype(a.dtype.newbyteorder("<")).tobytes()).hexdigest() == digest, key
 
 
 @pytest.mark.parametrize("case", load("reference_steps.json"), ids=lambda c: f"c{c['config']}-a{int(c['adaptive'])}")
