@@ -114,7 +114,18 @@ struct BatchOpts {
     double tol;
     int check_every, adaptive, restart, pad;
     long long handoff;  // iterations after which an LP leaves the batch (0: never)
+    // k^-0.3 and k^-0.6 for k < pow_count, computed on the host with std::pow
+    // (the adaptive step rule's exponents, src/cpu.cpp:156-163)
+    const double *pow3, *pow6;
+    long long pow_count;
 };
+
+__device__ __forceinline__ double pow_m3(const BatchOpts &o, long long k) {
+    return k < o.pow_count ? o.pow3[k] : pow(double(k), -.3);
+}
+__device__ __forceinline__ double pow_m6(const BatchOpts &o, long long k) {
+    return k < o.pow_count ? o.pow6[k] : pow(double(k), -.6);
+}
 
 // ------------------------------------------------------------ reductions --
 
@@ -464,15 +475,15 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
 }
 
 // Accept/reject and adaptive step factor (src/cpu.cpp:151-172).  Thread 0.
+// p3, p6: k^-0.3 and k^-0.6 for k = accepted + 2.
 __device__ void decide_batch(LpState &S, const BatchOpts &o, double dx2, double dy2, double coupling, double tau,
-                             double sigma, double f, int *accept_flag) {
+                             double sigma, double f, double p3, double p6, int *accept_flag) {
     const double energy = dx2 / tau + dy2 / sigma;
     const double limit = fabs(coupling) > 0 ? energy / (2 * fabs(coupling)) : INFINITY;
     const bool finite = isfinite(energy) && isfinite(coupling);
     const bool accept = finite && (!o.adaptive || limit >= 1);
     if (o.adaptive) {
-        const double k = double(S.accepted + 2);
-        double mult = mind((1 - pow(k, -.3)) * limit, 1 + pow(k, -.6));
+        double mult = mind((1 - p3) * limit, 1 + p6);
         if (!finite || !isfinite(mult) || mult <= 0) mult = .5;
         S.factor = clampd(f * mult, 1e-12, 1e12);
     }
@@ -521,7 +532,9 @@ __device__ void attempt(const LpDev &L, LpState &S, const BatchOpts &o, double t
                            axn[i] = a + delta;
                        });
     block_reduce(r, lds);
-    if (threadIdx.x == 0) decide_batch(S, o, r.s[0], r.s[1], r.s[2], tau, sigma, f, accept_flag);
+    if (threadIdx.x == 0)
+        decide_batch(S, o, r.s[0], r.s[1], r.s[2], tau, sigma, f, pow_m3(o, S.accepted + 2), pow_m6(o, S.accepted + 2),
+                     accept_flag);
     __syncthreads();
 }
 
@@ -625,7 +638,7 @@ __device__ __forceinline__ void rows_dot(const int *k, const int *e, const int *
 // kernel is bound by the per-CU rate of memory requests, not by bytes.  Each
 // row is still summed in storage order, so the sums equal rows_dot's.
 template <int NR>
-__device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const GLOBAL_AS int *sb,
+__device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const int *sb,
                                               const GLOBAL_AS int *si, const GLOBAL_AS double *sv, int slices, int b0,
                                               int nt, const double *vec, double *acc) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -690,7 +703,7 @@ __device__ __forceinline__ double tile_total(const double *tiles, int count) {
 template <int NT, int NR>
 __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, double tau0, double sigma0,
                             double *tiles, double *ylds, double *dxlds, const int *tplds, const int *aplds,
-                            int *accept_flag) {
+                            const int *tsblds, const int *asblds, double *powlds, int *accept_flag) {
     const int cur = S.cur;
     const double f = S.factor, tau = tau0 * f, sigma = sigma0 * f, w = f / (S.mass + f);
     const GLOBAL_AS double *x = gin(L.x[cur]), *xa = gin(L.xa[cur]), *y = gin(L.y[cur]), *ya = gin(L.ya[cur]),
@@ -699,6 +712,9 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
                      *yan = gout(L.ya[cur ^ 1]), *axn = gout(L.ax[cur ^ 1]);
     const GLOBAL_AS double *c = gin(L.c), *lb = gin(L.lb), *ub = gin(L.ub), *rl = gin(L.rl), *ru = gin(L.ru);
     for (int i = threadIdx.x; i < L.m; i += NT) ylds[i] = y[i];
+    // the step rule's powers load with y, off the decision's serial path
+    if (threadIdx.x == 64 % NT) powlds[0] = pow_m3(o, S.accepted + 2);
+    if (threadIdx.x == 65 % NT) powlds[1] = pow_m6(o, S.accepted + 2);
     __syncthreads();
     double *tile_dx = tiles, *tile_dy = tiles + kMaxTiles, *tile_cr = tiles + 2 * kMaxTiles;
     const int wave = threadIdx.x >> 6;
@@ -722,7 +738,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             lo_[q] = lb[j];
             hi_[q] = ub[j];
         }
-        rows_dot_sell<NR>(k, e, gin(L.tsb), gin(L.tsi), gin(L.tsv), (L.n + 63) / 64, b0, NT, ylds, g);
+        rows_dot_sell<NR>(k, e, tsblds, gin(L.tsi), gin(L.tsv), (L.n + 63) / 64, b0, NT, ylds, g);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             double d2 = 0;
@@ -761,7 +777,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             lo_[q] = rl[i];
             hi_[q] = ru[i];
         }
-        rows_dot_sell<NR>(k, e, gin(L.asb), gin(L.asi), gin(L.asv), (L.m + 63) / 64, b0, NT, dxlds, delta);
+        rows_dot_sell<NR>(k, e, asblds, gin(L.asi), gin(L.asv), (L.m + 63) / 64, b0, NT, dxlds, delta);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             double sq = 0, cr = 0;
@@ -791,7 +807,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
         const double dx2 = tile_total(tile_dx, (L.n + 63) / 64);
         const double dy2 = tile_total(tile_dy, (L.m + 63) / 64);
         const double cross = tile_total(tile_cr, (L.m + 63) / 64);
-        if (threadIdx.x == 0) decide_batch(S, o, dx2, dy2, cross, tau, sigma, f, accept_flag);
+        if (threadIdx.x == 0) decide_batch(S, o, dx2, dy2, cross, tau, sigma, f, powlds[0], powlds[1], accept_flag);
     }
     __syncthreads();
 }
@@ -804,6 +820,7 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     const int s = slots[blockIdx.x];
     __shared__ LpState S;
     __shared__ double tiles[3 * kMaxTiles];
+    __shared__ double powlds[2];
     __shared__ int flag;
     if (threadIdx.x == 0) S = states[s];
     __syncthreads();
@@ -818,12 +835,16 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
     double *ylds = dyn, *dxlds = dyn + L.m;
     int *tplds = reinterpret_cast<int *>(dyn + L.m + L.n), *aplds = tplds + L.n + 1;
-    const GLOBAL_AS int *tp = gin(L.tp), *ap = gin(L.ap);
+    const int ns = (L.n + 63) / 64, ms = (L.m + 63) / 64;
+    int *tsblds = aplds + L.m + 1, *asblds = tsblds + ns + 1;
+    const GLOBAL_AS int *tp = gin(L.tp), *ap = gin(L.ap), *tsb = gin(L.tsb), *asb = gin(L.asb);
     for (int j = threadIdx.x; j <= L.n; j += NT) tplds[j] = tp[j];
     for (int i = threadIdx.x; i <= L.m; i += NT) aplds[i] = ap[i];
+    for (int j = threadIdx.x; j <= ns; j += NT) tsblds[j] = tsb[j];
+    for (int i = threadIdx.x; i <= ms; i += NT) asblds[i] = asb[i];
     __syncthreads();
     for (long long done = 0; done < count;) {
-        attempt_lds<NT, NR>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, &flag);
+        attempt_lds<NT, NR>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds, &flag);
         if (flag == 1) ++done;
         else if (flag < 0) {
             if (threadIdx.x == 0) S.status = kExhausted;
@@ -1156,11 +1177,29 @@ class BatchEngine {
         hipDeviceProp_t prop;
         HIP_OK(hipGetDeviceProperties(&prop, device));
         const long long room = (long long)prop.sharedMemPerBlock - kStaticLds - 16;
-        for (int c = 0; c < kClasses - 1; ++c)
-            class_cap_[c] = int(std::max<long long>(0, std::min<long long>(kClassDoubles[c], room / 12)));
+        for (int c = 0; c < kClasses - 1; ++c) {
+            int cap = kClassDoubles[c];
+            while (cap > 0 && (long long)lds_bytes(cap) > room) --cap;
+            class_cap_[c] = cap;
+        }
         HIP_OK(hipMalloc(&d_desc_, sizeof(LpDev) * max_slots_));
         HIP_OK(hipMalloc(&d_state_, sizeof(LpState) * max_slots_));
         HIP_OK(hipMalloc(&d_slots_, sizeof(int) * max_slots_ * (kClasses + 1)));
+        // k^-0.3 and k^-0.6 by std::pow for every k the step rule can reach
+        // (k = accepted + 2 <= iteration_limit + 1), up to 2^22 entries.
+        pow_count_ = std::min<long long>(std::max<long long>(opt_.iteration_limit, 0) + 3, 1LL << 22);
+        {
+            std::vector<double> p3(static_cast<size_t>(pow_count_)), p6(static_cast<size_t>(pow_count_));
+            for (long long k = 0; k < pow_count_; ++k) {
+                p3[size_t(k)] = std::pow(double(k), -.3);
+                p6[size_t(k)] = std::pow(double(k), -.6);
+            }
+            HIP_OK(hipMalloc(&d_pow3_, sizeof(double) * size_t(pow_count_)));
+            HIP_OK(hipMalloc(&d_pow6_, sizeof(double) * size_t(pow_count_)));
+            HIP_OK(hipMemcpyAsync(d_pow3_, p3.data(), sizeof(double) * p3.size(), hipMemcpyHostToDevice, stream_));
+            HIP_OK(hipMemcpyAsync(d_pow6_, p6.data(), sizeof(double) * p6.size(), hipMemcpyHostToDevice, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+        }
         HIP_OK(hipEventCreate(&uploaded_));
 
         desc_.resize(max_slots_);
@@ -1183,6 +1222,8 @@ class BatchEngine {
         hipFree(d_desc_);
         hipFree(d_state_);
         hipFree(d_slots_);
+        hipFree(d_pow3_);
+        hipFree(d_pow6_);
         for (auto &list : class_events_)
             for (auto &e : list) hipEventDestroy(e);
         if (uploaded_) hipEventDestroy(uploaded_);
@@ -1266,7 +1307,8 @@ class BatchEngine {
                 HIP_OK(hipMemcpyAsync(d_slots_ + size_t(g.kind) * max_slots_, g.slots.data(), sizeof(int) * g.slots.size(),
                                       hipMemcpyHostToDevice, stream_));
             HIP_OK(hipEventRecord(uploaded_, stream_));
-            BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0, handoff_};
+            BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0, handoff_,
+                         d_pow3_,     d_pow6_,          pow_count_};
             auto t0 = Clock::now();
             std::vector<long long> before(active.size());
             for (size_t k = 0; k < active.size(); ++k)
@@ -1548,9 +1590,10 @@ class BatchEngine {
         }
     }
 
-    // Dynamic LDS of an LP with m + n <= cap: y and dx (doubles) and the two
-    // offset arrays (n + 1 and m + 1 ints).
-    static size_t lds_bytes(int cap) { return size_t(cap) * 12 + 16; }
+    // Dynamic LDS of an LP with m + n <= cap: y and dx (doubles), the two
+    // offset arrays (n + 1 and m + 1 ints) and the two slice-base arrays
+    // (ceil(n/64) + 1 and ceil(m/64) + 1 ints).
+    static size_t lds_bytes(int cap) { return size_t(cap) * 12 + size_t(cap / 64 + 3) * 8 + 16; }
 
     int lds_class(const Job &j) const {
         const nyk::Model &m = j.prep.model;
@@ -2066,6 +2109,8 @@ class BatchEngine {
     std::vector<double> slot_bytes_;
     hipStream_t class_stream_[kClasses + 1] = {};  // per class, then the tail group
     int tail_rows_ = 1;
+    double *d_pow3_ = nullptr, *d_pow6_ = nullptr;
+    long long pow_count_ = 0;
     int rows_per_thread_ = 2;  // measured best on gfx950 for the 3-8 nnz/column batch
     int class_cap_[kClasses - 1] = {};
     std::vector<hipEvent_t> class_events_[kClasses + 1];
