@@ -642,35 +642,57 @@ __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const 
                                               const GLOBAL_AS int *si, const GLOBAL_AS double *sv, int slices, int b0,
                                               int nt, const double *vec, double *acc) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int base[NR], len[NR], width = 0;
+    // Every lane of a slice owns `w` stored entries (padding is value 0,
+    // index 0), so loads below a slice's width need no per-lane test: only
+    // the sum skips a row's padding.  Widths are the same for the whole wave.
+    int base[NR], len[NR], w[NR], width = 0, common = 1 << 30;
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
-        const int s = (b0 + q * nt) / 64 + wave;  // the same for the whole wave
+        const int s = (b0 + q * nt) / 64 + wave;
         base[q] = 0;
+        w[q] = 0;
         len[q] = e[q] - k[q];
         if (s < slices) {
             const int b = sb[s];
             base[q] = b + lane;
-            width = max(width, (sb[s + 1] - b) / 64);
+            w[q] = (sb[s + 1] - b) / 64;
         }
+        w[q] = __builtin_amdgcn_readfirstlane(w[q]);
+        width = max(width, w[q]);
+        common = min(common, w[q]);
     }
-    for (int t = 0; t < width; t += kStep) {
+    // one chunk of kStep entries per row; `full` chunks lie within every
+    // row's slice and need no width test
+    auto chunk = [&](int t, bool full) {
         double v[NR][kStep];
         int ix[NR][kStep];
 #pragma unroll
-        for (int q = 0; q < NR; ++q)
+        for (int q = 0; q < NR; ++q) {
+            const GLOBAL_AS double *pv = sv + base[q] + t * 64;
+            const GLOBAL_AS int *pi = si + base[q] + t * 64;
 #pragma unroll
             for (int u = 0; u < kStep; ++u) {
-                const bool on = t + u < len[q];
-                v[q][u] = on ? sv[base[q] + (t + u) * 64] : 0.0;
-                ix[q][u] = on ? si[base[q] + (t + u) * 64] : 0;
+                if (full || t + u < w[q]) {
+                    v[q][u] = pv[u * 64];
+                    ix[q][u] = pi[u * 64];
+                } else {
+                    v[q][u] = 0.0;
+                    ix[q][u] = 0;
+                }
             }
+        }
 #pragma unroll
         for (int q = 0; q < NR; ++q)
 #pragma unroll
             for (int u = 0; u < kStep; ++u)
-                if (t + u < len[q]) acc[q] += v[q][u] * vec[ix[q][u]];
-    }
+                if (full || t + u < w[q]) {
+                    const double p = v[q][u] * vec[ix[q][u]];
+                    acc[q] = t + u < len[q] ? acc[q] + p : acc[q];
+                }
+    };
+    int t = 0;
+    for (; t + kStep <= common; t += kStep) chunk(t, true);
+    for (; t < width; t += kStep) chunk(t, false);
 }
 
 // Sums over rows are taken per tile of 64 consecutive rows (one wave, fixed
