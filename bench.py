@@ -198,7 +198,8 @@ def serve_stream(engine, qps, duration, seed=SEED, warm_fraction=0.2, hbm_budget
             "hbm_budget_bytes": hbm_budget}
 
 
-def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16, cpu_iterations=None):
+def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16, cpu_iterations=None,
+              matrix_precision="fp64", verbose=False):
     """One LP of config 0 / 1 / 4 on the grid-wide single-LP engine for a fixed
     iteration budget, beside the reference's CPU PDHG on the same instance
     (oracle/_ref/ref_driver, `cpu_threads` OpenMP threads, `cpu_iterations`
@@ -207,13 +208,16 @@ def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16,
     lp = inst.lp
     m, n, nnz = lp.rows, lp.cols, lp.nnz
     log(f"single: config {config} scale {scale}: {m} x {n}, {nnz} nonzeros")
-    r = engine.solve_lp(lp, iteration_limit=iterations, time_limit=1e9, tol=tol)
+    r = engine.solve_lp(lp, iteration_limit=iterations, time_limit=1e9, tol=tol, matrix_precision=matrix_precision,
+                        verbose=int(verbose))
     attempts = r["iterations"] + r["rejected_steps"]
     s_p = 4 if nnz < 2 ** 31 else 8
-    # algorithmic bytes of one attempt (FP64 values, int32 indices), original sizes
-    b_attempt = 2 * nnz * 12 + (n + m + 2) * s_p + m * 8 + n * 8 + 64 * (n + m)
+    s_v = 4 if matrix_precision == "mixed" else 8
+    # algorithmic bytes of one attempt (s_v-byte values, int32 indices), original sizes
+    b_attempt = 2 * nnz * (s_v + 4) + (n + m + 2) * s_p + m * 8 + n * 8 + 64 * (n + m)
     achieved = b_attempt * attempts / max(r["iteration_seconds"], 1e-12) / 1e9
     out = {"mode": "single", "config": config, "scale": scale, "rows": m, "cols": n, "nonzeros": nnz,
+           "matrix_precision": matrix_precision,
            "tol": tol, "status": r["status"], "iterations": r["iterations"], "attempts": attempts,
            "kkt": r["kkt_error"], "objective": r["objective"], "planted_objective": inst.planted_objective,
            "seconds": round(r["seconds"], 3), "iteration_seconds": round(r["iteration_seconds"], 3),
@@ -370,6 +374,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-iterations", type=int, default=None)
+    ap.add_argument("--verbose", action="store_true", help="single-LP mode: one progress line per check period")
+    ap.add_argument("--matrix-precision", default="fp64", choices=("fp64", "mixed"),
+                    help="single-LP mode: FP64 or FP32 matrix values (FP64 accumulation and iterates)")
     ap.add_argument("--qps", type=float, default=1.5)
     ap.add_argument("--duration", type=float, default=40.0)
     args = ap.parse_args()
@@ -403,7 +410,8 @@ def main():
         return
     if args.single is not None:
         print(json.dumps(single_lp(engine, args.single, args.scale, args.iterations, args.tol,
-                                   cpu_threads=args.cpu_threads, cpu_iterations=args.cpu_iterations)), flush=True)
+                                   cpu_threads=args.cpu_threads, cpu_iterations=args.cpu_iterations,
+                                   matrix_precision=args.matrix_precision, verbose=args.verbose)), flush=True)
         return
     insts = make_instances(engine, args.count)
     nnz = sum(i.lp.nnz for i in insts)

@@ -917,6 +917,35 @@ int chunk_count(long long vector_length) {
     return p > 1 ? p : 0;
 }
 
+// Mean number of distinct 128-byte lines of the gathered vector that a window
+// of 8,192 consecutive rows touches (rows longer than `threshold` excluded:
+// they run as segments), in bytes.  A matrix whose windows stay within a
+// fraction of one XCD's L2 (block-diagonal structure, banded rows) gathers
+// from L2 without column chunks, and chunking would only add the partial
+// planes' traffic.
+double gather_footprint(const int64_t *ptr, const int64_t *index, long long rows, long long cols,
+                        long long threshold) {
+    constexpr long long kWindow = 8192;
+    std::vector<int> stamp(static_cast<size_t>((cols + 15) / 16 + 1), -1);
+    double lines = 0;
+    long long windows = 0;
+    for (long long w0 = 0; w0 < rows; w0 += kWindow, ++windows) {
+        const int id = int(w0 / kWindow);
+        for (long long i = w0; i < std::min(rows, w0 + kWindow); ++i) {
+            if (ptr[i + 1] - ptr[i] > threshold) continue;
+            for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e) {
+                int &st = stamp[size_t(index[e] / 16)];
+                if (st != id) {
+                    st = id;
+                    lines += 1;
+                }
+            }
+        }
+    }
+    return windows ? 128.0 * lines / double(windows) : 0.0;
+}
+constexpr double kLocalFootprint = 1 << 20;  // bytes per window below which rows gather from L2 unchunked
+
 struct HostChunks {
     int P = 0;
     std::vector<int> ptr;  // P * (rows + 1)
@@ -926,16 +955,40 @@ struct HostChunks {
 };
 
 // Column chunks of a CSR matrix for partial_kernel; rows longer than
-// `threshold` (segment rows) are left out.
+// `threshold` (segment rows) are left out.  Chunk boundaries split the
+// columns so that every chunk holds about the same number of nonzeros: the
+// chunks run side by side on different XCDs with equal shares of the grid,
+// so a chunk that held the dense columns (the linking rows of a
+// block-angular LP, for A^T) would otherwise keep one XCD busy long after
+// the others finish.
 HostChunks make_chunks(const int64_t *ptr, const int64_t *index, const double *value, long long rows,
                        long long cols, long long threshold, int P) {
     HostChunks h;
     h.P = P;
-    const long long width = (cols + P - 1) / P;
+    std::vector<long long> per_col(static_cast<size_t>(cols) + 1, 0);
+    long long kept = 0;
+    for (long long i = 0; i < rows; ++i) {
+        if (ptr[i + 1] - ptr[i] > threshold) continue;
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e) per_col[size_t(index[e])]++;
+        kept += ptr[i + 1] - ptr[i];
+    }
+    // chunk_of[j]: the first k whose share of nonzeros the running count of
+    // columns < j has not yet filled
+    std::vector<unsigned char> chunk_of(static_cast<size_t>(cols));
+    {
+        long long run = 0;
+        int k = 0;
+        for (long long j = 0; j < cols; ++j) {
+            while (k < P - 1 && run >= (long long)((__int128)kept * (k + 1) / P)) ++k;
+            chunk_of[size_t(j)] = (unsigned char)k;
+            run += per_col[size_t(j)];
+        }
+    }
     std::vector<long long> count(size_t(P) * size_t(rows + 1), 0);
     for (long long i = 0; i < rows; ++i) {
         if (ptr[i + 1] - ptr[i] > threshold) continue;
-        for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e) count[size_t(index[e] / width) * (rows + 1) + i + 1]++;
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e)
+            count[size_t(chunk_of[size_t(index[e])]) * (rows + 1) + i + 1]++;
     }
     long long total = 0;
     h.ptr.resize(count.size());
@@ -956,7 +1009,7 @@ HostChunks make_chunks(const int64_t *ptr, const int64_t *index, const double *v
     for (long long i = 0; i < rows; ++i) {
         if (ptr[i + 1] - ptr[i] > threshold) continue;
         for (int64_t e = ptr[i]; e < ptr[i + 1]; ++e) {
-            const int k = int(index[e] / width);
+            const int k = chunk_of[size_t(index[e])];
             h.idx[fill[k]] = int(index[e]);
             h.val[fill[k]] = value[e];
             ++fill[k];
@@ -1416,7 +1469,8 @@ class Backend {
     void setup_chunks(const int64_t *ptr, const int64_t *index, const double *value, long long rows, long long cols,
                       long long threshold, ChunkedCsr &c, void *&val, int &group, int &grid, double *&part) {
         c = ChunkedCsr{};
-        const int P = chunk_count(cols);
+        int P = chunk_count(cols);
+        if (P && gather_footprint(ptr, index, rows, cols, threshold) < kLocalFootprint) P = 0;
         if (!P || rows == 0) return;
         HostChunks h = make_chunks(ptr, index, value, rows, cols, threshold, P);
         c.P = P;

@@ -248,19 +248,34 @@ __global__ void ruiz_columns_kernel(const int64_t *tptr, const int64_t *perm, do
 }
 
 // Objective scale (src/preprocess.cpp:336-345): max |c|, 1 when it is 0.
-__global__ void objective_scale_kernel(const double *c, long long n, double *scale) {
-    __shared__ double part[kWaves];
+// Level 1: each workgroup's max; level 2 (one workgroup): the max of those.
+__global__ void abs_max_kernel(const double *c, long long n, double *part) {
+    __shared__ double wave_part[kWaves];
     double v = 0;
-    for (long long j = threadIdx.x; j < n; j += kBlock) {
+    for (long long j = (long long)blockIdx.x * kBlock + threadIdx.x; j < n; j += (long long)gridDim.x * kBlock) {
         const double a = fabs(c[j]);
         v = v < a ? a : v;
     }
     v = wave_max(v);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    if ((threadIdx.x & 63) == 0) wave_part[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = 0;
-        for (int w = 0; w < kWaves; ++w) m = m < part[w] ? part[w] : m;
+        for (int w = 0; w < kWaves; ++w) m = m < wave_part[w] ? wave_part[w] : m;
+        part[blockIdx.x] = m;
+    }
+}
+
+__global__ void objective_scale_kernel(const double *part, int count, double *scale) {
+    __shared__ double wave_part[kWaves];
+    double v = 0;
+    for (int j = threadIdx.x; j < count; j += kBlock) v = v < part[j] ? part[j] : v;
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) wave_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0;
+        for (int w = 0; w < kWaves; ++w) m = m < wave_part[w] ? wave_part[w] : m;
         *scale = m > 0 ? m : 1.0;
     }
 }
@@ -348,6 +363,8 @@ void scale_lp(const npdhg_lp &lp, int passes, npdhg_scaled &out) {
     double *c = d.alloc<double>(n), *lb = d.alloc<double>(n), *ub = d.alloc<double>(n);
     double *rl = d.alloc<double>(m), *ru = d.alloc<double>(m);
     double *rs = d.alloc<double>(m), *cs = d.alloc<double>(n), *osc = d.alloc<double>(1);
+    const int max_blocks = grid_for(n);
+    double *cmax = d.alloc<double>(max_blocks);
     unsigned long long *count = d.alloc<unsigned long long>(n);
     int64_t *tptr = d.alloc<int64_t>(n + 1);
     const long long nsums = std::max<long long>(1, (n + kBlock * kScanItems - 1) / (kBlock * kScanItems));
@@ -392,7 +409,8 @@ void scale_lp(const npdhg_lp &lp, int passes, npdhg_scaled &out) {
         if (n) ruiz_columns_kernel<<<wave_grid(n), kBlock, 0, s>>>(tptr, perm, val, n, c, lb, ub, cs);
     }
     if (passes > 0) {
-        objective_scale_kernel<<<1, kBlock, 0, s>>>(c, n, osc);
+        abs_max_kernel<<<max_blocks, kBlock, 0, s>>>(c, n, cmax);
+        objective_scale_kernel<<<1, kBlock, 0, s>>>(cmax, max_blocks, osc);
         if (n) divide_kernel<<<grid_for(n), kBlock, 0, s>>>(c, n, osc);
     }
     if (nnz) gather_transpose_kernel<<<grid_for(nnz), kBlock, 0, s>>>(perm, nnz, row, val, t_index, t_value);
