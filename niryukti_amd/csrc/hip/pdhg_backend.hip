@@ -181,7 +181,7 @@ __device__ __forceinline__ double gather_sum(const double *values, int stride, i
 // row, the row index and its totals.
 template <bool ABS, class O, class V>
 __device__ bool segment_row(const DevCsr<O, V> &A, const Segments &sg, const double *v, double *lds, long long &row,
-                            double &total, double &total_abs, int s) {
+                            double &total, double &total_abs, int s, int *completed = nullptr) {
     const long long end = sg.end[s];
     double acc = 0, acca = 0;
     for (long long k = sg.begin[s] + threadIdx.x; k < end; k += kBlock) {
@@ -201,6 +201,7 @@ __device__ bool segment_row(const DevCsr<O, V> &A, const Segments &sg, const dou
     }
     __syncthreads();
     const int L = completes;
+    if (completed) *completed = L;
     if (L < 0) return false;
     // The whole block reads the row's segment partials at once (fixed order).
     const double *first = sg.partial + 2 * (long long)sg.first[L];
@@ -218,15 +219,26 @@ __device__ bool segment_row(const DevCsr<O, V> &A, const Segments &sg, const dou
 // Calls row(i, sum_k A[i,k] v[k], sum_k |A[i,k] v[k]|) for every row (the
 // second sum only with ABS, else 0): long rows through their segments, short
 // rows by lane groups of G over the remaining blocks.
+//
+// *slot (when given) receives where this block's share of any sum formed in
+// row() belongs: blockIdx.x for row-group blocks; for segment blocks, the
+// block of the row's first segment when this block completed a long row, and
+// -1 otherwise.  Which segment block completes a row depends on timing, so
+// writing the row's share under the completing block's own index would move
+// it between partial slots from run to run and change the rounding of the
+// fixed-order sum over slots.
 template <int G, bool ABS = false, class O, class V, class Row>
 __device__ __forceinline__ void for_rows(const DevCsr<O, V> &A, const Segments &sg, const double *v, double *lds,
-                                         Row &&row) {
+                                         Row &&row, long long *slot = nullptr) {
     if ((int)blockIdx.x < sg.nseg) {
         long long i = 0;
         double total = 0, total_abs = 0;
-        if (segment_row<ABS>(A, sg, v, lds, i, total, total_abs, blockIdx.x)) row(i, total, total_abs);
+        int completed = -1;
+        if (segment_row<ABS>(A, sg, v, lds, i, total, total_abs, blockIdx.x, &completed)) row(i, total, total_abs);
+        if (slot) *slot = completed >= 0 ? (long long)sg.first[completed] : -1;
         return;
     }
+    if (slot) *slot = blockIdx.x;
     constexpr int rows_per_block = kBlock / G;
     const int lane = threadIdx.x % G;
     const long long nb = (long long)gridDim.x - sg.nseg;
@@ -384,10 +396,11 @@ __global__ __launch_bounds__(kBlock) void primal_kernel(DevCsr<O, V> at, Segment
             xan[j] = v + (t - v) * w;
         }
     };
+    long long slot = blockIdx.x;
     if (a.part) for_rows_partials(at.rows, a.part, a.parts, update);
-    else for_rows<G>(at, sg, y, lds, update);
+    else for_rows<G>(at, sg, y, lds, update, &slot);
     const double s = block_sum(local, lds);
-    if (threadIdx.x == 0) store_shared(&a.partial[blockIdx.x], s);
+    if (threadIdx.x == 0 && slot >= 0) store_shared(&a.partial[slot], s);
 }
 
 // Accept/reject and step-factor update (src/cpu.cpp:151-202).  Thread 0 only.
@@ -478,13 +491,14 @@ __global__ __launch_bounds__(kBlock) void dual_kernel(DevCsr<O, V> A, Segments s
             axn[i] = next_ax;
         }
     };
+    long long slot = blockIdx.x;
     if (a.part) for_rows_partials(A.rows, a.part, a.parts, update);
-    else for_rows<G>(A, sg, a.dx, lds, update);
+    else for_rows<G>(A, sg, a.dx, lds, update, &slot);
     const double s1 = block_sum(sq, lds);
     const double s2 = block_sum(cross, lds);
-    if (threadIdx.x == 0) {
-        store_shared(&a.partial[2 * blockIdx.x], s1);
-        store_shared(&a.partial[2 * blockIdx.x + 1], s2);
+    if (threadIdx.x == 0 && slot >= 0) {
+        store_shared(&a.partial[2 * slot], s1);
+        store_shared(&a.partial[2 * slot + 1], s2);
     }
     // Large grids hand the decision to decide_kernel: one ticket per block on
     // a single address serialises at the memory side (tens of ns each).
@@ -1120,6 +1134,9 @@ class Backend {
             }
             part1_ = alloc_d(nb1_);
             part2_ = alloc_d(2LL * nb2_);
+            // segment slots other than each long row's first are never written
+            zero(part1_, nb1_);
+            zero(part2_, 2LL * nb2_);
             st_ = static_cast<State *>(alloc_bytes(sizeof(State)));
             State init{};
             init.factor = 1;

@@ -44,6 +44,33 @@ def test_deterministic_and_graph_invariant(engine):
     assert np.array_equal(a["y"], b["y"]) and np.array_equal(a["y"], c["y"])
 
 
+def long_row_lp(engine, seed=11, m=2000, n=6000, per_col=3, long_rows=24, long_len=3000):
+    """Planted LP whose A has rows far above the lane-group limit, so the
+    step kernels sum them by segments (and one of the segment workgroups,
+    whichever finishes last, completes each such row)."""
+    rng = np.random.default_rng(seed)
+    rows = [rng.choice(m, per_col, replace=False) for _ in range(n)]
+    r = np.concatenate(rows)
+    c_ = np.repeat(np.arange(n), per_col)
+    lr = np.repeat(rng.choice(m, long_rows, replace=False), long_len)
+    lc = np.concatenate([rng.choice(n, long_len, replace=False) for _ in range(long_rows)])
+    A = np.zeros((m, n))
+    np.add.at(A, (np.concatenate([r, lr]), np.concatenate([c_, lc])), rng.standard_normal(len(r) + len(lr)))
+    x0 = rng.uniform(0, 10, n) * (rng.random(n) > 0.3)
+    y0 = rng.standard_normal(m)
+    c = A.T @ y0 + np.where(x0 == 0, rng.uniform(0, 1, n), 0.0)
+    b = A @ x0
+    return engine.LP.from_dense(A, c, np.zeros(n), np.full(n, 10.0), b, b)
+
+
+def test_long_row_solves_are_deterministic(engine):
+    lp = long_row_lp(engine)
+    for checks in (0, 1):  # host checks, device checks
+        a, b = (engine.solve_lp(lp, iteration_limit=600, device_checks=checks) for _ in range(2))
+        assert a["iterations"] == b["iterations"] and a["rejected_steps"] == b["rejected_steps"]
+        assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
+
+
 def test_index_width_invariant(engine):
     inst = engine.Instance(2, 5, 1, 3)
     a = engine.solve_lp(inst.lp, gpu_indices="32")
