@@ -884,6 +884,18 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     }
 }
 
+// Many small device-to-host copies at once: piece b (source, offset, count)
+// is copied into one staging buffer, which then travels in a single copy.
+struct CopyPiece {
+    const double *src;
+    long long offset, count;
+};
+
+__global__ void gather_pieces_kernel(const CopyPiece *pieces, double *out) {
+    const CopyPiece p = pieces[blockIdx.x];
+    for (long long i = threadIdx.x; i < p.count; i += blockDim.x) out[p.offset + i] = p.src[i];
+}
+
 // Check period end (src/solver.cpp:550-725): candidate evaluation, best
 // tracking, parking for host certification / Farkas checks, restart rule.
 __global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps, LpState *states, const int *slots,
@@ -1246,6 +1258,9 @@ class BatchEngine {
         hipFree(d_slots_);
         hipFree(d_pow3_);
         hipFree(d_pow6_);
+        if (d_stage_) hipFree(d_stage_);
+        if (h_stage_) hipHostFree(h_stage_);
+        if (d_pieces_) hipFree(d_pieces_);
         for (auto &list : class_events_)
             for (auto &e : list) hipEventDestroy(e);
         if (uploaded_) hipEventDestroy(uploaded_);
@@ -1843,8 +1858,48 @@ class BatchEngine {
         return L.total;
     }
 
+    // Queues a device-to-host copy; flush_downloads() performs every queued
+    // copy with one gather kernel and one transfer (thousands of small copies
+    // per scheduling round otherwise cost a runtime call each).
     void download(double *host, const double *dev, size_t count) {
-        if (count) HIP_OK(hipMemcpyAsync(host, dev, count * 8, hipMemcpyDeviceToHost, stream_));
+        if (count) pending_.push_back(Pending{host, dev, count});
+    }
+    void flush_downloads() {
+        if (pending_.empty()) {
+            HIP_OK(hipStreamSynchronize(stream_));
+            return;
+        }
+        std::vector<CopyPiece> pieces;
+        pieces.reserve(pending_.size());
+        long long total = 0;
+        for (const Pending &p : pending_) {
+            pieces.push_back(CopyPiece{p.dev, total, (long long)p.count});
+            total += (long long)p.count;
+        }
+        if (total > stage_capacity_) {
+            if (d_stage_) HIP_OK(hipFree(d_stage_));
+            if (h_stage_) HIP_OK(hipHostFree(h_stage_));
+            d_stage_ = nullptr;
+            h_stage_ = nullptr;
+            stage_capacity_ = std::max<long long>(total, 2 * stage_capacity_);
+            HIP_OK(hipMalloc(&d_stage_, size_t(stage_capacity_) * 8));
+            HIP_OK(hipHostMalloc(&h_stage_, size_t(stage_capacity_) * 8, hipHostMallocDefault));
+        }
+        if ((long long)pieces.size() > piece_capacity_) {
+            if (d_pieces_) HIP_OK(hipFree(d_pieces_));
+            d_pieces_ = nullptr;
+            piece_capacity_ = std::max<long long>((long long)pieces.size(), 2 * piece_capacity_);
+            HIP_OK(hipMalloc(&d_pieces_, sizeof(CopyPiece) * size_t(piece_capacity_)));
+        }
+        HIP_OK(hipMemcpyAsync(d_pieces_, pieces.data(), sizeof(CopyPiece) * pieces.size(), hipMemcpyHostToDevice,
+                              stream_));
+        gather_pieces_kernel<<<int(pieces.size()), kThreads, 0, stream_>>>(d_pieces_, d_stage_);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(h_stage_, d_stage_, size_t(total) * 8, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        for (size_t k = 0; k < pending_.size(); ++k)
+            std::memcpy(pending_[k].host, h_stage_ + pieces[k].offset, pending_[k].count * 8);
+        pending_.clear();
     }
 
     // Handles every LP whose workgroup stopped: certification, Farkas checks,
@@ -1898,7 +1953,7 @@ class BatchEngine {
                 download(epy[k].data(), d.epy, d.m);
             }
         }
-        HIP_OK(hipStreamSynchronize(stream_));
+        flush_downloads();
         std::vector<int> resume(stopped.size(), 0);
         std::mutex mu;
         nyk::parallel_tasks(int64_t(ns), [&](int64_t k) {
@@ -2132,6 +2187,15 @@ class BatchEngine {
     hipStream_t class_stream_[kClasses + 1] = {};  // per class, then the tail group
     int tail_rows_ = 1;
     double *d_pow3_ = nullptr, *d_pow6_ = nullptr;
+    struct Pending {
+        double *host;
+        const double *dev;
+        size_t count;
+    };
+    std::vector<Pending> pending_;
+    double *d_stage_ = nullptr, *h_stage_ = nullptr;
+    CopyPiece *d_pieces_ = nullptr;
+    long long stage_capacity_ = 0, piece_capacity_ = 0;
     long long pow_count_ = 0;
     int rows_per_thread_ = 2;  // measured best on gfx950 for the 3-8 nnz/column batch
     int class_cap_[kClasses - 1] = {};
