@@ -67,6 +67,17 @@ void niryukti_default_options(niryukti_options *o);
 int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *o, niryukti_result *r);
 const char *niryukti_engine_last_error(void);
 
+/* The host prepare step on its own (src/preprocess.cpp:133-346: presolve,
+ * scaling, objective scale), for tools and tests.  Output arrays are sized
+ * by the ORIGINAL model: ptr rows+1, index/value nnz, c/lb/ub/column_scale/
+ * col_map cols, rl/ru/row_scale/row_map rows.  Returns 0 (out_rows/out_cols
+ * set), 1 when presolve proved the LP infeasible (message in
+ * niryukti_engine_last_error), -1 on error. */
+int niryukti_prepare_lp(const niryukti_lp *lp, const niryukti_options *o, int64_t *out_rows, int64_t *out_cols,
+                        int64_t *ptr, int64_t *index, double *value, double *c, double *lb, double *ub, double *rl,
+                        double *ru, double *column_scale, double *row_scale, int64_t *col_map, int64_t *row_map,
+                        double *objective_scale, double *offset);
+
 /* Models read by the reference's readers (native .json or .mps, src/io.cpp:84-140,
  * 299-585): exactly one of `path` and `json_text` is given.  NULL on error. */
 typedef struct niryukti_model niryukti_model;

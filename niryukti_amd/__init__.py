@@ -135,6 +135,10 @@ def load_library() -> C.CDLL:
     lib.npdhg_reset.argtypes = [C.c_void_p, _f64p, _f64p]
     lib.npdhg_reset_candidate.argtypes = [C.c_void_p, C.c_int]
     lib.npdhg_spmv.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p]
+    lib.niryukti_prepare_lp.argtypes = [C.POINTER(_LP), C.POINTER(_Options), C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64), _i64p, _i64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                        _f64p, _f64p, _i64p, _i64p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.niryukti_prepare_lp.restype = C.c_int
     lib.npdhg_scale_lp.argtypes = [C.POINTER(_PdhgLP), C.c_int, C.POINTER(_Scaled)]
     lib.npdhg_scale_lp.restype = C.c_int
     lib.npdhg_scale_last_error.restype = C.c_char_p
@@ -265,6 +269,37 @@ def solve_lp(lp: LP, **options) -> dict:
     if rc != 0:
         raise RuntimeError(lib.niryukti_engine_last_error().decode())
     return _result_dict(res, x, y)
+
+
+def prepare_lp(lp: LP, **options) -> dict:
+    """The engine's host prepare step (presolve + scaling) on its own; returns
+    the prepared arrays, scales and maps, or {"failure": message}."""
+    lib = load_library()
+    keep = []
+    opts = _options_struct(lib, options, keep)
+    m, n, nnz = lp.rows, lp.cols, lp.nnz
+    out = {k: np.zeros(max(size, 1)) for k, size in (("value", nnz), ("c", n), ("lb", n), ("ub", n), ("rl", m),
+                                                      ("ru", m), ("column_scale", n), ("row_scale", m))}
+    out.update(ptr=np.zeros(m + 1, np.int64), index=np.zeros(max(nnz, 1), np.int64), col_map=np.zeros(max(n, 1), np.int64),
+               row_map=np.zeros(max(m, 1), np.int64))
+    rows, cols, osc, off = C.c_int64(), C.c_int64(), C.c_double(), C.c_double()
+    held = lp._struct()
+    rc = lib.niryukti_prepare_lp(C.byref(held), C.byref(opts), C.byref(rows), C.byref(cols), _ptr(out["ptr"], _i64p),
+                                 _ptr(out["index"], _i64p), _ptr(out["value"]), _ptr(out["c"]), _ptr(out["lb"]),
+                                 _ptr(out["ub"]), _ptr(out["rl"]), _ptr(out["ru"]), _ptr(out["column_scale"]),
+                                 _ptr(out["row_scale"]), _ptr(out["col_map"], _i64p), _ptr(out["row_map"], _i64p),
+                                 C.byref(osc), C.byref(off))
+    if rc < 0:
+        raise RuntimeError(lib.niryukti_engine_last_error().decode())
+    if rc > 0:
+        return {"failure": lib.niryukti_engine_last_error().decode()}
+    r, c = rows.value, cols.value
+    k = int(out["ptr"][r])
+    return {"failure": "", "rows": r, "cols": c, "ptr": out["ptr"][:r + 1], "index": out["index"][:k],
+            "value": out["value"][:k], "c": out["c"][:c], "lb": out["lb"][:c], "ub": out["ub"][:c],
+            "rl": out["rl"][:r], "ru": out["ru"][:r], "column_scale": out["column_scale"][:c],
+            "row_scale": out["row_scale"][:r], "col_map": out["col_map"][:c], "row_map": out["row_map"][:r],
+            "objective_scale": osc.value, "offset": off.value}
 
 
 def scale_lp(lp: LP, passes: int = 5) -> dict:

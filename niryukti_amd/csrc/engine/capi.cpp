@@ -1,6 +1,7 @@
 // Array-level C-ABI of the engine (include/niryukti_engine.h).
 #include "niryukti_engine.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -94,6 +95,47 @@ void niryukti_default_options(niryukti_options *o) {
     o->graphs = 1;
     o->device_checks = -1;
     o->device_prepare = -1;
+}
+
+int niryukti_prepare_lp(const niryukti_lp *lp, const niryukti_options *opts, int64_t *out_rows, int64_t *out_cols,
+                        int64_t *ptr, int64_t *index, double *value, double *c, double *lb, double *ub, double *rl,
+                        double *ru, double *column_scale, double *row_scale, int64_t *col_map, int64_t *row_map,
+                        double *objective_scale, double *offset) {
+    try {
+        g_error.clear();
+        if (!lp || !out_rows || !out_cols || !objective_scale || !offset) throw std::runtime_error("Null argument");
+        const nyk::Model m = to_model(*lp);
+        const nyk::Options o = to_options(opts, *lp);
+        const nyk::Prepared p = nyk::prepare(m, o);
+        if (!p.failure.empty()) {
+            g_error = p.failure + ": " + p.reason;
+            return 1;
+        }
+        const nyk::Model &q = p.model;
+        *out_rows = q.rows();
+        *out_cols = q.cols();
+        auto put = [](auto *dst, const auto &v) {
+            if (dst && !v.empty()) std::copy(v.begin(), v.end(), dst);
+        };
+        put(ptr, q.A.ptr);
+        put(index, q.A.index);
+        put(value, q.A.value);
+        put(c, q.c);
+        put(lb, q.lb);
+        put(ub, q.ub);
+        put(rl, q.rl);
+        put(ru, q.ru);
+        put(column_scale, p.column_scale);
+        put(row_scale, p.row_scale);
+        put(col_map, p.cols);
+        put(row_map, p.rows);
+        *objective_scale = p.objective_scale;
+        *offset = q.offset;
+        return 0;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return -1;
+    }
 }
 
 int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *opts, niryukti_result *out) {

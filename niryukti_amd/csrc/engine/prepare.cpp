@@ -155,19 +155,6 @@ void ruiz_pass(Prepared &p) {
     apply_scaling(p, std::vector<double>(m.rows(), 1), col);
 }
 
-struct RowKeyHash {
-    size_t operator()(const std::vector<std::pair<int64_t, double>> &key) const {
-        uint64_t h = 1469598103934665603ULL;
-        for (const auto &e : key) {
-            h = (h ^ uint64_t(e.first)) * 1099511628211ULL;
-            uint64_t bits;
-            static_assert(sizeof bits == sizeof e.second);
-            __builtin_memcpy(&bits, &e.second, sizeof bits);
-            h = (h ^ bits) * 1099511628211ULL;
-        }
-        return size_t(h);
-    }
-};
 
 }  // namespace
 
@@ -204,11 +191,21 @@ Prepared prepare(const Model &src, const Options &o) {
             if (!src.names.empty()) m.names.push_back(src.names[j]);
         }
     }
-    std::vector<Entry> entries;
-    entries.reserve(src.A.value.size());
-    std::unordered_map<std::vector<std::pair<int64_t, double>>, size_t, RowKeyHash> duplicate_rows;
+    // Per-row analysis (src/preprocess.cpp:185-252) is independent from row
+    // to row, so it runs in parallel; the decisions that depend on earlier
+    // rows (the first failure, duplicate rows) are then taken in row order,
+    // exactly as the sequential loop takes them.
+    enum : uint8_t { kKeep, kDrop, kEmptyInfeasible, kDisjoint };
+    struct RowInfo {
+        double l, u;
+        uint64_t hash;  // of (column, value) pairs, for rows without removed columns
+        uint8_t action;
+        bool keyed;     // presolve and count == len: candidate for duplicate detection
+    };
+    std::vector<RowInfo> info(static_cast<size_t>(m0));
     constexpr long double eps = std::numeric_limits<long double>::epsilon();
-    for (int64_t i = 0; i < m0; ++i) {
+    parallel_for(m0, m0 > 4096, [&](int64_t i) {
+        RowInfo &r = info[size_t(i)];
         const int64_t begin = src.A.ptr[i], end = src.A.ptr[i + 1], len = end - begin;
         long double shift = 0, shift_magnitude = 0;
         int64_t count = 0;
@@ -221,16 +218,22 @@ Prepared prepare(const Model &src, const Options &o) {
             } else
                 ++count;
         }
-        double l = src.rl[i] - double(shift), u = src.ru[i] - double(shift);
+        r.l = src.rl[i] - double(shift);
+        r.u = src.ru[i] - double(shift);
+        r.action = kKeep;
+        r.keyed = false;
+        r.hash = 0;
         // Rows left empty by substitution: infeasible or dropped (:216-226).
         if (count == 0 && o.presolve) {
             long double guard = 8 * eps * (len + 1) * (1 + shift_magnitude);
             if (shift + guard < src.rl[i] || shift - guard > src.ru[i]) {
-                p.failure = "INFEASIBLE";
-                p.reason = "Empty row after fixed-variable substitution: " + src.row_name(i);
-                return p;
+                r.action = kEmptyInfeasible;
+                return;
             }
-            if (l <= 0 && u >= 0) continue;
+            if (r.l <= 0 && r.u >= 0) {
+                r.action = kDrop;
+                return;
+            }
         }
         // Activity bounds over the box prove infeasibility or redundancy (:227-252).
         long double amin = 0, amax = 0, magnitude = 0;
@@ -245,55 +248,158 @@ Prepared prepare(const Model &src, const Options &o) {
         if (o.presolve) {
             long double slack = 8 * eps * (len + 1) * (1 + magnitude);
             if (std::isfinite(amin) && std::isfinite(amax) && (amin > src.ru[i] + slack || amax < src.rl[i] - slack)) {
-                p.failure = "INFEASIBLE";
-                p.reason = "Verified disjoint row activity bounds: " + src.row_name(i);
-                return p;
+                r.action = kDisjoint;
+                return;
             }
         }
         if (o.presolve && std::isfinite(amin) && std::isfinite(amax)) {
             long double guard = 8 * eps * (len + 1) * (1 + magnitude);
-            if (amin - guard >= src.rl[i] && amax + guard <= src.ru[i]) continue;
+            if (amin - guard >= src.rl[i] && amax + guard <= src.ru[i]) {
+                r.action = kDrop;
+                return;
+            }
         }
-        // Exactly repeated rows are intersected into one (:253-283).
         if (o.presolve && count == len) {
-            std::vector<std::pair<int64_t, double>> key;
-            key.reserve(size_t(len));
-            for (int64_t k = begin; k < end; ++k) key.emplace_back(map[src.A.index[k]], src.A.value[k]);
-            auto found = duplicate_rows.find(key);
-            if (found != duplicate_rows.end()) {
-                size_t r = found->second;
-                if (l > m.rl[r]) {
-                    m.rl[r] = l;
-                    p.lower_rows[r] = i;
+            r.keyed = true;
+            uint64_t h = 1469598103934665603ULL;
+            for (int64_t k = begin; k < end; ++k) {
+                h = (h ^ uint64_t(map[src.A.index[k]])) * 1099511628211ULL;
+                uint64_t bits;
+                __builtin_memcpy(&bits, &src.A.value[k], sizeof bits);
+                h = (h ^ bits) * 1099511628211ULL;
+            }
+            r.hash = h;
+        }
+    });
+    // Exactly repeated rows are intersected into one (:253-283): rows match
+    // when their (column, value) sequences are equal.
+    auto same_row = [&](int64_t a, int64_t b) {
+        const int64_t la = src.A.ptr[a + 1] - src.A.ptr[a], lb = src.A.ptr[b + 1] - src.A.ptr[b];
+        if (la != lb) return false;
+        for (int64_t t = 0; t < la; ++t) {
+            const int64_t ka = src.A.ptr[a] + t, kb = src.A.ptr[b] + t;
+            if (map[src.A.index[ka]] != map[src.A.index[kb]] || !(src.A.value[ka] == src.A.value[kb])) return false;
+        }
+        return true;
+    };
+    // Only rows whose hash occurs more than once can repeat an earlier row.
+    std::vector<uint8_t> shared_hash(static_cast<size_t>(m0), 0);
+    {
+        std::vector<std::pair<uint64_t, int64_t>> keyed;
+        for (int64_t i = 0; i < m0; ++i)
+            if (info[size_t(i)].action == kKeep && info[size_t(i)].keyed) keyed.emplace_back(info[size_t(i)].hash, i);
+        std::sort(keyed.begin(), keyed.end());
+        for (size_t a = 0; a < keyed.size();) {
+            size_t b = a + 1;
+            while (b < keyed.size() && keyed[b].first == keyed[a].first) ++b;
+            if (b - a > 1)
+                for (size_t t = a; t < b; ++t) shared_hash[size_t(keyed[t].second)] = 1;
+            a = b;
+        }
+    }
+    std::unordered_map<uint64_t, std::vector<std::pair<int64_t, size_t>>> duplicate_rows;  // hash -> (row, kept index)
+    std::vector<int64_t> kept;
+    kept.reserve(size_t(m0));
+    for (int64_t i = 0; i < m0; ++i) {
+        const RowInfo &r = info[size_t(i)];
+        if (r.action == kEmptyInfeasible) {
+            p.failure = "INFEASIBLE";
+            p.reason = "Empty row after fixed-variable substitution: " + src.row_name(i);
+            return p;
+        }
+        if (r.action == kDisjoint) {
+            p.failure = "INFEASIBLE";
+            p.reason = "Verified disjoint row activity bounds: " + src.row_name(i);
+            return p;
+        }
+        if (r.action == kDrop) continue;
+        const double l = r.l, u = r.u;
+        if (r.keyed && shared_hash[size_t(i)]) {
+            auto &bucket = duplicate_rows[r.hash];
+            bool merged = false;
+            for (const auto &[other, rr] : bucket) {
+                if (!same_row(other, i)) continue;
+                if (l > m.rl[rr]) {
+                    m.rl[rr] = l;
+                    p.lower_rows[rr] = i;
                 }
-                if (u < m.ru[r]) {
-                    m.ru[r] = u;
-                    p.upper_rows[r] = i;
+                if (u < m.ru[rr]) {
+                    m.ru[rr] = u;
+                    p.upper_rows[rr] = i;
                 }
-                if (m.rl[r] > m.ru[r]) {
+                if (m.rl[rr] > m.ru[rr]) {
                     p.failure = "INFEASIBLE";
                     p.reason = "Disjoint bounds on identical constraint rows";
                     p.failure_ray.assign(m0, 0);
-                    p.failure_ray[p.lower_rows[r]] = -1;
-                    p.failure_ray[p.upper_rows[r]] += 1;
+                    p.failure_ray[p.lower_rows[rr]] = -1;
+                    p.failure_ray[p.upper_rows[rr]] += 1;
                     if (!(Verifier(src).infeasibility_bound(p.failure_ray) > 0)) p.failure_ray.clear();
                     return p;
                 }
-                continue;
+                merged = true;
+                break;
             }
-            duplicate_rows.emplace(std::move(key), p.rows.size());
+            if (merged) continue;
+            bucket.emplace_back(i, p.rows.size());
         }
-        const int64_t row = int64_t(p.rows.size());
         p.rows.push_back(i);
         p.lower_rows.push_back(i);
         p.upper_rows.push_back(i);
         m.rl.push_back(l);
         m.ru.push_back(u);
         if (!src.row_names.empty()) m.row_names.push_back(src.row_names[i]);
-        for (int64_t k = begin; k < end; ++k)
-            if (map[src.A.index[k]] >= 0) entries.push_back({row, map[src.A.index[k]], src.A.value[k]});
+        kept.push_back(i);
     }
-    m.A = Csr::from_entries(int64_t(m.rl.size()), int64_t(m.c.size()), std::move(entries));
+    // The kept rows' entries, columns renumbered: already a canonical CSR when
+    // every row's renumbered columns increase and no value is zero (the usual
+    // case, since the source matrix is canonical); otherwise the general
+    // builder sorts and merges (src/model.cpp:11-38).
+    const int64_t rows_kept = int64_t(kept.size());
+    bool canonical = true;
+    int64_t total = 0;
+    for (int64_t r = 0; r < rows_kept && canonical; ++r) {
+        int64_t prev = -1;
+        for (int64_t k = src.A.ptr[kept[r]]; k < src.A.ptr[kept[r] + 1]; ++k) {
+            const int64_t j = map[src.A.index[k]];
+            if (j < 0) continue;
+            if (j <= prev || src.A.value[k] == 0 || !std::isfinite(src.A.value[k])) {
+                canonical = false;
+                break;
+            }
+            prev = j;
+            ++total;
+        }
+    }
+    if (canonical) {
+        Csr &A = m.A;
+        A.rows = rows_kept;
+        A.cols = int64_t(m.c.size());
+        A.ptr.assign(size_t(rows_kept) + 1, 0);
+        for (int64_t r = 0; r < rows_kept; ++r) {
+            int64_t c = 0;
+            for (int64_t k = src.A.ptr[kept[r]]; k < src.A.ptr[kept[r] + 1]; ++k) c += map[src.A.index[k]] >= 0;
+            A.ptr[size_t(r) + 1] = A.ptr[size_t(r)] + c;
+        }
+        A.index.resize(size_t(total));
+        A.value.resize(size_t(total));
+        parallel_for(rows_kept, rows_kept > 4096, [&](int64_t r) {
+            int64_t at = A.ptr[size_t(r)];
+            for (int64_t k = src.A.ptr[kept[r]]; k < src.A.ptr[kept[r] + 1]; ++k) {
+                const int64_t j = map[src.A.index[k]];
+                if (j < 0) continue;
+                A.index[size_t(at)] = j;
+                A.value[size_t(at)] = src.A.value[k];
+                ++at;
+            }
+        });
+    } else {
+        std::vector<Entry> entries;
+        entries.reserve(src.A.value.size());
+        for (int64_t r = 0; r < rows_kept; ++r)
+            for (int64_t k = src.A.ptr[kept[r]]; k < src.A.ptr[kept[r] + 1]; ++k)
+                if (map[src.A.index[k]] >= 0) entries.push_back({r, map[src.A.index[k]], src.A.value[k]});
+        m.A = Csr::from_entries(rows_kept, int64_t(m.c.size()), std::move(entries));
+    }
     p.column_scale.assign(m.c.size(), 1);
     p.row_scale.assign(m.rl.size(), 1);
     const bool combined = o.scaling == "combined" && o.scaling_passes > 0;
