@@ -199,15 +199,15 @@ def serve_stream(engine, qps, duration, seed=SEED, warm_fraction=0.2, hbm_budget
 
 
 def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16, cpu_iterations=None,
-              matrix_precision="fp64", verbose=False):
+              matrix_precision="fp64", verbose=False, index=0):
     """One LP of config 0 / 1 / 4 on the grid-wide single-LP engine for a fixed
     iteration budget, beside the reference's CPU PDHG on the same instance
     (oracle/_ref/ref_driver, `cpu_threads` OpenMP threads, `cpu_iterations`
     iterations, by default the same budget)."""
-    inst = engine.Instance(config, seed, scale)
+    inst = engine.Instance(config, seed, scale, index)
     lp = inst.lp
     m, n, nnz = lp.rows, lp.cols, lp.nnz
-    log(f"single: config {config} scale {scale}: {m} x {n}, {nnz} nonzeros")
+    log(f"single: config {config} scale {scale} index {index}: {m} x {n}, {nnz} nonzeros")
     r = engine.solve_lp(lp, iteration_limit=iterations, time_limit=1e9, tol=tol, matrix_precision=matrix_precision,
                         verbose=int(verbose))
     attempts = r["iterations"] + r["rejected_steps"]
@@ -216,7 +216,7 @@ def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16,
     # algorithmic bytes of one attempt (s_v-byte values, int32 indices), original sizes
     b_attempt = 2 * nnz * (s_v + 4) + (n + m + 2) * s_p + m * 8 + n * 8 + 64 * (n + m)
     achieved = b_attempt * attempts / max(r["iteration_seconds"], 1e-12) / 1e9
-    out = {"mode": "single", "config": config, "scale": scale, "rows": m, "cols": n, "nonzeros": nnz,
+    out = {"mode": "single", "config": config, "scale": scale, "index": index, "rows": m, "cols": n, "nonzeros": nnz,
            "matrix_precision": matrix_precision,
            "tol": tol, "status": r["status"], "iterations": r["iterations"], "attempts": attempts,
            "kkt": r["kkt_error"], "objective": r["objective"], "planted_objective": inst.planted_objective,
@@ -228,7 +228,7 @@ def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16,
            "achieved_GBs": round(achieved, 1), "hbm_frac": round(achieved / HBM_PEAK_GBS, 4)}
     ref = ROOT / "oracle" / "_ref" / "ref_driver"
     if cpu_threads and ref.exists():
-        cmd = [str(ref), "solve", "--config", str(config), "--seed", str(seed), "--scale", str(scale),
+        cmd = [str(ref), "solve", "--config", str(config), "--seed", str(seed), "--scale", str(scale), "--index", str(index),
                "--iterations", str(cpu_iterations or iterations), "--threads", str(cpu_threads), "--tol", str(tol)]
         c = json.loads(subprocess.run(cmd, capture_output=True, text=True, check=True).stdout)
         out["cpu_baseline"] = {"kind": "reference", "threads": cpu_threads, "status": c["status"],
@@ -370,6 +370,7 @@ def main():
     ap.add_argument("--single", type=int, default=None, metavar="CONFIG",
                     help="one LP of config 0, 1 or 4 on the single-LP engine beside the reference CPU")
     ap.add_argument("--scale", type=int, default=1)
+    ap.add_argument("--index", type=int, default=0, help="single-LP mode: instance index (configs 2 and 3)")
     ap.add_argument("--iterations", type=int, default=1000)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -411,7 +412,8 @@ def main():
     if args.single is not None:
         print(json.dumps(single_lp(engine, args.single, args.scale, args.iterations, args.tol,
                                    cpu_threads=args.cpu_threads, cpu_iterations=args.cpu_iterations,
-                                   matrix_precision=args.matrix_precision, verbose=args.verbose)), flush=True)
+                                   matrix_precision=args.matrix_precision, verbose=args.verbose,
+                                   index=args.index)), flush=True)
         return
     insts = make_instances(engine, args.count)
     nnz = sum(i.lp.nnz for i in insts)

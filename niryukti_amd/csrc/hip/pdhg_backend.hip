@@ -922,10 +922,17 @@ HostSegments make_segments(const int64_t *ptr, long long rows, long long thresho
 // Gathered vectors up to this size stay in one XCD's L2 (4 MB) next to the
 // streamed matrix; larger ones are split into chunks of at most this size.
 constexpr long long kChunkBytes = 2LL << 20;
+// Below this size a gathered vector is served well enough by the L2s and the
+// memory-side cache that the partial planes of a chunked product cost more
+// than they save (measured on 2.9-8 MB vectors of 3-6M-nonzero LPs).
+constexpr long long kChunkMinVector = 12LL << 20;
 
 int chunk_count(long long vector_length) {
     if (const char *e = std::getenv("NIRYUKTI_XCD_CHUNKS"))
         if (std::atoi(e) == 0) return 0;
+    long long min_bytes = kChunkMinVector;
+    if (const char *e = std::getenv("NIRYUKTI_XCD_CHUNK_MIN_MB")) min_bytes = std::atoll(e) << 20;
+    if (vector_length * 8 < min_bytes) return 0;
     int p = 1;
     while (p < 8 && vector_length * 8 / p > kChunkBytes) p *= 2;
     return p > 1 ? p : 0;

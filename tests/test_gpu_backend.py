@@ -162,14 +162,16 @@ def test_spmv_long_rows_and_columns(engine):
 
 
 def test_xcd_chunked_products_track_oracle(engine, oracle, monkeypatch):
-    """Config 1 at half size: both gathered vectors exceed 2 MB, so A dx and
-    A^T y run as XCD column chunks plus fixed-order partial sums.  Three
-    adaptive chunks must track the oracle, and the chunked and direct paths
-    must agree to rounding."""
+    """Config 1 at half size with column chunks forced down to 2 MB vectors
+    (by default only vectors from 12 MB are chunked): A dx and A^T y run as
+    XCD column chunks plus fixed-order partial sums.  Three adaptive chunks
+    must track the oracle, and the chunked and direct paths must agree to
+    rounding."""
     inst = engine.Instance(1, 11, 2, 0)
     lp = inst.lp
     x0, y0 = np.zeros(lp.cols), np.zeros(lp.rows)
     cpu = oracle.Backend(lp, x0, y0, adaptive=True)
+    monkeypatch.setenv("NIRYUKTI_XCD_CHUNK_MIN_MB", "2")
     gpu = engine.Backend(lp, x0, y0, adaptive=True)
     monkeypatch.setenv("NIRYUKTI_XCD_CHUNKS", "0")
     direct = engine.Backend(lp, x0, y0, adaptive=True)
