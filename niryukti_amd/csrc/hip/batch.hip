@@ -1203,7 +1203,6 @@ class BatchEngine {
         stream_ = pool[0];
         for (int c = 0; c <= kClasses; ++c) class_stream_[c] = pool[1 + c];
         if (const char *env = std::getenv("NIRYUKTI_BATCH_ROWS_PER_THREAD")) rows_per_thread_ = std::atoi(env);
-        if (const char *env = std::getenv("NIRYUKTI_BATCH_TAIL_ROWS")) tail_rows_ = std::atoi(env);
         // Class capacities (m + n) bounded by what one workgroup may allocate
         // next to the kernel's static LDS.
         int device = 0;
@@ -1605,12 +1604,10 @@ class BatchEngine {
     void launch_lds_t(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
         batch_advance_lds_kernel<NT, NR><<<grid, NT, lds_bytes(cap), cs>>>(d_desc_, d_state_, slots, bo);
     }
+    // The tail group: one row per thread over 1,024 threads (2 or 4 rows per
+    // thread spill registers at this size and were measured slower).
     void launch_tail(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
-        switch (tail_rows_) {
-        case 4: launch_lds_t<kTailThreads, 4>(cap, grid, cs, slots, bo); break;
-        case 2: launch_lds_t<kTailThreads, 2>(cap, grid, cs, slots, bo); break;
-        default: launch_lds_t<kTailThreads, 1>(cap, grid, cs, slots, bo);
-        }
+        launch_lds_t<kTailThreads, 1>(cap, grid, cs, slots, bo);
     }
     // A crowded LDS class: 256-thread workgroups (512 for the largest class)
     // with rows_per_thread_ rows per thread.
@@ -2185,7 +2182,6 @@ class BatchEngine {
     std::vector<int> slot_job_, free_slots_;
     std::vector<double> slot_bytes_;
     hipStream_t class_stream_[kClasses + 1] = {};  // per class, then the tail group
-    int tail_rows_ = 1;
     double *d_pow3_ = nullptr, *d_pow6_ = nullptr;
     struct Pending {
         double *host;
