@@ -42,6 +42,7 @@ typedef struct {
     int presolve, restart, adaptive, graphs, verbose;
     int device_checks;       /* -1 auto (>= 100,000 nonzeros), 0 host checks, 1 device checks */
     int device_prepare;      /* Ruiz scaling + transpose: -1 auto (>= 100,000 nonzeros, ruiz), 0 host, 1 device */
+    int power_iterations;    /* 0..1000: power-iteration estimate of ||A||_2 for the first step (0 = off) */
     const double *initial_x; /* NULL or cols */
     const double *initial_y; /* NULL or rows */
 } niryukti_options;
@@ -58,6 +59,7 @@ typedef struct {
     double seconds, preprocess_seconds, transfer_seconds, iteration_seconds, verification_seconds;
     int index_bits;
     int device_prepare; /* 1 when scaling and transpose ran on the device */
+    double operator_norm_estimate; /* power-iteration estimate (0 when not requested) */
     double *x; /* NULL or cols: receives the primal solution */
     double *y; /* NULL or rows: receives the dual solution */
 } niryukti_result;
@@ -77,6 +79,11 @@ int niryukti_prepare_lp(const niryukti_lp *lp, const niryukti_options *o, int64_
                         int64_t *ptr, int64_t *index, double *value, double *c, double *lb, double *ub, double *rl,
                         double *ru, double *column_scale, double *row_scale, int64_t *col_map, int64_t *row_map,
                         double *objective_scale, double *offset);
+
+/* Power-iteration estimate of ||A||_2 of the LP's matrix (src/research.cpp:6-50:
+ * same seed, start vector and normalisation order as the reference); 0 when
+ * iterations < 1, the matrix is empty or the input is invalid. */
+double niryukti_power_norm(const niryukti_lp *lp, int iterations);
 
 /* Models read by the reference's readers (native .json or .mps, src/io.cpp:84-140,
  * 299-585): exactly one of `path` and `json_text` is given.  NULL on error. */

@@ -41,7 +41,8 @@ class _Options(C.Structure):
                 ("tol", C.c_double), ("time_limit", C.c_double), ("iteration_limit", C.c_int64),
                 ("check_every", C.c_int), ("scaling_passes", C.c_int), ("threads", C.c_int),
                 ("presolve", C.c_int), ("restart", C.c_int), ("adaptive", C.c_int), ("graphs", C.c_int),
-                ("verbose", C.c_int), ("device_checks", C.c_int), ("device_prepare", C.c_int), ("initial_x", _f64p),
+                ("verbose", C.c_int), ("device_checks", C.c_int), ("device_prepare", C.c_int),
+                ("power_iterations", C.c_int), ("initial_x", _f64p),
                 ("initial_y", _f64p)]
 
 
@@ -56,7 +57,8 @@ class _Result(C.Structure):
                 ("removed_rows", C.c_int64), ("removed_columns", C.c_int64), ("seconds", C.c_double),
                 ("preprocess_seconds", C.c_double), ("transfer_seconds", C.c_double),
                 ("iteration_seconds", C.c_double), ("verification_seconds", C.c_double),
-                ("index_bits", C.c_int), ("device_prepare", C.c_int), ("x", _f64p), ("y", _f64p)]
+                ("index_bits", C.c_int), ("device_prepare", C.c_int), ("operator_norm_estimate", C.c_double),
+                ("x", _f64p), ("y", _f64p)]
 
 
 class _PdhgLP(C.Structure):
@@ -139,6 +141,8 @@ def load_library() -> C.CDLL:
                                         C.POINTER(C.c_int64), _i64p, _i64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
                                         _f64p, _f64p, _i64p, _i64p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.niryukti_prepare_lp.restype = C.c_int
+    lib.niryukti_power_norm.argtypes = [C.POINTER(_LP), C.c_int]
+    lib.niryukti_power_norm.restype = C.c_double
     lib.npdhg_scale_lp.argtypes = [C.POINTER(_PdhgLP), C.c_int, C.POINTER(_Scaled)]
     lib.npdhg_scale_lp.restype = C.c_int
     lib.npdhg_scale_last_error.restype = C.c_char_p
@@ -300,6 +304,13 @@ def prepare_lp(lp: LP, **options) -> dict:
             "rl": out["rl"][:r], "ru": out["ru"][:r], "column_scale": out["column_scale"][:c],
             "row_scale": out["row_scale"][:r], "col_map": out["col_map"][:c], "row_map": out["row_map"][:r],
             "objective_scale": osc.value, "offset": off.value}
+
+
+def power_norm(lp: LP, iterations: int) -> float:
+    """Power-iteration estimate of ||A||_2 (the reference's --power-iterations)."""
+    lib = load_library()
+    s = lp._struct()
+    return float(lib.niryukti_power_norm(C.byref(s), int(iterations)))
 
 
 def scale_lp(lp: LP, passes: int = 5) -> dict:

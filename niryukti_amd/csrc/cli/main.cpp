@@ -33,7 +33,8 @@ void usage() {
                  "solve options: --device gpu|hip|auto --method pdhg|halpern|rhpdhg|r2hpdhg --tol X\n"
                  "       --time-limit S --iterations N --threads T --check-every K --scaling-passes P\n"
                  "       --scaling ruiz|combined --primal-weight displacement|pid --gpu-indices auto|32|64\n"
-                 "       --matrix-precision fp64|mixed --device-prepare auto|on|off --no-presolve --no-restart\n"
+                 "       --matrix-precision fp64|mixed --device-prepare auto|on|off --power-iterations N\n"
+                 "       --no-presolve --no-restart\n"
                  "       --no-adaptive --verbose\n"
                  "       --warm-start RESULT.json --allow-model-change --json-out FILE (--solution-out)\n"
                  "       --cuda-graphs and --gpu-monitor are accepted: graphs and device checks are always on\n"
@@ -63,6 +64,7 @@ int solve_command(int argc, char **argv, const nyk::Model &m) {
         else if (a == "--primal-weight") o.primal_weight = val();
         else if (a == "--gpu-indices") o.gpu_indices = val();
         else if (a == "--matrix-precision") o.matrix_precision = val();
+        else if (a == "--power-iterations") o.power_iterations = std::stoi(val());
         else if (a == "--device-prepare") {
             const std::string v = val();
             if (v != "auto" && v != "on" && v != "off") throw std::runtime_error("--device-prepare auto|on|off");

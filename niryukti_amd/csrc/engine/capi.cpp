@@ -58,6 +58,7 @@ nyk::Options to_options(const niryukti_options *in, const niryukti_lp &lp) {
     o.verbose = in->verbose != 0;
     o.device_checks = in->device_checks < 0 ? -1 : in->device_checks > 0 ? 1 : 0;
     o.device_prepare = in->device_prepare < 0 ? -1 : in->device_prepare > 0 ? 1 : 0;
+    o.power_iterations = in->power_iterations;
     if (in->initial_x) o.initial_x.assign(in->initial_x, in->initial_x + lp.cols);
     if (in->initial_y) o.initial_y.assign(in->initial_y, in->initial_y + lp.rows);
     return o;
@@ -95,6 +96,7 @@ void niryukti_default_options(niryukti_options *o) {
     o->graphs = 1;
     o->device_checks = -1;
     o->device_prepare = -1;
+    o->power_iterations = 0;
 }
 
 int niryukti_prepare_lp(const niryukti_lp *lp, const niryukti_options *opts, int64_t *out_rows, int64_t *out_cols,
@@ -138,6 +140,17 @@ int niryukti_prepare_lp(const niryukti_lp *lp, const niryukti_options *opts, int
     }
 }
 
+double niryukti_power_norm(const niryukti_lp *lp, int iterations) {
+    try {
+        g_error.clear();
+        if (!lp) throw std::runtime_error("Null argument");
+        return nyk::power_norm(to_model(*lp).A, iterations);
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return 0;
+    }
+}
+
 int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *opts, niryukti_result *out) {
     try {
         g_error.clear();
@@ -173,6 +186,7 @@ int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *opts, niryu
         out->verification_seconds = r.verification_seconds;
         out->index_bits = r.gpu_index_bits;
         out->device_prepare = r.device_prepare ? 1 : 0;
+        out->operator_norm_estimate = r.operator_norm_estimate;
         if (out->x && r.x.size() == size_t(lp->cols)) std::memcpy(out->x, r.x.data(), r.x.size() * sizeof(double));
         if (out->y && r.y.size() == size_t(lp->rows)) std::memcpy(out->y, r.y.data(), r.y.size() * sizeof(double));
         return 0;
@@ -272,6 +286,7 @@ const char *niryukti_result_json(const niryukti_model *h, const niryukti_result 
         r.verification_seconds = in->verification_seconds;
         r.gpu_index_bits = in->index_bits;
         r.device_prepare = in->device_prepare != 0;
+        r.operator_norm_estimate = in->operator_norm_estimate;
         r.estimated_gpu_bytes = 48. * m.A.nnz() + 240. * (m.rows() + m.cols() + 2);
         if (in->x) r.x.assign(in->x, in->x + m.cols());
         if (in->y) r.y.assign(in->y, in->y + m.rows());

@@ -408,20 +408,7 @@ Result solve_continuous(const Model &original, const Options &o) {
         return r;
     }
     // ||A||_2 <= sqrt(||A||_1 ||A||_inf) gives the guaranteed step (solver.cpp:299-322).
-    std::vector<double> colsum(m.cols(), 0);
-    double rowmax = 0;
-    for (int64_t i = 0; i < m.rows(); ++i) {
-        double s = 0;
-        for (int64_t k = m.A.ptr[i]; k < m.A.ptr[i + 1]; ++k) {
-            double v = std::abs(m.A.value[k]);
-            s += v;
-            colsum[m.A.index[k]] += v;
-        }
-        rowmax = std::max(rowmax, s);
-    }
-    double colmax = 0;
-    for (double s : colsum) colmax = std::max(colmax, s);
-    const double norm = std::sqrt(rowmax) * std::sqrt(colmax);
+    const double norm = step_norm(m, o, &r.operator_norm_estimate);
     double step = norm > 0 ? .9 / norm : 1, weight = 1;
     if ((o.adaptive || anchored(o)) && o.primal_weight != "pid") {  // solver.cpp:323-333
         long double c2 = 0, b2 = 0;
@@ -595,7 +582,8 @@ Result solve_continuous(const Model &original, const Options &o) {
 // src/solver.cpp:18-54, restricted to the options this engine implements.
 void validate_options(const Options &o) {
     if (!(o.tol > 0 && o.tol < 1) || !std::isfinite(o.tol) || o.iteration_limit < 0 || o.time_limit < 0 ||
-        std::isnan(o.time_limit) || o.check_every < 1 || o.scaling_passes < 0 || o.scaling_passes > 20 || o.threads < 1)
+        std::isnan(o.time_limit) || o.check_every < 1 || o.scaling_passes < 0 || o.scaling_passes > 20 || o.threads < 1 ||
+        o.power_iterations < 0 || o.power_iterations > 1000)
         throw std::runtime_error("Invalid solver options");
     if (o.device != "auto" && o.device != "cpu" && o.device != "cuda" && o.device != "hip" && o.device != "gpu")
         throw std::runtime_error("Unknown device");

@@ -528,7 +528,7 @@ Json result_json(const Model &m, const Result &r) {
     perf["weight_updates"] = (long long)r.weight_updates;
     perf["polishing_iterations"] = 0;
     perf["polishing_attempts"] = 0;
-    perf["operator_norm_estimate"] = 0.0;
+    perf["operator_norm_estimate"] = r.operator_norm_estimate;
     perf["restarts"] = (long long)r.restarts;
     perf["rejected_steps"] = (long long)r.rejected_steps;
     perf["monitor_checks"] = (long long)r.device_candidate_checks;

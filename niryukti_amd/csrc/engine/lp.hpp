@@ -69,6 +69,10 @@ struct Options {
     // models with >= 100,000 nonzeros and "ruiz" scaling, 0 on the host, 1 on
     // the device (presolve always runs on the host).
     int device_prepare = -1;
+    // Power iterations for an estimate of ||A||_2 (src/research.cpp:6-50);
+    // 0 (the default) skips it.  With adaptive steps the initial step uses
+    // min(bound, 1.05 x estimate) (src/solver.cpp:315-321).
+    int power_iterations = 0;
     std::vector<double> initial_x, initial_y;
     std::shared_ptr<std::atomic<bool>> cancellation;
 };
@@ -91,6 +95,7 @@ struct Result {
     int gpu_index_bits = 0;
     bool graph_execution = false;
     bool device_prepare = false;  // scaling and transpose ran on the device
+    double operator_norm_estimate = 0;
     std::string matrix_precision = "fp64", method_selected, device_reason;
     double estimated_gpu_bytes = 0;
     // The request as the caller made it (reported under "selection").
@@ -115,6 +120,14 @@ struct Prepared {
 };
 
 Prepared prepare(const Model &, const Options &);
+
+// Power-iteration estimate of ||A||_2 (src/research.cpp:6-50): 0 when
+// iterations < 1 or the matrix is empty.  Same seed, start vector and
+// normalisation order as the reference, so the estimate is the same.
+double power_norm(const Csr &A, int iterations);
+// The initial step's operator-norm bound sqrt(||A||_1 ||A||_inf), lowered to
+// 1.05 x the power estimate for adaptive steps (src/solver.cpp:299-321).
+double step_norm(const Model &scaled, const Options &o, double *estimate);
 
 class Verifier {
     const Model &model_;

@@ -12,6 +12,7 @@
 #include "parallel.hpp"
 
 #include <algorithm>
+#include <random>
 #include <stdexcept>
 #include <unordered_map>
 
@@ -415,6 +416,69 @@ Prepared prepare(const Model &src, const Options &o) {
     }
     m.validate();
     return p;
+}
+
+double power_norm(const Csr &A, int iterations) {
+    if (iterations < 1 || A.value.empty()) return 0;
+    double scale = 0;
+    for (double v : A.value) scale = std::max(scale, std::abs(v));
+    if (!(scale > 0) || !std::isfinite(scale)) return 0;
+    // On A / max|a|, so that A^T A cannot overflow from the units alone.
+    Csr a = A;
+    for (double &v : a.value) v /= scale;
+    const Csr at = a.transposed();
+    std::mt19937_64 random(0x56414e54414745ULL);
+    std::vector<double> x(size_t(a.cols));
+    for (double &v : x) v = double(random() >> 11) * 0x1p-53 - .5;
+    // Euclidean norm by running hypot, in index order.
+    auto norm_of = [](const std::vector<double> &v) {
+        double n = 0;
+        for (double e : v) n = std::hypot(n, e);
+        return n;
+    };
+    auto normalize = [&](std::vector<double> &v) {
+        const double n = norm_of(v);
+        if (!(n > 0) || !std::isfinite(n)) return false;
+        for (double &e : v) e /= n;
+        return true;
+    };
+    if (!normalize(x)) return 0;
+    double estimate = 0;
+    std::vector<double> y(size_t(a.rows));
+    for (int k = 0; k < iterations; ++k) {
+        a.multiply(x.data(), y.data());
+        estimate = std::max(estimate, norm_of(y));
+        if (!normalize(y)) break;
+        at.multiply(y.data(), x.data());
+        if (!normalize(x)) break;
+    }
+    const double result = scale * estimate;
+    return std::isfinite(result) ? result : 0;
+}
+
+double step_norm(const Model &m, const Options &o, double *estimate) {
+    std::vector<double> colsum(size_t(m.cols()), 0);
+    double rowmax = 0;
+    for (int64_t i = 0; i < m.rows(); ++i) {
+        double s = 0;
+        for (int64_t k = m.A.ptr[i]; k < m.A.ptr[i + 1]; ++k) {
+            double v = std::abs(m.A.value[k]);
+            s += v;
+            colsum[size_t(m.A.index[k])] += v;
+        }
+        rowmax = std::max(rowmax, s);
+    }
+    double colmax = 0;
+    for (double s : colsum) colmax = std::max(colmax, s);
+    double norm = std::sqrt(rowmax) * std::sqrt(colmax);
+    if (estimate) *estimate = 0;
+    if (o.power_iterations > 0) {
+        const double e = power_norm(m.A, o.power_iterations);
+        if (estimate) *estimate = e;
+        // an estimate is not a bound: only backtracking steps may use it
+        if (o.adaptive && e > 0) norm = std::min(norm, 1.05 * e);
+    }
+    return norm;
 }
 
 }  // namespace nyk

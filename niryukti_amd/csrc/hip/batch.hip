@@ -1564,20 +1564,7 @@ class BatchEngine {
             finish("TIME_LIMIT", "");
             return;
         }
-        std::vector<double> colsum(m.cols(), 0);
-        double rowmax = 0;
-        for (int64_t i = 0; i < m.rows(); ++i) {
-            double s = 0;
-            for (int64_t k = m.A.ptr[i]; k < m.A.ptr[i + 1]; ++k) {
-                double v = std::abs(m.A.value[k]);
-                s += v;
-                colsum[m.A.index[k]] += v;
-            }
-            rowmax = std::max(rowmax, s);
-        }
-        double colmax = 0;
-        for (double s : colsum) colmax = std::max(colmax, s);
-        const double norm = std::sqrt(rowmax) * std::sqrt(colmax);
+        const double norm = nyk::step_norm(m, o, &j.result.operator_norm_estimate);
         j.step = norm > 0 ? .9 / norm : 1;
         j.weight = 1;
         if (o.adaptive) {

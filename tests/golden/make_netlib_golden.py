@@ -3,7 +3,8 @@
 Runs the reference build (oracle/_ref/niryukti_ref, from oracle/ref.mk) with
 `solve MODEL --method pdhg --device cpu` and `inspect MODEL`, and stores the
 result documents under tests/golden/netlib/ as data; also keeps the reference's
-`convert afiro.mps afiro.reference_convert.mps` output.  Run from the repository
+`convert afiro.mps afiro.reference_convert.mps` output and its --power-iterations
+estimates (reference_power.json).  Run from the repository
 root after `make -f oracle/ref.mk REF=/root/reference`.
 """
 import json
@@ -27,3 +28,12 @@ for name in ("afiro", "adlittle", "israel", "e226"):
     print(name, doc["status"], doc["objective"], doc["performance"]["iterations"], inspect["fingerprint"])
 
 subprocess.run([str(CLI), "convert", str(OUT / "afiro.mps"), str(OUT / "afiro.reference_convert.mps")], check=True)
+
+# operator-norm estimates of the prepared models (--power-iterations, src/research.cpp:6-50)
+power = {}
+for name in ("afiro", "adlittle", "israel", "e226"):
+    for its in (1, 30):
+        res = subprocess.run([str(CLI), "solve", str(OUT / f"{name}.mps"), "--method", "pdhg", "--device", "cpu",
+                              "--power-iterations", str(its), "--iterations", "1"], capture_output=True, text=True)
+        power[f"{name}/{its}"] = json.loads(res.stdout)["performance"]["operator_norm_estimate"]
+(OUT / "reference_power.json").write_text(json.dumps(power, indent=1) + "\n")

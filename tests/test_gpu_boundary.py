@@ -99,3 +99,20 @@ def test_session_on_netlib_warm_start(engine):
         assert again["performance"]["iterations"] <= cold["performance"]["iterations"]
         fresh = s.solve(warm_start=False)
         assert fresh["performance"]["iterations"] == cold["performance"]["iterations"]
+
+
+def test_power_iterations_estimate_and_solve(engine):
+    """--power-iterations: the result reports the reference's estimate (the
+    same bits, tests/golden/netlib/reference_power.json), the solve with the
+    larger first step still certifies, and the option is range-checked like
+    the reference's (src/solver.cpp:20-25)."""
+    ref = json.loads((NETLIB / "reference_power.json").read_text())
+    out = run_cli(engine, "solve", NETLIB / "afiro.mps", "--power-iterations", "30")
+    assert out.returncode == 0, out.stderr
+    doc = json.loads(out.stdout)
+    assert doc["performance"]["operator_norm_estimate"] == ref["afiro/30"]
+    assert doc["status"] == "OPTIMAL"
+    assert run_cli(engine, "solve", NETLIB / "afiro.mps", "--power-iterations", "1001").returncode == 1
+    m = engine.Model(NETLIB / "adlittle.mps")
+    r = engine.solve_lp(m.lp, power_iterations=30)
+    assert r["operator_norm_estimate"] == ref["adlittle/30"] and r["status"] == "OPTIMAL"

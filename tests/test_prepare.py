@@ -3,6 +3,7 @@ the oracle's, bit for bit: the prepared arrays, scales and row/column maps
 (src/preprocess.cpp:133-346).  The oracle's prepare is itself pinned to the
 reference's output by hashes (tests/test_oracle.py).  Runs on the CPU.
 """
+import json
 from pathlib import Path
 
 import numpy as np
@@ -77,3 +78,28 @@ def test_presolve_infeasibility_matches_oracle(engine, oracle):
         ours = engine.prepare_lp(lp)
         ref = oracle.prepared_model(lp)
         assert ours["failure"].startswith("INFEASIBLE") and ref["failure"] == "INFEASIBLE"
+
+
+def prepared_lp(engine, lp):
+    p = engine.prepare_lp(lp)
+    return engine.LP(p["ptr"], p["index"], p["value"], p["c"], p["lb"], p["ub"], p["rl"], p["ru"], p["offset"], 1.0)
+
+
+@pytest.mark.parametrize("name", ["afiro", "adlittle", "israel", "e226"])
+def test_power_norm_matches_reference(engine, name):
+    """--power-iterations (src/research.cpp:6-50, src/solver.cpp:315-321): the
+    estimate on the prepared model equals the reference CLI's, bit for bit."""
+    ref = json.loads((NETLIB / "reference_power.json").read_text())
+    lp = prepared_lp(engine, engine.Model(NETLIB / f"{name}.mps").lp)
+    for its in (1, 30):
+        assert engine.power_norm(lp, its) == ref[f"{name}/{its}"], its
+
+
+def test_power_norm_closed_forms(engine):
+    # diagonal: the largest |entry|; rank one u v^T: |u| |v| once the start vector is in range
+    diag = engine.LP.from_dense(np.diag([3.0, -1.0, 2.0]), np.zeros(3), np.zeros(3), np.ones(3), np.zeros(3), np.ones(3))
+    assert abs(engine.power_norm(diag, 200) - 3.0) <= 1e-9
+    u, v = np.array([1.0, -2.0, 2.0]), np.array([0.5, 0.0, 1.5, -1.0])
+    rank1 = engine.LP.from_dense(np.outer(u, v), np.zeros(4), np.zeros(4), np.ones(4), np.zeros(3), np.ones(3))
+    assert abs(engine.power_norm(rank1, 3) - np.linalg.norm(u) * np.linalg.norm(v)) <= 1e-12 * 10
+    assert engine.power_norm(diag, 0) == 0.0
