@@ -43,6 +43,7 @@ typedef struct {
     int device_checks;       /* -1 auto (>= 100,000 nonzeros), 0 host checks, 1 device checks */
     int device_prepare;      /* Ruiz scaling + transpose: -1 auto (>= 100,000 nonzeros, ruiz), 0 host, 1 device */
     int power_iterations;    /* 0..1000: power-iteration estimate of ||A||_2 for the first step (0 = off) */
+    int polishing;           /* 1: feasibility polishing (src/solver.cpp:634-690), default 0 */
     const double *initial_x; /* NULL or cols */
     const double *initial_y; /* NULL or rows */
 } niryukti_options;
@@ -60,6 +61,7 @@ typedef struct {
     int index_bits;
     int device_prepare; /* 1 when scaling and transpose ran on the device */
     double operator_norm_estimate; /* power-iteration estimate (0 when not requested) */
+    int64_t polishing_iterations, polishing_attempts;
     double *x; /* NULL or cols: receives the primal solution */
     double *y; /* NULL or rows: receives the dual solution */
 } niryukti_result;

@@ -42,7 +42,7 @@ class _Options(C.Structure):
                 ("check_every", C.c_int), ("scaling_passes", C.c_int), ("threads", C.c_int),
                 ("presolve", C.c_int), ("restart", C.c_int), ("adaptive", C.c_int), ("graphs", C.c_int),
                 ("verbose", C.c_int), ("device_checks", C.c_int), ("device_prepare", C.c_int),
-                ("power_iterations", C.c_int), ("initial_x", _f64p),
+                ("power_iterations", C.c_int), ("polishing", C.c_int), ("initial_x", _f64p),
                 ("initial_y", _f64p)]
 
 
@@ -58,7 +58,7 @@ class _Result(C.Structure):
                 ("preprocess_seconds", C.c_double), ("transfer_seconds", C.c_double),
                 ("iteration_seconds", C.c_double), ("verification_seconds", C.c_double),
                 ("index_bits", C.c_int), ("device_prepare", C.c_int), ("operator_norm_estimate", C.c_double),
-                ("x", _f64p), ("y", _f64p)]
+                ("polishing_iterations", C.c_int64), ("polishing_attempts", C.c_int64), ("x", _f64p), ("y", _f64p)]
 
 
 class _PdhgLP(C.Structure):

@@ -34,7 +34,7 @@ void usage() {
                  "       --time-limit S --iterations N --threads T --check-every K --scaling-passes P\n"
                  "       --scaling ruiz|combined --primal-weight displacement|pid --gpu-indices auto|32|64\n"
                  "       --matrix-precision fp64|mixed --device-prepare auto|on|off --power-iterations N\n"
-                 "       --no-presolve --no-restart\n"
+                 "       --polishing --no-presolve --no-restart\n"
                  "       --no-adaptive --verbose\n"
                  "       --warm-start RESULT.json --allow-model-change --json-out FILE (--solution-out)\n"
                  "       --cuda-graphs and --gpu-monitor are accepted: graphs and device checks are always on\n"
@@ -45,7 +45,7 @@ int solve_command(int argc, char **argv, const nyk::Model &m) {
     nyk::Options o;
     o.method = "auto";
     std::string output, warm;
-    bool allow_model_change = false;
+    bool allow_model_change = false, cuda_graphs = false, gpu_monitor = false;
     for (int i = 3; i < argc; ++i) {
         std::string a = argv[i];
         auto val = [&]() {
@@ -65,6 +65,7 @@ int solve_command(int argc, char **argv, const nyk::Model &m) {
         else if (a == "--gpu-indices") o.gpu_indices = val();
         else if (a == "--matrix-precision") o.matrix_precision = val();
         else if (a == "--power-iterations") o.power_iterations = std::stoi(val());
+        else if (a == "--polishing") o.polishing = true;
         else if (a == "--device-prepare") {
             const std::string v = val();
             if (v != "auto" && v != "on" && v != "off") throw std::runtime_error("--device-prepare auto|on|off");
@@ -77,9 +78,10 @@ int solve_command(int argc, char **argv, const nyk::Model &m) {
         else if (a == "--no-presolve") o.presolve = false;
         else if (a == "--no-restart") o.restart = false;
         else if (a == "--no-adaptive") o.adaptive = false;
-        else if (a == "--cuda-graphs" || a == "--gpu-monitor" || a == "--auto") {
-            if (a == "--auto") o.method = "auto";
-        } else
+        else if (a == "--cuda-graphs") cuda_graphs = true;
+        else if (a == "--gpu-monitor") gpu_monitor = true;
+        else if (a == "--auto") o.method = "auto";
+        else
             throw std::runtime_error("Unknown option: " + a);
     }
     if (o.device == "auto") o.device = "gpu";
@@ -90,6 +92,26 @@ int solve_command(int argc, char **argv, const nyk::Model &m) {
     }
     auto r = nyk::solve(m, o);
     auto j = nyk::result_json(m, r);
+    // The options as requested (app/main.cpp:610-630; the branch-and-bound
+    // keys of that block have no counterpart in an LP engine).
+    auto &opt = j["options"];
+    opt = nyk::Json::object();
+    opt["tolerance"] = o.tol;
+    opt["time_limit"] = o.time_limit;
+    opt["iteration_limit"] = (long long)o.iteration_limit;
+    opt["threads"] = o.threads;
+    opt["scaling_passes"] = o.scaling_passes;
+    opt["scaling"] = o.scaling;
+    opt["method"] = o.method;
+    opt["primal_weight"] = o.primal_weight;
+    opt["power_iterations"] = o.power_iterations;
+    opt["cuda_graphs"] = cuda_graphs;
+    opt["gpu_monitor"] = gpu_monitor;
+    opt["gpu_indices"] = o.gpu_indices;
+    opt["matrix_precision"] = o.matrix_precision;
+    opt["polishing"] = o.polishing;
+    opt["restart"] = o.restart;
+    opt["adaptive"] = o.adaptive;
     const std::string text = j.dump(2);
     if (!output.empty()) {
         std::ofstream f(output);

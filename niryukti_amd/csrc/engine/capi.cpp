@@ -59,6 +59,7 @@ nyk::Options to_options(const niryukti_options *in, const niryukti_lp &lp) {
     o.device_checks = in->device_checks < 0 ? -1 : in->device_checks > 0 ? 1 : 0;
     o.device_prepare = in->device_prepare < 0 ? -1 : in->device_prepare > 0 ? 1 : 0;
     o.power_iterations = in->power_iterations;
+    o.polishing = in->polishing != 0;
     if (in->initial_x) o.initial_x.assign(in->initial_x, in->initial_x + lp.cols);
     if (in->initial_y) o.initial_y.assign(in->initial_y, in->initial_y + lp.rows);
     return o;
@@ -97,6 +98,7 @@ void niryukti_default_options(niryukti_options *o) {
     o->device_checks = -1;
     o->device_prepare = -1;
     o->power_iterations = 0;
+    o->polishing = 0;
 }
 
 int niryukti_prepare_lp(const niryukti_lp *lp, const niryukti_options *opts, int64_t *out_rows, int64_t *out_cols,
@@ -187,6 +189,8 @@ int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *opts, niryu
         out->index_bits = r.gpu_index_bits;
         out->device_prepare = r.device_prepare ? 1 : 0;
         out->operator_norm_estimate = r.operator_norm_estimate;
+        out->polishing_iterations = r.polishing_iterations;
+        out->polishing_attempts = r.polishing_attempts;
         if (out->x && r.x.size() == size_t(lp->cols)) std::memcpy(out->x, r.x.data(), r.x.size() * sizeof(double));
         if (out->y && r.y.size() == size_t(lp->rows)) std::memcpy(out->y, r.y.data(), r.y.size() * sizeof(double));
         return 0;
@@ -287,6 +291,8 @@ const char *niryukti_result_json(const niryukti_model *h, const niryukti_result 
         r.gpu_index_bits = in->index_bits;
         r.device_prepare = in->device_prepare != 0;
         r.operator_norm_estimate = in->operator_norm_estimate;
+        r.polishing_iterations = in->polishing_iterations;
+        r.polishing_attempts = in->polishing_attempts;
         r.estimated_gpu_bytes = 48. * m.A.nnz() + 240. * (m.rows() + m.cols() + 2);
         if (in->x) r.x.assign(in->x, in->x + m.cols());
         if (in->y) r.y.assign(in->y, in->y + m.rows());

@@ -178,6 +178,98 @@ bool certify_ray(Verifier &verifier, std::vector<double> ray, Result &r) {
     return true;
 }
 
+Result solve_continuous(const Model &original, const Options &o);
+
+// The dual feasibility problem of an LP (src/research.cpp:72-94): variables y
+// (one per row, sign-restricted where a row bound is infinite) and one
+// stationarity row per column, c_j + (A^T y)_j free or one-signed by the
+// column's bounds, zero objective.
+Model dual_feasibility_model(const Model &m) {
+    Model d;
+    d.name = m.name + "_dual_feasibility";
+    d.A = m.A.transposed();
+    d.c.assign(size_t(m.rows()), 0);
+    d.offset = 0;
+    d.sense = 1;
+    for (int64_t i = 0; i < m.rows(); ++i) {
+        d.names.push_back("y" + std::to_string(i));
+        d.lb.push_back(std::isfinite(m.rl[i]) ? -kInf : 0);
+        d.ub.push_back(std::isfinite(m.ru[i]) ? kInf : 0);
+    }
+    for (int64_t j = 0; j < m.cols(); ++j) {
+        d.row_names.push_back("stationarity" + std::to_string(j));
+        d.rl.push_back(std::isfinite(m.ub[j]) ? -kInf : -m.c[j]);
+        d.ru.push_back(std::isfinite(m.lb[j]) ? kInf : -m.c[j]);
+    }
+    d.validate();
+    return d;
+}
+
+// Feasibility polishing (src/solver.cpp:634-690).  From iteration 100 on, at
+// the first check past each doubling of the iteration count, when the chosen
+// candidate's gap is <= 1 %: solve the zero-objective primal feasibility
+// problem from its x, then the dual feasibility problem from its y, each with
+// up to 1/8 of the iterations so far; the pair (x, y) they return is kept when
+// the verifier rates it better.  The sub-solves' work counts in r.
+struct Polish {
+    int64_t next = 100;
+    bool due(const Options &o, const Result &r, const Accuracy &ca, Clock::time_point start) {
+        if (!o.polishing || r.iterations < next) return false;
+        next = r.iterations <= std::numeric_limits<int64_t>::max() / 2 ? 2 * r.iterations
+                                                                         : std::numeric_limits<int64_t>::max();
+        return ca.gap <= .01 && budget(o, r) > 0 && !stop_requested(o) && elapsed(start) < o.time_limit;
+    }
+    static int64_t budget(const Options &o, const Result &r) {
+        return std::min(r.iterations / 8, (o.iteration_limit - r.iterations) / 2);
+    }
+    // Returns true (with x, y, accuracy) when both sub-solves produced a pair.
+    static bool run(const Model &original, const Options &o, Verifier &verifier, const std::vector<double> &ox,
+                    const std::vector<double> &oy, Result &r, Clock::time_point start, std::vector<double> &x,
+                    std::vector<double> &y, Accuracy &accuracy) {
+        const int64_t limit = budget(o, r);
+        r.polishing_attempts++;
+        auto accumulate = [&](const Result &aux) {
+            r.iterations += aux.iterations;
+            r.polishing_iterations += aux.iterations;
+            r.restarts += aux.restarts;
+            r.rejected_steps += aux.rejected_steps;
+            r.weight_updates += aux.weight_updates;
+            r.preprocess_seconds += aux.preprocess_seconds;
+            r.transfer_seconds += aux.transfer_seconds;
+            r.iteration_seconds += aux.iteration_seconds;
+            r.verification_seconds += aux.verification_seconds;
+        };
+        Model primal = original;
+        std::fill(primal.c.begin(), primal.c.end(), 0.0);
+        primal.offset = 0;
+        Options po = o;
+        po.method = "pdhg";
+        po.adaptive = true;
+        po.polishing = false;
+        po.initial_x = ox;
+        po.initial_y.assign(size_t(original.rows()), 0);
+        po.iteration_limit = limit;
+        po.time_limit = std::max(0., o.time_limit - elapsed(start));
+        Result pr = solve_continuous(primal, po);
+        accumulate(pr);
+        if (!(pr.accuracy.finite && pr.accuracy.primal <= o.tol) || stop_requested(o) ||
+            elapsed(start) >= o.time_limit)
+            return false;
+        const Model dual = dual_feasibility_model(original);
+        po.initial_x = oy;
+        po.initial_y.assign(size_t(original.cols()), 0);
+        po.iteration_limit = std::min(limit, o.iteration_limit - r.iterations);
+        po.time_limit = std::max(0., o.time_limit - elapsed(start));
+        Result dr = solve_continuous(dual, po);
+        accumulate(dr);
+        if (dr.x.size() != size_t(original.rows())) return false;
+        accuracy = verifier.evaluate(pr.x, dr.x);
+        x = std::move(pr.x);
+        y = std::move(dr.x);
+        return true;
+    }
+};
+
 // The chunk loop of solve_continuous with the candidate checks on the device:
 // the same decisions in the same order (src/solver.cpp:540-726), fed by
 // npdhg_evaluate's original-space KKT figures.  The host downloads a point
@@ -192,6 +284,7 @@ void run_device_checks(const Model &original, const Prepared &prep, const Option
     int64_t since_restart = 0, previous_rejected = 0;
     double best_kkt = r.accuracy.kkt;
     bool have_best = false;
+    Polish polish;
     while (r.iterations < o.iteration_limit) {
         if (stop_requested(o)) {
             r.status = "INTERRUPTED";
@@ -265,6 +358,22 @@ void run_device_checks(const Model &original, const Prepared &prep, const Option
             }
             if (!found && backend.farkas_displacement(avg, ray)) found = certify_ray(verifier, ray, r);
             if (found) break;
+        }
+        if (polish.due(o, r, ca, start)) {
+            std::vector<double> ox(n0), oy(m0), px, py;
+            backend.original_point(avg ? 1 : 0, &ox, &oy);
+            Accuracy pa;
+            if (Polish::run(original, o, verifier, ox, oy, r, start, px, py, pa) && pa.finite && pa.kkt < best_kkt) {
+                r.x = std::move(px);
+                r.y = std::move(py);
+                r.accuracy = pa;
+                best_kkt = pa.kkt;
+                have_best = false;  // r holds the best pair until the device finds a better one
+                if (pa.kkt <= o.tol) {
+                    r.status = "OPTIMAL";
+                    return;
+                }
+            }
         }
         const bool restart_due = residual_restarts ? restart_requested
                                                    : since_restart >= int64_t(o.check_every) * 2 &&
@@ -448,6 +557,7 @@ Result solve_continuous(const Model &original, const Options &o) {
     }
     std::vector<double> epochx = x, epochy = y;
     std::vector<double> cx(m.cols()), cy(m.rows()), ax(m.cols()), ay(m.rows());
+    Polish polish;
     while (r.iterations < o.iteration_limit) {
         if (stop_requested(o)) {
             r.status = "INTERRUPTED";
@@ -536,6 +646,20 @@ Result solve_continuous(const Model &original, const Options &o) {
                 }
             }
             if (r.status == "INFEASIBLE") break;
+        }
+        if (polish.due(o, r, ca, start)) {
+            std::vector<double> px, py;
+            Accuracy pa;
+            if (Polish::run(original, o, verifier, ox, oy, r, start, px, py, pa) && pa.finite &&
+                pa.kkt < r.accuracy.kkt) {
+                r.x = std::move(px);
+                r.y = std::move(py);
+                r.accuracy = pa;
+                if (pa.kkt <= o.tol) {
+                    r.status = "OPTIMAL";
+                    break;
+                }
+            }
         }
         const bool restart_due = residual_restarts ? restart_requested
                                                    : since_restart >= int64_t(o.check_every) * 2 &&

@@ -526,8 +526,8 @@ Json result_json(const Model &m, const Result &r) {
     perf["verification_seconds"] = r.verification_seconds;
     perf["iterations"] = (long long)r.iterations;
     perf["weight_updates"] = (long long)r.weight_updates;
-    perf["polishing_iterations"] = 0;
-    perf["polishing_attempts"] = 0;
+    perf["polishing_iterations"] = (long long)r.polishing_iterations;
+    perf["polishing_attempts"] = (long long)r.polishing_attempts;
     perf["operator_norm_estimate"] = r.operator_norm_estimate;
     perf["restarts"] = (long long)r.restarts;
     perf["rejected_steps"] = (long long)r.rejected_steps;

@@ -73,6 +73,8 @@ struct Options {
     // 0 (the default) skips it.  With adaptive steps the initial step uses
     // min(bound, 1.05 x estimate) (src/solver.cpp:315-321).
     int power_iterations = 0;
+    // Feasibility polishing (src/solver.cpp:634-690): off by default.
+    bool polishing = false;
     std::vector<double> initial_x, initial_y;
     std::shared_ptr<std::atomic<bool>> cancellation;
 };
@@ -96,6 +98,7 @@ struct Result {
     bool graph_execution = false;
     bool device_prepare = false;  // scaling and transpose ran on the device
     double operator_norm_estimate = 0;
+    int64_t polishing_iterations = 0, polishing_attempts = 0;
     std::string matrix_precision = "fp64", method_selected, device_reason;
     double estimated_gpu_bytes = 0;
     // The request as the caller made it (reported under "selection").

@@ -13,6 +13,7 @@ import json
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -116,3 +117,51 @@ def test_power_iterations_estimate_and_solve(engine):
     m = engine.Model(NETLIB / "adlittle.mps")
     r = engine.solve_lp(m.lp, power_iterations=30)
     assert r["operator_norm_estimate"] == ref["adlittle/30"] and r["status"] == "OPTIMAL"
+
+
+def test_polishing_cli_and_options_block(engine, tmp_path):
+    """--polishing through the CLI (tests/test_cli.py:106-110): the solve still
+    certifies, the options block records the request (app/main.cpp:610-630),
+    and a quadratic model is refused with exit code 1."""
+    out = run_cli(engine, "solve", NETLIB / "afiro.mps", "--polishing", "--power-iterations", "20")
+    assert out.returncode == 0, out.stderr
+    doc = json.loads(out.stdout)
+    assert doc["status"] == "OPTIMAL" and doc["performance"]["operator_norm_estimate"] > 0
+    assert doc["options"]["polishing"] is True and doc["options"]["power_iterations"] == 20
+    assert doc["performance"]["polishing_iterations"] <= doc["performance"]["iterations"]
+    assert run_cli(engine, "solve", NETLIB / "afiro.mps", "--power-iterations", "-1").returncode == 1
+    qp = dict(linear_model(), objective={"linear": [3], "quadratic_diagonal": [0.2]})
+    path = tmp_path / "dispatch.json"
+    path.write_text(json.dumps(qp))
+    assert run_cli(engine, "solve", path, "--polishing").returncode == 1
+
+
+def test_polishing_shares_the_iteration_budget(engine, oracle):
+    """A badly scaled LP started next to its optimum (tests/test_research.cpp:
+    1084-1102): polishing runs, its iterations come out of the one iteration
+    limit, and an OPTIMAL answer passes the original-space check."""
+    lp = engine.LP(np.array([0, 1, 2], np.int64), np.array([0, 1], np.int64), np.array([1.0, 1e-3]),
+                   np.array([1.0, 1e-3]), np.zeros(2), np.full(2, np.inf), np.array([1.0, 1e-3]),
+                   np.array([1.0, 1e-3]), 0.0, 1.0)
+    r = engine.solve_lp(lp, polishing=1, scaling_passes=0, tol=1e-10, iteration_limit=2000,
+                        initial_x=np.array([0.999, 0.999]), initial_y=np.array([-1.0, -1.0]))
+    print("slow lp", r["status"], r["iterations"], r["polishing_attempts"], r["polishing_iterations"], flush=True)
+    assert r["polishing_attempts"] > 0
+    assert r["iterations"] <= 2000 and r["polishing_iterations"] <= r["iterations"]
+    if r["status"] == "OPTIMAL":
+        assert oracle.verify(lp, r["x"], r["y"])["kkt"] <= 1e-10
+
+
+@pytest.mark.parametrize("name", ("adlittle", "israel"))
+def test_polishing_on_netlib(engine, oracle, name):
+    """Polishing on real models: the answer still passes the original-space
+    check at the requested tolerance within the same iteration limit, and its
+    objective agrees with the reference CLI's result for the model."""
+    ref = json.loads((NETLIB / f"{name}.reference_result.json").read_text())
+    lp = engine.Model(NETLIB / f"{name}.mps").lp
+    r = engine.solve_lp(lp, polishing=1)
+    print(name, r["status"], r["iterations"], r["polishing_attempts"], r["polishing_iterations"], flush=True)
+    assert r["status"] == "OPTIMAL" and r["polishing_attempts"] > 0
+    assert r["polishing_iterations"] <= r["iterations"] <= 100000
+    assert oracle.verify(lp, r["x"], r["y"])["kkt"] <= 1e-6
+    assert abs(r["objective"] - ref["objective"]) <= 1e-5 * (1 + abs(ref["objective"]))
