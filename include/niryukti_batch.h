@@ -55,9 +55,13 @@ void niryukti_batch_destroy(niryukti_batch *b);
  * in original space (NULL for a cold start).  Returns its ticket. */
 int niryukti_batch_submit(niryukti_batch *b, const niryukti_lp *lp, const double *x0, const double *y0,
                           int64_t *ticket);
-/* One scheduling round: admit queued LPs into free slots, run one launch of
- * up to `checks_per_launch` check periods per resident LP, certify or retire
- * LPs that stopped.  *running receives the number still queued or resident. */
+/* One scheduling round: admit queued LPs into free slots, run
+ * `checks_per_launch` check periods per resident LP (LPs of a class with at
+ * most 256 LPs left run in one fused launch that continues past that number
+ * until about 32 ms have passed, so LPs of different speed do not wait for
+ * each other), certify or retire LPs that stopped.  Results do not depend on
+ * how periods fall into rounds.  *running receives the number still queued
+ * or resident. */
 int niryukti_batch_step(niryukti_batch *b, int checks_per_launch, int64_t *running);
 /* Admits queued LPs into free slots (uploading them to the arena) without
  * launching; returns the number admitted in *admitted. */

@@ -57,8 +57,9 @@ constexpr int kThreads = 256;
 // the last class runs the global-memory stream kernel.
 constexpr int kClasses = 4;
 constexpr int kClassDoubles[kClasses - 1] = {2048, 4096, 8000};
-// Upper bound of the LDS step kernel's static allocation (state, tile sums, flag).
-constexpr int kStaticLds = 4096;
+// Upper bound of the LDS step kernels' static allocation (state, tile sums,
+// flags; the fused tail kernel adds the check's reduction slots and stage).
+constexpr int kStaticLds = 24576;
 constexpr int64_t kShortRow = 128;
 // The largest LDS class runs 512-thread workgroups: fewer rows per thread,
 // twice the loads in flight per LP.
@@ -67,6 +68,12 @@ constexpr int kWideThreads = 512;
 // into one tail launch of kTailThreads-thread workgroups, one row per thread.
 constexpr int kTailLps = 256;
 constexpr int kTailThreads = 1024;
+// A tail launch runs each LP for at least the requested check periods, then
+// on until kTailBudgetMs have passed (at most kTailMaxPeriodsPerCheck times
+// the requested periods), so that LPs of different speed do not wait for
+// each other at every period.
+constexpr int kTailBudgetMs = 32;
+constexpr int kTailMaxPeriodsPerCheck = 1000;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxRejects = 40;
 
@@ -120,11 +127,15 @@ struct BatchOpts {
     long long pow_count;
 };
 
+// Beyond the tables (more than 2^22 accepted steps) the power is computed
+// here; as a call of its own its constants stay out of the step kernels'
+// registers.
+__device__ __noinline__ double pow_beyond_table(long long k, double e) { return pow(double(k), e); }
 __device__ __forceinline__ double pow_m3(const BatchOpts &o, long long k) {
-    return k < o.pow_count ? o.pow3[k] : pow(double(k), -.3);
+    return k < o.pow_count ? o.pow3[k] : pow_beyond_table(k, -.3);
 }
 __device__ __forceinline__ double pow_m6(const BatchOpts &o, long long k) {
-    return k < o.pow_count ? o.pow6[k] : pow(double(k), -.6);
+    return k < o.pow_count ? o.pow6[k] : pow_beyond_table(k, -.6);
 }
 
 // ------------------------------------------------------------ reductions --
@@ -161,7 +172,21 @@ struct Reducer {
     DD d[ND > 0 ? ND : 1];
 };
 
-template <int NS, int NM, int ND>
+// Sum (MAX: maximum) of lds[w * W + k] over waves B .. B+N-1 as a pairwise
+// tree (for four waves: (w0 + w1) + (w2 + w3)).
+template <int B, int N, bool MAX = false>
+__device__ __forceinline__ double wave_tree(const double *lds, int W, int k) {
+    if constexpr (N == 1) return lds[B * W + k];
+    else {
+        const double a = wave_tree<B, N / 2, MAX>(lds, W, k), b = wave_tree<B + N / 2, N / 2, MAX>(lds, W, k);
+        return MAX ? maxd(a, b) : a + b;
+    }
+}
+
+// Workgroup reduction over the first NW waves; the lanes of any further
+// waves take no part (a larger workgroup can run an NW-wave reduction and get
+// exactly the NW-wave result), but every thread reaches the barriers.
+template <int NW = kWaves, int NS, int NM, int ND>
 __device__ void block_reduce(Reducer<NS, NM, ND> &r, double *lds) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -178,7 +203,7 @@ __device__ void block_reduce(Reducer<NS, NM, ND> &r, double *lds) {
     }
     constexpr int W = NS + NM + 2 * ND;
     __syncthreads();
-    if (lane == 0) {
+    if (lane == 0 && wave < NW) {
 #pragma unroll
         for (int k = 0; k < NS; ++k) lds[wave * W + k] = r.s[k];
 #pragma unroll
@@ -191,15 +216,14 @@ __device__ void block_reduce(Reducer<NS, NM, ND> &r, double *lds) {
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NS; ++k) r.s[k] = (lds[k] + lds[W + k]) + (lds[2 * W + k] + lds[3 * W + k]);
+    for (int k = 0; k < NS; ++k) r.s[k] = wave_tree<0, NW>(lds, W, k);
 #pragma unroll
-    for (int k = 0; k < NM; ++k)
-        r.mx[k] = maxd(maxd(lds[NS + k], lds[W + NS + k]), maxd(lds[2 * W + NS + k], lds[3 * W + NS + k]));
+    for (int k = 0; k < NM; ++k) r.mx[k] = wave_tree<0, NW, true>(lds, W, NS + k);
 #pragma unroll
     for (int k = 0; k < ND; ++k) {
         const int b = NS + NM + 2 * k;
         DD acc{lds[b], lds[b + 1]};
-        for (int w = 1; w < kWaves; ++w) acc = dd_merge(acc, DD{lds[w * W + b], lds[w * W + b + 1]});
+        for (int w = 1; w < NW; ++w) acc = dd_merge(acc, DD{lds[w * W + b], lds[w * W + b + 1]});
         r.d[k] = acc;
     }
     __syncthreads();
@@ -271,6 +295,14 @@ __device__ void block_reduce_n(Reducer<NS, NM, ND> &r, double *lds) {
 // against the LDS wait counter, which makes every LDS wait drain all global
 // loads in flight.  Device code casts them back to global pointers.
 #define GLOBAL_AS __attribute__((address_space(1)))
+// A pointer that is the same in every lane, moved to scalar registers.
+template <class T>
+__device__ __forceinline__ T *uniform(T *p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(v));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(unsigned(v >> 32));
+    return reinterpret_cast<T *>((unsigned long long)hi << 32 | lo);
+}
 template <class T>
 __device__ __forceinline__ const GLOBAL_AS T *gin(const T *p) {
     return (const GLOBAL_AS T *)p;
@@ -282,7 +314,16 @@ __device__ __forceinline__ GLOBAL_AS T *gout(T *p) {
 
 constexpr int kChunk = 2048;
 
-template <bool ABS, class Init, class Epi>
+// Start of a thread's strided loop over [begin, end) in which only the first
+// NT threads of the workgroup take part (the others start at `end`).
+template <int NT>
+__device__ __forceinline__ int first_index(int begin, int end) {
+    return threadIdx.x < unsigned(NT) ? begin + int(threadIdx.x) : end;
+}
+
+// NT: threads of the workgroup; NTR (<= NT): threads that form the sum of a
+// row longer than kChunk, so that the sum does not depend on NT.
+template <bool ABS, int NT = kThreads, int NTR = NT, class Init, class Epi>
 __device__ void stream_rows(const int *ptr_, const int *idx_, const double *val_, const int *blk_, int nblk,
                             const double *vec_, double *stage, double *red, Init init, Epi epi) {
     const GLOBAL_AS int *ptr = gin(ptr_), *idx = gin(idx_), *blk = gin(blk_);
@@ -291,9 +332,9 @@ __device__ void stream_rows(const int *ptr_, const int *idx_, const double *val_
         const int r0 = blk[b], r1 = blk[b + 1];
         const int k0 = ptr[r0], k1 = ptr[r1];
         if (k1 - k0 <= kChunk) {
-            for (int k = k0 + threadIdx.x; k < k1; k += kThreads) stage[k - k0] = val[k] * vec[idx[k]];
+            for (int k = k0 + threadIdx.x; k < k1; k += NT) stage[k - k0] = val[k] * vec[idx[k]];
             __syncthreads();
-            for (int r = r0 + threadIdx.x; r < r1; r += kThreads) {
+            for (int r = r0 + threadIdx.x; r < r1; r += NT) {
                 double acc = init(r), a = 0;
                 const int e = ptr[r + 1] - k0;
                 for (int k = ptr[r] - k0; k < e; ++k) {
@@ -305,31 +346,34 @@ __device__ void stream_rows(const int *ptr_, const int *idx_, const double *val_
             __syncthreads();
         } else {
             Reducer<2, 0, 0> rr{};
-            for (int k = k0 + threadIdx.x; k < k1; k += kThreads) {
+            for (int k = first_index<NTR>(k0, k1); k < k1; k += NTR) {
                 const double t = val[k] * vec[idx[k]];
                 rr.s[0] += t;
                 if (ABS) rr.s[1] += fabs(t);
             }
-            block_reduce(rr, red);
+            block_reduce<NTR / 64>(rr, red);
             if (threadIdx.x == 0) epi(r0, init(r0) + rr.s[0], rr.s[1]);
             __syncthreads();
         }
     }
 }
 
+template <int NT = kThreads, int NTR = NT>
 __device__ void spmv(const int *ptr, const int *idx, const double *val, const int *blk, int nblk, const double *x,
                      double *out, double *stage, double *red) {
-    stream_rows<false>(ptr, idx, val, blk, nblk, x, stage, red, [](int) { return 0.0; },
+    stream_rows<false, NT, NTR>(ptr, idx, val, blk, nblk, x, stage, red, [](int) { return 0.0; },
                        [&](int r, double v, double) { out[r] = v; });
 }
+template <int NT = kThreads>
 __device__ void copy_vec(double *dst, const double *src, int n) {
     GLOBAL_AS double *d = gout(dst);
     const GLOBAL_AS double *v = gin(src);
-    for (int i = threadIdx.x; i < n; i += kThreads) d[i] = v[i];
+    for (int i = threadIdx.x; i < n; i += NT) d[i] = v[i];
 }
+template <int NT = kThreads>
 __device__ void zero_vec(double *dst, int n) {
     GLOBAL_AS double *d = gout(dst);
-    for (int i = threadIdx.x; i < n; i += kThreads) d[i] = 0;
+    for (int i = threadIdx.x; i < n; i += NT) d[i] = 0;
 }
 
 // Original-space accuracy of one candidate (the non-certifying evaluate of
@@ -353,9 +397,9 @@ __device__ __forceinline__ double phi(double g, double l, double u) {
 // Farkas pre-filter over columns for a ray whose A^T ray is in aty/atyabs.
 // Adds into r: s[0] += sum of per-column upper bounds, s[1] += their |.|,
 // s[2] += Lipschitz slack, mx[0] = 1 when a column bound is -inf.
-template <int NS, int NM, int ND>
+template <int NTR, int NS, int NM, int ND>
 __device__ void farkas_columns(const LpDev &L, Reducer<NS, NM, ND> &r) {
-    for (int j = threadIdx.x; j < L.n0; j += kThreads) {
+    for (int j = first_index<NTR>(0, L.n0); j < L.n0; j += NTR) {
         const double g = gin(L.aty)[j], ga = gin(L.atyabs)[j];
         const int len = gin(L.otp)[j + 1] - gin(L.otp)[j];
         const double e = double(len + 4) * 2 * kEps * ga;
@@ -372,20 +416,22 @@ __device__ void farkas_columns(const LpDev &L, Reducer<NS, NM, ND> &r) {
     }
 }
 
+// NT threads copy and multiply; the first NTR form every sum.
+template <int NT, int NTR>
 __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double *ox, double *oy, double *lds,
                         double *stage) {
     // restore_x / restore_y (src/preprocess.cpp:121-132)
-    copy_vec(ox, L.fixed, L.n0);
-    zero_vec(oy, L.m0);
+    copy_vec<NT>(ox, L.fixed, L.n0);
+    zero_vec<NT>(oy, L.m0);
     __syncthreads();
-    for (int j = threadIdx.x; j < L.n; j += kThreads) ox[gin(L.cols)[j]] = x[j] * gin(L.dc)[j];
-    for (int i = threadIdx.x; i < L.m; i += kThreads) {
+    for (int j = threadIdx.x; j < L.n; j += NT) ox[gin(L.cols)[j]] = x[j] * gin(L.dc)[j];
+    for (int i = threadIdx.x; i < L.m; i += NT) {
         const int t = y[i] >= 0 ? gin(L.upper_rows)[i] : gin(L.lower_rows)[i];
         oy[t] = 0.0 + y[i] * gin(L.dr)[i] * L.sobj;
     }
     __syncthreads();
-    spmv(L.op, L.oi, L.ov, L.oblk, L.nob, ox, L.act, stage, lds);
-    stream_rows<true>(L.otp, L.oti, L.otv, L.otblk, L.notb, oy, stage, lds, [](int) { return 0.0; },
+    spmv<NT, NTR>(L.op, L.oi, L.ov, L.oblk, L.nob, ox, L.act, stage, lds);
+    stream_rows<true, NT, NTR>(L.otp, L.oti, L.otv, L.otblk, L.notb, oy, stage, lds, [](int) { return 0.0; },
                       [&](int j, double v, double a) {
                           gout(L.aty)[j] = v;
                           gout(L.atyabs)[j] = a;
@@ -396,7 +442,7 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
     //     y != 0 at an infinite row endpoint, infinite Lagrangian box argument
     // d : support, complement, box_min, sum y b over rows
     Reducer<5, 8, 4> r{};
-    for (int i = threadIdx.x; i < L.m0; i += kThreads) {
+    for (int i = first_index<NTR>(0, L.m0); i < L.m0; i += NTR) {
         const double a = gin(L.act)[i], yi = oy[i], lo = gin(L.rl0)[i], hi = gin(L.ru0)[i];
         if (!isfinite(a) || !isfinite(yi)) r.mx[0] = 1;
         const double v = maxd(maxd(0., lo - a), a - hi);
@@ -417,7 +463,7 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
             }
         }
     }
-    for (int j = threadIdx.x; j < L.n0; j += kThreads) {
+    for (int j = first_index<NTR>(0, L.n0); j < L.n0; j += NTR) {
         const double xj = ox[j], l = gin(L.lb0)[j], u = gin(L.ub0)[j], c = gin(L.c0)[j];
         if (!isfinite(xj)) r.mx[0] = 1;
         const double v = maxd(maxd(0., l - xj), xj - u);
@@ -448,7 +494,7 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
             r.s[3] += ga * box;
         }
     }
-    block_reduce(r, lds);
+    block_reduce<NTR / 64>(r, lds);
     Acc a{};
     const double support = dd_value(r.d[0]), complement = dd_value(r.d[1]), box_min = dd_value(r.d[2]);
     a.objective = L.offset0 + r.s[0];
@@ -728,10 +774,15 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
                             const int *tsblds, const int *asblds, double *powlds, int *accept_flag) {
     const int cur = S.cur;
     const double f = S.factor, tau = tau0 * f, sigma = sigma0 * f, w = f / (S.mass + f);
-    const GLOBAL_AS double *x = gin(L.x[cur]), *xa = gin(L.xa[cur]), *y = gin(L.y[cur]), *ya = gin(L.ya[cur]),
-                           *ax = gin(L.ax[cur]);
-    GLOBAL_AS double *xn = gout(L.x[cur ^ 1]), *xan = gout(L.xa[cur ^ 1]), *yn = gout(L.y[cur ^ 1]),
-                     *yan = gout(L.ya[cur ^ 1]), *axn = gout(L.ax[cur ^ 1]);
+    // (selects rather than indexing by cur, so that a descriptor held in
+    // registers stays in registers)
+    const bool c1 = cur != 0;
+    const GLOBAL_AS double *x = gin(c1 ? L.x[1] : L.x[0]), *xa = gin(c1 ? L.xa[1] : L.xa[0]),
+                           *y = gin(c1 ? L.y[1] : L.y[0]), *ya = gin(c1 ? L.ya[1] : L.ya[0]),
+                           *ax = gin(c1 ? L.ax[1] : L.ax[0]);
+    GLOBAL_AS double *xn = gout(c1 ? L.x[0] : L.x[1]), *xan = gout(c1 ? L.xa[0] : L.xa[1]),
+                     *yn = gout(c1 ? L.y[0] : L.y[1]), *yan = gout(c1 ? L.ya[0] : L.ya[1]),
+                     *axn = gout(c1 ? L.ax[0] : L.ax[1]);
     const GLOBAL_AS double *c = gin(L.c), *lb = gin(L.lb), *ub = gin(L.ub), *rl = gin(L.rl), *ru = gin(L.ru);
     for (int i = threadIdx.x; i < L.m; i += NT) ylds[i] = y[i];
     // the step rule's powers load with y, off the decision's serial path
@@ -898,30 +949,27 @@ __global__ void gather_pieces_kernel(const CopyPiece *pieces, double *out) {
 
 // Check period end (src/solver.cpp:550-725): candidate evaluation, best
 // tracking, parking for host certification / Farkas checks, restart rule.
-__global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps, LpState *states, const int *slots,
-                                                               BatchOpts o) {
-    const int s = slots[blockIdx.x];
-    __shared__ LpState S;
-    __shared__ double lds[kWaves * 24];
-    __shared__ double stage[kChunk];
-    if (threadIdx.x == 0) S = states[s];
-    __syncthreads();
-    if (S.status != kRunning || S.phase == kAdvance) return;
-    const LpDev &L = lps[s];
+// For an LP whose phase is kCheck or kResume; S is the workgroup's shared
+// copy of its state.  NT threads copy and multiply, the first kThreads form
+// every sum, so the check decides the same in any workgroup size.  lds holds
+// kWaves * 24 doubles, stage kChunk.
+template <int NT>
+__device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, double *lds, double *stage) {
+    constexpr int NTR = kThreads;
     const int cur = S.cur;
     double cand_kkt;
     int avg;
     if (S.phase == kCheck) {
         // candidates(): rebase the activity recurrence (src/cpu.cpp:229-230)
-        spmv(L.ap, L.ai, L.av, L.ablk, L.nab, L.x[cur], L.ax[cur], stage, lds);
-        const Acc cur_acc = evaluate(L, L.x[cur], L.y[cur], L.ox[0], L.oy[0], lds, stage);
-        const Acc avg_acc = evaluate(L, L.xa[cur], L.ya[cur], L.ox[1], L.oy[1], lds, stage);
+        spmv<NT, NTR>(L.ap, L.ai, L.av, L.ablk, L.nab, L.x[cur], L.ax[cur], stage, lds);
+        const Acc cur_acc = evaluate<NT, NTR>(L, L.x[cur], L.y[cur], L.ox[0], L.oy[0], lds, stage);
+        const Acc avg_acc = evaluate<NT, NTR>(L, L.xa[cur], L.ya[cur], L.ox[1], L.oy[1], lds, stage);
         avg = avg_acc.kkt < cur_acc.kkt ? 1 : 0;
         const Acc ca = avg ? avg_acc : cur_acc;
         const bool better = ca.finite && ca.kkt < S.best_kkt;
         if (better) {
-            copy_vec(L.bx, L.ox[avg], L.n0);
-            copy_vec(L.by, L.oy[avg], L.m0);
+            copy_vec<NT>(L.bx, L.ox[avg], L.n0);
+            copy_vec<NT>(L.by, L.oy[avg], L.m0);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -933,31 +981,29 @@ __global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps,
             else if (ca.kkt <= o.tol) S.status = kParkOptimal;
         }
         __syncthreads();
-        if (S.status != kRunning) {
-            if (threadIdx.x == 0) states[s] = S;
-            return;
-        }
+        if (S.status != kRunning) return;
         if (S.iterations >= 1000 && ca.primal > o.tol) {
             bool possible = cur_acc.farkas_possible || avg_acc.farkas_possible;
             if (!possible) {
                 // displacement ray restore_y(cy - epochy) (src/solver.cpp:603-610)
                 const double *cy = avg ? L.ya[cur] : L.y[cur];
-                zero_vec(L.oy[0], L.m0);
+                zero_vec<NT>(L.oy[0], L.m0);
                 __syncthreads();
-                for (int i = threadIdx.x; i < L.m; i += kThreads) {
+                for (int i = threadIdx.x; i < L.m; i += NT) {
                     const double d = cy[i] - gin(L.epy)[i];
                     const int t = d >= 0 ? gin(L.upper_rows)[i] : gin(L.lower_rows)[i];
                     gout(L.oy[0])[t] = 0.0 + d * gin(L.dr)[i] * L.sobj;
                 }
                 __syncthreads();
-                stream_rows<true>(L.otp, L.oti, L.otv, L.otblk, L.notb, L.oy[0], stage, lds, [](int) { return 0.0; },
+                stream_rows<true, NT, NTR>(L.otp, L.oti, L.otv, L.otblk, L.notb, L.oy[0], stage, lds,
+                                           [](int) { return 0.0; },
                                   [&](int j, double v, double a) {
                                       gout(L.aty)[j] = v;
                                       gout(L.atyabs)[j] = a;
                                   });
                 __syncthreads();
                 Reducer<4, 2, 1> r{};
-                for (int i = threadIdx.x; i < L.m0; i += kThreads) {
+                for (int i = first_index<NTR>(0, L.m0); i < L.m0; i += NTR) {
                     const double yi = L.oy[0][i];
                     if (!isfinite(yi)) r.mx[1] = 1;
                     if (yi != 0) {
@@ -969,19 +1015,20 @@ __global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps,
                         }
                     }
                 }
-                farkas_columns(L, r);
-                block_reduce(r, lds);
+                farkas_columns<NTR>(L, r);
+                block_reduce<NTR / 64>(r, lds);
                 const double value = r.s[0] - dd_value(r.d[0]);
                 const double slack = double(L.n0 + L.m0 + 16) * 4 * kEps * (r.s[1] + r.s[2] + r.s[3]);
                 // all-zero and nonfinite rays are skipped by the reference
                 possible = !(r.mx[0] != 0 || r.mx[1] != 0 || value + slack <= 0);
             }
             if (possible) {
+                __syncthreads();
                 if (threadIdx.x == 0) {
                     S.status = kParkFarkas;
                     S.farkas_parks += 1;
-                    states[s] = S;
                 }
+                __syncthreads();
                 return;
             }
         }
@@ -1001,15 +1048,15 @@ __global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps,
         const double *cx = avg ? L.xa[cur] : L.x[cur];
         const double *cy = avg ? L.ya[cur] : L.y[cur];
         Reducer<0, 0, 2> r{};
-        for (int j = threadIdx.x; j < L.n; j += kThreads) {
+        for (int j = first_index<NTR>(0, L.n); j < L.n; j += NTR) {
             const double d = cx[j] - gin(L.epx)[j];
             dd_add(r.d[0], d * d);
         }
-        for (int i = threadIdx.x; i < L.m; i += kThreads) {
+        for (int i = first_index<NTR>(0, L.m); i < L.m; i += NTR) {
             const double d = cy[i] - gin(L.epy)[i];
             dd_add(r.d[1], d * d);
         }
-        block_reduce(r, lds);
+        block_reduce<NTR / 64>(r, lds);
         if (threadIdx.x == 0 && o.adaptive) {
             const double dx = dd_value(r.d[0]), dy = dd_value(r.d[1]);
             if (dx > 1e-24 && dy > 1e-24) {
@@ -1018,15 +1065,15 @@ __global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps,
             }
         }
         if (avg) {
-            copy_vec(L.x[cur], L.xa[cur], L.n);
-            copy_vec(L.y[cur], L.ya[cur], L.m);
+            copy_vec<NT>(L.x[cur], L.xa[cur], L.n);
+            copy_vec<NT>(L.y[cur], L.ya[cur], L.m);
             __syncthreads();
         }
-        spmv(L.ap, L.ai, L.av, L.ablk, L.nab, L.x[cur], L.ax[cur], stage, lds);
-        copy_vec(L.epx, L.x[cur], L.n);
-        copy_vec(L.epy, L.y[cur], L.m);
-        zero_vec(L.xa[cur], L.n);
-        zero_vec(L.ya[cur], L.m);
+        spmv<NT, NTR>(L.ap, L.ai, L.av, L.ablk, L.nab, L.x[cur], L.ax[cur], stage, lds);
+        copy_vec<NT>(L.epx, L.x[cur], L.n);
+        copy_vec<NT>(L.epy, L.y[cur], L.m);
+        zero_vec<NT>(L.xa[cur], L.n);
+        zero_vec<NT>(L.ya[cur], L.m);
         __syncthreads();
         if (threadIdx.x == 0) {
             S.mass = 0;
@@ -1039,6 +1086,127 @@ __global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps,
     // A long-running LP leaves the batch at a check, so the point where it
     // moves depends only on its own iteration count.
     if (threadIdx.x == 0 && o.handoff > 0 && S.status == kRunning && S.iterations >= o.handoff) S.status = kHandoff;
+    __syncthreads();
+}
+
+// check_period as a call of its own for the fused tail kernel, so that the
+// check's register needs do not shape the register allocation of the
+// attempts.  The pointers carry their address spaces (LDS, global) into the
+// call, so its LDS and global accesses stay direct.
+#define LDS_AS __attribute__((address_space(3)))
+template <int NT>
+__device__ __noinline__ void check_period_call(const GLOBAL_AS LpDev *L, LDS_AS LpState *S, BatchOpts o,
+                                               LDS_AS double *lds, LDS_AS double *stage) {
+    check_period<NT>(*(const LpDev *)L, *(LpState *)S, o, (double *)lds, (double *)stage);
+}
+
+__global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps, LpState *states, const int *slots,
+                                                               BatchOpts o) {
+    const int s = slots[blockIdx.x];
+    __shared__ LpState S;
+    __shared__ double lds[kWaves * 24];
+    __shared__ double stage[kChunk];
+    if (threadIdx.x == 0) S = states[s];
+    __syncthreads();
+    if (S.status != kRunning || S.phase == kAdvance) return;
+    check_period<kThreads>(lps[s], S, o, lds, stage);
+    if (threadIdx.x == 0) states[s] = S;
+}
+
+// The tail group in one launch: each workgroup takes its LP through whole
+// check periods, the attempts of batch_advance_lds_kernel followed by
+// check_period, until the LP stops running (parked for the host, finished,
+// handed off), `max_periods` periods are done, or at least `min_periods` are
+// done and `budget` ticks of the constant-rate clock have passed since the
+// workgroup started.  An LP's periods do not wait for the other LPs of the
+// launch, and its trajectory is the one the two-kernel cycle gives: the
+// attempts are the same code, and check_period forms its sums on the first
+// kThreads threads exactly as batch_check_kernel does.
+template <int NT>
+__global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpState *states, const int *slots,
+                                                        BatchOpts o, int min_periods, int max_periods,
+                                                        long long budget) {
+    extern __shared__ double dyn[];
+    const long long start = wall_clock64();
+    const int s = slots[blockIdx.x];
+    __shared__ LpState S;
+    __shared__ double tiles[3 * kMaxTiles];
+    __shared__ double powlds[2];
+    __shared__ double lds[kWaves * 24];
+    __shared__ double stage[kChunk];
+    __shared__ int flag, stop;
+    if (threadIdx.x == 0) S = states[s];
+    __syncthreads();
+    if (S.status != kRunning) return;
+    // The attempts' copy of the descriptor, read before any store (scalar
+    // loads) and kept in scalar registers across the check calls.
+    LpDev L = lps[s];
+    for (int h = 0; h < 2; ++h) {
+        L.x[h] = uniform(L.x[h]);
+        L.xa[h] = uniform(L.xa[h]);
+        L.y[h] = uniform(L.y[h]);
+        L.ya[h] = uniform(L.ya[h]);
+        L.ax[h] = uniform(L.ax[h]);
+    }
+    L.c = uniform(L.c);
+    L.lb = uniform(L.lb);
+    L.ub = uniform(L.ub);
+    L.rl = uniform(L.rl);
+    L.ru = uniform(L.ru);
+    L.tsi = uniform(L.tsi);
+    L.tsv = uniform(L.tsv);
+    L.asi = uniform(L.asi);
+    L.asv = uniform(L.asv);
+    L.n = __builtin_amdgcn_readfirstlane(L.n);
+    L.m = __builtin_amdgcn_readfirstlane(L.m);
+    double *ylds = dyn, *dxlds = dyn + L.m;
+    int *tplds = reinterpret_cast<int *>(dyn + L.m + L.n), *aplds = tplds + L.n + 1;
+    const int ns = (L.n + 63) / 64, ms = (L.m + 63) / 64;
+    int *tsblds = aplds + L.m + 1, *asblds = tsblds + ns + 1;
+    const GLOBAL_AS int *tp = gin(L.tp), *ap = gin(L.ap), *tsb = gin(L.tsb), *asb = gin(L.asb);
+    for (int j = threadIdx.x; j <= L.n; j += NT) tplds[j] = tp[j];
+    for (int i = threadIdx.x; i <= L.m; i += NT) aplds[i] = ap[i];
+    for (int j = threadIdx.x; j <= ns; j += NT) tsblds[j] = tsb[j];
+    for (int i = threadIdx.x; i <= ms; i += NT) asblds[i] = asb[i];
+    __syncthreads();
+    for (int period = 0;;) {
+        if (S.phase != kAdvance) {
+            check_period_call<NT>((const GLOBAL_AS LpDev *)(lps + s), (LDS_AS LpState *)&S, o, (LDS_AS double *)lds,
+                                  (LDS_AS double *)stage);
+            if (S.status != kRunning) break;
+        }
+        if (threadIdx.x == 0)
+            stop = period >= max_periods || (period >= min_periods && wall_clock64() - start >= budget);
+        __syncthreads();
+        if (stop) break;
+        const long long remaining = S.iteration_limit - S.iterations;
+        if (remaining <= 0) {
+            __syncthreads();
+            if (threadIdx.x == 0) S.status = kIterationLimit;
+            break;
+        }
+        const long long count = remaining < o.check_every ? remaining : o.check_every;
+        const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
+        for (long long done = 0; done < count;) {
+            attempt_lds<NT, 1>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds, &flag);
+            if (flag == 1) ++done;
+            else if (flag < 0) {
+                if (threadIdx.x == 0) S.status = kExhausted;
+                break;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && S.status == kRunning) {
+            S.iterations += count;
+            S.since_restart += count;
+            S.checks += 1;
+            S.phase = kCheck;
+        }
+        __syncthreads();
+        if (S.status != kRunning) break;
+        ++period;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) states[s] = S;
 }
 
@@ -1210,6 +1378,10 @@ class BatchEngine {
         hipDeviceProp_t prop;
         HIP_OK(hipGetDeviceProperties(&prop, device));
         const long long room = (long long)prop.sharedMemPerBlock - kStaticLds - 16;
+        int wall_khz = 0;
+        HIP_OK(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device));
+        if (wall_khz <= 0) throw std::runtime_error("The device reports no constant-rate clock");
+        tail_budget_ticks_ = (long long)wall_khz * kTailBudgetMs;
         for (int c = 0; c < kClasses - 1; ++c) {
             int cap = kClassDoubles[c];
             while (cap > 0 && (long long)lds_bytes(cap) > room) --cap;
@@ -1239,6 +1411,7 @@ class BatchEngine {
         state_.resize(max_slots_);
         slot_job_.assign(max_slots_, -1);
         slot_bytes_.assign(max_slots_, 0);
+        slot_check_bytes_.assign(max_slots_, 0);
         slot_class_.assign(max_slots_, kClasses - 1);
         slot_group_.assign(max_slots_, kClasses - 1);
         for (int s = max_slots_ - 1; s >= 0; --s) free_slots_.push_back(s);
@@ -1346,9 +1519,11 @@ class BatchEngine {
             BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0, handoff_,
                          d_pow3_,     d_pow6_,          pow_count_};
             auto t0 = Clock::now();
-            std::vector<long long> before(active.size());
-            for (size_t k = 0; k < active.size(); ++k)
+            std::vector<long long> before(active.size()), checks_before(active.size());
+            for (size_t k = 0; k < active.size(); ++k) {
                 before[k] = state_[active[k]].accepted + state_[active[k]].rejected;
+                checks_before[k] = state_[active[k]].checks;
+            }
             for (const Group &g : groups) {
                 auto &ev = class_events_[g.kind];
                 while (ev.size() < size_t(2 * checks + 2)) {
@@ -1361,25 +1536,36 @@ class BatchEngine {
                 const int grid = int(g.slots.size());
                 const int *slots = d_slots_ + size_t(g.kind) * max_slots_;
                 HIP_OK(hipEventRecord(ev[0], cs));
-                for (int k = 0; k < checks; ++k) {
-                    if (g.kind == kClasses)
-                        launch_tail(g.cap, grid, cs, slots, bo);
-                    else if (g.kind < kClasses - 1)
-                        launch_lds(g.cap, g.kind == kClasses - 2, grid, cs, slots, bo);
-                    else
-                        batch_advance_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
-                    HIP_OK(hipEventRecord(ev[2 * k + 1], cs));
-                    batch_check_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
-                    HIP_OK(hipEventRecord(ev[2 * k + 2], cs));
+                if (g.kind == kClasses) {
+                    launch_tail(g.cap, grid, cs, slots, bo, checks);
+                    HIP_OK(hipEventRecord(ev[1], cs));
+                } else {
+                    for (int k = 0; k < checks; ++k) {
+                        if (g.kind < kClasses - 1)
+                            launch_lds(g.cap, g.kind == kClasses - 2, grid, cs, slots, bo);
+                        else
+                            batch_advance_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
+                        HIP_OK(hipEventRecord(ev[2 * k + 1], cs));
+                        batch_check_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
+                        HIP_OK(hipEventRecord(ev[2 * k + 2], cs));
+                    }
                 }
                 HIP_OK(hipGetLastError());
-                HIP_OK(hipStreamWaitEvent(stream_, ev[2 * checks], 0));
+                HIP_OK(hipStreamWaitEvent(stream_, ev[g.kind == kClasses ? 1 : 2 * checks], 0));
             }
             HIP_OK(hipMemcpyAsync(state_.data(), d_state_, sizeof(LpState) * max_slots_, hipMemcpyDeviceToHost, stream_));
             HIP_OK(hipStreamSynchronize(stream_));
             for (const Group &g : groups) {
                 auto &ev = class_events_[g.kind];
-                for (int k = 0; k < checks; ++k) {
+                if (g.kind == kClasses) {
+                    // the fused launch: attempts and checks together
+                    float a = 0;
+                    HIP_OK(hipEventElapsedTime(&a, ev[0], ev[1]));
+                    advance_seconds_ += a * 1e-3;
+                    class_seconds_[g.kind] += a * 1e-3;
+                    class_launches_[g.kind] += 1;
+                }
+                for (int k = 0; k < (g.kind == kClasses ? 0 : checks); ++k) {
                     float a = 0, b = 0;
                     HIP_OK(hipEventElapsedTime(&a, ev[2 * k], ev[2 * k + 1]));
                     HIP_OK(hipEventElapsedTime(&b, ev[2 * k + 1], ev[2 * k + 2]));
@@ -1396,6 +1582,8 @@ class BatchEngine {
                 attempts_ += done;
                 advance_bytes_ += double(done) * slot_bytes_[active[k]];
                 class_bytes_[slot_group_[active[k]]] += double(done) * slot_bytes_[active[k]];
+                if (slot_group_[active[k]] == kClasses)  // the fused tail launch also did the checks
+                    class_bytes_[kClasses] += double(st.checks - checks_before[k]) * slot_check_bytes_[active[k]];
             }
             device_seconds_ += std::chrono::duration<double>(Clock::now() - t0).count();
             ++launches_;
@@ -1591,10 +1779,23 @@ class BatchEngine {
     void launch_lds_t(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
         batch_advance_lds_kernel<NT, NR><<<grid, NT, lds_bytes(cap), cs>>>(d_desc_, d_state_, slots, bo);
     }
-    // The tail group: one row per thread over 1,024 threads (2 or 4 rows per
-    // thread spill registers at this size and were measured slower).
-    void launch_tail(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
-        launch_lds_t<kTailThreads, 1>(cap, grid, cs, slots, bo);
+    // Algorithmic bytes of one check period's check: the scaled SpMV that
+    // rebases A x, then two original-space evaluations (restore, A x and A^T y
+    // of the original model, and the row and column passes of the verifier).
+    static double check_bytes(const Job &j) {
+        const nyk::Model &s = j.prep.model, &o = j.original;
+        const double n = double(s.cols()), m = double(s.rows()), nnz = double(s.A.nnz());
+        const double n0 = double(o.cols()), m0 = double(o.rows()), nnz0 = double(o.A.nnz());
+        return nnz * 12 + m * 12 + n * 8 + 2 * (24 * nnz0 + 68 * m0 + 100 * n0 + 28 * n + 32 * m);
+    }
+
+    // The tail group: one fused launch per scheduling round, one row per
+    // thread over 1,024 threads (2 or 4 rows per thread spill registers at
+    // this size and were measured slower).  Each LP runs at least `checks`
+    // periods and then continues until kTailBudgetMs have passed.
+    void launch_tail(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo, int checks) {
+        batch_tail_kernel<kTailThreads><<<grid, kTailThreads, lds_bytes(cap), cs>>>(
+            d_desc_, d_state_, slots, bo, checks, checks * kTailMaxPeriodsPerCheck, tail_budget_ticks_);
     }
     // A crowded LDS class: 256-thread workgroups (512 for the largest class)
     // with rows_per_thread_ rows per thread.
@@ -1658,6 +1859,7 @@ class BatchEngine {
             j.bytes = bytes;
             slot_job_[s] = int(j.ticket);
             slot_bytes_[s] = attempt_bytes(j);
+            slot_check_bytes_[s] = check_bytes(j);
             slot_class_[s] = lds_class(j);
             desc_[s] = d;
             LpState st{};
@@ -2167,7 +2369,8 @@ class BatchEngine {
     std::vector<LpDev> desc_;
     std::vector<LpState> state_;
     std::vector<int> slot_job_, free_slots_;
-    std::vector<double> slot_bytes_;
+    std::vector<double> slot_bytes_, slot_check_bytes_;
+    long long tail_budget_ticks_ = 0;
     hipStream_t class_stream_[kClasses + 1] = {};  // per class, then the tail group
     double *d_pow3_ = nullptr, *d_pow6_ = nullptr;
     struct Pending {

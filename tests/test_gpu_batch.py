@@ -138,10 +138,13 @@ def test_handoff_to_grid_engine(engine, oracle):
 
 def test_launch_shape_does_not_change_results(engine):
     """A crowded class runs 256/512-thread workgroups with several rows per
-    thread; once few LPs are left it switches to 1024-thread workgroups.  The
-    per-tile sums keep every step's arithmetic independent of that shape, so an
-    LP that changes shape mid-solve matches the same LP run alone throughout in
-    the tail shape, and the rows-per-thread setting changes nothing."""
+    thread and a separate check kernel; once few LPs are left it switches to
+    the fused tail kernel (1024-thread workgroups that also run the checks).
+    The per-tile sums keep every step's arithmetic independent of that shape,
+    and the fused check forms its sums on 256 threads as the check kernel
+    does, so an LP that changes shape mid-solve matches the same LP run alone
+    throughout in the tail shape, and the rows-per-thread setting changes
+    nothing."""
     insts = [engine.Instance(2, 5, 1, i) for i in range(900)]
     mix, stats = solve_batch(engine, [i.lp for i in insts], checks=1)
     assert stats["launches"] > 2
@@ -152,4 +155,6 @@ def test_launch_shape_does_not_change_results(engine):
     for k, a in zip(picks, alone):
         assert a["status"] == mix[k]["status"] == "OPTIMAL"
         assert a["iterations"] == mix[k]["iterations"]
+        # the checks decided alike: the same restarts and primal weight updates
+        assert a["restarts"] == mix[k]["restarts"] and a["weight_updates"] == mix[k]["weight_updates"]
         assert np.array_equal(a["x"], mix[k]["x"]) and np.array_equal(a["y"], mix[k]["y"])
