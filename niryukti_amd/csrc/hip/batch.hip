@@ -764,14 +764,17 @@ __device__ __forceinline__ double tile_total(const double *tiles, int count) {
 }
 
 // One PDHG attempt for an LP whose dual iterate and primal step fit in LDS:
-// y is staged into LDS once per attempt, dx never leaves LDS, the CSR offsets
+// y lives in LDS (staged from memory only when `ystaged` is false: at the
+// start, after a rejected attempt, after a check; an accepted attempt leaves
+// its new y there), dx never leaves LDS, the CSR offsets
 // of A and A^T sit in LDS for the whole launch (so a row's entries are one
 // memory round trip away, not two), and each thread carries NR rows at a
 // time (rows are short in this class).
 template <int NT, int NR>
 __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, double tau0, double sigma0,
                             double *tiles, double *ylds, double *dxlds, const int *tplds, const int *aplds,
-                            const int *tsblds, const int *asblds, double *powlds, int *accept_flag) {
+                            const int *tsblds, const int *asblds, double *powlds, int *accept_flag,
+                            bool &ystaged) {
     const int cur = S.cur;
     const double f = S.factor, tau = tau0 * f, sigma = sigma0 * f, w = f / (S.mass + f);
     // (selects rather than indexing by cur, so that a descriptor held in
@@ -784,11 +787,14 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
                      *yn = gout(c1 ? L.y[0] : L.y[1]), *yan = gout(c1 ? L.ya[0] : L.ya[1]),
                      *axn = gout(c1 ? L.ax[0] : L.ax[1]);
     const GLOBAL_AS double *c = gin(L.c), *lb = gin(L.lb), *ub = gin(L.ub), *rl = gin(L.rl), *ru = gin(L.ru);
-    for (int i = threadIdx.x; i < L.m; i += NT) ylds[i] = y[i];
     // the step rule's powers load with y, off the decision's serial path
+    // (the barriers after the two passes order them before the decision)
     if (threadIdx.x == 64 % NT) powlds[0] = pow_m3(o, S.accepted + 2);
     if (threadIdx.x == 65 % NT) powlds[1] = pow_m6(o, S.accepted + 2);
-    __syncthreads();
+    if (!ystaged) {
+        for (int i = threadIdx.x; i < L.m; i += NT) ylds[i] = y[i];
+        __syncthreads();
+    }
     double *tile_dx = tiles, *tile_dy = tiles + kMaxTiles, *tile_cr = tiles + 2 * kMaxTiles;
     const int wave = threadIdx.x >> 6;
     // g = c + A^T y and the projected primal step (src/cpu.cpp:107-132)
@@ -863,6 +869,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
                 sq = (!isfinite(bar) || !isfinite(t)) ? INFINITY : dy * dy;
                 cr = dy * delta[q];
                 yn[i] = t;
+                ylds[i] = t;  // only this thread reads row i in this pass
                 yan[i] = v + (t - v) * w;
                 axn[i] = a + delta[q];
             }
@@ -883,6 +890,8 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
         if (threadIdx.x == 0) decide_batch(S, o, dx2, dy2, cross, tau, sigma, f, powlds[0], powlds[1], accept_flag);
     }
     __syncthreads();
+    // accepted: LDS holds the new y[cur]; otherwise y[cur] is still the old one
+    ystaged = *accept_flag == 1;
 }
 
 // Hot loop for the LDS-resident class; dynamic LDS holds y (m) and dx (n).
@@ -916,8 +925,10 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     for (int j = threadIdx.x; j <= ns; j += NT) tsblds[j] = tsb[j];
     for (int i = threadIdx.x; i <= ms; i += NT) asblds[i] = asb[i];
     __syncthreads();
+    bool ystaged = false;
     for (long long done = 0; done < count;) {
-        attempt_lds<NT, NR>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds, &flag);
+        attempt_lds<NT, NR>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds, &flag,
+                            ystaged);
         if (flag == 1) ++done;
         else if (flag < 0) {
             if (threadIdx.x == 0) S.status = kExhausted;
@@ -1169,10 +1180,12 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
     for (int j = threadIdx.x; j <= ns; j += NT) tsblds[j] = tsb[j];
     for (int i = threadIdx.x; i <= ms; i += NT) asblds[i] = asb[i];
     __syncthreads();
+    bool ystaged = false;
     for (int period = 0;;) {
         if (S.phase != kAdvance) {
             check_period_call<NT>((const GLOBAL_AS LpDev *)(lps + s), (LDS_AS LpState *)&S, o, (LDS_AS double *)lds,
                                   (LDS_AS double *)stage);
+            ystaged = false;  // a restart may have replaced y
             if (S.status != kRunning) break;
         }
         if (threadIdx.x == 0)
@@ -1188,7 +1201,8 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
         const long long count = remaining < o.check_every ? remaining : o.check_every;
         const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
         for (long long done = 0; done < count;) {
-            attempt_lds<NT, 1>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds, &flag);
+            attempt_lds<NT, 1>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds, &flag,
+                               ystaged);
             if (flag == 1) ++done;
             else if (flag < 0) {
                 if (threadIdx.x == 0) S.status = kExhausted;
