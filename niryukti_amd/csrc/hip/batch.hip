@@ -58,8 +58,9 @@ constexpr int kThreads = 256;
 constexpr int kClasses = 4;
 constexpr int kClassDoubles[kClasses - 1] = {2048, 4096, 8000};
 // Upper bound of the LDS step kernels' static allocation (state, tile sums,
-// flags; the fused tail kernel adds the check's reduction slots and stage).
-constexpr int kStaticLds = 24576;
+// flags; the fused tail kernel adds the check's reduction slots and its
+// kWideChunk stage).
+constexpr int kStaticLds = 57344;
 constexpr int64_t kShortRow = 128;
 // The largest LDS class runs 512-thread workgroups: fewer rows per thread,
 // twice the loads in flight per LP.
@@ -104,10 +105,11 @@ struct LpDev {
     double *ox[2], *oy[2], *act, *aty, *atyabs;
     double *bx, *by;
     const int *ablk, *tblk, *oblk, *otblk;  // row blocks of A, A^T, original A, original A^T
+    const int *ablkw, *oblkw, *otblkw;      // the same for the tail kernel's wider check stage
     // A and A^T in 64-row slices for the LDS step kernel (see rows_dot_sell)
     const int *asb, *asi, *tsb, *tsi;
     const double *asv, *tsv;
-    int nab, ntb, nob, notb;
+    int nab, ntb, nob, notb, nabw, nobw, notbw;
 };
 
 struct LpState {
@@ -314,6 +316,12 @@ __device__ __forceinline__ GLOBAL_AS T *gout(T *p) {
 
 constexpr int kChunk = 2048;
 
+// Stage of the check in the fused tail kernel: its row blocks hold up to
+// kWideChunk nonzeros of rows that are each at most kChunk long, so a check
+// takes a third of the dependent passes (each pass is a chain of memory round
+// trips and two barriers, which is what bounds a check on one workgroup).
+constexpr int kWideChunk = 6144;
+
 // Start of a thread's strided loop over [begin, end) in which only the first
 // NT threads of the workgroup take part (the others start at `end`).
 template <int NT>
@@ -322,8 +330,11 @@ __device__ __forceinline__ int first_index(int begin, int end) {
 }
 
 // NT: threads of the workgroup; NTR (<= NT): threads that form the sum of a
-// row longer than kChunk, so that the sum does not depend on NT.
-template <bool ABS, int NT = kThreads, int NTR = NT, class Init, class Epi>
+// row longer than kChunk, so that the sum does not depend on NT.  Row blocks
+// (host row_blocks): consecutive rows of at most STAGE nonzeros in all, each
+// row at most kChunk long, or a single row longer than kChunk; every row up
+// to kChunk long is summed in storage order by one thread whatever STAGE is.
+template <bool ABS, int NT = kThreads, int NTR = NT, int STAGE = kChunk, class Init, class Epi>
 __device__ void stream_rows(const int *ptr_, const int *idx_, const double *val_, const int *blk_, int nblk,
                             const double *vec_, double *stage, double *red, Init init, Epi epi) {
     const GLOBAL_AS int *ptr = gin(ptr_), *idx = gin(idx_), *blk = gin(blk_);
@@ -331,7 +342,7 @@ __device__ void stream_rows(const int *ptr_, const int *idx_, const double *val_
     for (int b = 0; b < nblk; ++b) {
         const int r0 = blk[b], r1 = blk[b + 1];
         const int k0 = ptr[r0], k1 = ptr[r1];
-        if (k1 - k0 <= kChunk) {
+        if (k1 - k0 <= kChunk || r1 - r0 > 1) {
             for (int k = k0 + threadIdx.x; k < k1; k += NT) stage[k - k0] = val[k] * vec[idx[k]];
             __syncthreads();
             for (int r = r0 + threadIdx.x; r < r1; r += NT) {
@@ -358,10 +369,10 @@ __device__ void stream_rows(const int *ptr_, const int *idx_, const double *val_
     }
 }
 
-template <int NT = kThreads, int NTR = NT>
+template <int NT = kThreads, int NTR = NT, int STAGE = kChunk>
 __device__ void spmv(const int *ptr, const int *idx, const double *val, const int *blk, int nblk, const double *x,
                      double *out, double *stage, double *red) {
-    stream_rows<false, NT, NTR>(ptr, idx, val, blk, nblk, x, stage, red, [](int) { return 0.0; },
+    stream_rows<false, NT, NTR, STAGE>(ptr, idx, val, blk, nblk, x, stage, red, [](int) { return 0.0; },
                        [&](int r, double v, double) { out[r] = v; });
 }
 template <int NT = kThreads>
@@ -416,10 +427,12 @@ __device__ void farkas_columns(const LpDev &L, Reducer<NS, NM, ND> &r) {
     }
 }
 
-// NT threads copy and multiply; the first NTR form every sum.
-template <int NT, int NTR>
+// NT threads copy and multiply; the first NTR form every sum.  stage holds
+// STAGE doubles (kChunk, or kWideChunk with the wide row blocks).
+template <int NT, int NTR, int STAGE = kChunk>
 __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double *ox, double *oy, double *lds,
                         double *stage) {
+    constexpr bool wide = STAGE > kChunk;
     // restore_x / restore_y (src/preprocess.cpp:121-132)
     copy_vec<NT>(ox, L.fixed, L.n0);
     zero_vec<NT>(oy, L.m0);
@@ -430,8 +443,9 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
         oy[t] = 0.0 + y[i] * gin(L.dr)[i] * L.sobj;
     }
     __syncthreads();
-    spmv<NT, NTR>(L.op, L.oi, L.ov, L.oblk, L.nob, ox, L.act, stage, lds);
-    stream_rows<true, NT, NTR>(L.otp, L.oti, L.otv, L.otblk, L.notb, oy, stage, lds, [](int) { return 0.0; },
+    spmv<NT, NTR, STAGE>(L.op, L.oi, L.ov, wide ? L.oblkw : L.oblk, wide ? L.nobw : L.nob, ox, L.act, stage, lds);
+    stream_rows<true, NT, NTR, STAGE>(L.otp, L.oti, L.otv, wide ? L.otblkw : L.otblk, wide ? L.notbw : L.notb, oy,
+                                      stage, lds, [](int) { return 0.0; },
                       [&](int j, double v, double a) {
                           gout(L.aty)[j] = v;
                           gout(L.atyabs)[j] = a;
@@ -963,18 +977,20 @@ __global__ void gather_pieces_kernel(const CopyPiece *pieces, double *out) {
 // For an LP whose phase is kCheck or kResume; S is the workgroup's shared
 // copy of its state.  NT threads copy and multiply, the first kThreads form
 // every sum, so the check decides the same in any workgroup size.  lds holds
-// kWaves * 24 doubles, stage kChunk.
-template <int NT>
+// kWaves * 24 doubles, stage STAGE (with the matching row blocks, which sum
+// every row as the kChunk blocks do).
+template <int NT, int STAGE = kChunk>
 __device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, double *lds, double *stage) {
     constexpr int NTR = kThreads;
+    constexpr bool wide = STAGE > kChunk;
     const int cur = S.cur;
     double cand_kkt;
     int avg;
     if (S.phase == kCheck) {
         // candidates(): rebase the activity recurrence (src/cpu.cpp:229-230)
-        spmv<NT, NTR>(L.ap, L.ai, L.av, L.ablk, L.nab, L.x[cur], L.ax[cur], stage, lds);
-        const Acc cur_acc = evaluate<NT, NTR>(L, L.x[cur], L.y[cur], L.ox[0], L.oy[0], lds, stage);
-        const Acc avg_acc = evaluate<NT, NTR>(L, L.xa[cur], L.ya[cur], L.ox[1], L.oy[1], lds, stage);
+        spmv<NT, NTR, STAGE>(L.ap, L.ai, L.av, wide ? L.ablkw : L.ablk, wide ? L.nabw : L.nab, L.x[cur], L.ax[cur], stage, lds);
+        const Acc cur_acc = evaluate<NT, NTR, STAGE>(L, L.x[cur], L.y[cur], L.ox[0], L.oy[0], lds, stage);
+        const Acc avg_acc = evaluate<NT, NTR, STAGE>(L, L.xa[cur], L.ya[cur], L.ox[1], L.oy[1], lds, stage);
         avg = avg_acc.kkt < cur_acc.kkt ? 1 : 0;
         const Acc ca = avg ? avg_acc : cur_acc;
         const bool better = ca.finite && ca.kkt < S.best_kkt;
@@ -1006,7 +1022,8 @@ __device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, dou
                     gout(L.oy[0])[t] = 0.0 + d * gin(L.dr)[i] * L.sobj;
                 }
                 __syncthreads();
-                stream_rows<true, NT, NTR>(L.otp, L.oti, L.otv, L.otblk, L.notb, L.oy[0], stage, lds,
+                stream_rows<true, NT, NTR, STAGE>(L.otp, L.oti, L.otv, wide ? L.otblkw : L.otblk,
+                                                  wide ? L.notbw : L.notb, L.oy[0], stage, lds,
                                            [](int) { return 0.0; },
                                   [&](int j, double v, double a) {
                                       gout(L.aty)[j] = v;
@@ -1080,7 +1097,7 @@ __device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, dou
             copy_vec<NT>(L.y[cur], L.ya[cur], L.m);
             __syncthreads();
         }
-        spmv<NT, NTR>(L.ap, L.ai, L.av, L.ablk, L.nab, L.x[cur], L.ax[cur], stage, lds);
+        spmv<NT, NTR, STAGE>(L.ap, L.ai, L.av, wide ? L.ablkw : L.ablk, wide ? L.nabw : L.nab, L.x[cur], L.ax[cur], stage, lds);
         copy_vec<NT>(L.epx, L.x[cur], L.n);
         copy_vec<NT>(L.epy, L.y[cur], L.m);
         zero_vec<NT>(L.xa[cur], L.n);
@@ -1105,10 +1122,10 @@ __device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, dou
 // attempts.  The pointers carry their address spaces (LDS, global) into the
 // call, so its LDS and global accesses stay direct.
 #define LDS_AS __attribute__((address_space(3)))
-template <int NT>
+template <int NT, int STAGE>
 __device__ __noinline__ void check_period_call(const GLOBAL_AS LpDev *L, LDS_AS LpState *S, BatchOpts o,
                                                LDS_AS double *lds, LDS_AS double *stage) {
-    check_period<NT>(*(const LpDev *)L, *(LpState *)S, o, (double *)lds, (double *)stage);
+    check_period<NT, STAGE>(*(const LpDev *)L, *(LpState *)S, o, (double *)lds, (double *)stage);
 }
 
 __global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps, LpState *states, const int *slots,
@@ -1144,7 +1161,7 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
     __shared__ double tiles[3 * kMaxTiles];
     __shared__ double powlds[2];
     __shared__ double lds[kWaves * 24];
-    __shared__ double stage[kChunk];
+    __shared__ double stage[kWideChunk];
     __shared__ int flag, stop;
     if (threadIdx.x == 0) S = states[s];
     __syncthreads();
@@ -1183,7 +1200,7 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
     bool ystaged = false;
     for (int period = 0;;) {
         if (S.phase != kAdvance) {
-            check_period_call<NT>((const GLOBAL_AS LpDev *)(lps + s), (LDS_AS LpState *)&S, o, (LDS_AS double *)lds,
+            check_period_call<NT, kWideChunk>((const GLOBAL_AS LpDev *)(lps + s), (LDS_AS LpState *)&S, o, (LDS_AS double *)lds,
                                   (LDS_AS double *)stage);
             ystaged = false;  // a restart may have replaced y
             if (S.status != kRunning) break;
@@ -1307,10 +1324,10 @@ std::vector<hipStream_t> &stream_pool() {
     return pool;
 }
 
-// Splits CSR rows into consecutive blocks of at most kChunk nonzeros; a row
-// longer than kChunk is a block of its own.  Returns block start rows plus
-// the final row count.
-std::vector<int64_t> row_blocks(const std::vector<int64_t> &ptr) {
+// Splits CSR rows into consecutive blocks of at most `limit` nonzeros whose
+// rows are each at most kChunk long; a row longer than kChunk is a block of
+// its own.  Returns block start rows plus the final row count.
+std::vector<int64_t> row_blocks(const std::vector<int64_t> &ptr, int64_t limit = kChunk) {
     const int64_t rows = int64_t(ptr.size()) - 1;
     std::vector<int64_t> b{0};
     for (int64_t r = 0; r < rows;) {
@@ -1318,7 +1335,7 @@ std::vector<int64_t> row_blocks(const std::vector<int64_t> &ptr) {
         if (ptr[r + 1] - ptr[r] > kChunk)
             ++r;
         else
-            while (r < rows && ptr[r + 1] - ptr[start] <= kChunk) ++r;
+            while (r < rows && ptr[r + 1] - ptr[r] <= kChunk && ptr[r + 1] - ptr[start] <= limit) ++r;
         b.push_back(r);
     }
     return b;
@@ -1926,6 +1943,9 @@ class BatchEngine {
         const auto ba = row_blocks(m.A.ptr), bt = row_blocks(t_ptr(m)), bo = row_blocks(m0.A.ptr), bot = row_blocks(t0.ptr);
         size_t o_ab = L.add<int>(ba.size()), o_tb = L.add<int>(bt.size()), o_ob = L.add<int>(bo.size()),
                o_otb = L.add<int>(bot.size());
+        const auto baw = row_blocks(m.A.ptr, kWideChunk), bow = row_blocks(m0.A.ptr, kWideChunk),
+                   botw = row_blocks(t0.ptr, kWideChunk);
+        size_t o_abw = L.add<int>(baw.size()), o_obw = L.add<int>(bow.size()), o_otbw = L.add<int>(botw.size());
         nyk::Csr t = m.A.transposed();
         const Sliced sa = slice_rows(m.A.ptr, m.A.index, m.A.value), st = slice_rows(t.ptr, t.index, t.value);
         size_t o_asb = L.add<int>(sa.base.size()), o_asi = L.add<int>(sa.index.size()), o_asv = L.add<double>(sa.value.size());
@@ -2039,6 +2059,15 @@ class BatchEngine {
         d.tblk = (const int *)P(o_tb);
         d.oblk = (const int *)P(o_ob);
         d.otblk = (const int *)P(o_otb);
+        put_i(o_abw, baw);
+        put_i(o_obw, bow);
+        put_i(o_otbw, botw);
+        d.ablkw = (const int *)P(o_abw);
+        d.oblkw = (const int *)P(o_obw);
+        d.otblkw = (const int *)P(o_otbw);
+        d.nabw = int(baw.size()) - 1;
+        d.nobw = int(bow.size()) - 1;
+        d.notbw = int(botw.size()) - 1;
         put_i(o_asb, sa.base);
         put_i(o_asi, sa.index);
         put_d(o_asv, sa.value);
