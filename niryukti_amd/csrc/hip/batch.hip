@@ -57,6 +57,7 @@ constexpr int kThreads = 256;
 // the last class runs the global-memory stream kernel.
 constexpr int kClasses = 4;
 constexpr int kClassDoubles[kClasses - 1] = {2048, 4096, 8000};
+static_assert(kClassDoubles[kClasses - 2] <= 65536, "sliced entry indices are 16-bit");
 // Upper bound of the LDS step kernels' static allocation (state, tile sums,
 // flags; the fused tail kernel adds the check's reduction slots and its
 // kWideChunk stage).
@@ -108,7 +109,9 @@ struct LpDev {
     const int *ablk, *tblk, *oblk, *otblk;  // row blocks of A, A^T, original A, original A^T
     const int *ablkw, *oblkw, *otblkw;      // the same for the tail kernel's wider check stage
     // A and A^T in 64-row slices for the LDS step kernel (see rows_dot_sell)
-    const int *asb, *asi, *tsb, *tsi;
+    // (entry indices in 16 bits: an LDS-class LP has m + n <= 8,000)
+    const int *asb, *tsb;
+    const unsigned short *asi, *tsi;
     const double *asv, *tsv;
     int nab, ntb, nob, notb, nabw, nobw, notbw;
 };
@@ -700,7 +703,7 @@ __device__ __forceinline__ void rows_dot(const int *k, const int *e, const int *
 // row is still summed in storage order, so the sums equal rows_dot's.
 template <int NR>
 __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const int *sb,
-                                              const GLOBAL_AS int *si, const GLOBAL_AS double *sv, int slices, int b0,
+                                              const GLOBAL_AS unsigned short *si, const GLOBAL_AS double *sv, int slices, int b0,
                                               int nt, const double *vec, double *acc) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // Every lane of a slice owns `w` stored entries (padding is value 0,
@@ -730,7 +733,7 @@ __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const 
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             const GLOBAL_AS double *pv = sv + base[q] + t * 64;
-            const GLOBAL_AS int *pi = si + base[q] + t * 64;
+            const GLOBAL_AS unsigned short *pi = si + base[q] + t * 64;
 #pragma unroll
             for (int u = 0; u < kStep; ++u) {
                 if (full || t + u < w[q]) {
@@ -1802,9 +1805,12 @@ class BatchEngine {
     // streams (8-byte values, 4-byte indices and offsets), each gathered
     // vector once, and eight FP64 vector streams per side (read x, xa, c, lb,
     // ub / write x, xa, dx; read y, ya, ax, rl, ru / write y, ya, ax).
-    static double attempt_bytes(const Job &j) {
+    // (an entry is 8 value bytes and a 16-bit index in the sliced copies of
+    // the LDS classes, a 32-bit index in the global class's CSR)
+    double attempt_bytes(const Job &j) const {
         const double n = double(j.prep.model.cols()), m = double(j.prep.model.rows()), nnz = double(j.prep.model.A.nnz());
-        return 2 * nnz * 12 + (n + m + 2) * 4 + m * 8 + n * 8 + 64 * (n + m);
+        const double entry = lds_class(j) < kClasses - 1 ? 10 : 12;
+        return 2 * nnz * entry + (n + m + 2) * 4 + m * 8 + n * 8 + 64 * (n + m);
     }
 
     template <int NT, int NR>
@@ -1946,10 +1952,17 @@ class BatchEngine {
         const auto baw = row_blocks(m.A.ptr, kWideChunk), bow = row_blocks(m0.A.ptr, kWideChunk),
                    botw = row_blocks(t0.ptr, kWideChunk);
         size_t o_abw = L.add<int>(baw.size()), o_obw = L.add<int>(bow.size()), o_otbw = L.add<int>(botw.size());
+        // the sliced copies serve only the LDS classes (the global class walks the CSR)
+        Sliced sa, st;
         nyk::Csr t = m.A.transposed();
-        const Sliced sa = slice_rows(m.A.ptr, m.A.index, m.A.value), st = slice_rows(t.ptr, t.index, t.value);
-        size_t o_asb = L.add<int>(sa.base.size()), o_asi = L.add<int>(sa.index.size()), o_asv = L.add<double>(sa.value.size());
-        size_t o_tsb = L.add<int>(st.base.size()), o_tsi = L.add<int>(st.index.size()), o_tsv = L.add<double>(st.value.size());
+        if (lds_class(j) < kClasses - 1) {
+            sa = slice_rows(m.A.ptr, m.A.index, m.A.value);
+            st = slice_rows(t.ptr, t.index, t.value);
+        }
+        size_t o_asb = L.add<int>(sa.base.size()), o_asi = L.add<unsigned short>(sa.index.size()),
+               o_asv = L.add<double>(sa.value.size());
+        size_t o_tsb = L.add<int>(st.base.size()), o_tsi = L.add<unsigned short>(st.index.size()),
+               o_tsv = L.add<double>(st.value.size());
         if (!base) return L.total;
         image.assign(L.total, 0);
         auto put_i = [&](size_t off, const std::vector<int64_t> &v) {
@@ -2068,17 +2081,21 @@ class BatchEngine {
         d.nabw = int(baw.size()) - 1;
         d.nobw = int(bow.size()) - 1;
         d.notbw = int(botw.size()) - 1;
+        auto put_u16 = [&](size_t off, const std::vector<int64_t> &v) {
+            unsigned short *p = reinterpret_cast<unsigned short *>(image.data() + off);
+            for (size_t k = 0; k < v.size(); ++k) p[k] = (unsigned short)v[k];
+        };
         put_i(o_asb, sa.base);
-        put_i(o_asi, sa.index);
+        put_u16(o_asi, sa.index);
         put_d(o_asv, sa.value);
         put_i(o_tsb, st.base);
-        put_i(o_tsi, st.index);
+        put_u16(o_tsi, st.index);
         put_d(o_tsv, st.value);
         d.asb = (const int *)P(o_asb);
-        d.asi = (const int *)P(o_asi);
+        d.asi = (const unsigned short *)P(o_asi);
         d.asv = (const double *)P(o_asv);
         d.tsb = (const int *)P(o_tsb);
-        d.tsi = (const int *)P(o_tsi);
+        d.tsi = (const unsigned short *)P(o_tsi);
         d.tsv = (const double *)P(o_tsv);
         d.nab = int(ba.size()) - 1;
         d.ntb = int(bt.size()) - 1;
