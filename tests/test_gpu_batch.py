@@ -56,6 +56,25 @@ def test_batch_alone_equals_in_mix(engine):
         assert np.array_equal(alone[0]["x"], mix[k]["x"]) and np.array_equal(alone[0]["y"], mix[k]["y"])
 
 
+def test_global_class_beside_lds_classes(engine, oracle):
+    """An LP too large for the LDS classes (m + n = 15,000) runs in the global
+    class, which walks the CSR instead of the 16-bit sliced copies, next to
+    LDS-class LPs; all are certified, and the large one matches its solve
+    alone in the batch bit for bit."""
+    big = engine.Instance(1, 42, 200)
+    small = [engine.Instance(2, 7, 1, i) for i in range(3)]
+    assert big.lp.rows + big.lp.cols > 8000
+    mix, _ = solve_batch(engine, [big.lp] + [i.lp for i in small])
+    alone, _ = solve_batch(engine, [big.lp])
+    print("global class", mix[0]["status"], mix[0]["iterations"], flush=True)
+    for inst, r in zip([big] + small, mix):
+        assert r["status"] == "OPTIMAL" and r["kkt_error"] <= 1e-6
+        assert rel(r["objective"], inst.planted_objective) <= 1e-6
+        assert oracle.verify(inst.lp, r["x"], r["y"])["kkt"] <= 1e-6
+    assert alone[0]["iterations"] == mix[0]["iterations"]
+    assert np.array_equal(alone[0]["x"], mix[0]["x"]) and np.array_equal(alone[0]["y"], mix[0]["y"])
+
+
 def test_continuous_admission_small_arena(engine):
     """An arena that holds only a few LPs forces many admission rounds while
     other LPs keep iterating; results are unchanged."""
