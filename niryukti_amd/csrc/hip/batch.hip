@@ -30,6 +30,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <exception>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -1452,6 +1453,7 @@ class BatchEngine {
         for (int s = max_slots_ - 1; s >= 0; --s) free_slots_.push_back(s);
     }
     ~BatchEngine() {
+        if (settling_) settling_->worker.join();
         {
             std::lock_guard<std::mutex> lock(handoff_mu_);
             stopping_ = true;
@@ -2150,7 +2152,27 @@ class BatchEngine {
 
     // Handles every LP whose workgroup stopped: certification, Farkas checks,
     // limits and errors (src/solver.cpp:583-633, 727-731).
-    void settle(const std::vector<int> &active) {
+    // LPs that stopped in one round.  The host certifies them on a thread of
+    // its own while the next round runs on the device (their slots stay
+    // parked, so the device skips them), and the outcome is applied after
+    // that round: a finished LP is retired one round later, an LP whose
+    // certification fails resumes one round later.  Neither changes any
+    // result, only when it is reported.
+    struct Settling {
+        std::vector<int> slots;
+        std::vector<Job *> jobs;
+        std::vector<LpState> states;  // the parked states, as downloaded
+        std::vector<std::vector<double>> cx, cyo, cy, cya, epy;
+        std::vector<int> resume;  // 0 finished, 1 resume, 2 hand off
+        std::vector<double> cand_kkt;
+        int64_t certifications = 0, farkas_checks = 0;
+        std::thread worker;
+        std::exception_ptr error;
+    };
+
+    // Collects the LPs that stopped in the round just run (and running LPs
+    // past the time limit) and downloads what their certification needs.
+    std::unique_ptr<Settling> collect(const std::vector<int> &active) {
         std::vector<int> stopped;
         for (int s : active)
             if (state_[s].status != kRunning) stopped.push_back(s);
@@ -2163,26 +2185,38 @@ class BatchEngine {
                 stopped.push_back(s);
             }
         }
-        if (stopped.empty()) return;
+        if (stopped.empty()) return nullptr;
+        auto p = std::make_unique<Settling>();
         const size_t ns = stopped.size();
-        std::vector<std::vector<double>> cx(ns), cyo(ns), cy(ns), cya(ns), epy(ns);
+        p->slots = stopped;
+        p->jobs.resize(ns);
+        p->states.resize(ns);
+        p->cx.resize(ns);
+        p->cyo.resize(ns);
+        p->cy.resize(ns);
+        p->cya.resize(ns);
+        p->epy.resize(ns);
+        p->resume.assign(ns, 0);
+        p->cand_kkt.assign(ns, 0.0);
         for (size_t k = 0; k < ns; ++k) {
             const int s = stopped[k];
             const LpDev &d = desc_[s];
             const LpState &st = state_[s];
             Job &j = *jobs_[slot_job_[s]];
+            p->jobs[k] = &j;
+            p->states[k] = st;
             if (st.status == kHandoff) {
                 // the current (scaled) iterate; restored on the host below
-                cx[k].resize(d.n);
-                cyo[k].resize(d.m);
-                download(cx[k].data(), d.x[st.cur], d.n);
-                download(cyo[k].data(), d.y[st.cur], d.m);
+                p->cx[k].resize(d.n);
+                p->cyo[k].resize(d.m);
+                download(p->cx[k].data(), d.x[st.cur], d.n);
+                download(p->cyo[k].data(), d.y[st.cur], d.m);
             } else if (st.status == kParkOptimal) {
                 // the parked candidate is still in its evaluation buffers
-                cx[k].resize(d.n0);
-                cyo[k].resize(d.m0);
-                download(cx[k].data(), d.ox[st.avg], d.n0);
-                download(cyo[k].data(), d.oy[st.avg], d.m0);
+                p->cx[k].resize(d.n0);
+                p->cyo[k].resize(d.m0);
+                download(p->cx[k].data(), d.ox[st.avg], d.n0);
+                download(p->cyo[k].data(), d.oy[st.avg], d.m0);
             } else {
                 j.result.x.resize(d.n0);
                 j.result.y.resize(d.m0);
@@ -2191,55 +2225,58 @@ class BatchEngine {
             }
             if (st.status == kParkFarkas) {
                 const int c = st.cur;
-                cy[k].resize(d.m);
-                cya[k].resize(d.m);
-                epy[k].resize(d.m);
-                download(cy[k].data(), d.y[c], d.m);
-                download(cya[k].data(), d.ya[c], d.m);
-                download(epy[k].data(), d.epy, d.m);
+                p->cy[k].resize(d.m);
+                p->cya[k].resize(d.m);
+                p->epy[k].resize(d.m);
+                download(p->cy[k].data(), d.y[c], d.m);
+                download(p->cya[k].data(), d.ya[c], d.m);
+                download(p->epy[k].data(), d.epy, d.m);
             }
         }
         flush_downloads();
-        std::vector<int> resume(stopped.size(), 0);
+        return p;
+    }
+
+    // Host certification of a round's stopped LPs; touches only their jobs
+    // and the Settling record (never state_, which the next round's
+    // download rewrites meanwhile).
+    void certify(Settling &p) {
         std::mutex mu;
-        nyk::parallel_tasks(int64_t(ns), [&](int64_t k) {
-            const int s = stopped[k];
-            Job &j = *jobs_[slot_job_[s]];
-            LpState &st = state_[s];
+        nyk::parallel_tasks(int64_t(p.slots.size()), [&](int64_t k) {
+            Job &j = *p.jobs[k];
+            const LpState &st = p.states[k];
             auto &r = j.result;
             auto t0 = Clock::now();
             copy_counters(j, st);
             if (st.status == kHandoff) {
-                j.handoff_x = j.prep.restore_x(cx[k]);
-                j.handoff_y = j.prep.restore_y(cyo[k], j.original.rl.size());
-                resume[k] = 2;
+                j.handoff_x = j.prep.restore_x(p.cx[k]);
+                j.handoff_y = j.prep.restore_y(p.cyo[k], j.original.rl.size());
+                p.resume[k] = 2;
             } else if (st.status == kParkOptimal) {
                 // src/solver.cpp:588-596: OPTIMAL only after the safe re-evaluation
-                auto safe = j.verifier->evaluate(cx[k], cyo[k]);
+                auto safe = j.verifier->evaluate(p.cx[k], p.cyo[k]);
                 {
                     std::lock_guard<std::mutex> lock(mu);
-                    ++certifications_;
+                    ++p.certifications;
                 }
                 if (safe.finite && safe.kkt <= opt_.tol) {
-                    r.x = std::move(cx[k]);
-                    r.y = std::move(cyo[k]);
+                    r.x = std::move(p.cx[k]);
+                    r.y = std::move(p.cyo[k]);
                     r.accuracy = safe;
                     r.status = "OPTIMAL";
                 } else {
-                    st.cand_kkt = safe.kkt;  // the reference continues with the safe accuracy
-                    st.phase = 1;
-                    st.status = kRunning;
-                    resume[k] = 1;
+                    p.cand_kkt[k] = safe.kkt;  // the reference continues with the safe accuracy
+                    p.resume[k] = 1;
                 }
             } else if (st.status == kParkFarkas) {
                 {
                     std::lock_guard<std::mutex> lock(mu);
-                    ++farkas_checks_;
+                    ++p.farkas_checks;
                 }
                 const bool avg = st.avg != 0;
-                auto ysel = avg ? cya[k] : cy[k], yother = avg ? cy[k] : cya[k];
+                auto ysel = avg ? p.cya[k] : p.cy[k], yother = avg ? p.cy[k] : p.cya[k];
                 auto disp = ysel;
-                for (size_t i = 0; i < disp.size(); ++i) disp[i] -= epy[k][i];
+                for (size_t i = 0; i < disp.size(); ++i) disp[i] -= p.epy[k][i];
                 std::vector<std::vector<double>> rays{j.prep.restore_y(ysel, j.original.rl.size()),
                                                       j.prep.restore_y(yother, j.original.rl.size()),
                                                       j.prep.restore_y(disp, j.original.rl.size())};
@@ -2264,9 +2301,8 @@ class BatchEngine {
                     }
                 }
                 if (!infeasible) {
-                    st.phase = 1;
-                    st.status = kRunning;
-                    resume[k] = 1;
+                    p.cand_kkt[k] = st.cand_kkt;
+                    p.resume[k] = 1;
                 }
             } else if (st.status == kNumerical) {
                 r.status = "NUMERICAL_ERROR";
@@ -2278,16 +2314,56 @@ class BatchEngine {
                 r.status = "ITERATION_LIMIT";
             }
             r.verification_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
-            if (!resume[k]) {
+            if (!p.resume[k]) {
                 // final original-space verification (src/solver.cpp:727-731)
                 if (r.status != "OPTIMAL") r.accuracy = j.verifier->evaluate(r.x, r.y);
                 if (r.status == "OPTIMAL" && (!r.accuracy.finite || r.accuracy.kkt > opt_.tol)) r.status = "NUMERICAL_ERROR";
             }
         });
-        for (size_t k = 0; k < ns; ++k) {
-            if (resume[k] == 2) hand_off(*jobs_[slot_job_[stopped[k]]]);
-            else if (!resume[k]) retire(*jobs_[slot_job_[stopped[k]]]);
+    }
+
+    // Applies a finished certification: retire, resume (from the next
+    // round on) or hand off each LP.
+    void apply(Settling &p) {
+        certifications_ += p.certifications;
+        farkas_checks_ += p.farkas_checks;
+        for (size_t k = 0; k < p.slots.size(); ++k) {
+            Job &j = *p.jobs[k];
+            if (p.resume[k] == 1) {
+                LpState &st = state_[p.slots[k]];
+                st.cand_kkt = p.cand_kkt[k];
+                st.phase = 1;
+                st.status = kRunning;
+            } else if (p.resume[k] == 2) {
+                hand_off(j);
+            } else {
+                retire(j);
+            }
         }
+    }
+
+    // Waits for the previous round's certification and applies it.
+    void finish_settling() {
+        if (!settling_) return;
+        settling_->worker.join();
+        std::unique_ptr<Settling> p = std::move(settling_);
+        if (p->error) std::rethrow_exception(p->error);
+        apply(*p);
+    }
+
+    void settle(const std::vector<int> &active) {
+        finish_settling();
+        std::unique_ptr<Settling> p = collect(active);
+        if (!p) return;
+        Settling *raw = p.get();
+        p->worker = std::thread([this, raw] {
+            try {
+                certify(*raw);
+            } catch (...) {
+                raw->error = std::current_exception();
+            }
+        });
+        settling_ = std::move(p);
     }
 
     // Frees the LP's slot and arena space and queues it for a grid worker.
@@ -2430,6 +2506,7 @@ class BatchEngine {
     std::vector<LpState> state_;
     std::vector<int> slot_job_, free_slots_;
     std::vector<double> slot_bytes_, slot_check_bytes_;
+    std::unique_ptr<Settling> settling_;  // the certification running beside the next round
     long long tail_budget_ticks_ = 0;
     hipStream_t class_stream_[kClasses + 1] = {};  // per class, then the tail group
     double *d_pow3_ = nullptr, *d_pow6_ = nullptr;
