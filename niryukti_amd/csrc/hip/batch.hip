@@ -1171,8 +1171,10 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
     if (threadIdx.x == 0) S = states[s];
     __syncthreads();
     if (S.status != kRunning) return;
-    // The attempts' copy of the descriptor, read before any store (scalar
-    // loads) and kept in scalar registers across the check calls.
+    // The attempts' copy of the descriptor: its pointers and sizes moved to
+    // scalar registers, where they stay across the check calls (read through
+    // the descriptor after a call, they would be vector loads, and every
+    // address of the attempts would be formed in vector registers).
     LpDev L = lps[s];
     for (int h = 0; h < 2; ++h) {
         L.x[h] = uniform(L.x[h]);
