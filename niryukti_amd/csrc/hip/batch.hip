@@ -1697,6 +1697,10 @@ class BatchEngine {
         out->iteration_seconds = r.iteration_seconds;
         out->verification_seconds = r.verification_seconds;
         out->index_bits = 32;
+        out->device_prepare = 0;
+        out->operator_norm_estimate = r.operator_norm_estimate;
+        out->polishing_iterations = 0;
+        out->polishing_attempts = 0;
         if (out->x && r.x.size() == size_t(j.original.cols())) std::memcpy(out->x, r.x.data(), r.x.size() * 8);
         if (out->y && r.y.size() == size_t(j.original.rows())) std::memcpy(out->y, r.y.data(), r.y.size() * 8);
         return true;
@@ -2578,6 +2582,8 @@ nyk::Options options_from(const niryukti_options *in) {
     o.presolve = in->presolve != 0;
     o.restart = in->restart != 0;
     o.adaptive = in->adaptive != 0;
+    o.power_iterations = in->power_iterations;
+    o.polishing = in->polishing != 0;
     return o;
 }
 
@@ -2596,6 +2602,8 @@ int niryukti_batch_create(const niryukti_options *opts, int64_t arena_bytes, int
         nyk::validate_options(o);
         if (o.primal_weight != "displacement" || o.matrix_precision != "fp64")
             throw std::runtime_error("The batched engine supports the displacement weight and FP64 matrices");
+        if (o.polishing)
+            throw std::runtime_error("Feasibility polishing runs in the single-LP engine, not in the batched engine");
         *out = new niryukti_batch(o, arena_bytes, max_slots);
     });
 }

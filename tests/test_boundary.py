@@ -253,3 +253,10 @@ def test_model_handle_writes_reference_layout(engine, tmp_path):
     assert engine.Model(tmp_path / "afiro.json").fingerprint == m.fingerprint
     with pytest.raises(ValueError, match="Convert output"):
         m.write(tmp_path / "afiro.lp")
+
+
+def test_batch_refuses_polishing(engine):
+    """Feasibility polishing is a single-LP driver feature; the batched engine
+    refuses it at creation (before it looks for a device)."""
+    with pytest.raises(RuntimeError, match="polishing"):
+        engine.Batch(arena_bytes=1 << 20, max_slots=1, polishing=1)

@@ -178,3 +178,19 @@ def test_launch_shape_does_not_change_results(engine):
         # the checks decided alike: the same restarts and primal weight updates
         assert a["restarts"] == mix[k]["restarts"] and a["weight_updates"] == mix[k]["weight_updates"]
         assert np.array_equal(a["x"], mix[k]["x"]) and np.array_equal(a["y"], mix[k]["y"])
+
+
+def test_power_iterations_in_batch(engine):
+    """--power-iterations in the batched engine: the same host estimate of
+    ||A||_2 as the single-LP engine (the same seeded power iteration on the
+    same prepared matrix), the larger first step, and a certified answer that
+    is the same alone and in a mix."""
+    insts = [engine.Instance(2, 7, 1, i) for i in range(4)]
+    mix, _ = solve_batch(engine, [i.lp for i in insts], power_iterations=20)
+    alone, _ = solve_batch(engine, [insts[1].lp], power_iterations=20)
+    single = engine.solve_lp(insts[1].lp, power_iterations=20)
+    print("estimates", mix[1]["operator_norm_estimate"], single["operator_norm_estimate"], flush=True)
+    assert mix[1]["operator_norm_estimate"] == single["operator_norm_estimate"] > 0
+    for inst, r in zip(insts, mix):
+        assert r["status"] == "OPTIMAL" and rel(r["objective"], inst.planted_objective) <= 1e-6
+    assert alone[0]["iterations"] == mix[1]["iterations"] and np.array_equal(alone[0]["x"], mix[1]["x"])
