@@ -64,9 +64,11 @@ static_assert(kClassDoubles[kClasses - 2] <= 65536, "sliced entry indices are 16
 // kWideChunk stage).
 constexpr int kStaticLds = 57344;
 constexpr int64_t kShortRow = 128;
-// The largest LDS class runs 1,024-thread workgroups with one row per thread:
-// its LDS allows one workgroup per CU, and 16 waves keep more loads in flight
-// than 8 (measured 8-10 % faster per period than 512 threads x 2 rows).
+// The two larger LDS classes (m + n above 2,048) run 1,024-thread workgroups
+// with one row per thread: their LDS allows one or two workgroups per CU, and
+// 16 waves per workgroup keep more loads in flight (measured: the largest
+// class 8-10 % faster per period than 512 threads x 2 rows; the crowded
+// phase 3 % shorter with the middle class moved too).
 constexpr int kWideThreads = 1024;
 // LDS classes with at most kTailLps LPs left (one per CU or fewer) are merged
 // into one tail launch of kTailThreads-thread workgroups, one row per thread.
@@ -1581,7 +1583,7 @@ class BatchEngine {
                 } else {
                     for (int k = 0; k < checks; ++k) {
                         if (g.kind < kClasses - 1)
-                            launch_lds(g.cap, g.kind == kClasses - 2, grid, cs, slots, bo);
+                            launch_lds(g.cap, g.kind >= 1, grid, cs, slots, bo);
                         else
                             batch_advance_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
                         HIP_OK(hipEventRecord(ev[2 * k + 1], cs));
@@ -1844,7 +1846,7 @@ class BatchEngine {
             d_desc_, d_state_, slots, bo, checks, checks * kTailMaxPeriodsPerCheck, tail_budget_ticks_);
     }
     // A crowded LDS class: 256-thread workgroups with rows_per_thread_ rows
-    // per thread (the largest class: kWideThreads threads, one row each).
+    // per thread (the two larger classes: kWideThreads threads, one row each).
     void launch_lds(int cap, bool wide, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
         if (wide) {
             launch_lds_t<kWideThreads, 1>(cap, grid, cs, slots, bo);

@@ -157,8 +157,8 @@ def test_handoff_to_grid_engine(engine, oracle):
 
 def test_launch_shape_does_not_change_results(engine):
     """A crowded class runs 256-thread workgroups with several rows per thread
-    (1,024 threads, one row each, for the largest class) and a separate check
-    kernel; once few LPs are left it switches to
+    (1,024 threads, one row each, for the two larger classes) and a separate
+    check kernel; once few LPs are left it switches to
     the fused tail kernel (1024-thread workgroups that also run the checks).
     The per-tile sums keep every step's arithmetic independent of that shape,
     and the fused check forms its sums on 256 threads as the check kernel
