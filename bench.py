@@ -440,7 +440,7 @@ def main():
     dom = max(range(groups), key=lambda c: cls_b[c])
     adv_b, adv_s, launches = cls_b[dom], cls_s[dom], cls_n[dom]
     achieved = adv_b / adv_s / 1e9 if adv_s > 0 else 0.0
-    rows = int(os.environ.get("NIRYUKTI_BATCH_ROWS_PER_THREAD", "2"))
+    rows = int(os.environ.get("NIRYUKTI_BATCH_ROWS_PER_THREAD", "1"))
     symbol = [f"batch_advance_lds_kernel<256, {rows}>", "batch_advance_lds_kernel<1024, 1>",
               "batch_advance_lds_kernel<1024, 1>", "batch_advance_kernel", "batch_tail_kernel<1024>"][dom]
     kernel_name = symbol + [" (class m+n <= 2048, LDS)", " (class m+n <= 4096, LDS)", " (class m+n <= 8000, LDS)",

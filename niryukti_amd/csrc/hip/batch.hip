@@ -2528,7 +2528,7 @@ class BatchEngine {
     CopyPiece *d_pieces_ = nullptr;
     long long stage_capacity_ = 0, piece_capacity_ = 0;
     long long pow_count_ = 0;
-    int rows_per_thread_ = 2;  // measured best on gfx950 for the 3-8 nnz/column batch
+    int rows_per_thread_ = 1;  // measured best on gfx950 for the smallest class (m + n <= 2,048)
     int class_cap_[kClasses - 1] = {};
     std::vector<hipEvent_t> class_events_[kClasses + 1];
     hipEvent_t uploaded_ = nullptr;

@@ -155,7 +155,7 @@ def test_handoff_to_grid_engine(engine, oracle):
         assert np.array_equal(alone[0]["x"], mix[k]["x"]) and alone[0]["iterations"] == mix[k]["iterations"]
 
 
-def test_launch_shape_does_not_change_results(engine):
+def test_launch_shape_does_not_change_results(engine, monkeypatch):
     """A crowded class runs 256-thread workgroups with several rows per thread
     (1,024 threads, one row each, for the two larger classes) and a separate
     check kernel; once few LPs are left it switches to
@@ -178,6 +178,13 @@ def test_launch_shape_does_not_change_results(engine):
         # the checks decided alike: the same restarts and primal weight updates
         assert a["restarts"] == mix[k]["restarts"] and a["weight_updates"] == mix[k]["weight_updates"]
         assert np.array_equal(a["x"], mix[k]["x"]) and np.array_equal(a["y"], mix[k]["y"])
+    # 2 or 4 rows per thread in the smallest class (read when a batch is created)
+    for rows in ("2", "4"):
+        monkeypatch.setenv("NIRYUKTI_BATCH_ROWS_PER_THREAD", rows)
+        other, _ = solve_batch(engine, [i.lp for i in insts[:300]], checks=1)
+        for k in (0, 101, 257):
+            assert other[k]["iterations"] == mix[k]["iterations"], rows
+            assert np.array_equal(other[k]["x"], mix[k]["x"]) and np.array_equal(other[k]["y"], mix[k]["y"]), rows
 
 
 def test_power_iterations_in_batch(engine):
