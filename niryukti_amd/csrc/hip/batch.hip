@@ -79,6 +79,9 @@ constexpr int kTailThreads = 1024;
 // the requested periods), so that LPs of different speed do not wait for
 // each other at every period.
 constexpr int kTailBudgetMs = 32;
+// Rounds in which at most this many LPs stop certify them before returning;
+// larger sets are certified on a host thread beside the next round.
+constexpr int kInlineCertify = 32;
 constexpr int kTailMaxPeriodsPerCheck = 1000;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxRejects = 40;
@@ -2363,6 +2366,13 @@ class BatchEngine {
         finish_settling();
         std::unique_ptr<Settling> p = collect(active);
         if (!p) return;
+        if (p->slots.size() <= size_t(kInlineCertify)) {
+            // a few LPs: certify now (a millisecond or so), so their results
+            // are not held back a round (what a served request would wait)
+            certify(*p);
+            apply(*p);
+            return;
+        }
         Settling *raw = p.get();
         p->worker = std::thread([this, raw] {
             try {
