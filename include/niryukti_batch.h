@@ -59,9 +59,10 @@ int niryukti_batch_submit(niryukti_batch *b, const niryukti_lp *lp, const double
  * `checks_per_launch` check periods per resident LP (LPs of a class with at
  * most 256 LPs left run in one fused launch that continues past that number
  * until about 32 ms have passed, so LPs of different speed do not wait for
- * each other).  LPs that stopped in a round are certified on a host thread
- * while the next round runs on the device and are retired (or resume) at the
- * end of the next call.  Results do not depend on how periods fall into
+ * each other).  LPs that stopped in a round are certified before the call
+ * returns when they are few (at most 32); a larger set is certified on a host
+ * thread while the next round runs on the device and is retired (or resumes)
+ * at the end of the next call.  Results do not depend on how periods fall into
  * rounds.  *running receives the number still queued, resident or being
  * certified. */
 int niryukti_batch_step(niryukti_batch *b, int checks_per_launch, int64_t *running);
