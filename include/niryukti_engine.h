@@ -62,6 +62,11 @@ typedef struct {
     int device_prepare; /* 1 when scaling and transpose ran on the device */
     double operator_norm_estimate; /* power-iteration estimate (0 when not requested) */
     int64_t polishing_iterations, polishing_attempts;
+    /* Decision trace (src/solver.cpp:299-333, 692-725): the first step and
+     * primal weight, the iteration of the first restart and the primal weight
+     * right after it (both 0 when no restart happened). */
+    double initial_step, initial_weight, first_restart_weight;
+    int64_t first_restart_iteration;
     double *x; /* NULL or cols: receives the primal solution */
     double *y; /* NULL or rows: receives the dual solution */
 } niryukti_result;

@@ -58,7 +58,9 @@ class _Result(C.Structure):
                 ("preprocess_seconds", C.c_double), ("transfer_seconds", C.c_double),
                 ("iteration_seconds", C.c_double), ("verification_seconds", C.c_double),
                 ("index_bits", C.c_int), ("device_prepare", C.c_int), ("operator_norm_estimate", C.c_double),
-                ("polishing_iterations", C.c_int64), ("polishing_attempts", C.c_int64), ("x", _f64p), ("y", _f64p)]
+                ("polishing_iterations", C.c_int64), ("polishing_attempts", C.c_int64),
+                ("initial_step", C.c_double), ("initial_weight", C.c_double), ("first_restart_weight", C.c_double),
+                ("first_restart_iteration", C.c_int64), ("x", _f64p), ("y", _f64p)]
 
 
 class _PdhgLP(C.Structure):
@@ -194,10 +196,24 @@ class LP:
     sense: float = 1.0
 
     def __post_init__(self):
-        self.ptr = np.ascontiguousarray(self.ptr, dtype=np.int64)
-        self.index = np.ascontiguousarray(self.index, dtype=np.int64)
+        self.ptr = np.ascontiguousarray(self.ptr, dtype=np.int64).reshape(-1)
+        self.index = np.ascontiguousarray(self.index, dtype=np.int64).reshape(-1)
         for name in ("value", "c", "lb", "ub", "rl", "ru"):
-            setattr(self, name, _f64(getattr(self, name)))
+            setattr(self, name, _f64(getattr(self, name)).reshape(-1))
+        # The native side reads exactly these extents from the raw pointers.
+        m, n = len(self.rl), len(self.c)
+        if len(self.ptr) != m + 1:
+            raise ValueError(f"ptr has {len(self.ptr)} entries; rows + 1 = {m + 1} expected")
+        if self.ptr[0] != 0 or np.any(np.diff(self.ptr) < 0):
+            raise ValueError("ptr must start at 0 and be nondecreasing")
+        nnz = int(self.ptr[-1])
+        if len(self.index) != nnz or len(self.value) != nnz:
+            raise ValueError(f"index and value must have ptr[-1] = {nnz} entries "
+                             f"(got {len(self.index)} and {len(self.value)})")
+        if len(self.lb) != n or len(self.ub) != n:
+            raise ValueError(f"lb and ub must have cols = {n} entries (got {len(self.lb)} and {len(self.ub)})")
+        if len(self.ru) != m:
+            raise ValueError(f"ru must have rows = {m} entries (got {len(self.ru)})")
 
     @property
     def rows(self) -> int:
@@ -225,12 +241,23 @@ class LP:
         return cls(np.cumsum(ptr), cols, A[rows, cols], c, lb, ub, rl, ru, offset, sense)
 
 
-def _options_struct(lib, options: dict, keep: list) -> _Options:
+def _warm_array(val, size: int, what: str) -> np.ndarray:
+    """A warm-start vector of exactly `size` entries (the native side copies
+    that many doubles from the pointer)."""
+    arr = _f64(val).reshape(-1)
+    if arr.size != size:
+        raise ValueError(f"{what} has {arr.size} entries; {size} expected")
+    return arr
+
+
+def _options_struct(lib, options: dict, keep: list, lp: "LP | None" = None) -> _Options:
     opts = _Options()
     lib.niryukti_default_options(C.byref(opts))
     for key, val in options.items():
         if key in ("initial_x", "initial_y"):
-            arr = _f64(val)
+            if lp is None:
+                raise ValueError(f"{key} needs the LP it warm-starts")
+            arr = _warm_array(val, lp.cols if key == "initial_x" else lp.rows, key)
             keep.append(arr)
             setattr(opts, key, _ptr(arr))
         elif key in _OPTION_STRINGS:
@@ -262,7 +289,7 @@ def solve_lp(lp: LP, **options) -> dict:
     """Solves `lp` with the GPU engine; keyword options mirror the reference's Options."""
     lib = load_library()
     keep = []
-    opts = _options_struct(lib, options, keep)
+    opts = _options_struct(lib, options, keep, lp)
     x = np.zeros(lp.cols)
     y = np.zeros(lp.rows)
     res = _Result()
@@ -280,7 +307,7 @@ def prepare_lp(lp: LP, **options) -> dict:
     the prepared arrays, scales and maps, or {"failure": message}."""
     lib = load_library()
     keep = []
-    opts = _options_struct(lib, options, keep)
+    opts = _options_struct(lib, options, keep, lp)
     m, n, nnz = lp.rows, lp.cols, lp.nnz
     out = {k: np.zeros(max(size, 1)) for k, size in (("value", nnz), ("c", n), ("lb", n), ("ub", n), ("rl", m),
                                                       ("ru", m), ("column_scale", n), ("row_scale", m))}
@@ -545,13 +572,16 @@ class Batch:
         return rc
 
     def submit(self, lp: LP, x0=None, y0=None) -> int:
+        """Queues `lp` (copied) for prepare and admission; returns its ticket.
+        x0 / y0: optional original-space warm start of cols / rows entries."""
         s = lp._struct()
-        px = _ptr(_f64(x0)) if x0 is not None else None
-        py = _ptr(_f64(y0)) if y0 is not None else None
-        keep = (x0, y0)
+        wx = _warm_array(x0, lp.cols, "x0") if x0 is not None else None
+        wy = _warm_array(y0, lp.rows, "y0") if y0 is not None else None
+        px = _ptr(wx) if wx is not None else None
+        py = _ptr(wy) if wy is not None else None
         t = C.c_int64()
         self._check(self._lib.niryukti_batch_submit(self._h, C.byref(s), px, py, C.byref(t)))
-        del keep
+        del wx, wy
         self._shapes[t.value] = (lp.cols, lp.rows)
         return t.value
 

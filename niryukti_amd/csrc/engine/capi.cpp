@@ -191,6 +191,10 @@ int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *opts, niryu
         out->operator_norm_estimate = r.operator_norm_estimate;
         out->polishing_iterations = r.polishing_iterations;
         out->polishing_attempts = r.polishing_attempts;
+        out->initial_step = r.initial_step;
+        out->initial_weight = r.initial_weight;
+        out->first_restart_weight = r.first_restart_weight;
+        out->first_restart_iteration = r.first_restart_iteration;
         if (out->x && r.x.size() == size_t(lp->cols)) std::memcpy(out->x, r.x.data(), r.x.size() * sizeof(double));
         if (out->y && r.y.size() == size_t(lp->rows)) std::memcpy(out->y, r.y.data(), r.y.size() * sizeof(double));
         return 0;

@@ -396,7 +396,10 @@ void run_device_checks(const Model &original, const Prepared &prep, const Option
             backend.set_epoch(false);
             restart_kkt = ca.kkt;
             since_restart = 0;
-            r.restarts++;
+            if (r.restarts++ == 0) {
+                r.first_restart_iteration = r.iterations;
+                r.first_restart_weight = weight;
+            }
         }
     }
     auto check = Clock::now();
@@ -528,6 +531,8 @@ Result solve_continuous(const Model &original, const Options &o) {
         }
         if (c2 > 1e-24 && b2 > 1e-24) weight = std::clamp(std::sqrt(double(c2 / b2)), 1e-4, 1e4);
     }
+    r.initial_step = step;
+    r.initial_weight = weight;
     const bool residual_restarts = o.method == "rhpdhg" || o.method == "r2hpdhg";
     auto transfer = Clock::now();
     npdhg_config cfg{};
@@ -687,7 +692,10 @@ Result solve_continuous(const Model &original, const Options &o) {
             epochy = cy;
             restart_kkt = ca.kkt;
             since_restart = 0;
-            r.restarts++;
+            if (r.restarts++ == 0) {
+                r.first_restart_iteration = r.iterations;
+                r.first_restart_weight = weight;
+            }
         }
         cx.resize(m.cols());
         cy.resize(m.rows());

@@ -99,6 +99,11 @@ struct Result {
     bool device_prepare = false;  // scaling and transpose ran on the device
     double operator_norm_estimate = 0;
     int64_t polishing_iterations = 0, polishing_attempts = 0;
+    // Decision trace: the first step and primal weight (src/solver.cpp:299-333),
+    // and the iteration of the first restart with the weight after it
+    // (src/solver.cpp:692-725; 0 and 0 when no restart happened).
+    double initial_step = 0, initial_weight = 0, first_restart_weight = 0;
+    int64_t first_restart_iteration = 0;
     std::string matrix_precision = "fp64", method_selected, device_reason;
     double estimated_gpu_bytes = 0;
     // The request as the caller made it (reported under "selection").

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cmath>
@@ -124,7 +125,9 @@ struct LpDev {
 
 struct LpState {
     double step, weight, factor, mass, restart_kkt, best_kkt, cand_kkt;
+    double first_weight;  // primal weight after the first restart
     long long iterations, since_restart, accepted, rejected, restarts, weight_updates, checks, iteration_limit;
+    long long first_restart;  // iteration of the first restart (0: none yet)
     int cur, status, phase, avg;
     int consecutive_rejects, farkas_parks;
 };
@@ -1118,6 +1121,10 @@ __device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, dou
             S.restart_kkt = cand_kkt;
             S.since_restart = 0;
             S.restarts += 1;
+            if (S.restarts == 1) {
+                S.first_restart = S.iterations;
+                S.first_weight = S.weight;
+            }
         }
     }
     __syncthreads();
@@ -1317,12 +1324,20 @@ struct Job {
     nyk::Prepared prep;
     std::unique_ptr<nyk::Verifier> verifier;
     nyk::Result result;
-    bool finished = false;
+    bool finished = false;  // written by the scheduling thread only
     std::vector<double> x, y;  // scaled start point
+    std::vector<double> warm_x, warm_y;  // original-space warm start as submitted (empty: cold)
     double step = 1, weight = 1;
     Clock::time_point start;
     int slot = -1;
     size_t offset = 0, bytes = 0;
+    // Filled by a prepare worker before the job reaches the ready list.
+    LpDev dev{};               // device pointers into the job's arena range
+    int cls = 0;               // launch class
+    double attempt_b = 0, check_b = 0;
+    bool terminal = false;     // the host decided the result without iterating
+    bool resident = false;     // the image is in the arena at `offset`
+    std::string prepare_error; // non-empty: prepare or upload failed
 };
 
 // One main stream plus one stream per launch class, created on first use and
@@ -1468,6 +1483,17 @@ class BatchEngine {
         }
         handoff_cv_.notify_all();
         for (auto &w : workers_) w.join();
+        {
+            std::lock_guard<std::mutex> lock(prep_mu_);
+            prep_stop_ = true;
+            prep_todo_.clear();
+        }
+        prep_cv_.notify_all();
+        {
+            std::lock_guard<std::mutex> lock(arena_mu_);
+        }
+        arena_cv_.notify_all();
+        for (auto &w : preparers_) w.join();
         if (stream_) hipStreamSynchronize(stream_);
         hipFree(d_desc_);
         hipFree(d_state_);
@@ -1505,18 +1531,137 @@ class BatchEngine {
         m0.validate();
         if (m0.rows() >= (1LL << 31) || m0.cols() >= (1LL << 31) || nnz >= (1LL << 31))
             throw std::runtime_error("Batched LPs must have fewer than 2^31 rows, columns and nonzeros");
-        nyk::Options o = opt_;
-        if (x0) o.initial_x.assign(x0, x0 + lp.cols);
-        if (y0) o.initial_y.assign(y0, y0 + lp.rows);
-        for (double v : o.initial_x)
+        if (x0) job->warm_x.assign(x0, x0 + lp.cols);
+        if (y0) job->warm_y.assign(y0, y0 + lp.rows);
+        for (double v : job->warm_x)
             if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
-        for (double v : o.initial_y)
+        for (double v : job->warm_y)
             if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
-        setup(*job, o);
+        start_preparers();
         jobs_.push_back(std::move(job));
-        Job &j = *jobs_.back();
-        if (!j.finished) queue_.push_back(j.ticket);
-        return j.ticket;
+        Job *j = jobs_.back().get();
+        {
+            std::lock_guard<std::mutex> lock(prep_mu_);
+            prep_todo_.push_back(j);
+        }
+        ++prep_outstanding_;
+        prep_cv_.notify_one();
+        return j->ticket;
+    }
+
+    // ---- host prepare off the launch path ----
+    //
+    // Presolve, scaling, the start-point checks, the device image (sliced
+    // copies, row blocks, transposes) and its upload into the arena run on
+    // prepare workers, so the scheduling thread only assigns slots and
+    // launches.  A worker that finds no arena room waits until a retired LP
+    // frees some.  Results do not depend on when an LP is admitted.
+    void start_preparers() {
+        if (!preparers_.empty()) return;
+        int device = 0;
+        HIP_OK(hipGetDevice(&device));
+        const int workers = std::max(1, nyk::host_threads());
+        for (int w = 0; w < workers; ++w) preparers_.emplace_back([this, device] { prepare_loop(device); });
+    }
+
+    void prepare_loop(int device) {
+        (void)hipSetDevice(device);
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
+        for (;;) {
+            Job *j = nullptr;
+            {
+                std::unique_lock<std::mutex> lock(prep_mu_);
+                prep_cv_.wait(lock, [&] { return prep_stop_ || !prep_todo_.empty(); });
+                if (prep_stop_) break;
+                j = prep_todo_.front();
+                prep_todo_.pop_front();
+            }
+            try {
+                if (!st) throw std::runtime_error("Could not create a prepare stream");
+                prepare_job(*j, st);
+            } catch (const std::exception &e) {
+                j->prepare_error = e.what();
+                if (j->prepare_error.empty()) j->prepare_error = "unknown error";
+            }
+            {
+                std::lock_guard<std::mutex> lock(prep_mu_);
+                prep_ready_.push_back(j);
+            }
+            ready_cv_.notify_all();
+        }
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    // setup() + the device image + its upload, on a prepare worker.
+    void prepare_job(Job &j, hipStream_t st) {
+        nyk::Options o = opt_;
+        o.initial_x = j.warm_x;
+        o.initial_y = j.warm_y;
+        j.terminal = setup(j, o);
+        if (j.terminal) return;
+        j.cls = lds_class(j);
+        j.attempt_b = attempt_bytes(j);
+        j.check_b = check_bytes(j);
+        std::vector<char> image;
+        const size_t bytes = build_image(j, image, j.dev, [&](size_t total) {
+            j.offset = reserve_arena(total);
+            j.bytes = total;
+            j.resident = true;
+            return arena_.base() + j.offset;
+        });
+        HIP_OK(hipMemcpyAsync(arena_.base() + j.offset, image.data(), bytes, hipMemcpyHostToDevice, st));
+        HIP_OK(hipStreamSynchronize(st));
+    }
+
+    // First-fit room for `bytes` in the arena; waits while resident LPs hold
+    // it.  An LP larger than the whole arena can never be admitted.
+    size_t reserve_arena(size_t bytes) {
+        std::unique_lock<std::mutex> lock(arena_mu_);
+        if (align_up(bytes) > arena_.size())
+            throw std::runtime_error("LP does not fit the batch arena (" + std::to_string(bytes) + " bytes)");
+        size_t offset = 0;
+        while (!arena_.allocate(bytes, offset)) {
+            if (prep_stop_) throw std::runtime_error("Batch is closing");
+            arena_cv_.wait(lock);
+        }
+        return offset;
+    }
+    void release_arena(size_t offset, size_t bytes) {
+        {
+            std::lock_guard<std::mutex> lock(arena_mu_);
+            arena_.release(offset, bytes);
+        }
+        arena_cv_.notify_all();
+    }
+
+    // Moves prepared jobs to the admission queue or, when the host already
+    // decided them, to the finished list (scheduling thread only).  With
+    // `wait`, blocks until at least one prepare has finished.
+    void collect_prepared(bool wait) {
+        std::vector<Job *> ready;
+        {
+            std::unique_lock<std::mutex> lock(prep_mu_);
+            if (wait && prep_outstanding_ > 0)
+                ready_cv_.wait(lock, [&] { return !prep_ready_.empty(); });
+            ready.swap(prep_ready_);
+        }
+        std::sort(ready.begin(), ready.end(), [](const Job *a, const Job *b) { return a->ticket < b->ticket; });
+        for (Job *j : ready) {
+            --prep_outstanding_;
+            if (!j->prepare_error.empty()) {
+                if (j->resident) release_arena(j->offset, j->bytes);
+                j->resident = false;
+                const bool room = j->prepare_error.rfind("LP does not fit", 0) == 0;
+                j->result.message = room ? j->prepare_error : "Prepare failed: " + j->prepare_error;
+                retire_host(*j, room ? "UNSUPPORTED" : "NUMERICAL_ERROR");
+            } else if (j->terminal) {
+                j->finished = true;
+                done_.push_back(j->ticket);
+            } else {
+                queue_.push_back(j->ticket);
+            }
+        }
     }
 
     int64_t step(int checks) {
@@ -1634,9 +1779,11 @@ class BatchEngine {
         }
         settle(active);
         drain_handoffs(active.empty());
-        int64_t running = int64_t(queue_.size()) + handoffs_outstanding_;
-        for (int s = 0; s < max_slots_; ++s) running += slot_job_[s] >= 0;
-        return running;
+        int64_t resident = 0;
+        for (int s = 0; s < max_slots_; ++s) resident += slot_job_[s] >= 0;
+        // Nothing to launch yet: wait for a prepare worker instead of spinning.
+        if (!resident && queue_.empty() && prep_outstanding_ > 0) collect_prepared(true);
+        return int64_t(queue_.size()) + handoffs_outstanding_ + prep_outstanding_ + resident;
     }
 
     // Handoff of long-running LPs to the grid-wide single-LP engine: after
@@ -1706,6 +1853,10 @@ class BatchEngine {
         out->operator_norm_estimate = r.operator_norm_estimate;
         out->polishing_iterations = 0;
         out->polishing_attempts = 0;
+        out->initial_step = r.initial_step;
+        out->initial_weight = r.initial_weight;
+        out->first_restart_weight = r.first_restart_weight;
+        out->first_restart_iteration = r.first_restart_iteration;
         if (out->x && r.x.size() == size_t(j.original.cols())) std::memcpy(out->x, r.x.data(), r.x.size() * 8);
         if (out->y && r.y.size() == size_t(j.original.rows())) std::memcpy(out->y, r.y.data(), r.y.size() * 8);
         return true;
@@ -1713,13 +1864,16 @@ class BatchEngine {
 
     void stats(niryukti_batch_stats *s) const {
         s->submitted = int64_t(jobs_.size());
-        s->queued = int64_t(queue_.size());
+        s->queued = int64_t(queue_.size()) + prep_outstanding_;
         s->active = 0;
         for (int v : slot_job_) s->active += v >= 0;
         s->finished = 0;
         for (const auto &j : jobs_) s->finished += !j || j->finished;
         s->arena_bytes = int64_t(arena_.size());
-        s->arena_used_bytes = int64_t(arena_.used());
+        {
+            std::lock_guard<std::mutex> lock(arena_mu_);
+            s->arena_used_bytes = int64_t(arena_.used());
+        }
         s->launches = launches_;
         s->host_certifications = certifications_;
         s->host_farkas_checks = farkas_checks_;
@@ -1737,16 +1891,22 @@ class BatchEngine {
         }
     }
 
+    // Waits for every submitted LP's prepare, then admits what the slots hold.
     int64_t admit_only() {
-        const size_t before = queue_.size();
+        while (prep_outstanding_ > 0) collect_prepared(true);
+        int64_t before = 0, after = 0;
+        for (int v : slot_job_) before += v >= 0;
         admit();
-        return int64_t(before - queue_.size());
+        for (int v : slot_job_) after += v >= 0;
+        return after - before;
     }
 
   private:
     // Host part of src/solver.cpp:215-333 for one LP: prepare, start point,
     // start-point verification, unboundedness and trivial optimality, steps.
-    void setup(Job &j, const nyk::Options &o) {
+    // Runs on a prepare worker; returns true when the result is decided
+    // without iterating (the scheduling thread then marks the job finished).
+    bool setup(Job &j, const nyk::Options &o) {
         auto &r = j.result;
         r.method_selected = "pdhg";
         r.backend = "hip-batch";
@@ -1758,15 +1918,13 @@ class BatchEngine {
         auto finish = [&](const char *status, std::string message) {
             r.status = status;
             r.message = std::move(message);
-            j.finished = true;
-            done_.push_back(j.ticket);
             r.seconds = std::chrono::duration<double>(Clock::now() - j.start).count();
+            return true;
         };
         if (!j.prep.failure.empty()) {
             r.infeasibility_ray = j.prep.failure_ray;
             if (!r.infeasibility_ray.empty()) r.certificate_margin = j.verifier->infeasibility_bound(r.infeasibility_ray);
-            finish(j.prep.failure.c_str(), j.prep.reason);
-            return;
+            return finish(j.prep.failure.c_str(), j.prep.reason);
         }
         const nyk::Model &m = j.prep.model;
         r.removed_columns = j.original.cols() - m.cols();
@@ -1787,19 +1945,12 @@ class BatchEngine {
             std::vector<uint8_t> used(m.cols(), 0);
             for (auto c : m.A.index) used[c] = 1;
             for (int64_t c = 0; c < m.cols(); ++c)
-                if (!used[c] && ((m.c[c] < 0 && m.ub[c] == nyk::kInf) || (m.c[c] > 0 && m.lb[c] == -nyk::kInf))) {
-                    finish("UNBOUNDED", "Feasible point and isolated improving recession direction: " + m.column_name(c));
-                    return;
-                }
+                if (!used[c] && ((m.c[c] < 0 && m.ub[c] == nyk::kInf) || (m.c[c] > 0 && m.lb[c] == -nyk::kInf)))
+                    return finish("UNBOUNDED",
+                                  "Feasible point and isolated improving recession direction: " + m.column_name(c));
         }
-        if (r.accuracy.finite && r.accuracy.kkt <= o.tol) {
-            finish("OPTIMAL", "");
-            return;
-        }
-        if (o.time_limit == 0) {
-            finish("TIME_LIMIT", "");
-            return;
-        }
+        if (r.accuracy.finite && r.accuracy.kkt <= o.tol) return finish("OPTIMAL", "");
+        if (o.time_limit == 0) return finish("TIME_LIMIT", "");
         const double norm = nyk::step_norm(m, o, &j.result.operator_norm_estimate);
         j.step = norm > 0 ? .9 / norm : 1;
         j.weight = 1;
@@ -1812,6 +1963,9 @@ class BatchEngine {
             }
             if (c2 > 1e-24 && b2 > 1e-24) j.weight = std::clamp(std::sqrt(double(c2 / b2)), 1e-4, 1e4);
         }
+        r.initial_step = j.step;
+        r.initial_weight = j.weight;
+        return false;
     }
 
     // Algorithmic HBM bytes of one PDHG attempt on the scaled LP: both CSR
@@ -1822,7 +1976,7 @@ class BatchEngine {
     // the LDS classes, a 32-bit index in the global class's CSR)
     double attempt_bytes(const Job &j) const {
         const double n = double(j.prep.model.cols()), m = double(j.prep.model.rows()), nnz = double(j.prep.model.A.nnz());
-        const double entry = lds_class(j) < kClasses - 1 ? 10 : 12;
+        const double entry = j.cls < kClasses - 1 ? 10 : 12;
         return 2 * nnz * entry + (n + m + 2) * 4 + m * 8 + n * 8 + 64 * (n + m);
     }
 
@@ -1880,38 +2034,26 @@ class BatchEngine {
         return kClasses - 1;
     }
 
-    // Admits queued LPs while a slot and arena room are free (FIFO).
+    // Admits prepared LPs (already resident in the arena) into free slots (FIFO).
     void admit() {
+        collect_prepared(false);
         while (!queue_.empty() && !free_slots_.empty()) {
             Job &j = *jobs_[queue_.front()];
+            queue_.pop_front();
             if (std::chrono::duration<double>(Clock::now() - j.start).count() >= opt_.time_limit) {
-                queue_.pop_front();
+                release_arena(j.offset, j.bytes);
+                j.resident = false;
                 retire_host(j, "TIME_LIMIT");
                 continue;
             }
-            std::vector<char> image;
-            LpDev d{};
-            size_t bytes = build_image(j, image, d, nullptr);
-            size_t offset = 0;
-            if (!arena_.allocate(bytes, offset)) {
-                if (arena_.used() == 0)
-                    throw std::runtime_error("LP does not fit the batch arena (" + std::to_string(bytes) + " bytes)");
-                break;
-            }
-            build_image(j, image, d, arena_.base() + offset);
-            HIP_OK(hipMemcpyAsync(arena_.base() + offset, image.data(), bytes, hipMemcpyHostToDevice, stream_));
-            HIP_OK(hipStreamSynchronize(stream_));  // `image` is a temporary
             const int s = free_slots_.back();
             free_slots_.pop_back();
-            queue_.pop_front();
             j.slot = s;
-            j.offset = offset;
-            j.bytes = bytes;
             slot_job_[s] = int(j.ticket);
-            slot_bytes_[s] = attempt_bytes(j);
-            slot_check_bytes_[s] = check_bytes(j);
-            slot_class_[s] = lds_class(j);
-            desc_[s] = d;
+            slot_bytes_[s] = j.attempt_b;
+            slot_check_bytes_[s] = j.check_b;
+            slot_class_[s] = j.cls;
+            desc_[s] = j.dev;
             LpState st{};
             st.step = j.step;
             st.weight = j.weight;
@@ -1925,8 +2067,11 @@ class BatchEngine {
         }
     }
 
-    // Lays out (and, with a base pointer, fills) the device image of one LP.
-    size_t build_image(const Job &j, std::vector<char> &image, LpDev &d, char *base) {
+    // Lays out and fills the device image of one LP; `place(total bytes)`
+    // returns the device address the image will be copied to, which the
+    // descriptor's pointers refer to.
+    template <class Place>
+    size_t build_image(const Job &j, std::vector<char> &image, LpDev &d, Place &&place) {
         const nyk::Model &m = j.prep.model;
         const nyk::Model &m0 = j.original;
         const nyk::Csr &t0 = j.verifier->transpose();
@@ -1968,7 +2113,7 @@ class BatchEngine {
         // the sliced copies serve only the LDS classes (the global class walks the CSR)
         Sliced sa, st;
         nyk::Csr t = m.A.transposed();
-        if (lds_class(j) < kClasses - 1) {
+        if (j.cls < kClasses - 1) {
             sa = slice_rows(m.A.ptr, m.A.index, m.A.value);
             st = slice_rows(t.ptr, t.index, t.value);
         }
@@ -1976,7 +2121,7 @@ class BatchEngine {
                o_asv = L.add<double>(sa.value.size());
         size_t o_tsb = L.add<int>(st.base.size()), o_tsi = L.add<unsigned short>(st.index.size()),
                o_tsv = L.add<double>(st.value.size());
-        if (!base) return L.total;
+        char *const base = place(L.total);
         image.assign(L.total, 0);
         auto put_i = [&](size_t off, const std::vector<int64_t> &v) {
             int *p = reinterpret_cast<int *>(image.data() + off);
@@ -2386,7 +2531,8 @@ class BatchEngine {
 
     // Frees the LP's slot and arena space and queues it for a grid worker.
     void hand_off(Job &j) {
-        arena_.release(j.offset, j.bytes);
+        release_arena(j.offset, j.bytes);
+        j.resident = false;
         slot_job_[j.slot] = -1;
         free_slots_.push_back(j.slot);
         state_[j.slot] = LpState{};
@@ -2493,10 +2639,13 @@ class BatchEngine {
         r.rejected_steps = st.rejected;
         r.weight_updates = st.weight_updates;
         r.host_candidate_checks = st.checks;
+        r.first_restart_iteration = st.first_restart;
+        r.first_restart_weight = st.first_weight;
     }
 
     void retire(Job &j) {
-        arena_.release(j.offset, j.bytes);
+        release_arena(j.offset, j.bytes);
+        j.resident = false;
         slot_job_[j.slot] = -1;
         free_slots_.push_back(j.slot);
         state_[j.slot] = LpState{};
@@ -2549,6 +2698,16 @@ class BatchEngine {
     int64_t attempts_ = 0;
     std::vector<std::unique_ptr<Job>> jobs_;
     std::deque<int64_t> queue_, done_;
+    // prepare workers: jobs to prepare, prepared jobs, and how many of the
+    // submitted jobs the scheduling thread has not yet collected
+    std::vector<std::thread> preparers_;
+    std::mutex prep_mu_;
+    mutable std::mutex arena_mu_;
+    std::condition_variable prep_cv_, ready_cv_, arena_cv_;
+    std::deque<Job *> prep_todo_;
+    std::vector<Job *> prep_ready_;
+    std::atomic<bool> prep_stop_{false};
+    int64_t prep_outstanding_ = 0;
     // handoff to the grid-wide engine
     int64_t handoff_ = 0;
     int handoff_workers_ = 4;

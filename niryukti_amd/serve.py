@@ -166,6 +166,7 @@ class Server:
 
     # -------------------------------------------------------------- loop --
     def _loop(self):
+        incoming: list[Request] = []
         try:
             while True:
                 with self._cv:
@@ -174,8 +175,15 @@ class Server:
                     if self._closing and not self._incoming and not self._tickets:
                         return
                     incoming, self._incoming = self._incoming, []
-                for req in incoming:
-                    self._tickets[self._batch.submit(req.lp, req.x0, req.y0)] = req
+                while incoming:
+                    req = incoming.pop(0)
+                    try:
+                        ticket = self._batch.submit(req.lp, req.x0, req.y0)
+                    except (ValueError, RuntimeError) as error:
+                        # this request alone is rejected (e.g. a nonfinite warm start)
+                        req._complete(error)
+                        continue
+                    self._tickets[ticket] = req
                 if self._tickets:
                     self._batch.step(self.checks_per_launch)
                     for t in self._batch.finished():
@@ -185,7 +193,7 @@ class Server:
         except BaseException as error:  # surface to every waiter
             self._error = error
             with self._cv:
-                pending = list(self._tickets.values()) + self._incoming
+                pending = list(self._tickets.values()) + incoming + self._incoming
                 self._tickets.clear()
                 self._incoming = []
             for req in pending:

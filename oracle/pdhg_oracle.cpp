@@ -768,6 +768,10 @@ struct Result {
     int64_t iterations = 0, restarts = 0, rejected_steps = 0, removed_rows = 0, removed_columns = 0,
             weight_updates = 0, host_candidate_checks = 0;
     double certificate_margin = 0;
+    // decision trace (not in the reference's Result): first step and weight,
+    // first restart iteration and the weight right after it
+    double initial_step = 0, initial_weight = 0, first_restart_weight = 0;
+    int64_t first_restart_iteration = 0;
 };
 
 static bool anchored_method(const Options &o) { return o.method == "halpern" || o.method == "rhpdhg" || o.method == "r2hpdhg"; }
@@ -861,6 +865,8 @@ static Result solve_continuous(const Model &original, const Options &o) {
         }
         if (c2 > 1e-24 && b2 > 1e-24) weight = std::clamp(std::sqrt(double(c2 / b2)), 1e-4, 1e4);
     }
+    r.initial_step = step;
+    r.initial_weight = weight;
     bool residual_restarts = o.method == "rhpdhg" || o.method == "r2hpdhg";
     CpuBackend backend(m, x, y, o.adaptive, anchored_method(o), o.method == "r2hpdhg" ? 1 : 0,
                        residual_restarts && o.restart);
@@ -980,7 +986,10 @@ static Result solve_continuous(const Model &original, const Options &o) {
             epochy = cy;
             restart_kkt = ca.kkt;
             since_restart = 0;
-            r.restarts++;
+            if (r.restarts++ == 0) {
+                r.first_restart_iteration = r.iterations;
+                r.first_restart_weight = weight;
+            }
         }
     }
     auto check = Clock::now();
@@ -1107,6 +1116,10 @@ int oracle_solve(const oracle_lp *lp, const oracle_options *opts, oracle_result 
         out->iteration_seconds = r.iteration_seconds;
         out->preprocess_seconds = r.preprocess_seconds;
         out->verification_seconds = r.verification_seconds;
+        out->initial_step = r.initial_step;
+        out->initial_weight = r.initial_weight;
+        out->first_restart_weight = r.first_restart_weight;
+        out->first_restart_iteration = r.first_restart_iteration;
         if (out->x && r.x.size() == size_t(lp->cols)) std::copy(r.x.begin(), r.x.end(), out->x);
         if (out->y && r.y.size() == size_t(lp->rows)) std::copy(r.y.begin(), r.y.end(), out->y);
         return 0;
@@ -1230,4 +1243,10 @@ int oracle_backend_restart_requested(void *b) {
     return static_cast<oracle::BackendHandle *>(b)->backend->restart_requested() ? 1 : 0;
 }
 void oracle_backend_destroy(void *b) { delete static_cast<oracle::BackendHandle *>(b); }
+
+void *oracle_pid_create(void) { return new oracle::PrimalWeightController(); }
+double oracle_pid_update(void *pid, double weight, double dx, double dy) {
+    return static_cast<oracle::PrimalWeightController *>(pid)->update(weight, (long double)dx, (long double)dy);
+}
+void oracle_pid_destroy(void *pid) { delete static_cast<oracle::PrimalWeightController *>(pid); }
 }

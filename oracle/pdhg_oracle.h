@@ -49,6 +49,10 @@ typedef struct {
     int64_t iterations, restarts, rejected_steps, weight_updates, host_candidate_checks,
         removed_rows, removed_columns;
     double seconds, iteration_seconds, preprocess_seconds, verification_seconds;
+    /* decision trace: first step and primal weight, iteration of the first
+     * restart and the primal weight right after it (0 when none) */
+    double initial_step, initial_weight, first_restart_weight;
+    int64_t first_restart_iteration;
     double *x; /* NULL or cols: receives the returned primal point */
     double *y; /* NULL or rows: receives the returned dual point */
 } oracle_result;
@@ -79,6 +83,13 @@ void oracle_backend_reset(void *b, const double *x, const double *y);
 int64_t oracle_backend_rejected(void *b);
 int oracle_backend_restart_requested(void *b);
 void oracle_backend_destroy(void *b);
+
+/* The PID primal-weight controller (src/research.cpp:52-70) on its own:
+ * update() feeds one epoch displacement (dx = ||x - x_epoch||^2,
+ * dy = ||y - y_epoch||^2) and returns the next weight. */
+void *oracle_pid_create(void);
+double oracle_pid_update(void *pid, double weight, double dx, double dy);
+void oracle_pid_destroy(void *pid);
 
 #ifdef __cplusplus
 }

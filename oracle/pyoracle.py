@@ -42,7 +42,8 @@ class _Result(C.Structure):
         [(k, C.c_int64) for k in ("iterations", "restarts", "rejected_steps", "weight_updates",
                                   "host_candidate_checks", "removed_rows", "removed_columns")] + \
         [(k, C.c_double) for k in ("seconds", "iteration_seconds", "preprocess_seconds", "verification_seconds")] + \
-        [("x", _f64p), ("y", _f64p)]
+        [(k, C.c_double) for k in ("initial_step", "initial_weight", "first_restart_weight")] + \
+        [("first_restart_iteration", C.c_int64), ("x", _f64p), ("y", _f64p)]
 
 
 _lib = None
@@ -74,6 +75,10 @@ def lib() -> C.CDLL:
         l.oracle_backend_rejected.restype = C.c_int64
         l.oracle_backend_restart_requested.argtypes = [C.c_void_p]
         l.oracle_backend_destroy.argtypes = [C.c_void_p]
+        l.oracle_pid_create.restype = C.c_void_p
+        l.oracle_pid_update.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        l.oracle_pid_update.restype = C.c_double
+        l.oracle_pid_destroy.argtypes = [C.c_void_p]
         _lib = l
     return _lib
 
@@ -251,4 +256,20 @@ class Backend:
     def __del__(self):
         if getattr(self, "_h", None):
             self._l.oracle_backend_destroy(self._h)
+            self._h = None
+
+
+class Pid:
+    """The oracle's PID primal-weight controller (src/research.cpp:52-70)."""
+
+    def __init__(self):
+        self._l = lib()
+        self._h = self._l.oracle_pid_create()
+
+    def update(self, weight, dx, dy) -> float:
+        return self._l.oracle_pid_update(self._h, float(weight), float(dx), float(dy))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._l.oracle_pid_destroy(self._h)
             self._h = None

@@ -10,7 +10,16 @@
 //                      [--no-adaptive] [--no-restart] [--method M] [--xy]
 //   ref_driver prepare (--config ... | --model FILE) [--no-presolve] [--passes P]
 //   ref_driver steps   (--config ... | --model FILE) --count K --tau T --sigma S
-//                      [--adaptive] [--halpern] [--reflection R] [--chunks N]
+//                      [--adaptive] [--halpern] [--reflection R] [--residual-restarts]
+//                      [--reset-on-restart] [--chunks N]
+//                      (--reset-on-restart: after a chunk that requested a restart,
+//                      reset to the T(z) candidate, as the driver does)
+//   ref_driver solve-range --config 2 --seed S --first I --stride K --budget SEC [--threads T]
+//                      solves LPs I, I+K, I+2K, ... of the batched workload one after
+//                      another until their summed solve() time reaches SEC; one JSON
+//                      line per LP (instance generation is outside the timed solve)
+//   ref_driver pid     --weight W --updates "dx,dy;dx,dy;..."   (PrimalWeightController,
+//                      src/research.cpp:52-70: each update's weight feeds the next)
 //
 // This program is part of the checker only; nothing in the product links it.
 #include "instances.hpp"
@@ -124,6 +133,7 @@ Options options() {
     o.scaling_passes = int(getl("--passes", 5));
     o.check_every = int(getl("--check-every", 100));
     o.scaling = get("--scaling", "ruiz");
+    o.primal_weight = get("--primal-weight", "displacement");
     return o;
 }
 
@@ -208,7 +218,8 @@ int cmd_steps() {
     const auto &m = l.model;
     std::vector<double> x(m.c.size()), y(m.rl.size());
     for (size_t j = 0; j < x.size(); ++j) x[j] = std::clamp(0.0, m.lb[j], m.ub[j]);
-    auto b = cpu_backend(m, x, y, flag("--adaptive"), flag("--halpern"), getd("--reflection", 0), false);
+    auto b = cpu_backend(m, x, y, flag("--adaptive"), flag("--halpern"), getd("--reflection", 0),
+                         flag("--residual-restarts"));
     int count = int(getl("--count", 10)), chunks = int(getl("--chunks", 3));
     double tau = getd("--tau", 0.1), sigma = getd("--sigma", 0.1);
     std::printf("{\"chunks\":[");
@@ -216,13 +227,62 @@ int cmd_steps() {
         int acc = b->advance(count, tau, sigma);
         std::vector<double> cx, cy, ax, ay;
         b->candidates(cx, cy, ax, ay);
-        std::printf("%s{\"accepted\":%d,\"rejected\":%lld,", k ? "," : "", acc,
-                    (long long)b->rejected_steps());
+        std::printf("%s{\"accepted\":%d,\"rejected\":%lld,\"restart_requested\":%d,", k ? "," : "", acc,
+                    (long long)b->rejected_steps(), b->restart_requested() ? 1 : 0);
         vec("x", cx);
         vec("y", cy);
         vec("xa", ax);
         vec("ya", ay, false);
         std::printf("}");
+        if (flag("--reset-on-restart") && b->restart_requested()) b->reset(ax, ay);
+    }
+    std::printf("]}\n");
+    return 0;
+}
+int cmd_solve_range() {
+    const uint64_t seed = uint64_t(getl("--seed", 42));
+    const int64_t first = getl("--first", 0), stride = std::max<int64_t>(1, getl("--stride", 1));
+    const int64_t total = getl("--total", 4096);
+    const double budget = getd("--budget", 10);
+    auto o = options();
+    const auto shapes = nyk_gen::batch_shapes(int(total), seed);
+    double spent = 0;
+    for (int64_t i = first; i < total && spent < budget; i += stride) {
+        Model m = from_instance(nyk_gen::batch_instance(shapes[size_t(i)]));
+        auto t0 = std::chrono::steady_clock::now();
+        auto r = solve(m, o);
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        spent += dt;
+        std::printf("{\"index\":%lld,\"status\":\"%s\",\"iterations\":%lld,\"seconds\":%.9g,\"kkt\":",
+                    (long long)i, r.status.c_str(), (long long)r.iterations, dt);
+        num(r.accuracy.kkt);
+        std::printf("}\n");
+        std::fflush(stdout);
+    }
+    return 0;
+}
+
+// The reference's PID primal-weight controller on a sequence of epoch
+// displacements; prints the weight after each update.
+int cmd_pid() {
+    PrimalWeightController pid;
+    double w = getd("--weight", 1);
+    std::string seq = get("--updates", "");
+    std::printf("{\"weights\":[");
+    size_t at = 0;
+    bool first = true;
+    while (at < seq.size()) {
+        size_t end = seq.find(';', at);
+        if (end == std::string::npos) end = seq.size();
+        std::string item = seq.substr(at, end - at);
+        size_t comma = item.find(',');
+        if (comma == std::string::npos) throw std::runtime_error("pid: updates are dx,dy pairs");
+        long double dx = std::stold(item.substr(0, comma)), dy = std::stold(item.substr(comma + 1));
+        w = pid.update(w, (long double)(double)dx, (long double)(double)dy);
+        std::printf(first ? "" : ",");
+        num(w);
+        first = false;
+        at = end + 1;
     }
     std::printf("]}\n");
     return 0;
@@ -241,7 +301,8 @@ int main(int argc, char **argv) {
         if (k == "--scale" || k == "--seed" || k == "--config" || k == "--tau" || k == "--sigma" ||
             k == "--reflection" || k == "--alpha" || k == "--zipf" || k == "--maxdeg" ||
             k == "--upper" || k == "--zero" || k == "--m" || k == "--n" || k == "--lo" ||
-            k == "--hi" || k == "--eq" || k == "--mindeg" || k == "--scaling" || k == "--hubs")
+            k == "--hi" || k == "--eq" || k == "--mindeg" || k == "--scaling" || k == "--hubs" ||
+            k == "--weight" || k == "--updates" || k == "--primal-weight" || k == "--method")
             boolean = i + 1 >= argc;
         args[k] = boolean ? "1" : argv[++i];
     }
@@ -250,6 +311,8 @@ int main(int argc, char **argv) {
         if (c == "solve") return cmd_solve();
         if (c == "prepare") return cmd_prepare();
         if (c == "steps") return cmd_steps();
+        if (c == "pid") return cmd_pid();
+        if (c == "solve-range") return cmd_solve_range();
         std::fprintf(stderr, "unknown command\n");
         return 1;
     } catch (const std::exception &e) {

@@ -5,7 +5,7 @@ sources by oracle/ref.mk) and stores its results as JSON data under
 tests/golden/.  Run from the repository root after
 `make -f oracle/ref.mk REF=/root/reference`:
 
-    python tests/golden/make_golden.py [solves] [prepare] [steps]   (default: all)
+    python tests/golden/make_golden.py [solves] [prepare] [steps] [controls]   (default: all)
 """
 import hashlib
 import json
@@ -74,13 +74,15 @@ def prepares():
 
 
 def main():
-    parts = set(sys.argv[1:]) or {"solves", "prepare", "steps"}
+    parts = set(sys.argv[1:]) or {"solves", "prepare", "steps", "controls"}
     if "solves" in parts:
         solves()
     if "prepare" in parts:
         prepares()
     if "steps" in parts:
         steps()
+    if "controls" in parts:
+        controls()
 
 
 def solves():
@@ -105,6 +107,53 @@ def steps():
                  sigma=0.05)
         steps.append(r)
     (OUT / "reference_steps.json").write_text(json.dumps(steps) + "\n")
+
+
+# The optional controls: anchored (Halpern, reflected, residual-restart)
+# methods and the PID primal weight, as whole solves, raw backend chunks and
+# controller sequences.
+CONTROL_SOLVES = [
+    # (config, seed, scale, index, method, primal_weight, adaptive)
+    (0, 3, 20, 0, "halpern", "displacement", False),
+    (0, 3, 20, 0, "rhpdhg", "displacement", False),
+    (0, 3, 20, 0, "r2hpdhg", "displacement", False),
+    (0, 3, 20, 0, "halpern", "pid", False),
+    (0, 3, 20, 0, "rhpdhg", "pid", False),
+    (0, 3, 20, 0, "r2hpdhg", "pid", False),
+    (0, 3, 20, 0, "pdhg", "pid", True),
+    (2, 3, 1, 1, "pdhg", "pid", True),
+    (2, 3, 1, 0, "r2hpdhg", "displacement", False),
+]
+PID_SEQUENCES = [
+    (1.0, [(100, 1), (0, 1)] + [(1e20, 1)] * 40),
+    (0.37, [(2.5, 0.125), (1e-3, 4.0), (3.0, 3.0), (1e-30, 1.0), (7.0, 1e-9), (0.5, 0.25)] * 6),
+    (250.0, [(1e-6, 1e6)] * 30 + [(1e6, 1e-6)] * 30),
+]
+
+
+def controls():
+    out = {"solves": [], "steps": [], "pid": []}
+    for config, seed, scale, index, method, weight, adaptive in CONTROL_SOLVES:
+        args = ["solve", *case_args(config, seed, scale, index), "--method", method, "--primal-weight", weight,
+                "--iterations", 30000]
+        if not adaptive:
+            args.append("--no-adaptive")
+        r = run(*args)
+        r.update(config=config, seed=seed, scale=scale, index=index, method=method, primal_weight=weight,
+                 adaptive=adaptive)
+        out["solves"].append(r)
+        print("control", config, method, weight, r["status"], r["iterations"], r["restarts"], r["weight_updates"])
+    for reflection, residual in ((1.0, False), (0.0, True), (1.0, True)):
+        r = run("steps", *case_args(0, 3, 20, 0), "--count", 25, "--chunks", 8, "--tau", 0.05, "--sigma", 0.05,
+                "--halpern", "--reflection", reflection,
+                *(["--residual-restarts", "--reset-on-restart"] if residual else []))
+        r.update(config=0, seed=3, scale=20, index=0, reflection=reflection, residual_restarts=residual,
+                 count=25, tau=0.05, sigma=0.05)
+        out["steps"].append(r)
+    for w0, seq in PID_SEQUENCES:
+        r = run("pid", "--weight", w0, "--updates", ";".join(f"{dx!r},{dy!r}" for dx, dy in seq))
+        out["pid"].append({"weight": w0, "updates": seq, "weights": r["weights"]})
+    (OUT / "reference_controls.json").write_text(json.dumps(out) + "\n")
 
 
 if __name__ == "__main__":

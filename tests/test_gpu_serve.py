@@ -63,7 +63,7 @@ def test_stream_matches_alone(engine, oracle):
         assert r["status"] == "OPTIMAL" and r["kkt_error"] <= 1e-6
         ref = oracle.solve(lp2)
         assert ref["status"] == "OPTIMAL"
-        assert rel(r["objective"], ref["objective"]) <= 1e-5
+        assert rel(r["objective"], ref["objective"]) <= 1e-6
         assert oracle.verify(lp2, r["x"], r["y"])["kkt"] <= 1e-6
         if req.route == "batch":
             alone = alone_batch(engine, lp2, x0, y0)
@@ -78,7 +78,7 @@ def test_solve_request_shape(engine):
     with engine.Server(hbm_budget=1 << 30) as server:
         code, body = server.solve_request({"model": model, "options": {"tol": 1e-6}})
         assert code == 200 and body["status"] == "OPTIMAL"
-        assert abs(body["objective"] - 6) <= 1e-5
+        assert abs(body["objective"] - 6) <= 1e-6 * 7
         assert body["serving"]["route"] == "batch"
         code, body = server.solve_request({"model": model, "options": {"iterations": 50000}})
         assert code == 200 and body["serving"]["route"] == "grid" and body["status"] == "OPTIMAL"

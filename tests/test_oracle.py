@@ -180,3 +180,104 @@ def test_oracle_matches_highs_on_random_lps(engine, oracle):
         h = linprog(c, A_ub=A[~eq], b_ub=ru[~eq], A_eq=A[eq], b_eq=b[eq], bounds=[(0, 5)] * n, method="highs")
         assert h.status == 0 and r["status"] == "OPTIMAL"
         assert abs(r["objective"] - h.fun) <= 1e-5 * (1 + abs(h.fun))
+
+
+# ------------------------------------------------------- optional controls
+
+def test_pid_controller_closed_forms(oracle):
+    """tests/test_research.cpp:1016-1022, plus two hand-derived steps.
+
+    From a fresh controller, update(1, e^2, 1) has error = 0.5 * ln(e^2) = 1,
+    integral 1, no derivative, correction 0.3 + 0.01 = 0.31: the weight is
+    e^-0.31.  A second identical update has error 1 - 0.31 = 0.69, integral
+    1.69, derivative -0.31, correction 0.207 + 0.0169 - 0.031 = 0.1929."""
+    pid = oracle.Pid()
+    w = pid.update(1, 100, 1)
+    assert 0 < w < 1, "displacement feedback direction"
+    assert pid.update(w, 0, 1) == w, "no update without a primal displacement"
+    for _ in range(1000):
+        w = pid.update(w, 1e20, 1)
+    assert np.isfinite(w) and w >= 1e-4 * (1 - 1e-12), "saturation and anti-windup"
+    fresh = oracle.Pid()
+    w1 = fresh.update(1.0, np.exp(2.0), 1.0)
+    assert abs(np.log(w1) + 0.31) <= 1e-14
+    w2 = fresh.update(w1, np.exp(2.0), 1.0)
+    assert abs(np.log(w2) - (np.log(w1) - 0.1929)) <= 1e-13
+    # the per-update correction is clamped to +-ln 4
+    big = oracle.Pid()
+    assert abs(big.update(1.0, 1e12, 1e-12) - 0.25) <= 1e-15
+    # anti-windup: while clamped at 1e-4 the integral stops growing, so a
+    # reversal moves the weight back up by the full step at once
+    wind = oracle.Pid()
+    w = 1.0
+    for _ in range(50):
+        w = wind.update(w, 1e20, 1)
+    assert abs(w - 1e-4) <= 1e-16
+    assert wind.update(w, 1e-20, 1) > w * 3.99
+
+
+def test_pid_controller_matches_reference(oracle):
+    for case in load("reference_controls.json")["pid"]:
+        pid, w = oracle.Pid(), case["weight"]
+        for (dx, dy), want in zip(case["updates"], case["weights"]):
+            w = pid.update(w, dx, dy)
+            assert w == want
+
+
+def test_reflected_halpern_closed_form(engine, oracle):
+    # tests/test_research.cpp:1023-1041: reflected Halpern on min_x max_y (x-1) y
+    lp = dense_lp(engine, [[1]], [0], [-np.inf], [np.inf], [1], [1])
+    b = oracle.Backend(lp, [0.0], [0.0], adaptive=False, halpern=True, reflection=1.0)
+    x = y = 0.0
+    for k in range(20):
+        tx = x - .5 * y
+        ty = y + .5 * (2 * tx - x - 1)
+        w = (k + 1) / (k + 2)
+        x, y = w * (2 * tx - x), w * (2 * ty - y)
+        b.advance(1, .5, .5)
+        cx, cy, rx, ry = b.candidates()
+        for got, want in ((cx[0], x), (cy[0], y), (rx[0], tx), (ry[0], ty)):
+            assert abs(got - want) <= 1e-11 * (1 + abs(want))
+
+
+def test_fixed_point_restart_interrupts_and_clears(engine, oracle):
+    # tests/test_research.cpp:1042-1046
+    lp = dense_lp(engine, [[1]], [0], [-np.inf], [np.inf], [1], [1])
+    b = oracle.Backend(lp, [0.0], [0.0], adaptive=False, halpern=True, reflection=1.0, residual_restarts=True)
+    assert b.advance(100, .5, .5) < 100 and b.restart_requested(), "fixed-point restart interrupts a chunk"
+    b.reset([0.0], [0.0])
+    assert not b.restart_requested(), "restart state cleared"
+    # without the residual test the same chunk runs to the end
+    plain = oracle.Backend(lp, [0.0], [0.0], adaptive=False, halpern=True, reflection=1.0)
+    assert plain.advance(100, .5, .5) == 100 and not plain.restart_requested()
+
+
+def test_anchored_steps_match_reference(engine, oracle):
+    """Raw backend chunks of the reflected and residual-restart Halpern
+    iterations, bit for bit against the reference build (a reset to the T(z)
+    candidate after each chunk that requested a restart)."""
+    for case in load("reference_controls.json")["steps"]:
+        lp = instance(engine, case).lp
+        b = oracle.Backend(lp, np.clip(np.zeros(lp.cols), lp.lb, lp.ub), np.zeros(lp.rows), adaptive=False,
+                           halpern=True, reflection=case["reflection"], residual_restarts=case["residual_restarts"])
+        for chunk in case["chunks"]:
+            assert b.advance(case["count"], case["tau"], case["sigma"]) == chunk["accepted"]
+            assert b.restart_requested() == bool(chunk["restart_requested"])
+            x, y, xa, ya = b.candidates()
+            for got, key in ((x, "x"), (y, "y"), (xa, "xa"), (ya, "ya")):
+                assert np.array_equal(got, chunk[key]), key
+            if case["residual_restarts"] and b.restart_requested():
+                b.reset(xa, ya)
+
+
+@pytest.mark.parametrize("case", load("reference_controls.json")["solves"],
+                         ids=lambda c: f"c{c['config']}-{c['method']}-{c['primal_weight']}")
+def test_control_solves_match_reference(engine, oracle, case):
+    inst = instance(engine, case)
+    r = oracle.solve(inst.lp, threads=1, time_limit=1e9, method=case["method"], primal_weight=case["primal_weight"],
+                     adaptive=case["adaptive"], iteration_limit=30000)
+    assert r["status"] == case["status"] == "OPTIMAL"
+    for key in ("iterations", "restarts", "rejected_steps", "weight_updates"):
+        assert r[key] == case[key], key
+    assert r["objective"] == case["objective"] and r["kkt"] == case["kkt"]
+    assert abs(case["objective"] - inst.planted_objective) <= 1e-6 * (1 + abs(inst.planted_objective))
