@@ -46,6 +46,10 @@ typedef struct {
     int polishing;           /* 1: feasibility polishing (src/solver.cpp:634-690), default 0 */
     const double *initial_x; /* NULL or cols */
     const double *initial_y; /* NULL or rows */
+    /* Continuous checkpoints (src/solver.cpp:384-509, the reference's
+     * --checkpoint-out / --resume / --checkpoint-nodes): NULL or a path. */
+    const char *checkpoint_path, *resume_path;
+    int64_t checkpoint_nodes; /* iterations between checkpoint writes, default 100 */
 } niryukti_options;
 
 typedef struct {
@@ -86,6 +90,17 @@ int niryukti_prepare_lp(const niryukti_lp *lp, const niryukti_options *o, int64_
                         int64_t *ptr, int64_t *index, double *value, double *c, double *lb, double *ub, double *rl,
                         double *ru, double *column_scale, double *row_scale, int64_t *col_map, int64_t *row_map,
                         double *objective_scale, double *offset);
+
+/* The original-space accuracy of a point (x: cols, y: rows) as the certifying
+ * check computes it: vantage::verify (src/verify.cpp:241-244, Verifier::evaluate
+ * at :103-240).  With safe_bound != 0 the gap includes the outward-rounded
+ * Lagrangian bound (src/verify.cpp:12-88), as for a certificate.  0 or -1. */
+typedef struct {
+    double objective, primal, dual, gap, kkt, complementarity, lower_bound, primal_absolute, dual_absolute;
+    int finite;
+} niryukti_accuracy;
+int niryukti_verify_lp(const niryukti_lp *lp, const double *x, const double *y, int safe_bound,
+                       niryukti_accuracy *out);
 
 /* Power-iteration estimate of ||A||_2 of the LP's matrix (src/research.cpp:6-50:
  * same seed, start vector and normalisation order as the reference); 0 when

@@ -84,6 +84,18 @@ int npdhg_reset(npdhg_backend *b, const double *x, const double *y);
  * (src/cuda.cu:892-930). */
 int npdhg_reset_candidate(npdhg_backend *b, int average);
 
+/* Resumable backend state for checkpoints: the current iterate, averages,
+ * activity, Halpern anchors, step factor, average mass, residual history and
+ * counters, plus (once npdhg_attach_original has run) the epoch point and the
+ * best restored candidate; npdhg_snapshot_size gives its length in doubles.
+ * npdhg_restore puts a snapshot of the same backend configuration back, after
+ * which advance continues exactly as from the snapshot's moment.  Replace
+ * IterationBackend::snapshot / restore (src/internal.hpp:106-107,
+ * src/cpu.cpp:49-89, src/cuda.cu:739-770). */
+int npdhg_snapshot_size(npdhg_backend *b, int64_t *doubles);
+int npdhg_snapshot(npdhg_backend *b, double *state);
+int npdhg_restore(npdhg_backend *b, const double *state);
+
 /* ---- Device-side candidate checks ----
  *
  * The reference checks both candidates on the host every chunk: restore to the

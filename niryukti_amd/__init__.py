@@ -43,7 +43,8 @@ class _Options(C.Structure):
                 ("presolve", C.c_int), ("restart", C.c_int), ("adaptive", C.c_int), ("graphs", C.c_int),
                 ("verbose", C.c_int), ("device_checks", C.c_int), ("device_prepare", C.c_int),
                 ("power_iterations", C.c_int), ("polishing", C.c_int), ("initial_x", _f64p),
-                ("initial_y", _f64p)]
+                ("initial_y", _f64p), ("checkpoint_path", C.c_char_p), ("resume_path", C.c_char_p),
+                ("checkpoint_nodes", C.c_int64)]
 
 
 class _Result(C.Structure):
@@ -61,6 +62,11 @@ class _Result(C.Structure):
                 ("polishing_iterations", C.c_int64), ("polishing_attempts", C.c_int64),
                 ("initial_step", C.c_double), ("initial_weight", C.c_double), ("first_restart_weight", C.c_double),
                 ("first_restart_iteration", C.c_int64), ("x", _f64p), ("y", _f64p)]
+
+
+class _Accuracy(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("objective", "primal", "dual", "gap", "kkt", "complementarity",
+                                          "lower_bound", "primal_absolute", "dual_absolute")] + [("finite", C.c_int)]
 
 
 class _PdhgLP(C.Structure):
@@ -143,6 +149,8 @@ def load_library() -> C.CDLL:
                                         C.POINTER(C.c_int64), _i64p, _i64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
                                         _f64p, _f64p, _i64p, _i64p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     lib.niryukti_prepare_lp.restype = C.c_int
+    lib.niryukti_verify_lp.argtypes = [C.POINTER(_LP), _f64p, _f64p, C.c_int, C.POINTER(_Accuracy)]
+    lib.niryukti_verify_lp.restype = C.c_int
     lib.niryukti_power_norm.argtypes = [C.POINTER(_LP), C.c_int]
     lib.niryukti_power_norm.restype = C.c_double
     lib.npdhg_scale_lp.argtypes = [C.POINTER(_PdhgLP), C.c_int, C.POINTER(_Scaled)]
@@ -282,7 +290,8 @@ def _result_dict(res: _Result, x, y) -> dict:
     return out
 
 
-_OPTION_STRINGS = ("device", "method", "scaling", "primal_weight", "matrix_precision", "gpu_indices")
+_OPTION_STRINGS = ("device", "method", "scaling", "primal_weight", "matrix_precision", "gpu_indices",
+                   "checkpoint_path", "resume_path")
 
 
 def solve_lp(lp: LP, **options) -> dict:
@@ -331,6 +340,18 @@ def prepare_lp(lp: LP, **options) -> dict:
             "rl": out["rl"][:r], "ru": out["ru"][:r], "column_scale": out["column_scale"][:c],
             "row_scale": out["row_scale"][:r], "col_map": out["col_map"][:c], "row_map": out["row_map"][:r],
             "objective_scale": osc.value, "offset": off.value}
+
+
+def verify_lp(lp: LP, x, y, safe_bound: bool = True) -> dict:
+    """Original-space accuracy of (x, y) by the engine's certifying verifier
+    (the reference's vantage::verify)."""
+    lib = load_library()
+    x, y = _warm_array(x, lp.cols, "x"), _warm_array(y, lp.rows, "y")
+    s = lp._struct()
+    a = _Accuracy()
+    if lib.niryukti_verify_lp(C.byref(s), _ptr(x), _ptr(y), int(bool(safe_bound)), C.byref(a)) != 0:
+        raise RuntimeError(lib.niryukti_engine_last_error().decode())
+    return {k: getattr(a, k) for k, _ in _Accuracy._fields_}
 
 
 def power_norm(lp: LP, iterations: int) -> float:

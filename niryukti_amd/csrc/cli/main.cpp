@@ -37,6 +37,7 @@ void usage() {
                  "       --polishing --no-presolve --no-restart\n"
                  "       --no-adaptive --verbose\n"
                  "       --warm-start RESULT.json --allow-model-change --json-out FILE (--solution-out)\n"
+                 "       --checkpoint-out FILE --resume FILE --checkpoint-nodes N\n"
                  "       --cuda-graphs and --gpu-monitor are accepted: graphs and device checks are always on\n"
                  "models: .mps or native .json\n";
 }
@@ -72,6 +73,9 @@ int solve_command(int argc, char **argv, const nyk::Model &m) {
             o.device_prepare = v == "auto" ? -1 : v == "on" ? 1 : 0;
         }
         else if (a == "--json-out" || a == "--solution-out") output = val();
+        else if (a == "--checkpoint-out") o.checkpoint_path = val();
+        else if (a == "--resume") o.resume_path = val();
+        else if (a == "--checkpoint-nodes") o.checkpoint_nodes = std::stoll(val());
         else if (a == "--warm-start") warm = val();
         else if (a == "--allow-model-change") allow_model_change = true;
         else if (a == "--verbose") o.verbose = true;

@@ -62,6 +62,9 @@ nyk::Options to_options(const niryukti_options *in, const niryukti_lp &lp) {
     o.polishing = in->polishing != 0;
     if (in->initial_x) o.initial_x.assign(in->initial_x, in->initial_x + lp.cols);
     if (in->initial_y) o.initial_y.assign(in->initial_y, in->initial_y + lp.rows);
+    if (in->checkpoint_path) o.checkpoint_path = in->checkpoint_path;
+    if (in->resume_path) o.resume_path = in->resume_path;
+    o.checkpoint_nodes = in->checkpoint_nodes;
     return o;
 }
 }  // namespace
@@ -99,6 +102,9 @@ void niryukti_default_options(niryukti_options *o) {
     o->device_prepare = -1;
     o->power_iterations = 0;
     o->polishing = 0;
+    o->checkpoint_path = nullptr;
+    o->resume_path = nullptr;
+    o->checkpoint_nodes = 100;
 }
 
 int niryukti_prepare_lp(const niryukti_lp *lp, const niryukti_options *opts, int64_t *out_rows, int64_t *out_cols,
@@ -135,6 +141,32 @@ int niryukti_prepare_lp(const niryukti_lp *lp, const niryukti_options *opts, int
         put(row_map, p.rows);
         *objective_scale = p.objective_scale;
         *offset = q.offset;
+        return 0;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+int niryukti_verify_lp(const niryukti_lp *lp, const double *x, const double *y, int safe_bound,
+                       niryukti_accuracy *out) {
+    try {
+        g_error.clear();
+        if (!lp || !x || !y || !out) throw std::runtime_error("Null argument");
+        auto m = to_model(*lp);
+        nyk::Verifier v(m);
+        const auto a = v.evaluate(std::vector<double>(x, x + lp->cols), std::vector<double>(y, y + lp->rows),
+                                  safe_bound != 0);
+        out->objective = a.finite ? m.sense * a.objective : NAN;
+        out->primal = a.primal;
+        out->dual = a.dual;
+        out->gap = a.gap;
+        out->kkt = a.kkt;
+        out->complementarity = a.complementarity;
+        out->lower_bound = a.lower_bound;
+        out->primal_absolute = a.primal_absolute;
+        out->dual_absolute = a.dual_absolute;
+        out->finite = a.finite ? 1 : 0;
         return 0;
     } catch (const std::exception &e) {
         g_error = e.what();

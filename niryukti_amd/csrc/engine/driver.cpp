@@ -8,6 +8,7 @@
 // safe re-evaluation, Farkas ray tests after 1000 iterations, the restart rule
 // and the displacement / PID primal-weight update.  Iterations run in the HIP
 // backend behind the npdhg_* C-ABI (include/niryukti_pdhg.h).
+#include "checkpoint.hpp"
 #include "lp.hpp"
 #include "niryukti_pdhg.h"
 #include "niryukti_prepare.h"
@@ -53,6 +54,13 @@ class WeightPid {
         previous_ = error;
         initialized_ = true;
         return std::exp(clipped);
+    }
+    std::vector<double> state() const { return {integral_, previous_, initialized_ ? 1. : 0.}; }
+    void restore(const std::vector<double> &v) {
+        if (v.size() != 3 || !std::isfinite(v[0]) || !std::isfinite(v[1])) throw std::runtime_error("PID checkpoint state");
+        integral_ = v[0];
+        previous_ = v[1];
+        initialized_ = v[2] != 0;
     }
 };
 
@@ -130,6 +138,19 @@ class Device {
         dy = b;
     }
     void set_epoch(bool average) { check(npdhg_set_epoch(b_, average ? 1 : 0)); }
+    std::vector<double> snapshot() {
+        int64_t size = 0;
+        check(npdhg_snapshot_size(b_, &size));
+        std::vector<double> s(static_cast<size_t>(size));
+        check(npdhg_snapshot(b_, s.data()));
+        return s;
+    }
+    void restore(const std::vector<double> &s) {
+        int64_t size = 0;
+        check(npdhg_snapshot_size(b_, &size));
+        if (s.size() != size_t(size)) throw std::runtime_error("Continuous checkpoint backend dimensions");
+        check(npdhg_restore(b_, s.data()));
+    }
 
   private:
     static void check(int rc) {
@@ -158,6 +179,59 @@ Accuracy from_device(const npdhg_accuracy &d) {
     a.kkt = d.kkt;
     return a;
 }
+
+// Checkpoint files of one solve: the backend snapshot is stored as its
+// arrays (iterates, averages, activity, anchors, scalars, and with device
+// checks the epoch and best points), as the reference stores its backend's.
+class Checkpointer {
+  public:
+    Checkpointer(const Options &o, const Model &original, Device &backend, int64_t n, int64_t m, bool device_checks)
+        : o_(o), original_(original), backend_(backend), config_(checkpoint_configuration(o, device_checks)) {
+        parts_ = {n, m, n, m, m, n, m, m, 8};
+        if (device_checks) parts_.insert(parts_.end(), {n, m, original.cols(), original.rows(), 1});
+        if (o.checkpoint_nodes < 1) throw std::runtime_error("checkpoint_nodes must be at least 1");
+    }
+    bool writing() const { return !o_.checkpoint_path.empty(); }
+    bool resuming() const { return !o_.resume_path.empty(); }
+    // At the top of a chunk: write when the iteration count has reached the next mark.
+    bool due(int64_t iterations) {
+        if (!writing() || iterations < next_) return false;
+        next_ = iterations + o_.checkpoint_nodes;
+        return true;
+    }
+    void save(LoopState &s) {
+        const auto flat = backend_.snapshot();
+        s.backend.clear();
+        size_t at = 0;
+        for (int64_t len : parts_) {
+            s.backend.emplace_back(flat.begin() + long(at), flat.begin() + long(at + size_t(len)));
+            at += size_t(len);
+        }
+        write_checkpoint(o_.checkpoint_path, original_, config_, s);
+    }
+    // Reads the resume file and restores the backend from it.
+    LoopState load() {
+        LoopState s = read_checkpoint(o_.resume_path, original_, config_);
+        if (s.backend.size() != parts_.size()) throw std::runtime_error("Continuous checkpoint backend state count");
+        std::vector<double> flat;
+        for (size_t k = 0; k < parts_.size(); ++k) {
+            if (s.backend[k].size() != size_t(parts_[k]))
+                throw std::runtime_error("Continuous checkpoint backend dimensions");
+            flat.insert(flat.end(), s.backend[k].begin(), s.backend[k].end());
+        }
+        backend_.restore(flat);
+        next_ = s.iterations;
+        return s;
+    }
+
+  private:
+    const Options &o_;
+    const Model &original_;
+    Device &backend_;
+    Json config_;
+    std::vector<int64_t> parts_;
+    int64_t next_ = 0;
+};
 
 // Host-side Farkas test of one ray (src/solver.cpp:599-633): normalise by the
 // largest entry, then the exact outward-rounded bound decides.
@@ -243,6 +317,8 @@ struct Polish {
         std::fill(primal.c.begin(), primal.c.end(), 0.0);
         primal.offset = 0;
         Options po = o;
+        po.checkpoint_path.clear();  // the sub-solves are not checkpointed
+        po.resume_path.clear();
         po.method = "pdhg";
         po.adaptive = true;
         po.polishing = false;
@@ -278,14 +354,64 @@ struct Polish {
 // candidate.
 void run_device_checks(const Model &original, const Prepared &prep, const Options &o, Verifier &verifier,
                        Device &backend, Result &r, Clock::time_point start, double step, double weight,
-                       double restart_kkt, WeightPid &controller) {
+                       double restart_kkt, WeightPid &controller, Checkpointer &ck) {
     const bool residual_restarts = o.method == "rhpdhg" || o.method == "r2hpdhg";
     const int64_t n0 = original.cols(), m0 = original.rows();
     int64_t since_restart = 0, previous_rejected = 0;
     double best_kkt = r.accuracy.kkt;
     bool have_best = false;
     Polish polish;
+    if (ck.resuming()) {
+        LoopState s = ck.load();
+        r.x = std::move(s.best_x);
+        r.y = std::move(s.best_y);
+        r.accuracy = verifier.evaluate(r.x, r.y);
+        step = s.step;
+        weight = s.weight;
+        restart_kkt = s.restart_kkt;
+        since_restart = s.since_restart;
+        previous_rejected = s.previous_rejected;
+        polish.next = s.next_polish;
+        best_kkt = s.best_kkt;
+        have_best = s.have_best;
+        controller.restore(s.pid);
+        r.iterations = s.iterations;
+        r.restarts = s.restarts;
+        r.rejected_steps = s.rejected_steps;
+        r.weight_updates = s.weight_updates;
+        r.host_candidate_checks = s.host_checks;
+        r.device_candidate_checks = s.device_checks;
+        r.polishing_iterations = s.polishing_iterations;
+        r.polishing_attempts = s.polishing_attempts;
+        r.first_restart_iteration = s.first_restart_iteration;
+        r.first_restart_weight = s.first_restart_weight;
+    }
     while (r.iterations < o.iteration_limit) {
+        if (ck.due(r.iterations)) {
+            LoopState s;
+            s.best_x = r.x;
+            s.best_y = r.y;
+            s.pid = controller.state();
+            s.step = step;
+            s.weight = weight;
+            s.restart_kkt = restart_kkt;
+            s.since_restart = since_restart;
+            s.next_polish = polish.next;
+            s.previous_rejected = previous_rejected;
+            s.iterations = r.iterations;
+            s.restarts = r.restarts;
+            s.rejected_steps = r.rejected_steps;
+            s.weight_updates = r.weight_updates;
+            s.host_checks = r.host_candidate_checks;
+            s.device_checks = r.device_candidate_checks;
+            s.polishing_iterations = r.polishing_iterations;
+            s.polishing_attempts = r.polishing_attempts;
+            s.first_restart_iteration = r.first_restart_iteration;
+            s.first_restart_weight = r.first_restart_weight;
+            s.best_kkt = best_kkt;
+            s.have_best = have_best;
+            ck.save(s);
+        }
         if (stop_requested(o)) {
             r.status = "INTERRUPTED";
             break;
@@ -555,15 +681,73 @@ Result solve_continuous(const Model &original, const Options &o) {
     int64_t since_restart = 0, previous_rejected = 0;
     WeightPid controller;
     const bool want_device = o.device_checks > 0 || (o.device_checks < 0 && original.A.nnz() >= kDeviceCheckNonzeros);
-    if (want_device && backend.attach(original, prep)) {
-        run_device_checks(original, prep, o, verifier, backend, r, start, step, weight, restart_kkt, controller);
+    const bool device_checks = want_device && backend.attach(original, prep);
+    Checkpointer ck(o, original, backend, m.cols(), m.rows(), device_checks);
+    if (device_checks) {
+        run_device_checks(original, prep, o, verifier, backend, r, start, step, weight, restart_kkt, controller, ck);
         r.seconds = elapsed(start);
         return r;
     }
     std::vector<double> epochx = x, epochy = y;
     std::vector<double> cx(m.cols()), cy(m.rows()), ax(m.cols()), ay(m.rows());
     Polish polish;
+    bool best_certified = true;  // r holds the start point's figures, certified bound included
+    if (ck.resuming()) {
+        LoopState s = ck.load();
+        if (s.epoch_x.size() != size_t(m.cols()) || s.epoch_y.size() != size_t(m.rows()))
+            throw std::runtime_error("Continuous checkpoint epoch dimensions");
+        r.x = std::move(s.best_x);
+        r.y = std::move(s.best_y);
+        best_certified = s.best_certified;
+        r.accuracy = verifier.evaluate(r.x, r.y, best_certified);
+        if (!r.accuracy.finite) throw std::runtime_error("Invalid continuous checkpoint candidate");
+        epochx = std::move(s.epoch_x);
+        epochy = std::move(s.epoch_y);
+        step = s.step;
+        weight = s.weight;
+        restart_kkt = s.restart_kkt;
+        since_restart = s.since_restart;
+        previous_rejected = s.previous_rejected;
+        polish.next = s.next_polish;
+        controller.restore(s.pid);
+        r.iterations = s.iterations;
+        r.restarts = s.restarts;
+        r.rejected_steps = s.rejected_steps;
+        r.weight_updates = s.weight_updates;
+        r.host_candidate_checks = s.host_checks;
+        r.device_candidate_checks = s.device_checks;
+        r.polishing_iterations = s.polishing_iterations;
+        r.polishing_attempts = s.polishing_attempts;
+        r.first_restart_iteration = s.first_restart_iteration;
+        r.first_restart_weight = s.first_restart_weight;
+    }
     while (r.iterations < o.iteration_limit) {
+        if (ck.due(r.iterations)) {
+            LoopState s;
+            s.best_x = r.x;
+            s.best_y = r.y;
+            s.best_certified = best_certified;
+            s.epoch_x = epochx;
+            s.epoch_y = epochy;
+            s.pid = controller.state();
+            s.step = step;
+            s.weight = weight;
+            s.restart_kkt = restart_kkt;
+            s.since_restart = since_restart;
+            s.next_polish = polish.next;
+            s.previous_rejected = previous_rejected;
+            s.iterations = r.iterations;
+            s.restarts = r.restarts;
+            s.rejected_steps = r.rejected_steps;
+            s.weight_updates = r.weight_updates;
+            s.host_checks = r.host_candidate_checks;
+            s.device_checks = r.device_candidate_checks;
+            s.polishing_iterations = r.polishing_iterations;
+            s.polishing_attempts = r.polishing_attempts;
+            s.first_restart_iteration = r.first_restart_iteration;
+            s.first_restart_weight = r.first_restart_weight;
+            ck.save(s);
+        }
         if (stop_requested(o)) {
             r.status = "INTERRUPTED";
             break;
@@ -608,6 +792,7 @@ Result solve_continuous(const Model &original, const Options &o) {
             r.x = ox;
             r.y = oy;
             r.accuracy = ca;
+            best_certified = false;
         }
         r.verification_seconds += elapsed(tick);
         if (o.verbose)
@@ -660,6 +845,7 @@ Result solve_continuous(const Model &original, const Options &o) {
                 r.x = std::move(px);
                 r.y = std::move(py);
                 r.accuracy = pa;
+                best_certified = true;
                 if (pa.kkt <= o.tol) {
                     r.status = "OPTIMAL";
                     break;
@@ -765,6 +951,7 @@ Result solve(const Model &m, const Options &options) {
         if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
     for (double v : o.initial_y)
         if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
+    if (!o.resume_path.empty()) check_checkpoint_model(o.resume_path, m, checkpoint_configuration(o, false));
     if (npdhg_device_count() < 1) {
         Result r;
         requested(r);

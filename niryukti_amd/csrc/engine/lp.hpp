@@ -76,6 +76,10 @@ struct Options {
     // Feasibility polishing (src/solver.cpp:634-690): off by default.
     bool polishing = false;
     std::vector<double> initial_x, initial_y;
+    // Continuous checkpoints (src/solver.cpp:384-509): written to
+    // checkpoint_path every checkpoint_nodes iterations, read by resume_path.
+    std::string checkpoint_path, resume_path;
+    int64_t checkpoint_nodes = 100;
     std::shared_ptr<std::atomic<bool>> cancellation;
 };
 

@@ -1,14 +1,19 @@
 // Original-space KKT verification on the host: the certifying check.
 //
-// Formulae follow the reference verifier (src/verify.cpp:12-240) term for term:
-// per-row/per-column normalised primal residual, projected-gradient dual
-// residual scaled by max(1 + |c_j|), complementarity and Lagrangian-bound gap,
-// with long-double accumulation where the reference uses it and the
-// outward-rounded long-double lower bound for the safe certificate.  The two
-// SpMVs run in parallel by rows (thread-count independent); the accumulations
-// stay sequential so the result is bit-identical to the reference.
+// The check is laid out like the device evaluation (eval_kernel in
+// hip/pdhg_backend.hip): one pass over the rows and one over the columns feed
+// a small set of accumulators (maxima for the residuals, long-double sums for
+// the objective support, complementarity and the box minimum of the
+// Lagrangian), and a final step turns them into the rel-KKT figures.  The
+// figures are the reference's (src/verify.cpp:103-240): normalised primal
+// violations, the projected reduced-cost dual residual scaled by max(1+|c_j|),
+// and the larger of the complementarity gap and the Lagrangian-bound gap.
+// Sums run sequentially over rows, then columns, in long double, which is the
+// order the reference's results depend on.  The certificate bound is an
+// outward-rounded long-double interval evaluation of the Lagrangian dual
+// function (src/verify.cpp:12-88), so it is a rigorous lower bound.
 //
-// The GPU evaluates the same quantities between certifications; this host
+// The device evaluates the same quantities between certifications; this host
 // version is the one that may declare OPTIMAL.
 #include "lp.hpp"
 
@@ -16,42 +21,128 @@
 
 namespace nyk {
 namespace {
-long double down(long double v) { return std::nextafter(v, -std::numeric_limits<long double>::infinity()); }
-long double up(long double v) { return std::nextafter(v, std::numeric_limits<long double>::infinity()); }
+using Wide = long double;
+constexpr Wide kWideInf = std::numeric_limits<Wide>::infinity();
 
-// Outward-rounded Lagrangian lower bound (src/verify.cpp:12-88, LP form).
-double lagrangian_lower_bound(const Model &m, const Csr &at, const std::vector<double> &y, bool feasibility) {
-    long double lower = feasibility ? 0 : m.offset;
-    for (size_t i = 0; i < y.size(); ++i)
-        if (y[i] != 0) {
-            double b = y[i] > 0 ? m.ru[i] : m.rl[i];
-            if (!std::isfinite(b)) return -kInf;
-            if (b == 0) continue;
-            lower = down(lower - up((long double)y[i] * b));
-        }
-    for (size_t j = 0; j < m.c.size(); ++j) {
-        long double gl = feasibility ? 0 : m.c[j], gu = gl;
-        for (int64_t k = at.ptr[j]; k < at.ptr[j + 1]; ++k)
-            if (y[at.index[k]] != 0 && at.value[k] != 0) {
-                long double v = (long double)at.value[k] * y[at.index[k]];
-                gl = down(gl + down(v));
-                gu = up(gu + up(v));
-            }
-        if ((!std::isfinite(m.lb[j]) && gu > 0) || (!std::isfinite(m.ub[j]) && gl < 0)) return -kInf;
-        long double term = std::numeric_limits<long double>::infinity();
-        if (gl <= 0 && gu >= 0) term = 0;
-        for (double b : {m.lb[j], m.ub[j]})
-            if (std::isfinite(b)) {
-                term = std::min(term, (b == 0 || gl == 0) ? 0 : down(gl * b));
-                term = std::min(term, (b == 0 || gu == 0) ? 0 : down(gu * b));
-            }
-        if (gl == 0 && gu == 0) term = 0;
-        if (!std::isfinite(term)) return -kInf;
-        if (term != 0) lower = down(lower + term);
+Wide lower_of(Wide v) { return std::nextafter(v, -kWideInf); }
+Wide upper_of(Wide v) { return std::nextafter(v, kWideInf); }
+
+// A long-double interval widened outward by one unit in the last place at
+// every operation, so that it encloses the exact real value.
+struct Enclosure {
+    Wide lo, hi;
+    void add_product(double a, double b) {
+        const Wide p = Wide(a) * b;
+        lo = lower_of(lo + lower_of(p));
+        hi = upper_of(hi + upper_of(p));
     }
-    double result = double(lower);
-    if (!std::isfinite(result)) return -kInf;
-    return std::nextafter(result, -kInf);
+    bool holds_zero() const { return lo <= 0 && hi >= 0; }
+};
+
+// Smallest value of g * x over the box [l, u] for every g in `g`, rounded
+// down; +inf when the box is unbounded in the direction g decreases.
+Wide box_minimum(const Enclosure &g, double l, double u) {
+    if (g.lo == 0 && g.hi == 0) return 0;
+    Wide best = g.holds_zero() ? 0 : kWideInf;
+    for (const double end : {l, u}) {
+        if (!std::isfinite(end)) continue;
+        for (const Wide slope : {g.lo, g.hi}) best = std::min(best, (end == 0 || slope == 0) ? 0 : lower_of(slope * end));
+    }
+    return best;
+}
+
+// Rigorous lower bound of the LP's Lagrangian dual function at y (with the
+// objective and offset dropped in feasibility mode, for Farkas rays).  -inf
+// when y pays for an infinite row bound or the columns' box cannot bound it.
+double certified_lower_bound(const Model &m, const Csr &at, const std::vector<double> &y, bool feasibility) {
+    Wide bound = feasibility ? 0 : m.offset;
+    for (size_t i = 0; i < y.size(); ++i) {
+        if (y[i] == 0) continue;
+        const double side = y[i] > 0 ? m.ru[i] : m.rl[i];
+        if (!std::isfinite(side)) return -kInf;
+        if (side != 0) bound = lower_of(bound - upper_of(Wide(y[i]) * side));
+    }
+    for (int64_t j = 0; j < m.cols(); ++j) {
+        const Wide cost = feasibility ? 0 : m.c[size_t(j)];
+        Enclosure g{cost, cost};
+        for (int64_t k = at.ptr[size_t(j)]; k < at.ptr[size_t(j) + 1]; ++k) {
+            const double yi = y[size_t(at.index[size_t(k)])], a = at.value[size_t(k)];
+            if (yi != 0 && a != 0) g.add_product(a, yi);
+        }
+        const double l = m.lb[size_t(j)], u = m.ub[size_t(j)];
+        if ((!std::isfinite(l) && g.hi > 0) || (!std::isfinite(u) && g.lo < 0)) return -kInf;
+        const Wide term = box_minimum(g, l, u);
+        if (!std::isfinite(term)) return -kInf;
+        if (term != 0) bound = lower_of(bound + term);
+    }
+    const double out = double(bound);
+    return std::isfinite(out) ? std::nextafter(out, -kInf) : -kInf;
+}
+
+// Running figures of one evaluation.
+struct Tally {
+    double primal = 0, primal_abs = 0, dual = 0, dual_abs = 0;
+    double objective = 0;
+    Wide support = 0;     // sum of y_i times its active row bound
+    Wide complement = 0;  // sum of |y_i (b_i - a_i x)| and |g_j (x_j - bound_j)|
+    Wide box = 0;         // minimum of (c + A^T y)^T x over the box
+    bool bounded = true;  // every active bound finite: the Lagrangian bound exists
+    bool finite = true;
+};
+
+void tally_rows(const Model &m, const std::vector<double> &ax, const std::vector<double> &y, Tally &t) {
+    for (size_t i = 0; i < y.size(); ++i) {
+        const double lo = m.rl[i], hi = m.ru[i], row = ax[i];
+        const double below = std::max(0., lo - row), above = std::max(0., row - hi);
+        if (std::isfinite(lo)) t.primal = std::max(t.primal, below / (1 + std::abs(lo)));
+        if (std::isfinite(hi)) t.primal = std::max(t.primal, above / (1 + std::abs(hi)));
+        t.primal_abs = std::max(t.primal_abs, std::max(below, above));
+        if (y[i] == 0) continue;
+        const double side = y[i] > 0 ? hi : lo;
+        if (std::isfinite(side)) {
+            t.support += Wide(y[i]) * side;
+            t.complement += std::abs(Wide(y[i]) * (side - row));
+        } else {
+            // a price on a side with no bound: dual infeasible by |y_i|
+            t.bounded = false;
+            t.dual = std::max(t.dual, std::abs(y[i]));
+            t.dual_abs = std::max(t.dual_abs, std::abs(y[i]));
+        }
+    }
+}
+
+void tally_columns(const Model &m, const std::vector<double> &x, const std::vector<double> &aty, double cost_scale,
+                   Tally &t) {
+    for (size_t j = 0; j < x.size(); ++j) {
+        const double xj = x[j], l = m.lb[j], u = m.ub[j];
+        const double outside = std::max({0., l - xj, xj - u});
+        t.primal = std::max(t.primal, outside / (1 + std::abs(xj)));
+        t.primal_abs = std::max(t.primal_abs, outside);
+        t.objective += m.c[j] * xj;
+        const double reduced = m.c[j] + aty[j];  // gradient of the Lagrangian in x_j
+        if (!std::isfinite(reduced)) {
+            t.finite = false;
+            return;
+        }
+        const double projected = std::clamp(reduced, xj - u, xj - l);
+        t.dual = std::max(t.dual, std::abs(projected) / cost_scale);
+        t.dual_abs = std::max(t.dual_abs, std::abs(projected));
+        if (reduced > 0 && std::isfinite(l)) t.complement += std::abs(Wide(reduced) * (xj - l));
+        if (reduced < 0 && std::isfinite(u)) t.complement += std::abs(Wide(reduced) * (u - xj));
+        const double minimiser = reduced > 0 ? l : reduced < 0 ? u : std::clamp(0., l, u);
+        if (std::isfinite(minimiser))
+            t.box += Wide(reduced) * minimiser;
+        else
+            t.bounded = false;
+    }
+}
+
+double relative_gap(double objective, double bound) {
+    return std::abs(objective - bound) / (1 + std::abs(objective) + std::abs(bound));
+}
+
+bool all_finite(const std::vector<double> &v) {
+    return std::all_of(v.begin(), v.end(), [](double e) { return std::isfinite(e); });
 }
 }  // namespace
 
@@ -59,85 +150,45 @@ Verifier::Verifier(const Model &model)
     : model_(model), transpose_(model.A.transposed()), activity_(model.A.rows), transpose_product_(model.A.cols) {}
 
 double Verifier::infeasibility_bound(const std::vector<double> &y) const {
-    if (y.size() != model_.rl.size()) return -kInf;
-    for (double v : y)
-        if (!std::isfinite(v)) return -kInf;
-    return lagrangian_lower_bound(model_, transpose_, y, true);
+    if (y.size() != model_.rl.size() || !all_finite(y)) return -kInf;
+    return certified_lower_bound(model_, transpose_, y, true);
 }
 
 Accuracy Verifier::evaluate(const std::vector<double> &x, const std::vector<double> &y, bool safe_bound) {
     const Model &m = model_;
-    Accuracy a;
-    if (x.size() != m.c.size() || y.size() != m.rl.size()) return a;
-    for (double v : x)
-        if (!std::isfinite(v)) return a;
-    for (double v : y)
-        if (!std::isfinite(v)) return a;
+    if (x.size() != m.c.size() || y.size() != m.rl.size() || !all_finite(x) || !all_finite(y)) return Accuracy{};
     m.A.multiply(x.data(), activity_.data());
     transpose_.multiply(y.data(), transpose_product_.data());
-    const auto &ax = activity_;
-    const auto &aty = transpose_product_;
-    for (double v : ax)
-        if (!std::isfinite(v)) return Accuracy{};
-    for (double v : aty)
-        if (!std::isfinite(v)) return Accuracy{};
-    a.objective = m.offset;
-    a.primal = a.dual = a.primal_absolute = a.dual_absolute = 0;
-    long double support = 0, box_min = 0, complement = 0;
-    bool bound_ok = true;
-    for (size_t i = 0; i < y.size(); ++i) {
-        double v = std::max({0., m.rl[i] - ax[i], ax[i] - m.ru[i]});
-        if (std::isfinite(m.rl[i])) a.primal = std::max(a.primal, std::max(0., m.rl[i] - ax[i]) / (1 + std::abs(m.rl[i])));
-        if (std::isfinite(m.ru[i])) a.primal = std::max(a.primal, std::max(0., ax[i] - m.ru[i]) / (1 + std::abs(m.ru[i])));
-        a.primal_absolute = std::max(a.primal_absolute, v);
-        double endpoint = y[i] > 0 ? m.ru[i] : m.rl[i];
-        if (y[i] != 0) {
-            if (std::isfinite(endpoint)) {
-                support += (long double)y[i] * endpoint;
-                complement += std::abs((long double)y[i] * (endpoint - ax[i]));
-            } else {
-                bound_ok = false;
-                a.dual_absolute = std::max(a.dual_absolute, std::abs(y[i]));
-                a.dual = std::max(a.dual, std::abs(y[i]));
-            }
-        }
-    }
-    double cscale = 1;
-    for (double v : m.c) cscale = std::max(cscale, 1 + std::abs(v));
-    for (size_t j = 0; j < x.size(); ++j) {
-        double v = std::max({0., m.lb[j] - x[j], x[j] - m.ub[j]});
-        a.primal = std::max(a.primal, v / (1 + std::abs(x[j])));
-        a.primal_absolute = std::max(a.primal_absolute, v);
-        // The zero quadratic terms are kept so that overflow behaves as in the reference.
-        a.objective += m.c[j] * x[j] + .5 * 0.0 * x[j] * x[j] + .5 * x[j] * 0.0;
-        double linear = m.c[j] + aty[j], g = linear + 0.0 * x[j] + 0.0;
-        if (!std::isfinite(linear) || !std::isfinite(g)) return Accuracy{};
-        double d = std::clamp(g, x[j] - m.ub[j], x[j] - m.lb[j]);
-        a.dual_absolute = std::max(a.dual_absolute, std::abs(d));
-        a.dual = std::max(a.dual, std::abs(d) / cscale);
-        if (g > 0 && std::isfinite(m.lb[j])) complement += std::abs((long double)g * (x[j] - m.lb[j]));
-        if (g < 0 && std::isfinite(m.ub[j])) complement += std::abs((long double)g * (m.ub[j] - x[j]));
-        double arg = linear > 0 ? m.lb[j] : linear < 0 ? m.ub[j] : std::clamp(0., m.lb[j], m.ub[j]);
-        if (!std::isfinite(arg))
-            bound_ok = false;
-        else
-            box_min += (long double)linear * arg + .5L * 0.0 * arg * arg;
-    }
-    a.complementarity = double(complement);
+    if (!all_finite(activity_) || !all_finite(transpose_product_)) return Accuracy{};
+    double cost_scale = 1;
+    for (double c : m.c) cost_scale = std::max(cost_scale, 1 + std::abs(c));
+
+    Tally t;
+    t.objective = m.offset;
+    tally_rows(m, activity_, y, t);
+    tally_columns(m, x, transpose_product_, cost_scale, t);
+    if (!t.finite) return Accuracy{};
+
+    Accuracy a;
+    a.objective = t.objective;
+    a.primal = t.primal;
+    a.primal_absolute = t.primal_abs;
+    a.dual = t.dual;
+    a.dual_absolute = t.dual_abs;
+    a.complementarity = double(t.complement);
     a.gap = a.complementarity / (1 + std::abs(a.objective - m.offset));
-    if (bound_ok) {
-        long double lower = (long double)m.offset - support + box_min;
-        long double guard = 64 * std::numeric_limits<double>::epsilon() *
-                            (1 + std::abs((long double)m.offset) + std::abs(support) + std::abs(box_min));
-        a.lower_bound = double(lower - guard);
-        a.gap = std::max(a.gap, std::abs(a.objective - a.lower_bound) / (1 + std::abs(a.objective) + std::abs(a.lower_bound)));
+    if (t.bounded) {
+        // the Lagrangian value at y, lowered by a guard for its rounding
+        const Wide value = Wide(m.offset) - t.support + t.box;
+        const Wide guard = 64 * std::numeric_limits<double>::epsilon() *
+                           (1 + std::abs(Wide(m.offset)) + std::abs(t.support) + std::abs(t.box));
+        a.gap = std::max(a.gap, relative_gap(a.objective, double(value - guard)));
     }
-    a.lower_bound = safe_bound ? lagrangian_lower_bound(m, transpose_, y, false) : -kInf;
-    if (std::isfinite(a.lower_bound))
-        a.gap = std::max(a.gap, std::abs(a.objective - a.lower_bound) / (1 + std::abs(a.objective) + std::abs(a.lower_bound)));
+    a.lower_bound = safe_bound ? certified_lower_bound(m, transpose_, y, false) : -kInf;
+    if (std::isfinite(a.lower_bound)) a.gap = std::max(a.gap, relative_gap(a.objective, a.lower_bound));
     a.kkt = std::max({a.primal, a.dual, a.gap});
-    a.finite = std::isfinite(a.objective) && std::isfinite(a.kkt) && std::isfinite(a.gap) && std::isfinite(a.primal) &&
-               std::isfinite(a.dual);
+    a.finite = std::isfinite(a.objective) && std::isfinite(a.primal) && std::isfinite(a.dual) && std::isfinite(a.gap) &&
+               std::isfinite(a.kkt);
     if (!a.finite) {
         a.kkt = kInf;
         a.lower_bound = -kInf;

@@ -1394,6 +1394,106 @@ class Backend {
         *dy2 = out[1];
     }
 
+    // Resumable state (IterationBackend::snapshot / restore,
+    // src/cpu.cpp:49-89, src/cuda.cu:739-770), in doubles:
+    //   x (n) | y (m) | xa (n) | ya (m) | ax (m) | anchor x (n) | anchor y (m) |
+    //   anchor ax (m) | factor, average mass, initial residual, previous
+    //   residual, accepted, rejected, epoch steps, restart requested |
+    // and with device checks attached:
+    //   epoch x (n) | epoch y (m) | best x (n0) | best y (m0) | best kept (0/1).
+    // The anchors are zero unless the backend runs Halpern iterations.
+    static constexpr int kSnapshotScalars = 8;
+    long long snapshot_size() const {
+        long long s = 3 * n_ + 5 * m_ + kSnapshotScalars;
+        if (attached_) s += n_ + m_ + n0_ + m0_ + 1;
+        return s;
+    }
+
+    void snapshot(double *out) {
+        sync_state();
+        const int c = host_.cur;
+        double *at = out;
+        auto take = [&](const double *dev, long long count) {
+            if (dev)
+                download(at, dev, count);
+            else
+                std::fill(at, at + count, 0.0);
+            at += count;
+        };
+        take(x_[c], n_);
+        take(y_[c], m_);
+        take(xa_[c], n_);
+        take(ya_[c], m_);
+        take(ax_[c], m_);
+        take(flags_.halpern ? anchor_x_ : nullptr, n_);
+        take(flags_.halpern ? anchor_y_ : nullptr, m_);
+        take(flags_.halpern ? anchor_ax_ : nullptr, m_);
+        const double scalars[kSnapshotScalars] = {host_.factor,           host_.average_mass,
+                                                  host_.initial_residual, host_.previous_residual,
+                                                  double(host_.accepted), double(host_.rejected),
+                                                  double(host_.epoch_steps), double(host_.restart_requested)};
+        std::copy(scalars, scalars + kSnapshotScalars, at);
+        at += kSnapshotScalars;
+        if (attached_) {
+            take(ex_, n_);
+            take(ey_, m_);
+            take(bx_, n0_);
+            take(by_, m0_);
+            *at++ = best_ ? 1.0 : 0.0;
+        }
+    }
+
+    void restore(const double *in) {
+        const long long total = snapshot_size();
+        for (long long k = 0; k < total; ++k)
+            if (std::isnan(in[k])) throw std::runtime_error("NaN in backend checkpoint state");
+        const double *q = in + 3 * n_ + 5 * m_;
+        auto counter = [](double v) { return v >= 0 && v < 9.0e15 && v == std::floor(v); };
+        if (!(q[0] > 0) || !std::isfinite(q[0]) || !(q[1] >= 0) || !std::isfinite(q[1]) || !counter(q[4]) ||
+            !counter(q[5]) || !counter(q[6]) || (q[7] != 0 && q[7] != 1))
+            throw std::runtime_error("Invalid backend checkpoint counters");
+        sync_state();
+        const double *at = in;
+        auto put = [&](double *dev, long long count) {
+            if (dev && count) HIP_OK(hipMemcpyAsync(dev, at, size_t(count) * sizeof(double), hipMemcpyHostToDevice, stream_));
+            at += count;
+        };
+        put(x_[0], n_);
+        put(y_[0], m_);
+        put(xa_[0], n_);
+        put(ya_[0], m_);
+        put(ax_[0], m_);
+        put(flags_.halpern ? anchor_x_ : nullptr, n_);
+        put(flags_.halpern ? anchor_y_ : nullptr, m_);
+        put(flags_.halpern ? anchor_ax_ : nullptr, m_);
+        at += kSnapshotScalars;
+        State s = host_;
+        s.factor = q[0];
+        s.average_mass = q[1];
+        s.initial_residual = q[2];
+        s.previous_residual = q[3];
+        s.accepted = (long long)q[4];
+        s.rejected = (long long)q[5];
+        s.epoch_steps = (long long)q[6];
+        s.restart_requested = q[7] != 0;
+        s.cur = 0;
+        s.stop = kDone;
+        s.chunk_accepted = s.chunk_target = 0;
+        s.consecutive_rejects = 0;
+        s.ticket = 0;
+        HIP_OK(hipMemcpyAsync(st_, &s, sizeof s, hipMemcpyHostToDevice, stream_));
+        if (attached_) {
+            put(ex_, n_);
+            put(ey_, m_);
+            put(bx_, n0_);
+            put(by_, m0_);
+            best_ = *at++ != 0;
+            evaluated_ = false;
+        }
+        HIP_OK(hipStreamSynchronize(stream_));
+        host_ = s;
+    }
+
     void set_epoch(bool average) {
         need_original();
         sync_state();
@@ -1865,6 +1965,27 @@ int npdhg_device_name(int device, char *name, int len) {
         HIP_OK(hipGetDeviceProperties(&prop, device));
         std::snprintf(name, size_t(len), "%s (%s, %d CUs, %.0f GiB)", prop.name, prop.gcnArchName,
                       prop.multiProcessorCount, double(prop.totalGlobalMem) / (1024.0 * 1024 * 1024));
+    });
+}
+
+int npdhg_snapshot_size(npdhg_backend *b, int64_t *doubles) {
+    return guarded([&] {
+        if (!b || !doubles) throw std::runtime_error("Null argument");
+        *doubles = b->impl.snapshot_size();
+    });
+}
+
+int npdhg_snapshot(npdhg_backend *b, double *state) {
+    return guarded([&] {
+        if (!b || !state) throw std::runtime_error("Null argument");
+        b->impl.snapshot(state);
+    });
+}
+
+int npdhg_restore(npdhg_backend *b, const double *state) {
+    return guarded([&] {
+        if (!b || !state) throw std::runtime_error("Null argument");
+        b->impl.restore(state);
     });
 }
 
