@@ -47,6 +47,20 @@ typedef struct {
 int npdhg_scale_lp(const npdhg_lp *lp, int passes, npdhg_scaled *out);
 const char *npdhg_scale_last_error(void);
 
+/* npdhg_scale_lp for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It has no counterpart in the reference.  Every array of `lp` and of `out`
+ * is a DEVICE pointer (lp->rows / lp->cols and out->objective_scale are host
+ * values); outputs are written in place.  `stream` is a hipStream_t (NULL for
+ * the null stream); the call synchronises it before returning.  Same kernels
+ * and bit-identical results as npdhg_scale_lp; device_seconds and seconds are
+ * 0.  The work arrays live in one static device array of 128 MiB per device
+ * (held until the process ends): an LP needing more than
+ * 24 bytes x nonzeros + 8 bytes x columns (+ a few KiB) is refused with an
+ * error.  It calls no hipMalloc or hipFree.  Every call on a device shares
+ * that device's array, so calls that overlap on one device race. */
+int npdhg_scale_lp_noalloc(const npdhg_lp *lp, int passes, npdhg_scaled *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

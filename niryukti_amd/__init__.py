@@ -156,6 +156,8 @@ def load_library() -> C.CDLL:
     lib.npdhg_scale_lp.argtypes = [C.POINTER(_PdhgLP), C.c_int, C.POINTER(_Scaled)]
     lib.npdhg_scale_lp.restype = C.c_int
     lib.npdhg_scale_last_error.restype = C.c_char_p
+    lib.npdhg_scale_lp_noalloc.argtypes = [C.POINTER(_PdhgLP), C.c_int, C.POINTER(_Scaled), C.c_void_p]
+    lib.npdhg_scale_lp_noalloc.restype = C.c_int
     lib.niryukti_batch_create.argtypes = [C.POINTER(_Options), C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]
     lib.niryukti_batch_destroy.argtypes = [C.c_void_p]
     lib.niryukti_batch_submit.argtypes = [C.c_void_p, C.POINTER(_LP), _f64p, _f64p, C.POINTER(C.c_int64)]

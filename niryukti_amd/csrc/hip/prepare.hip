@@ -21,6 +21,8 @@
 // npdhg_scale_lp refuses matrices where that exceeds kRankBudget.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -441,11 +443,219 @@ void scale_lp(const npdhg_lp &lp, int passes, npdhg_scaled &out) {
     out.seconds = std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
+// ------------------------------------------------ no-allocation variant --
+//
+// npdhg_scale_lp_noalloc runs the same kernels on LP arrays that are already
+// in device memory and writes into caller-provided device arrays; its work
+// arrays (column counts, slots, ranks, row of each entry, scan sums, the
+// objective-scale partials and the validation flags) live in one static
+// device array, `scratch`, so it never allocates device memory.
+
+constexpr size_t kScratchBytes = size_t(128) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Per-device address of `scratch` (each device holds its own copy of the
+// array); filled on the first call on a device with that device current.
+constexpr int kMaxDevices = 64;
+std::atomic<unsigned char *> g_scratch[kMaxDevices];
+
+unsigned char *scratch_address() {
+    int device = 0;
+    HIP_OK(hipGetDevice(&device));
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("Device ordinal out of range for the scratch table");
+    unsigned char *p = g_scratch[device].load(std::memory_order_acquire);
+    if (!p) {
+        void *addr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !addr)
+            throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        p = static_cast<unsigned char *>(addr);
+        g_scratch[device].store(p, std::memory_order_release);
+    }
+    return p;
+}
+
+__global__ void fill_u64_kernel(unsigned long long *p, long long n, unsigned long long v) {
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) p[i] = v;
+}
+__global__ void fill_f64_kernel(double *p, long long n, double v) {
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) p[i] = v;
+}
+
+// flags[0]: a row's offsets decrease or leave [0, nnz]; flags[1]: a column
+// index outside [0, n); flags[2]: a non-finite coefficient.
+__global__ void validate_kernel(const int64_t *ptr, long long m, const int64_t *idx, const double *val, long long nnz,
+                                long long n, unsigned int *flags) {
+    const long long stride = (long long)gridDim.x * kBlock;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < m; i += stride)
+        if (ptr[i] > ptr[i + 1] || ptr[i] < 0 || ptr[i + 1] > nnz) atomicOr(&flags[0], 1u);
+    for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < nnz; k += stride) {
+        const int64_t j = idx[k];
+        if (j < 0 || j >= n) atomicOr(&flags[1], 1u);
+        if (!isfinite(val[k])) atomicOr(&flags[2], 1u);
+    }
+}
+
+// Sum of squared column degrees (the rank step's cost), one atomic per block.
+__global__ void degree_work_kernel(const unsigned long long *count, long long n, unsigned long long *work) {
+    __shared__ unsigned long long part[kBlock];
+    unsigned long long w = 0;
+    for (long long j = (long long)blockIdx.x * kBlock + threadIdx.x; j < n; j += (long long)gridDim.x * kBlock)
+        w += count[j] * count[j];
+    part[threadIdx.x] = w;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h /= 2) {
+        if (int(threadIdx.x) < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(work, part[0]);
+}
+
+// Byte layout of the work arrays in `scratch` for an LP of n columns and nnz
+// nonzeros; total() beyond kScratchBytes is refused.
+struct ScratchPlan {
+    size_t flags, work, osc, cmax, count, sums, slots, perm, row, end;
+    ScratchPlan(long long n, long long nnz, int cmax_blocks, long long nsums) {
+        size_t at = 0;
+        auto take = [&](size_t bytes) {
+            const size_t here = at;
+            at = (at + std::max<size_t>(bytes, 8) + 255) & ~size_t(255);
+            return here;
+        };
+        flags = take(4 * sizeof(unsigned int));
+        work = take(sizeof(unsigned long long));
+        osc = take(sizeof(double));
+        cmax = take(size_t(cmax_blocks) * sizeof(double));
+        count = take(size_t(n) * sizeof(unsigned long long));
+        sums = take(size_t(nsums) * sizeof(int64_t));
+        slots = take(size_t(nnz) * sizeof(int64_t));
+        perm = take(size_t(nnz) * sizeof(int64_t));
+        row = take(size_t(nnz) * sizeof(int64_t));
+        end = at;
+    }
+};
+
+void scale_lp_noalloc(const npdhg_lp &lp, int passes, npdhg_scaled &out, hipStream_t s) {
+    const long long m = lp.rows, n = lp.cols;
+    if (m < 0 || n < 0 || m >= (1LL << 31) || n >= (1LL << 31)) throw std::runtime_error("LP dimensions out of range");
+    if (passes < 0) throw std::runtime_error("Negative scaling passes");
+    if (!lp.ptr || (n && (!lp.c || !lp.lb || !lp.ub)) || (m && (!lp.rl || !lp.ru)))
+        throw std::runtime_error("Null LP array");
+    if (!out.value || (n && (!out.c || !out.lb || !out.ub || !out.column_scale)) ||
+        (m && (!out.rl || !out.ru || !out.row_scale)) || !out.t_ptr)
+        throw std::runtime_error("Null output array");
+    unsigned char *base = scratch_address();
+    // the matrix's extent, from its device offsets
+    int64_t ends[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(&ends[0], lp.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&ends[1], lp.ptr + m, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (ends[0] != 0) throw std::runtime_error("CSR offsets must start at 0");
+    const long long nnz = ends[1];
+    if (nnz < 0 || nnz >= (1LL << 31)) throw std::runtime_error("Device scaling needs 0 <= nonzeros < 2^31");
+    if (nnz && (!lp.index || !lp.value || !out.t_index || !out.t_value)) throw std::runtime_error("Null matrix array");
+    const int cmax_blocks = grid_for(n);
+    const long long nsums = std::max<long long>(1, (n + kBlock * kScanItems - 1) / (kBlock * kScanItems));
+    const ScratchPlan plan(n, nnz, cmax_blocks, nsums);
+    if (plan.end > kScratchBytes)
+        throw std::runtime_error("LP needs " + std::to_string(plan.end) + " bytes of scratch; the no-allocation entry "
+                                 "point holds " + std::to_string(kScratchBytes) + " (24 bytes per nonzero + 8 per column)");
+    auto *flags = reinterpret_cast<unsigned int *>(base + plan.flags);
+    auto *work = reinterpret_cast<unsigned long long *>(base + plan.work);
+    auto *osc = reinterpret_cast<double *>(base + plan.osc);
+    auto *cmax = reinterpret_cast<double *>(base + plan.cmax);
+    auto *count = reinterpret_cast<unsigned long long *>(base + plan.count);
+    auto *sums = reinterpret_cast<int64_t *>(base + plan.sums);
+    auto *slots = reinterpret_cast<int64_t *>(base + plan.slots);
+    auto *perm = reinterpret_cast<int64_t *>(base + plan.perm);
+    auto *row = reinterpret_cast<int64_t *>(base + plan.row);
+
+    // flags and the degree sum start at zero on every call (fill kernels:
+    // the scratch keeps the previous call's contents)
+    fill_u64_kernel<<<1, kBlock, 0, s>>>(reinterpret_cast<unsigned long long *>(flags), 2, 0);
+    fill_u64_kernel<<<1, kBlock, 0, s>>>(work, 1, 0);
+    validate_kernel<<<grid_for(std::max(m, nnz)), kBlock, 0, s>>>(lp.ptr, m, lp.index, lp.value, nnz, n, flags);
+    HIP_OK(hipGetLastError());
+    unsigned int seen[3] = {0, 0, 0};
+    HIP_OK(hipMemcpyAsync(seen, flags, sizeof seen, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (seen[0]) throw std::runtime_error("CSR offsets not monotone");
+    if (seen[1]) throw std::runtime_error("CSR column index out of range");
+    if (seen[2]) throw std::runtime_error("Non-finite coefficient");
+
+    // outputs start as copies of the inputs and are scaled in place
+    auto copy = [&](double *dst, const double *src, long long count_) {
+        if (count_) HIP_OK(hipMemcpyAsync(dst, src, size_t(count_) * 8, hipMemcpyDeviceToDevice, s));
+    };
+    copy(out.value, lp.value, nnz);
+    copy(out.c, lp.c, n);
+    copy(out.lb, lp.lb, n);
+    copy(out.ub, lp.ub, n);
+    copy(out.rl, lp.rl, m);
+    copy(out.ru, lp.ru, m);
+    if (m) fill_f64_kernel<<<grid_for(m), kBlock, 0, s>>>(out.row_scale, m, 1.0);
+    if (n) fill_f64_kernel<<<grid_for(n), kBlock, 0, s>>>(out.column_scale, n, 1.0);
+    if (n) fill_u64_kernel<<<grid_for(n), kBlock, 0, s>>>(count, n, 0);
+    if (nnz) count_columns_kernel<<<grid_for(nnz), kBlock, 0, s>>>(lp.index, nnz, count);
+    if (n) degree_work_kernel<<<grid_for(n), kBlock, 0, s>>>(count, n, work);
+    HIP_OK(hipGetLastError());
+    unsigned long long rank_work = 0;
+    HIP_OK(hipMemcpyAsync(&rank_work, work, sizeof rank_work, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (double(rank_work) > kRankBudget) throw std::runtime_error("Columns too long for the device transpose");
+
+    int64_t *tptr = out.t_ptr;
+    double *val = out.value;
+    scan_blocks_kernel<<<int(nsums), kBlock, 0, s>>>(count, tptr, n, sums);
+    scan_sums_kernel<<<1, kBlock, 0, s>>>(sums, nsums);
+    scan_add_kernel<<<grid_for(n + 1), kBlock, 0, s>>>(tptr, n, sums, int64_t(nnz));
+    if (n) fill_u64_kernel<<<grid_for(n), kBlock, 0, s>>>(count, n, 0);
+    if (nnz) {
+        place_kernel<<<grid_for(nnz), kBlock, 0, s>>>(lp.index, nnz, tptr, count, slots);
+        rank_kernel<<<grid_for(nnz), kBlock, 0, s>>>(lp.index, nnz, tptr, slots, perm);
+    }
+    if (m) row_of_kernel<<<wave_grid(m), kBlock, 0, s>>>(lp.ptr, m, row);
+    for (int pass = 0; pass < passes; ++pass) {
+        if (m) ruiz_rows_kernel<<<wave_grid(m), kBlock, 0, s>>>(lp.ptr, val, m, out.rl, out.ru, out.row_scale);
+        if (n)
+            ruiz_columns_kernel<<<wave_grid(n), kBlock, 0, s>>>(tptr, perm, val, n, out.c, out.lb, out.ub,
+                                                                out.column_scale);
+    }
+    if (passes > 0) {
+        abs_max_kernel<<<cmax_blocks, kBlock, 0, s>>>(out.c, n, cmax);
+        objective_scale_kernel<<<1, kBlock, 0, s>>>(cmax, cmax_blocks, osc);
+        if (n) divide_kernel<<<grid_for(n), kBlock, 0, s>>>(out.c, n, osc);
+    }
+    if (nnz) gather_transpose_kernel<<<grid_for(nnz), kBlock, 0, s>>>(perm, nnz, row, val, out.t_index, out.t_value);
+    HIP_OK(hipGetLastError());
+    double scale = 1.0;
+    if (passes > 0) HIP_OK(hipMemcpyAsync(&scale, osc, sizeof scale, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    out.objective_scale = scale;
+    out.device_seconds = 0;
+    out.seconds = 0;
+}
+
 }  // namespace
 
 extern "C" {
 
 const char *npdhg_scale_last_error(void) { return g_error.c_str(); }
+
+int npdhg_scale_lp_noalloc(const npdhg_lp *lp, int passes, npdhg_scaled *out, void *stream) {
+    try {
+        g_error.clear();
+        if (!lp || !out) throw std::runtime_error("Null argument");
+        scale_lp_noalloc(*lp, passes, *out, static_cast<hipStream_t>(stream));
+        return 0;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return -1;
+    } catch (...) {
+        g_error = "unknown native error";
+        return -1;
+    }
+}
 
 int npdhg_scale_lp(const npdhg_lp *lp, int passes, npdhg_scaled *out) {
     try {
