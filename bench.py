@@ -236,7 +236,7 @@ def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16,
                         verbose=int(verbose))
     attempts = r["iterations"] + r["rejected_steps"]
     s_p = 4 if nnz < 2 ** 31 else 8
-    s_v = 4 if matrix_precision == "mixed" else 8
+    s_v = 4 if matrix_precision in ("mixed", "fp32") else 8
     # algorithmic bytes of one attempt (s_v-byte values, int32 indices), original sizes
     b_attempt = 2 * nnz * (s_v + 4) + (n + m + 2) * s_p + m * 8 + n * 8 + 64 * (n + m)
     achieved = b_attempt * attempts / max(r["iteration_seconds"], 1e-12) / 1e9
@@ -248,6 +248,7 @@ def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16,
            "preprocess_seconds": round(r["preprocess_seconds"], 3),
            "verification_seconds": round(r["verification_seconds"], 3),
            "ms_per_attempt": round(1e3 * r["iteration_seconds"] / max(attempts, 1), 4),
+           "fp32_iterations": r["fp32_iterations"],
            "algorithmic_bytes_per_attempt": b_attempt,
            "achieved_GBs": round(achieved, 1), "hbm_frac": round(achieved / HBM_PEAK_GBS, 4)}
     ref = ROOT / "oracle" / "_ref" / "ref_driver"
@@ -400,8 +401,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-iterations", type=int, default=None)
     ap.add_argument("--verbose", action="store_true", help="single-LP mode: one progress line per check period")
-    ap.add_argument("--matrix-precision", default="fp64", choices=("fp64", "mixed"),
-                    help="single-LP mode: FP64 or FP32 matrix values (FP64 accumulation and iterates)")
+    ap.add_argument("--matrix-precision", default="fp64", choices=("fp64", "mixed", "fp32"),
+                    help="single-LP mode: FP64; mixed = FP32 matrix values (FP64 accumulation and iterates); "
+                         "fp32 = FP32 SpMV arithmetic over FP64 iterates, then FP64 refinement")
     ap.add_argument("--qps", type=float, default=1.5)
     ap.add_argument("--duration", type=float, default=40.0)
     args = ap.parse_args()
@@ -464,9 +466,11 @@ def main():
     dom = max(range(groups), key=lambda c: cls_b[c])
     adv_b, adv_s, launches = cls_b[dom], cls_s[dom], cls_n[dom]
     achieved = adv_b / adv_s / 1e9 if adv_s > 0 else 0.0
-    rows = int(os.environ.get("NIRYUKTI_BATCH_ROWS_PER_THREAD", "1"))
-    symbol = [f"batch_advance_lds_kernel<256, {rows}>", "batch_advance_lds_kernel<1024, 1>",
-              "batch_advance_lds_kernel<1024, 1>", "batch_advance_kernel", "batch_tail_kernel<1024>"][dom]
+    last_stats = stats_all[-1]
+    shapes = [f"batch_advance_lds_kernel<{last_stats['class_threads'][c]}, {last_stats['class_rows'][c]}>"
+              for c in range(3)]
+    symbol = shapes + ["batch_advance_kernel", "batch_tail_kernel<1024>"]
+    symbol = symbol[dom]
     kernel_name = symbol + [" (class m+n <= 2048, LDS)", " (class m+n <= 4096, LDS)", " (class m+n <= 8000, LDS)",
                             " (global class)", " (tail group: attempts and checks in one launch)"][dom]
     traffic = committed_traffic(symbol, [0, 0, 1, 0, 0][dom])
@@ -499,6 +503,8 @@ def main():
         "unsolved_messages": sorted({r["message"][:120] for r in last if r["status"] != "OPTIMAL" and r["message"]})[:5],
         "iterations": {"mean": float(its.mean()), "max": int(its.max())},
         "handoffs": int(stats_all[-1].get("handoffs", 0)),
+        "launch_shapes": {f"class{c}": {"threads": last_stats["class_threads"][c], "rows": last_stats["class_rows"][c],
+                                        "tuned": bool(last_stats["class_tuned"][c])} for c in range(3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": TRAFFIC_PROFILE.name if traffic is not None else None,

@@ -44,6 +44,10 @@ typedef struct {
     int64_t class_launches[5];
     int64_t handoffs;         /* LPs moved to the grid-wide engine so far */
     int64_t handoffs_running; /* of those, still running there */
+    /* Launch shape of each crowded LDS class's step kernel (threads per
+     * workgroup, rows per thread): the autotuner's current choice, and 1 once
+     * its trials are over.  Shapes never change results. */
+    int32_t class_threads[3], class_rows[3], class_tuned[3], pad;
 } niryukti_batch_stats;
 
 /* opts: solver options shared by every LP of the batch (NULL = defaults).

@@ -34,7 +34,7 @@ typedef struct {
     const char *method;           /* "pdhg" | "halpern" | "rhpdhg" | "r2hpdhg" */
     const char *scaling;          /* "ruiz" | "combined" */
     const char *primal_weight;    /* "displacement" | "pid" */
-    const char *matrix_precision; /* "fp64" | "mixed" */
+    const char *matrix_precision; /* "fp64" | "mixed" (FP32 values) | "fp32" (FP32 SpMV arithmetic, FP64 refinement) */
     const char *gpu_indices;      /* "auto" | "32" | "64" */
     double tol, time_limit;
     int64_t iteration_limit;
@@ -71,6 +71,7 @@ typedef struct {
      * right after it (both 0 when no restart happened). */
     double initial_step, initial_weight, first_restart_weight;
     int64_t first_restart_iteration;
+    int64_t fp32_iterations; /* matrix_precision "fp32": iterations before the FP64 refinement phase */
     double *x; /* NULL or cols: receives the primal solution */
     double *y; /* NULL or rows: receives the dual solution */
 } niryukti_result;

@@ -39,6 +39,9 @@ typedef struct {
     int graphs;            /* replay chunks as hipGraphs */
     int index_bits;        /* 0 = auto, 32 or 64 (device CSR offsets) */
     int fp32_values;       /* 1 = FP32 matrix values, FP64 accumulation and iterates */
+    int fp32_arithmetic;   /* 1 = FP32 values, gathered vectors and per-lane sums in both products
+                              (iterates, steps and reductions FP64); FP64 values kept for the
+                              refinement phase (npdhg_set_fp32_phase) */
 } npdhg_config;
 
 /* Status of the last advance. */
@@ -47,6 +50,8 @@ typedef struct {
     int restart_requested;
     int index_bits;
     double factor; /* current adaptive step factor */
+    int fp32_phase; /* 1 while the attempts run in FP32 arithmetic */
+    int pad;
 } npdhg_status;
 
 const char *npdhg_last_error(void);
@@ -83,6 +88,13 @@ int npdhg_reset(npdhg_backend *b, const double *x, const double *y);
  * candidate without an upload.  Replaces IterationBackend::reset_candidate
  * (src/cuda.cu:892-930). */
 int npdhg_reset_candidate(npdhg_backend *b, int average);
+
+/* Moves a backend created with fp32_arithmetic between its FP32 arithmetic
+ * phase (on = 1, where it starts) and FP64 arithmetic (on = 0: the FP64
+ * refinement phase); the iterates, averages, step factor and counters carry
+ * on.  There is no counterpart in the reference, whose GPU path runs one
+ * precision for a whole solve (src/cuda.cu:142-160). */
+int npdhg_set_fp32_phase(npdhg_backend *b, int on);
 
 /* Resumable backend state for checkpoints: the current iterate, averages,
  * activity, Halpern anchors, step factor, average mass, residual history and

@@ -61,7 +61,7 @@ class _Result(C.Structure):
                 ("index_bits", C.c_int), ("device_prepare", C.c_int), ("operator_norm_estimate", C.c_double),
                 ("polishing_iterations", C.c_int64), ("polishing_attempts", C.c_int64),
                 ("initial_step", C.c_double), ("initial_weight", C.c_double), ("first_restart_weight", C.c_double),
-                ("first_restart_iteration", C.c_int64), ("x", _f64p), ("y", _f64p)]
+                ("first_restart_iteration", C.c_int64), ("fp32_iterations", C.c_int64), ("x", _f64p), ("y", _f64p)]
 
 
 class _Accuracy(C.Structure):
@@ -77,7 +77,7 @@ class _PdhgLP(C.Structure):
 class _PdhgConfig(C.Structure):
     _fields_ = [("adaptive", C.c_int), ("halpern", C.c_int), ("reflection", C.c_double),
                 ("residual_restarts", C.c_int), ("graphs", C.c_int), ("index_bits", C.c_int),
-                ("fp32_values", C.c_int)]
+                ("fp32_values", C.c_int), ("fp32_arithmetic", C.c_int)]
 
 
 class _PdhgOriginal(C.Structure):
@@ -94,7 +94,7 @@ class _PdhgAccuracy(C.Structure):
 
 class _PdhgStatus(C.Structure):
     _fields_ = [("accepted_total", C.c_int64), ("rejected_total", C.c_int64), ("restart_requested", C.c_int),
-                ("index_bits", C.c_int), ("factor", C.c_double)]
+                ("index_bits", C.c_int), ("factor", C.c_double), ("fp32_phase", C.c_int), ("pad", C.c_int)]
 
 
 class _Scaled(C.Structure):
@@ -110,7 +110,9 @@ class _BatchStats(C.Structure):
                                          "host_farkas_checks")] + \
         [(k, C.c_double) for k in ("device_seconds", "advance_seconds", "check_seconds", "advance_bytes")] + \
         [("attempts", C.c_int64), ("class_seconds", C.c_double * 5), ("class_bytes", C.c_double * 5),
-         ("class_launches", C.c_int64 * 5), ("handoffs", C.c_int64), ("handoffs_running", C.c_int64)]
+         ("class_launches", C.c_int64 * 5), ("handoffs", C.c_int64), ("handoffs_running", C.c_int64),
+         ("class_threads", C.c_int32 * 3), ("class_rows", C.c_int32 * 3), ("class_tuned", C.c_int32 * 3),
+         ("pad", C.c_int32)]
 
 
 _lib = None
@@ -144,6 +146,7 @@ def load_library() -> C.CDLL:
     lib.npdhg_candidates.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p]
     lib.npdhg_reset.argtypes = [C.c_void_p, _f64p, _f64p]
     lib.npdhg_reset_candidate.argtypes = [C.c_void_p, C.c_int]
+    lib.npdhg_set_fp32_phase.argtypes = [C.c_void_p, C.c_int]
     lib.npdhg_spmv.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p]
     lib.niryukti_prepare_lp.argtypes = [C.POINTER(_LP), C.POINTER(_Options), C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int64), _i64p, _i64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
@@ -483,13 +486,13 @@ class Backend:
     """Direct handle on the HIP iteration backend (the npdhg_* seam)."""
 
     def __init__(self, lp: LP, x0, y0, adaptive=True, halpern=False, reflection=0.0, residual_restarts=False,
-                 graphs=True, index_bits=0, fp32_values=False):
+                 graphs=True, index_bits=0, fp32_values=False, fp32_arithmetic=False):
         self._lib = load_library()
         self.lp = lp
         self._s = _PdhgLP(lp.rows, lp.cols, _ptr(lp.ptr, _i64p), _ptr(lp.index, _i64p), _ptr(lp.value),
                           _ptr(lp.c), _ptr(lp.lb), _ptr(lp.ub), _ptr(lp.rl), _ptr(lp.ru))
         cfg = _PdhgConfig(int(adaptive), int(halpern), float(reflection), int(residual_restarts), int(graphs),
-                          int(index_bits), int(fp32_values))
+                          int(index_bits), int(fp32_values), int(fp32_arithmetic))
         x0, y0 = _f64(x0), _f64(y0)
         h = C.c_void_p()
         self._check(self._lib.npdhg_create(C.byref(self._s), _ptr(x0), _ptr(y0), C.byref(cfg), C.byref(h)))
@@ -518,6 +521,10 @@ class Backend:
     def reset(self, x, y):
         x, y = _f64(x), _f64(y)
         self._check(self._lib.npdhg_reset(self._h, _ptr(x), _ptr(y)))
+
+    def set_fp32_phase(self, on: bool):
+        """FP32 arithmetic phase on / off (backends created with fp32_arithmetic)."""
+        self._check(self._lib.npdhg_set_fp32_phase(self._h, int(bool(on))))
 
     def reset_candidate(self, average: bool):
         self._check(self._lib.npdhg_reset_candidate(self._h, int(bool(average))))

@@ -227,6 +227,7 @@ int niryukti_solve_lp(const niryukti_lp *lp, const niryukti_options *opts, niryu
         out->initial_weight = r.initial_weight;
         out->first_restart_weight = r.first_restart_weight;
         out->first_restart_iteration = r.first_restart_iteration;
+        out->fp32_iterations = r.fp32_iterations;
         if (out->x && r.x.size() == size_t(lp->cols)) std::memcpy(out->x, r.x.data(), r.x.size() * sizeof(double));
         if (out->y && r.y.size() == size_t(lp->rows)) std::memcpy(out->y, r.y.data(), r.y.size() * sizeof(double));
         return 0;

@@ -124,6 +124,10 @@ void write_checkpoint(const std::string &path, const Model &original, const Json
     extra["best_certified"] = s.best_certified;
     extra["best_kkt"] = pack({s.best_kkt});
     extra["have_best"] = s.have_best;
+    extra["fp32_phase"] = s.fp32_phase;
+    extra["fp32_best"] = pack({s.fp32_best});
+    extra["fp32_since"] = (long long)s.fp32_since;
+    extra["fp32_iterations"] = (long long)s.fp32_iterations;
     doc["engine"] = extra;
     const std::string temporary = path + ".tmp";
     {
@@ -199,6 +203,13 @@ LoopState read_checkpoint(const std::string &path, const Model &original, const 
         throw std::runtime_error("Invalid continuous checkpoint extras");
     s.best_certified = extra.at("best_certified").boolean();
     s.have_best = extra.at("have_best").boolean();
+    if (!extra.at("fp32_phase").is_bool()) throw std::runtime_error("Invalid continuous checkpoint extras");
+    s.fp32_phase = extra.at("fp32_phase").boolean();
+    const auto fb = unpack(extra.at("fp32_best"));
+    if (fb.size() != 1) throw std::runtime_error("Invalid continuous checkpoint extras");
+    s.fp32_best = fb[0];
+    s.fp32_since = count_of(extra, "fp32_since");
+    s.fp32_iterations = count_of(extra, "fp32_iterations");
     if (s.best_x.size() != size_t(original.cols()) || s.best_y.size() != size_t(original.rows()))
         throw std::runtime_error("Continuous checkpoint solution dimensions");
     return s;

@@ -26,6 +26,9 @@ struct LoopState {
     bool best_certified = true;  // host checks: the best pair's figures include the certified bound
     double best_kkt = 0;         // device checks: rel-KKT of the best candidate on the device
     bool have_best = false;      // device checks: the device holds a best candidate
+    bool fp32_phase = false;     // fp32 precision: still in the FP32 arithmetic phase
+    double fp32_best = kInf;     // fp32 precision: best rel-KKT seen in that phase
+    int64_t fp32_since = 0, fp32_iterations = 0;
 };
 
 // The solver settings a checkpoint belongs to (a resumed solve must use the

@@ -64,6 +64,47 @@ class WeightPid {
     }
 };
 
+// --matrix-precision fp32: the attempts start in FP32 arithmetic (FP32
+// values, gathered vectors and per-lane sums; FP64 iterates, steps, checks
+// and decisions) and move to FP64 for the rest of the solve, the refinement
+// phase, at the first check whose candidate reaches kSwitchKkt (about a
+// hundred times the rel-KKT resolution of FP32 sums) or after the FP32 phase
+// has gone kStallChecks checks without halving its best rel-KKT.
+class Refinement {
+  public:
+    static constexpr double kSwitchKkt = 1e-5;
+    static constexpr int kStallChecks = 20;
+    explicit Refinement(const Options &o) : active_(o.matrix_precision == "fp32") {}
+    // After a check with candidate rel-KKT `kkt`: true when the refinement phase starts now.
+    bool switch_now(double kkt) {
+        if (!active_) return false;
+        if (kkt < 0.5 * best_) {
+            best_ = kkt;
+            since_ = 0;
+        } else {
+            ++since_;
+        }
+        if (kkt <= kSwitchKkt || since_ >= kStallChecks) {
+            active_ = false;
+            return true;
+        }
+        return false;
+    }
+    bool in_fp32() const { return active_; }
+    void restore(bool fp32, double best, int since) {
+        active_ = fp32;
+        best_ = best;
+        since_ = since;
+    }
+    double best() const { return best_; }
+    int since() const { return since_; }
+
+  private:
+    bool active_;
+    double best_ = kInf;
+    int since_ = 0;
+};
+
 // RAII owner of a device backend.
 class Device {
   public:
@@ -138,6 +179,7 @@ class Device {
         dy = b;
     }
     void set_epoch(bool average) { check(npdhg_set_epoch(b_, average ? 1 : 0)); }
+    void leave_fp32_phase() { check(npdhg_set_fp32_phase(b_, 0)); }
     std::vector<double> snapshot() {
         int64_t size = 0;
         check(npdhg_snapshot_size(b_, &size));
@@ -361,6 +403,7 @@ void run_device_checks(const Model &original, const Prepared &prep, const Option
     double best_kkt = r.accuracy.kkt;
     bool have_best = false;
     Polish polish;
+    Refinement refine(o);
     if (ck.resuming()) {
         LoopState s = ck.load();
         r.x = std::move(s.best_x);
@@ -374,6 +417,9 @@ void run_device_checks(const Model &original, const Prepared &prep, const Option
         polish.next = s.next_polish;
         best_kkt = s.best_kkt;
         have_best = s.have_best;
+        refine.restore(s.fp32_phase, s.fp32_best, int(s.fp32_since));
+        if (o.matrix_precision == "fp32" && !s.fp32_phase) backend.leave_fp32_phase();
+        r.fp32_iterations = s.fp32_iterations;
         controller.restore(s.pid);
         r.iterations = s.iterations;
         r.restarts = s.restarts;
@@ -410,6 +456,10 @@ void run_device_checks(const Model &original, const Prepared &prep, const Option
             s.first_restart_weight = r.first_restart_weight;
             s.best_kkt = best_kkt;
             s.have_best = have_best;
+            s.fp32_phase = refine.in_fp32();
+            s.fp32_best = refine.best();
+            s.fp32_since = refine.since();
+            s.fp32_iterations = r.fp32_iterations;
             ck.save(s);
         }
         if (stop_requested(o)) {
@@ -446,6 +496,8 @@ void run_device_checks(const Model &original, const Prepared &prep, const Option
             have_best = true;
         }
         r.verification_seconds += elapsed(tick);
+        if (refine.in_fp32()) r.fp32_iterations = r.iterations;
+        if (refine.switch_now(ca.kkt)) backend.leave_fp32_phase();
         if (o.verbose)
             std::fprintf(stderr, "iter=%lld objective=%.12g primal=%.3e dual=%.3e gap=%.3e seconds=%.3f\n",
                          (long long)r.iterations, original.sense * ca.objective, ca.primal, ca.dual, ca.gap,
@@ -669,6 +721,7 @@ Result solve_continuous(const Model &original, const Options &o) {
     cfg.graphs = o.graphs;
     cfg.index_bits = o.gpu_indices == "64" ? 64 : o.gpu_indices == "32" ? 32 : 0;
     cfg.fp32_values = o.matrix_precision == "mixed";
+    cfg.fp32_arithmetic = o.matrix_precision == "fp32";
     Device backend(m, x, y, cfg, transposed);
     transposed = Csr{};
     r.transfer_seconds = elapsed(transfer);
@@ -691,6 +744,7 @@ Result solve_continuous(const Model &original, const Options &o) {
     std::vector<double> epochx = x, epochy = y;
     std::vector<double> cx(m.cols()), cy(m.rows()), ax(m.cols()), ay(m.rows());
     Polish polish;
+    Refinement refine(o);
     bool best_certified = true;  // r holds the start point's figures, certified bound included
     if (ck.resuming()) {
         LoopState s = ck.load();
@@ -709,6 +763,9 @@ Result solve_continuous(const Model &original, const Options &o) {
         since_restart = s.since_restart;
         previous_rejected = s.previous_rejected;
         polish.next = s.next_polish;
+        refine.restore(s.fp32_phase, s.fp32_best, int(s.fp32_since));
+        if (o.matrix_precision == "fp32" && !s.fp32_phase) backend.leave_fp32_phase();
+        r.fp32_iterations = s.fp32_iterations;
         controller.restore(s.pid);
         r.iterations = s.iterations;
         r.restarts = s.restarts;
@@ -746,6 +803,10 @@ Result solve_continuous(const Model &original, const Options &o) {
             s.polishing_attempts = r.polishing_attempts;
             s.first_restart_iteration = r.first_restart_iteration;
             s.first_restart_weight = r.first_restart_weight;
+            s.fp32_phase = refine.in_fp32();
+            s.fp32_best = refine.best();
+            s.fp32_since = refine.since();
+            s.fp32_iterations = r.fp32_iterations;
             ck.save(s);
         }
         if (stop_requested(o)) {
@@ -795,6 +856,8 @@ Result solve_continuous(const Model &original, const Options &o) {
             best_certified = false;
         }
         r.verification_seconds += elapsed(tick);
+        if (refine.in_fp32()) r.fp32_iterations = r.iterations;
+        if (refine.switch_now(ca.kkt)) backend.leave_fp32_phase();
         if (o.verbose)
             std::fprintf(stderr, "iter=%lld objective=%.12g primal=%.3e dual=%.3e gap=%.3e seconds=%.3f\n",
                          (long long)r.iterations, original.sense * ca.objective, ca.primal, ca.dual, ca.gap,
@@ -912,9 +975,9 @@ void validate_options(const Options &o) {
     if (o.scaling != "ruiz" && o.scaling != "combined") throw std::runtime_error("Unknown scaling: use ruiz or combined");
     if (o.gpu_indices != "auto" && o.gpu_indices != "32" && o.gpu_indices != "64")
         throw std::runtime_error("GPU indices must be auto, 32 or 64");
-    if (o.matrix_precision != "fp64" && o.matrix_precision != "mixed")
-        throw std::runtime_error("Matrix precision must be fp64 or mixed");
-    if (anchored(o) && o.matrix_precision == "mixed") throw std::runtime_error("Fixed Halpern operators require FP64 matrices");
+    if (o.matrix_precision != "fp64" && o.matrix_precision != "mixed" && o.matrix_precision != "fp32")
+        throw std::runtime_error("Matrix precision must be fp64, mixed or fp32");
+    if (anchored(o) && o.matrix_precision != "fp64") throw std::runtime_error("Fixed Halpern operators require FP64 matrices");
 }
 
 // src/solver.cpp:55-105 for continuous LPs.

@@ -538,7 +538,9 @@ Json result_json(const Model &m, const Result &r) {
     Json hw = Json::object();
     hw["backend"] = r.backend;
     hw["device"] = r.device_name;
-    hw["precision"] = r.matrix_precision == "mixed" ? "FP32 matrix values, FP64 iterates" : "FP64";
+    hw["precision"] = r.matrix_precision == "mixed" ? "FP32 matrix values, FP64 iterates"
+                      : r.matrix_precision == "fp32" ? "FP32 SpMV arithmetic, FP64 iterates, FP64 refinement"
+                                                     : "FP64";
     j["hardware"] = std::move(hw);
     Json pre = Json::object();
     pre["removed_rows"] = (long long)r.removed_rows;
@@ -568,6 +570,7 @@ Json result_json(const Model &m, const Result &r) {
     gx["index_bits"] = r.gpu_index_bits;
     gx["cuda_graphs"] = r.graph_execution;
     gx["matrix_precision"] = r.matrix_precision;
+    if (r.matrix_precision == "fp32") gx["fp32_iterations"] = (long long)r.fp32_iterations;
     gx["device_prepare"] = r.device_prepare;
     j["gpu_execution"] = std::move(gx);
     if (!r.infeasibility_ray.empty()) {

@@ -108,6 +108,9 @@ struct Result {
     // (src/solver.cpp:692-725; 0 and 0 when no restart happened).
     double initial_step = 0, initial_weight = 0, first_restart_weight = 0;
     int64_t first_restart_iteration = 0;
+    // --matrix-precision fp32: iterations run in FP32 arithmetic before the
+    // FP64 refinement phase (all of them when it never started)
+    int64_t fp32_iterations = 0;
     std::string matrix_precision = "fp64", method_selected, device_reason;
     double estimated_gpu_bytes = 0;
     // The request as the caller made it (reported under "selection").

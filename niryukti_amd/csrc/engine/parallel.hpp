@@ -5,7 +5,10 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
+#include <exception>
+#include <stdexcept>
 #include <thread>
+#include <utility>
 #include <vector>
 
 namespace nyk {
@@ -53,5 +56,44 @@ void parallel_tasks(int64_t count, F &&body) {
         });
     for (auto &th : pool) th.join();
 }
+
+// One task on a thread of its own beside the caller's work (the batched
+// engine certifies a round's stopped LPs this way while the next round runs).
+// wait() joins the task and rethrows what it threw; the task must touch only
+// data the caller leaves alone until wait() returns.
+class Beside {
+  public:
+    Beside() = default;
+    Beside(const Beside &) = delete;
+    Beside &operator=(const Beside &) = delete;
+    ~Beside() {
+        if (thread_.joinable()) thread_.join();
+    }
+    template <class F>
+    void start(F &&task) {
+        if (thread_.joinable()) throw std::logic_error("a task is already running beside");
+        error_ = nullptr;
+        thread_ = std::thread([this, f = std::forward<F>(task)]() mutable {
+            try {
+                f();
+            } catch (...) {
+                error_ = std::current_exception();
+            }
+        });
+    }
+    bool running() const { return thread_.joinable(); }
+    void wait() {
+        if (thread_.joinable()) thread_.join();
+        if (error_) {
+            std::exception_ptr e = error_;
+            error_ = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+
+  private:
+    std::thread thread_;
+    std::exception_ptr error_;
+};
 
 }  // namespace nyk

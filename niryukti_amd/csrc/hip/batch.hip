@@ -1414,6 +1414,55 @@ struct Layout {
     }
 };
 
+// Launch shapes of a crowded LDS class's step kernel: threads per workgroup
+// and rows per thread.  Every shape gives the same results (tile-ordered
+// sums), so the choice is free to follow measurements.
+struct Shape {
+    int threads, rows;
+};
+constexpr Shape kShapes[] = {{256, 1}, {256, 2}, {256, 4}, {1024, 1}};
+constexpr int kShapeCount = int(sizeof kShapes / sizeof kShapes[0]);
+
+// Autotuner of one crowded LDS class: the class's first scheduling rounds
+// try each shape once (every LP of the class in the launch, so the rounds
+// still do full work), then the class keeps the shape with the lowest
+// step-kernel time per attempt, measured by HIP events.  A round whose class
+// ran fewer than kMinTrialLps LPs does not count as a trial.
+class ShapeTuner {
+  public:
+    static constexpr int kMinTrialLps = 64;
+    void fix(int shape) {
+        fixed_ = true;
+        chosen_ = shape;
+    }
+    bool decided() const { return fixed_ || tried_ == kShapeCount; }
+    int next() const { return decided() ? chosen_ : order_[tried_]; }
+    void record(int shape, double seconds, long long attempts, int lps) {
+        if (decided() || shape != order_[tried_] || lps < kMinTrialLps || attempts <= 0) return;
+        per_attempt_[shape] = seconds / double(attempts);
+        if (++tried_ == kShapeCount) {
+            chosen_ = 0;
+            for (int k = 1; k < kShapeCount; ++k)
+                if (per_attempt_[k] < per_attempt_[chosen_]) chosen_ = k;
+        }
+    }
+    double per_attempt(int shape) const { return per_attempt_[shape]; }
+    // The measured-best shape is tried first, so a short batch already runs it.
+    void start_with(int shape) {
+        chosen_ = shape;
+        int k = 0;
+        order_[k++] = shape;
+        for (int s = 0; s < kShapeCount; ++s)
+            if (s != shape) order_[k++] = s;
+    }
+
+  private:
+    bool fixed_ = false;
+    int tried_ = 0, chosen_ = 0;
+    int order_[kShapeCount] = {0, 1, 2, 3};
+    double per_attempt_[kShapeCount] = {};
+};
+
 class BatchEngine {
   public:
     BatchEngine(const nyk::Options &o, int64_t arena_bytes, int64_t max_slots) : opt_(o) {
@@ -1428,7 +1477,18 @@ class BatchEngine {
         auto &pool = stream_pool();
         stream_ = pool[0];
         for (int c = 0; c <= kClasses; ++c) class_stream_[c] = pool[1 + c];
-        if (const char *env = std::getenv("NIRYUKTI_BATCH_ROWS_PER_THREAD")) rows_per_thread_ = std::atoi(env);
+        // Shapes measured best on gfx950 (tried first): 256 threads x 1 row
+        // up to m + n = 2,048, 1,024 x 1 above.  NIRYUKTI_BATCH_ROWS_PER_THREAD
+        // (1, 2 or 4) fixes the smallest class at 256 threads x that many rows;
+        // NIRYUKTI_BATCH_TUNE=0 keeps the measured shapes without trials.
+        for (int c = 0; c < kClasses - 1; ++c) tuner_[c].start_with(c == 0 ? 0 : 3);
+        if (const char *env = std::getenv("NIRYUKTI_BATCH_ROWS_PER_THREAD")) {
+            const int r = std::atoi(env);
+            tuner_[0].fix(r == 4 ? 2 : r == 2 ? 1 : 0);
+        }
+        if (const char *env = std::getenv("NIRYUKTI_BATCH_TUNE"))
+            if (std::atoi(env) == 0)
+                for (int c = 0; c < kClasses - 1; ++c) tuner_[c].fix(c == 0 ? tuner_[0].next() : 3);
         // Class capacities (m + n) bounded by what one workgroup may allocate
         // next to the kernel's static LDS.
         int device = 0;
@@ -1475,7 +1535,12 @@ class BatchEngine {
         for (int s = max_slots_ - 1; s >= 0; --s) free_slots_.push_back(s);
     }
     ~BatchEngine() {
-        if (settling_) settling_->worker.join();
+        if (settling_) {
+            try {
+                settling_->task.wait();
+            } catch (...) {
+            }
+        }
         {
             std::lock_guard<std::mutex> lock(handoff_mu_);
             stopping_ = true;
@@ -1713,6 +1778,8 @@ class BatchEngine {
                 before[k] = state_[active[k]].accepted + state_[active[k]].rejected;
                 checks_before[k] = state_[active[k]].checks;
             }
+            int shape_of[kClasses - 1];
+            for (int c = 0; c < kClasses - 1; ++c) shape_of[c] = tuner_[c].next();
             for (const Group &g : groups) {
                 auto &ev = class_events_[g.kind];
                 while (ev.size() < size_t(2 * checks + 2)) {
@@ -1731,7 +1798,7 @@ class BatchEngine {
                 } else {
                     for (int k = 0; k < checks; ++k) {
                         if (g.kind < kClasses - 1)
-                            launch_lds(g.cap, g.kind >= 1, grid, cs, slots, bo);
+                            launch_lds(g.cap, shape_of[g.kind], grid, cs, slots, bo);
                         else
                             batch_advance_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
                         HIP_OK(hipEventRecord(ev[2 * k + 1], cs));
@@ -1765,15 +1832,30 @@ class BatchEngine {
                 }
                 for (int s : g.slots) slot_group_[s] = g.kind;
             }
+            double group_seconds[kClasses + 1] = {};
+            long long group_attempts[kClasses + 1] = {};
+            int group_lps[kClasses + 1] = {};
+            for (const Group &g : groups) group_lps[g.kind] = int(g.slots.size());
+            for (const Group &g : groups) {
+                auto &ev = class_events_[g.kind];
+                for (int k = 0; k < (g.kind == kClasses ? 0 : checks); ++k) {
+                    float a = 0;
+                    HIP_OK(hipEventElapsedTime(&a, ev[2 * k], ev[2 * k + 1]));
+                    group_seconds[g.kind] += a * 1e-3;
+                }
+            }
             for (size_t k = 0; k < active.size(); ++k) {
                 const LpState &st = state_[active[k]];
                 const long long done = st.accepted + st.rejected - before[k];
+                group_attempts[slot_group_[active[k]]] += done;
                 attempts_ += done;
                 advance_bytes_ += double(done) * slot_bytes_[active[k]];
                 class_bytes_[slot_group_[active[k]]] += double(done) * slot_bytes_[active[k]];
                 if (slot_group_[active[k]] == kClasses)  // the fused tail launch also did the checks
                     class_bytes_[kClasses] += double(st.checks - checks_before[k]) * slot_check_bytes_[active[k]];
             }
+            for (int c = 0; c < kClasses - 1; ++c)
+                if (group_lps[c]) tuner_[c].record(shape_of[c], group_seconds[c], group_attempts[c], group_lps[c]);
             device_seconds_ += std::chrono::duration<double>(Clock::now() - t0).count();
             ++launches_;
         }
@@ -1857,6 +1939,7 @@ class BatchEngine {
         out->initial_weight = r.initial_weight;
         out->first_restart_weight = r.first_restart_weight;
         out->first_restart_iteration = r.first_restart_iteration;
+        out->fp32_iterations = 0;
         if (out->x && r.x.size() == size_t(j.original.cols())) std::memcpy(out->x, r.x.data(), r.x.size() * 8);
         if (out->y && r.y.size() == size_t(j.original.rows())) std::memcpy(out->y, r.y.data(), r.y.size() * 8);
         return true;
@@ -1889,6 +1972,13 @@ class BatchEngine {
             s->class_bytes[c] = class_bytes_[c];
             s->class_launches[c] = class_launches_[c];
         }
+        for (int c = 0; c < kClasses - 1; ++c) {
+            const Shape sh = kShapes[tuner_[c].next()];
+            s->class_threads[c] = sh.threads;
+            s->class_rows[c] = sh.rows;
+            s->class_tuned[c] = tuner_[c].decided() ? 1 : 0;
+        }
+        s->pad = 0;
     }
 
     // Waits for every submitted LP's prepare, then admits what the slots hold.
@@ -2002,16 +2092,12 @@ class BatchEngine {
         batch_tail_kernel<kTailThreads><<<grid, kTailThreads, lds_bytes(cap), cs>>>(
             d_desc_, d_state_, slots, bo, checks, checks * kTailMaxPeriodsPerCheck, tail_budget_ticks_);
     }
-    // A crowded LDS class: 256-thread workgroups with rows_per_thread_ rows
-    // per thread (the two larger classes: kWideThreads threads, one row each).
-    void launch_lds(int cap, bool wide, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
-        if (wide) {
-            launch_lds_t<kWideThreads, 1>(cap, grid, cs, slots, bo);
-            return;
-        }
-        switch (rows_per_thread_) {
-        case 4: launch_lds_t<kThreads, 4>(cap, grid, cs, slots, bo); break;
-        case 2: launch_lds_t<kThreads, 2>(cap, grid, cs, slots, bo); break;
+    // A crowded LDS class in the launch shape its tuner picked.
+    void launch_lds(int cap, int shape, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
+        switch (shape) {
+        case 3: launch_lds_t<kWideThreads, 1>(cap, grid, cs, slots, bo); break;
+        case 2: launch_lds_t<kThreads, 4>(cap, grid, cs, slots, bo); break;
+        case 1: launch_lds_t<kThreads, 2>(cap, grid, cs, slots, bo); break;
         default: launch_lds_t<kThreads, 1>(cap, grid, cs, slots, bo);
         }
     }
@@ -2322,8 +2408,7 @@ class BatchEngine {
         std::vector<int> resume;  // 0 finished, 1 resume, 2 hand off
         std::vector<double> cand_kkt;
         int64_t certifications = 0, farkas_checks = 0;
-        std::thread worker;
-        std::exception_ptr error;
+        nyk::Beside task;  // the certification, when it runs beside the next round
     };
 
     // Collects the LPs that stopped in the round just run (and running LPs
@@ -2501,9 +2586,8 @@ class BatchEngine {
     // Waits for the previous round's certification and applies it.
     void finish_settling() {
         if (!settling_) return;
-        settling_->worker.join();
         std::unique_ptr<Settling> p = std::move(settling_);
-        if (p->error) std::rethrow_exception(p->error);
+        p->task.wait();  // rethrows a certification failure
         apply(*p);
     }
 
@@ -2519,13 +2603,7 @@ class BatchEngine {
             return;
         }
         Settling *raw = p.get();
-        p->worker = std::thread([this, raw] {
-            try {
-                certify(*raw);
-            } catch (...) {
-                raw->error = std::current_exception();
-            }
-        });
+        p->task.start([this, raw] { certify(*raw); });
         settling_ = std::move(p);
     }
 
@@ -2687,7 +2765,7 @@ class BatchEngine {
     CopyPiece *d_pieces_ = nullptr;
     long long stage_capacity_ = 0, piece_capacity_ = 0;
     long long pow_count_ = 0;
-    int rows_per_thread_ = 1;  // measured best on gfx950 for the smallest class (m + n <= 2,048)
+    ShapeTuner tuner_[kClasses - 1];
     int class_cap_[kClasses - 1] = {};
     std::vector<hipEvent_t> class_events_[kClasses + 1];
     hipEvent_t uploaded_ = nullptr;

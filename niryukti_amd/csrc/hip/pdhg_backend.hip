@@ -100,7 +100,9 @@ struct PrimalArgs {
     const double *c, *lb, *ub;
     double *x[2], *xa[2];
     double *dx;
+    float *dx32;           // FP32 arithmetic phase: dx as gathered by A dx
     const double *y[2];
+    const float *y32[2];   // FP32 arithmetic phase: the FP32 copy of y gathered by A^T y
     const double *anchor_x;
     double *partial;
     const double *part;  // chunk partials of A^T y, or null
@@ -112,7 +114,9 @@ struct PrimalArgs {
 struct DualArgs {
     const double *rl, *ru;
     double *y[2], *ya[2], *ax[2];
+    float *y32[2];          // FP32 arithmetic phase: FP32 copy of y, written beside y
     const double *dx;
+    const float *dx32;
     const double *anchor_y, *anchor_ax;
     double *partial;  // 2 per block: |dy|^2, dy.delta
     const double *primal_partial;
@@ -126,8 +130,8 @@ struct DualArgs {
 
 // ---------------------------------------------------------------- device --
 
-template <int G>
-__device__ __forceinline__ double group_sum(double v) {
+template <int G, class T = double>
+__device__ __forceinline__ T group_sum(T v) {
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
@@ -179,18 +183,21 @@ __device__ __forceinline__ double gather_sum(const double *values, int stride, i
 // Segment `blockIdx.x` of a long row: partial sum (and, with ABS, the sum of
 // absolute products), ticket, and in thread 0 of the block that completes the
 // row, the row index and its totals.
-template <bool ABS, class O, class V>
-__device__ bool segment_row(const DevCsr<O, V> &A, const Segments &sg, const double *v, double *lds, long long &row,
+// T is the type of the gathered vector and of each thread's running sum:
+// double, or float in the FP32 arithmetic phase (the partials across the
+// threads of a block, and across segments, are added in double).
+template <bool ABS, class O, class V, class T = double>
+__device__ bool segment_row(const DevCsr<O, V> &A, const Segments &sg, const T *v, double *lds, long long &row,
                             double &total, double &total_abs, int s, int *completed = nullptr) {
     const long long end = sg.end[s];
-    double acc = 0, acca = 0;
+    T acc = 0, acca = 0;
     for (long long k = sg.begin[s] + threadIdx.x; k < end; k += kBlock) {
-        const double p = double(A.val[k]) * v[A.idx[k]];
+        const T p = T(A.val[k]) * v[A.idx[k]];
         acc += p;
         if (ABS) acca += fabs(p);
     }
-    const double part = block_sum(acc, lds);
-    const double parta = ABS ? block_sum(acca, lds) : 0.0;
+    const double part = block_sum(double(acc), lds);
+    const double parta = ABS ? block_sum(double(acca), lds) : 0.0;
     __shared__ int completes;  // long-row slot this block completes, else -1
     if (threadIdx.x == 0) {
         store_shared(&sg.partial[2 * s], part);
@@ -227,8 +234,8 @@ __device__ bool segment_row(const DevCsr<O, V> &A, const Segments &sg, const dou
 // writing the row's share under the completing block's own index would move
 // it between partial slots from run to run and change the rounding of the
 // fixed-order sum over slots.
-template <int G, bool ABS = false, class O, class V, class Row>
-__device__ __forceinline__ void for_rows(const DevCsr<O, V> &A, const Segments &sg, const double *v, double *lds,
+template <int G, bool ABS = false, class O, class V, class T, class Row>
+__device__ __forceinline__ void for_rows(const DevCsr<O, V> &A, const Segments &sg, const T *v, double *lds,
                                          Row &&row, long long *slot = nullptr) {
     if ((int)blockIdx.x < sg.nseg) {
         long long i = 0;
@@ -246,20 +253,20 @@ __device__ __forceinline__ void for_rows(const DevCsr<O, V> &A, const Segments &
          base += nb * rows_per_block) {
         const long long i = base + threadIdx.x / G;
         bool valid = i < A.rows;
-        double acc = 0, acca = 0;
+        T acc = 0, acca = 0;
         if (valid) {
             const O begin = A.ptr[i], end = A.ptr[i + 1];
             valid = (long long)(end - begin) <= sg.threshold;
             if (valid)
                 for (O k = begin + lane; k < end; k += G) {
-                    const double p = double(A.val[k]) * v[A.idx[k]];
+                    const T p = T(A.val[k]) * v[A.idx[k]];
                     acc += p;
                     if (ABS) acca += fabs(p);
                 }
         }
-        acc = group_sum<G>(acc);
-        if (ABS) acca = group_sum<G>(acca);
-        if (valid && lane == 0) row(i, acc, acca);
+        acc = group_sum<G, T>(acc);
+        if (ABS) acca = group_sum<G, T>(acca);
+        if (valid && lane == 0) row(i, double(acc), double(acca));
     }
 }
 
@@ -280,8 +287,9 @@ struct ChunkedCsr {
     int P;
 };
 
+template <class T>
 struct VecPair {
-    const double *p[2];  // the gathered vector by iterate parity (both the same when not double-buffered)
+    const T *p[2];  // the gathered vector by iterate parity (both the same when not double-buffered)
 };
 
 constexpr int kChunkRows = 4;  // rows per thread per pass of partial_kernel
@@ -292,12 +300,12 @@ constexpr int kChunkRows = 4;  // rows per thread per pass of partial_kernel
 // Blocks [0, grid_chunks) work on the chunks; the blocks after them run the
 // long-row segments of the whole matrix and write each long row's total into
 // plane P, which is zero for every other row.
-template <class O, class V>
+template <class O, class V, class T>
 __global__ __launch_bounds__(kBlock) void partial_kernel(ChunkedCsr C, const V *val, DevCsr<O, V> A, Segments sg,
-                                                         int grid_chunks, const State *st, VecPair vec,
+                                                         int grid_chunks, const State *st, VecPair<T> vec,
                                                          double *part) {
     if (st->stop != kRun) return;
-    const double *v = vec.p[st->cur];
+    const T *v = vec.p[st->cur];
     if ((int)blockIdx.x >= grid_chunks) {
         __shared__ double lds[4];
         long long i = 0;
@@ -325,23 +333,25 @@ __global__ __launch_bounds__(kBlock) void partial_kernel(ChunkedCsr C, const V *
             end[r] = i < C.rows ? ptr[i + 1] : 0;
         }
         int c0[kChunkRows], c1[kChunkRows];
-        double v0[kChunkRows], v1[kChunkRows];
+        T v0[kChunkRows], v1[kChunkRows];
 #pragma unroll
         for (int r = 0; r < kChunkRows; ++r) {
             const bool h0 = beg[r] < end[r], h1 = beg[r] + 1 < end[r];
             c0[r] = h0 ? idx[beg[r]] : 0;
-            v0[r] = h0 ? double(vals[beg[r]]) : 0.0;
+            v0[r] = h0 ? T(vals[beg[r]]) : T(0);
             c1[r] = h1 ? idx[beg[r] + 1] : 0;
-            v1[r] = h1 ? double(vals[beg[r] + 1]) : 0.0;
+            v1[r] = h1 ? T(vals[beg[r] + 1]) : T(0);
         }
+        T sum[kChunkRows];
 #pragma unroll
         for (int r = 0; r < kChunkRows; ++r) {
-            const double g0 = beg[r] < end[r] ? v[c0[r]] : 0.0;
-            const double g1 = beg[r] + 1 < end[r] ? v[c1[r]] : 0.0;
-            acc[r] = 0.0;
-            if (beg[r] < end[r]) acc[r] += v0[r] * g0;
-            if (beg[r] + 1 < end[r]) acc[r] += v1[r] * g1;
-            for (int e = beg[r] + 2; e < end[r]; ++e) acc[r] += double(vals[e]) * v[idx[e]];
+            const T g0 = beg[r] < end[r] ? v[c0[r]] : T(0);
+            const T g1 = beg[r] + 1 < end[r] ? v[c1[r]] : T(0);
+            sum[r] = 0;
+            if (beg[r] < end[r]) sum[r] += v0[r] * g0;
+            if (beg[r] + 1 < end[r]) sum[r] += v1[r] * g1;
+            for (int e = beg[r] + 2; e < end[r]; ++e) sum[r] += T(vals[e]) * v[idx[e]];
+            acc[r] = double(sum[r]);
         }
 #pragma unroll
         for (int r = 0; r < kChunkRows; ++r) {
@@ -363,12 +373,29 @@ __device__ __forceinline__ void for_rows_partials(long long rows, const double *
     }
 }
 
-template <int G, class O, class V>
+// The vector a product gathers: the FP64 iterate, or in the FP32 arithmetic
+// phase (T = float) its FP32 copy.
+template <class T>
+__device__ __forceinline__ const T *gathered(const double *d, const float *f);
+template <>
+__device__ __forceinline__ const double *gathered<double>(const double *d, const float *) {
+    return d;
+}
+template <>
+__device__ __forceinline__ const float *gathered<float>(const double *, const float *f) {
+    return f;
+}
+
+// T: arithmetic of A^T y (double, or float in the FP32 arithmetic phase,
+// where A^T y gathers an FP32 copy of y, sums in FP32 per lane and the
+// projected step itself stays FP64).
+template <int G, class O, class V, class T = double>
 __global__ __launch_bounds__(kBlock) void primal_kernel(DevCsr<O, V> at, Segments sg, PrimalArgs a) {
+    constexpr bool kSingle = sizeof(T) == 4;
     const State *st = a.st;
     if (st->stop != kRun) return;
     const int cur = st->cur;
-    const double *y = a.y[cur];
+    const T *y = gathered<T>(a.y[cur], a.y32[cur]);
     const double *x = a.x[cur];
     double *xn = a.x[cur ^ 1];
     const double *xa = a.xa[cur];
@@ -386,7 +413,10 @@ __global__ __launch_bounds__(kBlock) void primal_kernel(DevCsr<O, V> at, Segment
         const double t = clampd(xj - tau * g, a.lb[j], a.ub[j]);
         const double d = t - xj;
         local += d * d;
-        a.dx[j] = d;
+        if (kSingle)
+            a.dx32[j] = float(d);
+        else
+            a.dx[j] = d;
         if (a.f.halpern) {
             xan[j] = t;
             xn[j] = lambda * a.anchor_x[j] + (1 - lambda) * ((1 + rho) * t - rho * xj);
@@ -451,13 +481,17 @@ __device__ void decide(State *st, const Flags &fl, double dx2, double dy2, doubl
     if (st->chunk_accepted >= st->chunk_target) st->stop = kDone;
 }
 
-template <int G, class O, class V>
+// T: arithmetic of A dx (as for primal_kernel); in the FP32 phase the new y
+// is also stored as FP32 for the next A^T y.
+template <int G, class O, class V, class T = double>
 __global__ __launch_bounds__(kBlock) void dual_kernel(DevCsr<O, V> A, Segments sg, DualArgs a) {
+    constexpr bool kSingle = sizeof(T) == 4;
     State *st = a.st;
     if (st->stop != kRun) return;
     const int cur = st->cur;
     const double *y = a.y[cur];
     double *yn = a.y[cur ^ 1];
+    float *yn32 = a.y32[cur ^ 1];
     const double *ya = a.ya[cur];
     double *yan = a.ya[cur ^ 1];
     const double *ax = a.ax[cur];
@@ -482,10 +516,13 @@ __global__ __launch_bounds__(kBlock) void dual_kernel(DevCsr<O, V> A, Segments s
         const double next_ax = axi + delta;
         if (a.f.halpern) {
             yan[i] = t;
-            yn[i] = lambda * a.anchor_y[i] + (1 - lambda) * ((1 + rho) * t - rho * yi);
+            const double next = lambda * a.anchor_y[i] + (1 - lambda) * ((1 + rho) * t - rho * yi);
+            yn[i] = next;
+            if (kSingle) yn32[i] = float(next);
             axn[i] = lambda * a.anchor_ax[i] + (1 - lambda) * ((1 + rho) * next_ax - rho * axi);
         } else {
             yn[i] = t;
+            if (kSingle) yn32[i] = float(t);
             const double v = ya[i];
             yan[i] = v + (t - v) * w;
             axn[i] = next_ax;
@@ -493,7 +530,7 @@ __global__ __launch_bounds__(kBlock) void dual_kernel(DevCsr<O, V> A, Segments s
     };
     long long slot = blockIdx.x;
     if (a.part) for_rows_partials(A.rows, a.part, a.parts, update);
-    else for_rows<G>(A, sg, a.dx, lds, update, &slot);
+    else for_rows<G>(A, sg, gathered<T>(a.dx, a.dx32), lds, update, &slot);
     const double s1 = block_sum(sq, lds);
     const double s2 = block_sum(cross, lds);
     if (threadIdx.x == 0 && slot >= 0) {
@@ -570,6 +607,11 @@ __global__ void restart_state_kernel(State *st, int halpern) {
         st->previous_residual = INFINITY;
         st->restart_requested = 0;
     }
+}
+
+__global__ void to_float_kernel(const double *src, float *dst, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = float(src[i]);
 }
 
 __global__ void fill_kernel(double *p, long long n, double v) {
@@ -1088,6 +1130,9 @@ class Backend {
         flags_.reflection = cfg.reflection;
         graphs_ = cfg.graphs != 0;
         fp32_ = cfg.fp32_values != 0;
+        fp32_arith_ = cfg.fp32_arithmetic != 0;
+        if (fp32_arith_ && fp32_) throw std::runtime_error("FP32 arithmetic keeps FP64 master values: not with fp32_values");
+        if (fp32_arith_ && cfg.halpern) throw std::runtime_error("Fixed Halpern operators require FP64 arithmetic");
         const bool fits32 = nnz_ < (1LL << 31);
         if (cfg.index_bits == 32 && !fits32) throw std::runtime_error("CSR offsets do not fit 32 bits");
         idx64_ = cfg.index_bits == 64 || !fits32;
@@ -1118,9 +1163,17 @@ class Backend {
             nb1_ = sg_at_.nseg + grid_for(n_, g_at_);
             nb2_ = sg_a_.nseg + grid_for(m_, g_a_);
             setup_chunks(lp.ptr, lp.index, lp.value, m_, n_, row_threshold(g_a_), ch_a_, ch_a_val_, g_ch_a_,
-                         nbp_a_, part_a_);
+                         nbp_a_, part_a_, ch_a_nnz_);
             setup_chunks(tp, ti, tv, n_, m_, row_threshold(g_at_), ch_at_,
-                         ch_at_val_, g_ch_at_, nbp_at_, part_at_);
+                         ch_at_val_, g_ch_at_, nbp_at_, part_at_, ch_at_nnz_);
+            if (fp32_arith_) {
+                // FP32 copies of the values for the FP32 arithmetic phase; the
+                // FP64 values stay for the refinement phase and the rebases
+                a32_val_ = narrow(static_cast<const double *>(a_.val), a_.nnz);
+                at32_val_ = narrow(static_cast<const double *>(at_.val), at_.nnz);
+                if (ch_a_.P) ch_a32_val_ = narrow(static_cast<const double *>(ch_a_val_), ch_a_nnz_);
+                if (ch_at_.P) ch_at32_val_ = narrow(static_cast<const double *>(ch_at_val_), ch_at_nnz_);
+            }
             c_ = upload(lp.c, n_);
             lb_ = upload(lp.lb, n_);
             ub_ = upload(lp.ub, n_);
@@ -1134,6 +1187,11 @@ class Backend {
                 ax_[h] = alloc_d(m_);
             }
             dx_ = alloc_d(n_);
+            if (fp32_arith_) {
+                for (int h = 0; h < 2; ++h) y32_[h] = static_cast<float *>(alloc_bytes(size_t(std::max<long long>(m_, 1)) * 4));
+                dx32_ = static_cast<float *>(alloc_bytes(size_t(std::max<long long>(n_, 1)) * 4));
+                phase32_ = true;
+            }
             if (flags_.halpern) {
                 anchor_x_ = alloc_d(n_);
                 anchor_y_ = alloc_d(m_);
@@ -1202,6 +1260,8 @@ class Backend {
         s->restart_requested = host_.restart_requested;
         s->index_bits = idx64_ ? 64 : 32;
         s->factor = host_.factor;
+        s->fp32_phase = phase32_ ? 1 : 0;
+        s->pad = 0;
     }
 
     void candidates(double *x, double *y, double *xa, double *ya) {
@@ -1482,6 +1542,7 @@ class Backend {
         s.consecutive_rejects = 0;
         s.ticket = 0;
         HIP_OK(hipMemcpyAsync(st_, &s, sizeof s, hipMemcpyHostToDevice, stream_));
+        refresh_y32(0);
         if (attached_) {
             put(ex_, n_);
             put(ey_, m_);
@@ -1591,7 +1652,9 @@ class Backend {
     }
 
     void setup_chunks(const int64_t *ptr, const int64_t *index, const double *value, long long rows, long long cols,
-                      long long threshold, ChunkedCsr &c, void *&val, int &group, int &grid, double *&part) {
+                      long long threshold, ChunkedCsr &c, void *&val, int &group, int &grid, double *&part,
+                      long long &entries) {
+        entries = 0;
         c = ChunkedCsr{};
         int P = chunk_count(cols);
         if (P && gather_footprint(ptr, index, rows, cols, threshold) < kLocalFootprint) P = 0;
@@ -1604,6 +1667,7 @@ class Backend {
         c.idx = upload_vector(h.idx);
         if (fp32_) val = upload_vector(std::vector<float>(h.val.begin(), h.val.end()));
         else val = upload_vector(h.val);
+        entries = (long long)h.val.size();
         group = 1;
         const long long rpb = (long long)kBlock * kChunkRows;
         const long long per_chunk = std::min<long long>((rows + rpb - 1) / rpb, 4096 / P);
@@ -1612,6 +1676,22 @@ class Backend {
         // P chunk planes plus the long-row plane, which stays zero except at long rows.
         part = alloc_d((long long)(P + 1) * rows);
         fill_kernel<<<elementwise_grid((P + 1) * rows), 256, 0, stream_>>>(part, (P + 1) * rows, 0.0);
+        HIP_OK(hipGetLastError());
+    }
+
+    // An FP32 copy of an FP64 device array.
+    float *narrow(const double *src, long long n) {
+        float *dst = static_cast<float *>(alloc_bytes(size_t(std::max<long long>(n, 1)) * sizeof(float)));
+        if (n) {
+            to_float_kernel<<<elementwise_grid(n), 256, 0, stream_>>>(src, dst, n);
+            HIP_OK(hipGetLastError());
+        }
+        return dst;
+    }
+    // FP32 copy of y_[c] (the vector A^T y gathers in the FP32 phase).
+    void refresh_y32(int c) {
+        if (!phase32_ || !m_) return;
+        to_float_kernel<<<elementwise_grid(m_), 256, 0, stream_>>>(y_[c], y32_[c], m_);
         HIP_OK(hipGetLastError());
     }
 
@@ -1640,6 +1720,10 @@ class Backend {
     DevCsr<O, V> view(const DeviceMatrix &d) const {
         return DevCsr<O, V>{static_cast<const O *>(d.ptr), d.idx, static_cast<const V *>(d.val), d.rows};
     }
+    template <class O, class V>
+    DevCsr<O, V> view_values(const DeviceMatrix &d, const void *values) const {
+        return DevCsr<O, V>{static_cast<const O *>(d.ptr), d.idx, static_cast<const V *>(values), d.rows};
+    }
 
     PrimalArgs primal_args() const {
         PrimalArgs p{};
@@ -1650,8 +1734,10 @@ class Backend {
             p.x[h] = x_[h];
             p.xa[h] = xa_[h];
             p.y[h] = y_[h];
+            p.y32[h] = y32_[h];
         }
         p.dx = dx_;
+        p.dx32 = dx32_;
         p.anchor_x = anchor_x_;
         p.partial = part1_;
         p.part = ch_at_.P ? part_at_ : nullptr;
@@ -1668,8 +1754,10 @@ class Backend {
             d.y[h] = y_[h];
             d.ya[h] = ya_[h];
             d.ax[h] = ax_[h];
+            d.y32[h] = y32_[h];
         }
         d.dx = dx_;
+        d.dx32 = dx32_;
         d.anchor_y = anchor_y_;
         d.anchor_ax = anchor_ax_;
         d.partial = part2_;
@@ -1683,43 +1771,78 @@ class Backend {
         return d;
     }
 
-    template <class O, class V>
+    // One attempt: A^T y with the primal step, A dx with the dual step and the
+    // decision.  T = float is the FP32 arithmetic phase (FP32 value copies,
+    // FP32 gathered vectors); the iterates stay FP64 either way.
+    template <class O, class V, class T = double>
     void launch_attempt_t() {
-        auto at = view<O, V>(at_);
-        auto a = view<O, V>(a_);
+        constexpr bool single = sizeof(T) == 4;
+        auto at = single ? view_values<O, V>(at_, at32_val_) : view<O, V>(at_);
+        auto a = single ? view_values<O, V>(a_, a32_val_) : view<O, V>(a_);
+        const void *ch_at_val = single ? static_cast<const void *>(ch_at32_val_) : ch_at_val_;
+        const void *ch_a_val = single ? static_cast<const void *>(ch_a32_val_) : ch_a_val_;
         auto pa = primal_args();
         auto da = dual_args();
-        if (ch_at_.P) launch_partial(ch_at_, ch_at_val_, at, sg_at_, nbp_at_, VecPair{{y_[0], y_[1]}}, part_at_);
+        VecPair<T> yv, dxv;
+        if constexpr (single) {
+            yv = VecPair<T>{{y32_[0], y32_[1]}};
+            dxv = VecPair<T>{{dx32_, dx32_}};
+        } else {
+            yv = VecPair<T>{{y_[0], y_[1]}};
+            dxv = VecPair<T>{{dx_, dx_}};
+        }
+        if (ch_at_.P) launch_partial(ch_at_, ch_at_val, at, sg_at_, nbp_at_, yv, part_at_);
         switch (g_at_) {
 #define PRIMAL_CASE(G) \
-    case G: primal_kernel<G, O, V><<<nb1_, kBlock, 0, stream_>>>(at, sg_at_, pa); break;
+    case G: primal_kernel<G, O, V, T><<<nb1_, kBlock, 0, stream_>>>(at, sg_at_, pa); break;
             PRIMAL_CASE(1) PRIMAL_CASE(2) PRIMAL_CASE(4) PRIMAL_CASE(8) PRIMAL_CASE(16) PRIMAL_CASE(32) PRIMAL_CASE(64)
 #undef PRIMAL_CASE
         }
-        if (ch_a_.P) launch_partial(ch_a_, ch_a_val_, a, sg_a_, nbp_a_, VecPair{{dx_, dx_}}, part_a_);
+        if (ch_a_.P) launch_partial(ch_a_, ch_a_val, a, sg_a_, nbp_a_, dxv, part_a_);
         switch (g_a_) {
 #define DUAL_CASE(G) \
-    case G: dual_kernel<G, O, V><<<nb2_, kBlock, 0, stream_>>>(a, sg_a_, da); break;
+    case G: dual_kernel<G, O, V, T><<<nb2_, kBlock, 0, stream_>>>(a, sg_a_, da); break;
             DUAL_CASE(1) DUAL_CASE(2) DUAL_CASE(4) DUAL_CASE(8) DUAL_CASE(16) DUAL_CASE(32) DUAL_CASE(64)
 #undef DUAL_CASE
         }
         if (!da.inline_decide) decide_kernel<<<1, kBlock, 0, stream_>>>(da, nb2_);
     }
-    template <class O, class V>
+    template <class O, class V, class T>
     void launch_partial(const ChunkedCsr &c, const void *val, const DevCsr<O, V> &m, const Segments &sg, int grid,
-                        VecPair vec, double *part) {
-        partial_kernel<O, V><<<grid + sg.nseg, kBlock, 0, stream_>>>(c, static_cast<const V *>(val), m, sg, grid, st_,
-                                                                    vec, part);
+                        VecPair<T> vec, double *part) {
+        partial_kernel<O, V, T><<<grid + sg.nseg, kBlock, 0, stream_>>>(c, static_cast<const V *>(val), m, sg, grid,
+                                                                       st_, vec, part);
     }
 
     void launch_attempt() {
         if (idx64_) {
-            if (fp32_) launch_attempt_t<int64_t, float>(); else launch_attempt_t<int64_t, double>();
+            if (phase32_) launch_attempt_t<int64_t, float, float>();
+            else if (fp32_) launch_attempt_t<int64_t, float>();
+            else launch_attempt_t<int64_t, double>();
         } else {
-            if (fp32_) launch_attempt_t<int32_t, float>(); else launch_attempt_t<int32_t, double>();
+            if (phase32_) launch_attempt_t<int32_t, float, float>();
+            else if (fp32_) launch_attempt_t<int32_t, float>();
+            else launch_attempt_t<int32_t, double>();
         }
         HIP_OK(hipGetLastError());
     }
+
+  public:
+    // Switches the attempts between the FP32 arithmetic phase and FP64 (the
+    // refinement phase); the iterates, averages and counters carry on.
+    void set_fp32_phase(bool on) {
+        if (on && !fp32_arith_) throw std::runtime_error("Backend was created without FP32 arithmetic");
+        if (on == phase32_) return;
+        sync_state();
+        for (auto &kv : graphs_cache_) HIP_OK(hipGraphExecDestroy(kv.second));
+        graphs_cache_.clear();
+        phase32_ = on;
+        refresh_y32(host_.cur);
+        HIP_OK(hipStreamSynchronize(stream_));
+    }
+    bool fp32_phase() const { return phase32_; }
+
+  private:
 
     template <class O, class V>
     void spmv_t(bool transpose, const double *x, double *out) {
@@ -1857,6 +1980,7 @@ class Backend {
         if (n_) HIP_OK(hipMemcpyAsync(x_[0], x0, n_ * sizeof(double), hipMemcpyHostToDevice, stream_));
         if (m_) HIP_OK(hipMemcpyAsync(y_[0], y0, m_ * sizeof(double), hipMemcpyHostToDevice, stream_));
         spmv(false, x_[0], ax_[0]);
+        refresh_y32(0);
         if (flags_.halpern) {
             copy(anchor_x_, x_[0], n_);
             copy(anchor_y_, y_[0], m_);
@@ -1867,6 +1991,7 @@ class Backend {
     // x_[c], y_[c] hold the restart point; rebuild activity, clear averages.
     void restart_resident(int c) {
         spmv(false, x_[c], ax_[c]);
+        refresh_y32(c);
         zero(xa_[c], n_);
         zero(ya_[c], m_);
         if (flags_.halpern) {
@@ -1892,6 +2017,12 @@ class Backend {
     hipStream_t stream_ = nullptr;
     long long n_ = 0, m_ = 0, nnz_ = 0;
     bool idx64_ = false, fp32_ = false, graphs_ = true;
+    // FP32 arithmetic phase: FP32 value copies, FP32 copies of y (by parity)
+    // and dx; phase32_ says which arithmetic the attempts use now.
+    bool fp32_arith_ = false, phase32_ = false;
+    float *a32_val_ = nullptr, *at32_val_ = nullptr, *ch_a32_val_ = nullptr, *ch_at32_val_ = nullptr;
+    float *y32_[2] = {}, *dx32_ = nullptr;
+    long long ch_a_nnz_ = 0, ch_at_nnz_ = 0;
     int g_a_ = 1, g_at_ = 1, nb1_ = 1, nb2_ = 1;
     Flags flags_{};
     DeviceMatrix a_, at_;
@@ -1965,6 +2096,13 @@ int npdhg_device_name(int device, char *name, int len) {
         HIP_OK(hipGetDeviceProperties(&prop, device));
         std::snprintf(name, size_t(len), "%s (%s, %d CUs, %.0f GiB)", prop.name, prop.gcnArchName,
                       prop.multiProcessorCount, double(prop.totalGlobalMem) / (1024.0 * 1024 * 1024));
+    });
+}
+
+int npdhg_set_fp32_phase(npdhg_backend *b, int on) {
+    return guarded([&] {
+        if (!b) throw std::runtime_error("Null argument");
+        b->impl.set_fp32_phase(on != 0);
     });
 }
 

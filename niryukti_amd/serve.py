@@ -3,14 +3,17 @@
 The reference serves `POST /v1/solve {"model": <native JSON>, "options": {...}}`
 (python/niryukti/service.py:54-69) with one synchronous solve per worker and
 answers 429 when every worker is busy.  Here requests are queued instead and
-admitted as device memory frees up:
+admitted as device memory frees up, by the native server of
+include/niryukti_serve.h (a C++ scheduler thread and grid workers; this module
+only binds it):
 
-* LPs below `large_nonzeros` go to one `Batch` (include/niryukti_batch.h): each
-  is a workgroup of the batched kernel, admitted into the HBM arena between
-  launches, certified on the host and retired while the others keep running.
-* Larger LPs run on the grid-wide single-LP engine (`solve_lp`), a few at a
-  time on worker threads, each on its own HIP stream, admitted against the
-  rest of the HBM budget.
+* LPs below `large_nonzeros` go to one batched engine: each is a workgroup of
+  the batched kernel, prepared on the engine's host workers, admitted into the
+  HBM arena between launches, certified on the host and retired while the
+  others keep running.
+* Larger LPs run on the grid-wide single-LP engine, a few at a time on worker
+  threads, each on its own HIP stream, admitted against the rest of the HBM
+  budget.
 
 Both paths are deterministic per LP (fixed-order reductions, no cross-LP
 state), so a result is the same alone and in any mix.  `solve_request`
@@ -18,12 +21,12 @@ keeps the reference's request and response shape without the HTTP layer.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 import threading
 import time
-from concurrent.futures import ThreadPoolExecutor
 
-from . import LP, Batch, Model, solve_lp
+from . import LP, Model, _f64, _options_struct, _ptr, _Result, _result_dict, _warm_array, load_library
 
 _REQUEST_KEYS = {"model", "options"}
 _OPTION_KEYS = {"device", "method", "tol", "time_limit", "threads", "iterations"}
@@ -68,30 +71,83 @@ def parse_request(request, tol: float = 1e-6, time_limit: float = 60.0, iteratio
     return model, extra
 
 
-class Request:
-    """One submitted LP; `wait()` returns its result dict."""
+class _Limits(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("batch_fraction", C.c_double), ("large_nonzeros", C.c_int64),
+                ("grid_workers", C.c_int), ("checks_per_launch", C.c_int), ("max_slots", C.c_int64)]
 
-    def __init__(self, lp: LP, x0, y0, route: str):
+
+class _Stats(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("submitted", "finished", "batch_routed", "grid_routed", "grid_running",
+                                         "grid_bytes_in_use", "grid_budget_bytes", "batch_rounds")]
+
+
+def _bind(lib):
+    if getattr(lib, "_serve_bound", False):
+        return lib
+    lib.niryukti_server_default_limits.argtypes = [C.POINTER(_Limits)]
+    lib.niryukti_server_create.argtypes = [C.c_void_p, C.POINTER(_Limits), C.POINTER(C.c_void_p)]
+    lib.niryukti_server_destroy.argtypes = [C.c_void_p]
+    lib.niryukti_server_submit.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                           C.POINTER(C.c_int64)]
+    lib.niryukti_server_wait.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.POINTER(_Result),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.niryukti_server_release.argtypes = [C.c_void_p, C.c_int64]
+    lib.niryukti_server_stats_get.argtypes = [C.c_void_p, C.POINTER(_Stats)]
+    lib.niryukti_server_last_error.restype = C.c_char_p
+    lib._serve_bound = True
+    return lib
+
+
+class Request:
+    """One submitted LP; `wait()` returns its result dict (or raises its error)."""
+
+    def __init__(self, server: "Server", rid: int, lp: LP, x0, y0, route: str):
+        self._server, self.id = server, rid
         self.lp, self.x0, self.y0, self.route = lp, x0, y0, route
         self.arrival = time.perf_counter()
         self.finish = None
         self.result = None
-        self._done = threading.Event()
-
-    def _complete(self, result: dict):
-        self.result = result
-        self.finish = time.perf_counter()
-        self._done.set()
+        self._latency = math.nan
+        self._lock = threading.Lock()
 
     @property
     def latency(self) -> float:
-        return (self.finish - self.arrival) if self.finish is not None else math.nan
+        """Seconds from submit to completion, measured by the native server."""
+        return self._latency
+
+    def _collect(self, timeout) -> bool:
+        with self._lock:
+            return self._collect_locked(timeout)
+
+    def _collect_locked(self, timeout) -> bool:
+        if self.finish is not None:
+            return True
+        x, y = _f64([0.0] * self.lp.cols), _f64([0.0] * self.lp.rows)
+        res = _Result()
+        res.x, res.y = _ptr(x), _ptr(y)
+        latency, route = C.c_double(), C.c_int()
+        rc = self._server._lib.niryukti_server_wait(self._server._h, self.id, -1.0 if timeout is None else timeout,
+                                                    C.byref(res), C.byref(latency), C.byref(route))
+        if rc == 0:
+            return False
+        if rc < 0:
+            message = self._server._lib.niryukti_server_last_error().decode()
+            if message == "Unknown request":
+                raise RuntimeError(message)
+            self.result = RuntimeError(message)
+        else:
+            self.result = _result_dict(res, x, y)
+        self._latency = latency.value
+        self.route = "grid" if route.value == 1 else "batch"
+        self.finish = self.arrival + self._latency
+        self._server._release(self)
+        return True
 
     def done(self) -> bool:
-        return self._done.is_set()
+        return self._collect(0.0)
 
     def wait(self, timeout=None) -> dict:
-        if not self._done.wait(timeout):
+        if not self._collect(timeout):
             raise TimeoutError("request still running")
         if isinstance(self.result, BaseException):
             raise self.result
@@ -99,52 +155,71 @@ class Request:
 
 
 class Server:
-    """Continuous-batching LP server on the current HIP device."""
+    """Continuous-batching LP server on the current HIP device (the native
+    server of include/niryukti_serve.h)."""
 
     def __init__(self, hbm_budget: int = 64 << 30, batch_fraction: float = 0.75, large_nonzeros: int = 200_000,
                  grid_workers: int = 2, checks_per_launch: int = 1, max_slots: int = 4096, tol: float = 1e-6,
                  time_limit: float = 60.0, iteration_limit: int = 100_000):
         if not 0 < batch_fraction < 1 or hbm_budget <= 0 or grid_workers < 1:
             raise ValueError("invalid server limits")
+        self._lib = _bind(load_library())
         self.tol, self.time_limit, self.iteration_limit = tol, time_limit, iteration_limit
         self.large_nonzeros = large_nonzeros
-        self.checks_per_launch = checks_per_launch
-        self._batch = Batch(arena_bytes=int(hbm_budget * batch_fraction), max_slots=max_slots, tol=tol,
-                            time_limit=time_limit, iteration_limit=iteration_limit)
-        self._grid_budget = int(hbm_budget * (1 - batch_fraction))
-        self._grid_used = 0
-        self._grid_running = 0
-        self._grid_cv = threading.Condition()
-        self._pool = ThreadPoolExecutor(grid_workers, thread_name_prefix="niryukti-grid")
-        self._cv = threading.Condition()
-        self._incoming: list[Request] = []
-        self._tickets: dict[int, Request] = {}
-        self._closing = False
-        self._error = None
-        self._thread = threading.Thread(target=self._loop, name="niryukti-scheduler", daemon=True)
-        self._thread.start()
+        self._keep = []
+        opts = _options_struct(self._lib, dict(tol=tol, time_limit=time_limit, iteration_limit=iteration_limit),
+                               self._keep)
+        limits = _Limits(int(hbm_budget), float(batch_fraction), int(large_nonzeros), int(grid_workers),
+                         int(checks_per_launch), int(max_slots))
+        h = C.c_void_p()
+        if self._lib.niryukti_server_create(C.byref(opts), C.byref(limits), C.byref(h)) != 0:
+            raise RuntimeError(self._lib.niryukti_server_last_error().decode())
+        self._h = h
+        self._open = {}  # requests whose results have not been collected yet
+        self._open_lock = threading.Lock()
 
     # ------------------------------------------------------------ submit --
     def submit(self, lp: LP, x0=None, y0=None, *, route: str | None = None, **options) -> Request:
         """Queues one LP (optionally warm-started from an original-space x0/y0).
         Options other than the server's own tol / time_limit / iteration_limit
         send the LP to the grid engine, which takes per-solve options."""
-        if self._closing:
+        if not getattr(self, "_h", None):
             raise RuntimeError("server is closed")
-        if self._error is not None:
-            raise RuntimeError(f"scheduler failed: {self._error}")
-        if route is None:
-            route = "grid" if lp.nnz >= self.large_nonzeros or options else "batch"
-        if route not in ("batch", "grid"):
+        if route not in (None, "batch", "grid"):
             raise ValueError("route must be 'batch' or 'grid'")
-        req = Request(lp, x0, y0, route)
-        if route == "grid":
-            self._pool.submit(self._grid_solve, req, options)
-        else:
-            with self._cv:
-                self._incoming.append(req)
-                self._cv.notify()
+        wx = _warm_array(x0, lp.cols, "x0") if x0 is not None else None
+        wy = _warm_array(y0, lp.rows, "y0") if y0 is not None else None
+        keep = []
+        opts = None
+        if options:
+            merged = dict(tol=self.tol, time_limit=self.time_limit, iteration_limit=self.iteration_limit)
+            merged.update(options)
+            opts = _options_struct(self._lib, merged, keep, lp)
+        s = lp._struct()
+        rid = C.c_int64()
+        code = {None: -1, "batch": 0, "grid": 1}[route]
+        rc = self._lib.niryukti_server_submit(self._h, C.byref(s), _ptr(wx) if wx is not None else None,
+                                              _ptr(wy) if wy is not None else None, code,
+                                              C.byref(opts) if opts is not None else None, C.byref(rid))
+        if rc != 0:
+            raise ValueError(self._lib.niryukti_server_last_error().decode())
+        guess = "grid" if (route == "grid" or options or (route is None and lp.nnz >= self.large_nonzeros)) \
+            else "batch"
+        req = Request(self, rid.value, lp, x0, y0, guess)
+        with self._open_lock:
+            self._open[req.id] = req
         return req
+
+    def _release(self, req: Request):
+        self._lib.niryukti_server_release(self._h, int(req.id))
+        with self._open_lock:
+            self._open.pop(req.id, None)
+
+    def stats(self) -> dict:
+        st = _Stats()
+        if self._lib.niryukti_server_stats_get(self._h, C.byref(st)) != 0:
+            raise RuntimeError(self._lib.niryukti_server_last_error().decode())
+        return {k: getattr(st, k) for k, _ in _Stats._fields_}
 
     # --------------------------------------------------------- /v1/solve --
     def solve_request(self, request) -> tuple:
@@ -164,75 +239,22 @@ class Server:
         body["serving"] = {"route": req.route, "latency_seconds": req.latency}
         return 200, body
 
-    # -------------------------------------------------------------- loop --
-    def _loop(self):
-        incoming: list[Request] = []
-        try:
-            while True:
-                with self._cv:
-                    while not self._incoming and not self._tickets and not self._closing:
-                        self._cv.wait()
-                    if self._closing and not self._incoming and not self._tickets:
-                        return
-                    incoming, self._incoming = self._incoming, []
-                while incoming:
-                    req = incoming.pop(0)
-                    try:
-                        ticket = self._batch.submit(req.lp, req.x0, req.y0)
-                    except (ValueError, RuntimeError) as error:
-                        # this request alone is rejected (e.g. a nonfinite warm start)
-                        req._complete(error)
-                        continue
-                    self._tickets[ticket] = req
-                if self._tickets:
-                    self._batch.step(self.checks_per_launch)
-                    for t in self._batch.finished():
-                        req = self._tickets.pop(t)
-                        req._complete(self._batch.result(t))
-                        self._batch.release(t)
-        except BaseException as error:  # surface to every waiter
-            self._error = error
-            with self._cv:
-                pending = list(self._tickets.values()) + incoming + self._incoming
-                self._tickets.clear()
-                self._incoming = []
-            for req in pending:
-                req._complete(error)
-
-    def _grid_solve(self, req: Request, options: dict):
-        need = grid_footprint(req.lp)
-        with self._grid_cv:
-            # Admit when it fits, or alone when it never could.
-            while self._grid_running and self._grid_used + need > self._grid_budget:
-                self._grid_cv.wait()
-            self._grid_used += need
-            self._grid_running += 1
-        try:
-            opts = dict(tol=self.tol, time_limit=self.time_limit, iteration_limit=self.iteration_limit)
-            opts.update(options)
-            if req.x0 is not None:
-                opts["initial_x"] = req.x0
-            if req.y0 is not None:
-                opts["initial_y"] = req.y0
-            req._complete(solve_lp(req.lp, **opts))
-        except BaseException as error:
-            req._complete(error)
-        finally:
-            with self._grid_cv:
-                self._grid_used -= need
-                self._grid_running -= 1
-                self._grid_cv.notify_all()
-
     def close(self):
-        with self._cv:
-            self._closing = True
-            self._cv.notify()
-        self._thread.join()
-        self._pool.shutdown(wait=True)
-        self._batch.close()
+        """Finishes every submitted request (their results stay readable from
+        their Request objects), then stops the server."""
+        if getattr(self, "_h", None):
+            with self._open_lock:
+                pending = list(self._open.values())
+            for req in pending:
+                req._collect(None)
+            self._lib.niryukti_server_destroy(self._h)
+            self._h = None
 
     def __enter__(self):
         return self
 
     def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
         self.close()

@@ -5,7 +5,8 @@ sources by oracle/ref.mk) and stores its results as JSON data under
 tests/golden/.  Run from the repository root after
 `make -f oracle/ref.mk REF=/root/reference`:
 
-    python tests/golden/make_golden.py [solves] [prepare] [steps] [controls]   (default: all)
+    python tests/golden/make_golden.py [solves] [prepare] [steps] [controls] [stalls]
+    (default: all but stalls, which runs the reference for about 15 minutes)
 """
 import hashlib
 import json
@@ -83,6 +84,8 @@ def main():
         steps()
     if "controls" in parts:
         controls()
+    if "stalls" in parts:
+        stalls()
 
 
 def solves():
@@ -154,6 +157,26 @@ def controls():
         r = run("pid", "--weight", w0, "--updates", ";".join(f"{dx!r},{dy!r}" for dx, dy in seq))
         out["pid"].append({"weight": w0, "updates": seq, "weights": r["weights"]})
     (OUT / "reference_controls.json").write_text(json.dumps(out) + "\n")
+
+
+# Where the reference's own PDHG stops converging within its 100,000-iteration
+# default: the block-angular workload at shrinking scales, tol 1e-8, and the
+# power-law workload at 1/50 scale, tol 1e-6 (status, iterations and rel-KKT
+# reached; 4 OpenMP threads, so trajectories are not bit-exact pins).
+STALLS = [(4, 42, 32, 1e-8), (4, 42, 16, 1e-8), (4, 42, 8, 1e-8), (4, 42, 4, 1e-8), (1, 42, 50, 1e-6)]
+
+
+def stalls():
+    out = []
+    for config, seed, scale, tol in STALLS:
+        r = run("solve", "--config", config, "--seed", seed, "--scale", scale, "--threads", 4, "--tol", tol,
+                "--iterations", 100000)
+        keep = {k: r[k] for k in ("status", "iterations", "restarts", "kkt", "primal", "dual", "gap", "objective",
+                                  "planted_objective", "rows", "cols", "nnz", "seconds")}
+        keep.update(config=config, seed=seed, scale=scale, tol=tol, threads=4)
+        out.append(keep)
+        print("stall", config, scale, tol, r["status"], r["iterations"], r["kkt"], flush=True)
+    (OUT / "reference_stalls.json").write_text(json.dumps(out, indent=1) + "\n")
 
 
 if __name__ == "__main__":

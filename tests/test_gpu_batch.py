@@ -201,3 +201,20 @@ def test_power_iterations_in_batch(engine):
     for inst, r in zip(insts, mix):
         assert r["status"] == "OPTIMAL" and rel(r["objective"], inst.planted_objective) <= 1e-6
     assert alone[0]["iterations"] == mix[1]["iterations"] and np.array_equal(alone[0]["x"], mix[1]["x"])
+
+
+def test_autotuner_keeps_results(engine, monkeypatch):
+    """The launch-shape autotuner tries every shape on the first rounds of a
+    crowded class and keeps the fastest; results are bit-identical to a run
+    with the tuner off (shapes never change the arithmetic)."""
+    insts = [engine.Instance(2, 5, 1, i) for i in range(700)]
+    lps = [i.lp for i in insts]
+    tuned, stats = solve_batch(engine, lps, checks=1)
+    print("tuned shapes", stats["class_threads"], stats["class_rows"], stats["class_tuned"], flush=True)
+    assert stats["class_tuned"][0] == 1
+    monkeypatch.setenv("NIRYUKTI_BATCH_TUNE", "0")
+    fixed, fstats = solve_batch(engine, lps, checks=1)
+    assert list(fstats["class_threads"]) == [256, 1024, 1024] and list(fstats["class_rows"]) == [1, 1, 1]
+    for a, b in zip(tuned, fixed):
+        assert a["status"] == b["status"] and a["iterations"] == b["iterations"]
+        assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
