@@ -210,7 +210,10 @@ def serve_stream(engine, qps, duration, seed=SEED, warm_fraction=0.2, hbm_budget
         statuses[res["status"]] = statuses.get(res["status"], 0) + 1
         routes[r.route] = routes.get(r.route, 0) + 1
     nnz = np.array([r.lp.nnz for r in reqs])
+    backlog = sum(1 for r in reqs if r.finish is None or r.finish - t0 > duration)
     return {"mode": "serve", "qps": qps, "duration_s": duration, "requests": len(reqs),
+            "arrival_rate_per_s": round(len(reqs) / duration, 3),
+            "finished_after_arrivals_end": backlog,
             "warm_resolves": int(sum(k[0] == "warm" for k in kinds)), "routes": routes, "statuses": statuses,
             "certified": int(ok.sum()), "fresh_at_planted_optimum": planted_ok,
             "solves_per_s": round(float(ok.sum()) / wall, 3), "wall_s": round(wall, 2),
@@ -404,8 +407,9 @@ def main():
     ap.add_argument("--matrix-precision", default="fp64", choices=("fp64", "mixed", "fp32"),
                     help="single-LP mode: FP64; mixed = FP32 matrix values (FP64 accumulation and iterates); "
                          "fp32 = FP32 SpMV arithmetic over FP64 iterates, then FP64 refinement")
-    ap.add_argument("--qps", type=float, default=1.5)
-    ap.add_argument("--duration", type=float, default=40.0)
+    ap.add_argument("--qps", type=float, default=1.0, help="serving stream: Poisson arrival rate")
+    ap.add_argument("--duration", type=float, default=30.0, help="serving stream: arrival window in seconds")
+    ap.add_argument("--no-serve", action="store_true", help="leave the serving stream out of the default run")
     args = ap.parse_args()
     if args.summarize_trace:
         print(summarize_trace(args.summarize_trace))
@@ -530,6 +534,11 @@ def main():
             line["cpu_baseline"]["host"] = host
             line["cpu_baseline"]["one_thread"] = one
             line["vs_cpu_all_cores"] = round(value / max(1e-9, line["cpu_baseline"]["value"]), 2)
+    if not args.no_serve:
+        # the serving stream (Poisson arrivals of LPs of 1e3..5e6 nonzeros, 20 %
+        # warm re-solves): an extra key beside the batch headline
+        line["serving"] = serve_stream(engine, args.qps, args.duration)
+        log(f"serving: {line['serving']}")
     print(json.dumps(line), flush=True)
 
 

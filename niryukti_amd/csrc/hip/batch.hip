@@ -2055,7 +2055,74 @@ class BatchEngine {
         }
         r.initial_step = j.step;
         r.initial_weight = j.weight;
+        sort_by_length(j.prep, j.x, j.y);
         return false;
+    }
+
+    // Reorders the prepared LP's rows and columns by decreasing length (a
+    // stable sort; the step and weight above are already taken), so that the
+    // 64-row slices of the sliced copies hold rows of similar length: the
+    // padding a slice carries to its longest row falls from about 1.5-1.8
+    // times the nonzeros to a few percent on the batched workload, and with
+    // it the loads of every step.  Each row keeps its entries in their order
+    // (so every row's dot product is the same sum), the prepare maps are
+    // composed with the reordering (restore_x / restore_y, the device's
+    // restore and certification see the original LP), and the start point is
+    // reordered with the LP.  The order depends only on the LP, so a result
+    // is the same alone and in any mix.
+    static void sort_by_length(nyk::Prepared &p, std::vector<double> &x, std::vector<double> &y) {
+        nyk::Model &m = p.model;
+        const int64_t rows = m.rows(), cols = m.cols();
+        std::vector<int64_t> col_len(static_cast<size_t>(cols), 0);
+        for (int64_t c : m.A.index) ++col_len[size_t(c)];
+        std::vector<int64_t> rperm(static_cast<size_t>(rows)), cperm(static_cast<size_t>(cols));
+        for (int64_t i = 0; i < rows; ++i) rperm[size_t(i)] = i;
+        for (int64_t c = 0; c < cols; ++c) cperm[size_t(c)] = c;
+        std::stable_sort(rperm.begin(), rperm.end(), [&](int64_t a, int64_t b) {
+            return m.A.ptr[size_t(a) + 1] - m.A.ptr[size_t(a)] > m.A.ptr[size_t(b) + 1] - m.A.ptr[size_t(b)];
+        });
+        std::stable_sort(cperm.begin(), cperm.end(),
+                         [&](int64_t a, int64_t b) { return col_len[size_t(a)] > col_len[size_t(b)]; });
+        std::vector<int64_t> cnew(static_cast<size_t>(cols));
+        for (int64_t c = 0; c < cols; ++c) cnew[size_t(cperm[size_t(c)])] = c;
+        nyk::Csr a;
+        a.rows = rows;
+        a.cols = cols;
+        a.ptr.assign(size_t(rows) + 1, 0);
+        a.index.reserve(m.A.index.size());
+        a.value.reserve(m.A.value.size());
+        for (int64_t r = 0; r < rows; ++r) {
+            const int64_t i = rperm[size_t(r)];
+            for (int64_t k = m.A.ptr[size_t(i)]; k < m.A.ptr[size_t(i) + 1]; ++k) {
+                a.index.push_back(cnew[size_t(m.A.index[size_t(k)])]);
+                a.value.push_back(m.A.value[size_t(k)]);
+            }
+            a.ptr[size_t(r) + 1] = int64_t(a.index.size());
+        }
+        m.A = std::move(a);
+        auto by_col = [&](auto &v) {
+            auto old = v;
+            for (int64_t c = 0; c < cols; ++c) v[size_t(c)] = old[size_t(cperm[size_t(c)])];
+        };
+        auto by_row = [&](auto &v) {
+            auto old = v;
+            for (int64_t r = 0; r < rows; ++r) v[size_t(r)] = old[size_t(rperm[size_t(r)])];
+        };
+        by_col(m.c);
+        by_col(m.lb);
+        by_col(m.ub);
+        by_col(p.column_scale);
+        by_col(p.cols);
+        by_col(x);
+        by_row(m.rl);
+        by_row(m.ru);
+        by_row(p.row_scale);
+        by_row(p.rows);
+        by_row(p.lower_rows);
+        by_row(p.upper_rows);
+        by_row(y);
+        if (!m.names.empty()) by_col(m.names);
+        if (!m.row_names.empty()) by_row(m.row_names);
     }
 
     // Algorithmic HBM bytes of one PDHG attempt on the scaled LP: both CSR
