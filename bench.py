@@ -410,6 +410,8 @@ def main():
     ap.add_argument("--qps", type=float, default=1.0, help="serving stream: Poisson arrival rate")
     ap.add_argument("--duration", type=float, default=30.0, help="serving stream: arrival window in seconds")
     ap.add_argument("--no-serve", action="store_true", help="leave the serving stream out of the default run")
+    ap.add_argument("--no-solve-phase", action="store_true",
+                    help="leave the solve-phase step (LPs prepared before the clock) out of the default run")
     args = ap.parse_args()
     if args.summarize_trace:
         print(summarize_trace(args.summarize_trace))
@@ -517,11 +519,12 @@ def main():
                      "avg_launch_ms": round(1e3 * adv_s / max(1, launches), 3),
                      "launches": launches},
     }
-    # the solve phase alone: every LP prepared and uploaded before the clock starts
-    el, solved, _, _, _ = batch_step(engine, insts, args.checks, args.handoff, prepared=True)
-    line["solve_phase"] = {"lps_per_s": round(solved / el, 2), "seconds": round(el, 3), "solved": solved,
-                           "note": "host prepare and upload before the timed region (not the headline)"}
-    log(f"solve phase: {el:.3f} s, {solved} solved")
+    if not args.no_solve_phase:
+        # the solve phase alone: every LP prepared and uploaded before the clock starts
+        el, solved, _, _, _ = batch_step(engine, insts, args.checks, args.handoff, prepared=True)
+        line["solve_phase"] = {"lps_per_s": round(solved / el, 2), "seconds": round(el, 3), "solved": solved,
+                               "note": "host prepare and upload before the timed region (not the headline)"}
+        log(f"solve phase: {el:.3f} s, {solved} solved")
     if not args.no_baselines:
         line["plain_gpu"] = plain_sample(engine, insts)
         line["served_vs_plain"] = round(value / max(1e-9, line["plain_gpu"]["lps_per_s"]), 2)
