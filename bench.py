@@ -7,18 +7,24 @@ by the batched engine (one workgroup per LP, FP64 iterates and matrices).  A
 solve counts only when the engine reports OPTIMAL, i.e. after the host's FP64
 original-space verifier with the outward-rounded bound certified kkt <= 1e-6.
 
-One step = one pass of the engine over the whole batch, timed from the first
-submit to the last certified result: host presolve and scaling (on the
-engine's prepare workers, beside the device rounds), upload, iteration,
-checks and certification are all inside.  Also reported: p50/p99 latency
-(each LP's own submit to its certified result), the solve phase alone
-(`solve_phase`: LPs prepared and uploaded before the clock starts), the plain
-one-LP-at-a-time GPU engine on a bounded sample, the step kernel's roofline,
-and the CPU baseline (the reference's own CPU PDHG, or the oracle when that
-build is absent) on the host cores of this machine on a bounded sample, at one
-thread and on all cores.
+One step = one pass of the engine over the whole batch.  The passes stream
+through one engine the way a server takes arriving work: pass k+1 is submitted
+once fewer than --stream-threshold LPs of the earlier passes are still
+running, so the next pass's prepares and its crowded phase overlap the few
+slow LPs that end a pass.  The timed region runs from the first submit of the
+first timed pass to the last certified result of the last one: host presolve
+and scaling (on the engine's prepare workers, beside the device rounds),
+upload, iteration, checks and certification are all inside, for every LP.
+Also reported: p50/p99 latency (each LP's own submit to its certified
+result), one pass alone on a fresh engine (`one_pass_alone`), the solve phase
+alone (`solve_phase`: LPs prepared and uploaded before the clock starts), the
+plain one-LP-at-a-time GPU engine on a bounded sample, the step kernel's
+roofline, and the CPU baseline (the reference's own CPU PDHG, or the oracle
+when that build is absent) on the host cores of this machine on a bounded
+sample, at one thread and on all cores.  --no-stream runs the passes one
+after another, each on a fresh engine.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--count 4096]
+    python bench.py [--gpus 1] [--steps 4] [--warmup 1] [--count 4096]
 """
 from __future__ import annotations
 
@@ -75,6 +81,88 @@ def batch_step(engine, insts, checks, handoff=0, prepared=False):
             # an LP's `seconds` run from its own submit to its certified result
             latencies.append(r["seconds"])
     return elapsed, solved, results, stats, latencies
+
+
+def batch_stream(b, insts, passes, checks, threshold):
+    """`passes` passes of the batch streamed through one engine `b`: pass 0 is
+    submitted at once, and pass k+1 as soon as fewer than `threshold` LPs of
+    the earlier passes are still running, so its prepares and its crowded
+    phase overlap the slowest LPs of the pass before (continuous batching: the
+    engine admits the new LPs beside the old ones).  A submitter thread hands
+    the LPs to the engine (niryukti_batch_submit is safe beside a running
+    step) while this thread runs the scheduling rounds and collects results.
+    Timed from the first submit to the last certified result; every LP's
+    latency runs from its own submit.  Returns (elapsed, per-pass results)."""
+    import threading
+
+    lock = threading.Condition()
+    tickets = [[None] * len(insts) for _ in range(passes)]
+    state = {"submitted": 0, "finished": 0, "done_submitting": False, "error": None, "abort": False}
+
+    def submitter():
+        try:
+            for k in range(passes):
+                with lock:
+                    lock.wait_for(lambda: state["abort"] or k == 0
+                                  or state["submitted"] - state["finished"] < threshold)
+                    if state["abort"]:
+                        return
+                for i, inst in enumerate(insts):
+                    t = b.submit(inst.lp)
+                    tickets[k][i] = t
+                    with lock:
+                        state["submitted"] += 1
+        except BaseException as e:  # reported by the scheduling thread
+            state["error"] = e
+        finally:
+            with lock:
+                state["done_submitting"] = True
+
+    out = {}
+    switch = sys.getswitchinterval()
+    sys.setswitchinterval(1e-4)  # the scheduling thread's bookkeeping should not wait on the submitter
+    t0 = time.perf_counter()
+    th = threading.Thread(target=submitter, daemon=True)
+    th.start()
+    try:
+        while True:
+            running = b.step(checks)
+            fin = b.finished(1 << 16)
+            for t in fin:
+                r = b.result(t)
+                out[t] = {k: r[k] for k in ("status", "kkt_error", "objective", "seconds", "iterations", "message")}
+                b.release(t)
+            with lock:
+                state["finished"] += len(fin)
+                lock.notify_all()
+                finished_all = state["done_submitting"] and state["finished"] == state["submitted"]
+            if state["error"] is not None:
+                raise state["error"]
+            if finished_all:
+                break
+            if running == 0 and not fin:
+                time.sleep(1e-4)  # the submitter is between passes
+        elapsed = time.perf_counter() - t0
+    finally:
+        sys.setswitchinterval(switch)
+        with lock:
+            state["abort"] = True
+            lock.notify_all()
+        th.join()
+    return elapsed, [[out[t] for t in row] for row in tickets]
+
+
+def score(insts, results):
+    """Certified solves of one pass (OPTIMAL, kkt <= 1e-6, objective within
+    1e-6 of the planted optimum) and their latencies."""
+    solved, latencies = 0, []
+    for inst, r in zip(insts, results):
+        ok = r["status"] == "OPTIMAL" and r["kkt_error"] <= 1e-6 and \
+            abs(r["objective"] - inst.planted_objective) <= 1e-6 * (1 + abs(inst.planted_objective))
+        solved += ok
+        if ok:
+            latencies.append(r["seconds"])
+    return solved, latencies
 
 
 def plain_sample(engine, insts, budget_s=15.0):
@@ -381,10 +469,14 @@ def _cpu_model():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--checks", type=int, default=5, help="check periods per launch")
+    ap.add_argument("--stream-threshold", type=int, default=1024,
+                    help="submit the next pass once fewer than this many LPs of earlier passes run")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="passes one after another, each on a fresh engine (no overlap between passes)")
     ap.add_argument("--handoff", type=int, default=0,
                     help="iterations after which an LP moves to the grid-wide engine (0: never)")
     ap.add_argument("--no-baselines", action="store_true")
@@ -450,18 +542,42 @@ def main():
     insts = make_instances(engine, args.count)
     nnz = sum(i.lp.nnz for i in insts)
     log(f"instances: {len(insts)} LPs, {nnz} nonzeros")
-    for w in range(args.warmup):
-        el, solved, _, _, _ = batch_step(engine, insts, args.checks, args.handoff)
-        log(f"warmup {w}: {el:.3f} s, {solved} solved")
     total_t, total_solved, lat, stats_all, last = 0.0, 0, [], [], None
-    for s in range(args.steps):
-        el, solved, results, stats, latencies = batch_step(engine, insts, args.checks, args.handoff)
-        log(f"step {s}: {el:.3f} s, {solved}/{len(insts)} solved, stats {stats}")
-        total_t += el
-        total_solved += solved
-        lat += latencies
-        stats_all.append(stats)
-        last = results
+    if args.no_stream:
+        for w in range(args.warmup):
+            el, solved, _, _, _ = batch_step(engine, insts, args.checks, args.handoff)
+            log(f"warmup {w}: {el:.3f} s, {solved} solved")
+        for s in range(args.steps):
+            el, solved, results, stats, latencies = batch_step(engine, insts, args.checks, args.handoff)
+            log(f"step {s}: {el:.3f} s, {solved}/{len(insts)} solved, stats {stats}")
+            total_t += el
+            total_solved += solved
+            lat += latencies
+            stats_all.append(stats)
+            last = results
+    else:
+        # one engine for every pass, warm-up passes included (its launch-shape
+        # tuner settles there); the timed passes start once they have drained
+        b = engine.Batch(arena_bytes=96 << 30, max_slots=2 * len(insts), time_limit=1e9)
+        b.set_handoff(args.handoff, 4)
+        if args.warmup:
+            el, passes = batch_stream(b, insts, args.warmup, args.checks, args.stream_threshold)
+            log(f"warmup: {args.warmup} passes streamed in {el:.3f} s, "
+                f"{sum(score(insts, r)[0] for r in passes)} solved")
+        before = b.stats()
+        total_t, passes = batch_stream(b, insts, args.steps, args.checks, args.stream_threshold)
+        after = b.stats()
+        b.close()
+        for k, results in enumerate(passes):
+            solved, latencies = score(insts, results)
+            total_solved += solved
+            lat += latencies
+            log(f"pass {k}: {solved}/{len(insts)} solved")
+        log(f"{args.steps} passes streamed in {total_t:.3f} s, stats {after}")
+        stats_all.append({key: [a - c for a, c in zip(after[key], before[key])]
+                          for key in ("class_bytes", "class_seconds", "class_launches")})
+        stats_all[0].update({key: after[key] for key in ("class_threads", "class_rows", "class_tuned", "handoffs")})
+        last = passes[-1]
     value = total_solved / total_t
     # Dominant kernel: the launch class (one step-kernel instantiation) that
     # moved the most algorithmic bytes; its rate = bytes / its event time.
@@ -479,7 +595,11 @@ def main():
     symbol = symbol[dom]
     kernel_name = symbol + [" (class m+n <= 2048, LDS)", " (class m+n <= 4096, LDS)", " (class m+n <= 8000, LDS)",
                             " (global class)", " (tail group: attempts and checks in one launch)"][dom]
-    traffic = committed_traffic(symbol, [0, 0, 1, 0, 0][dom])
+    # classes that share a kernel symbol differ in dynamic LDS (it grows with
+    # the class), so the profile's entries for that symbol, sorted by LDS,
+    # follow the class order
+    rank = sum(1 for c in range(min(dom, 3)) if shapes[c] == symbol)
+    traffic = committed_traffic(symbol, rank)
     statuses = {}
     for r in last:
         statuses[r["status"]] = statuses.get(r["status"], 0) + 1
@@ -501,6 +621,9 @@ def main():
                                f"[2m,3m], 3-8 nnz/col, N(0,1), box [0,U], U~U(1,50), planted optimum, seed {SEED}",
                    "lps": args.count, "nonzeros": int(nnz), "tol": 1e-6, "checks_per_launch": args.checks,
                    "handoff_iterations": args.handoff,
+                   "passes": "one engine, pass k+1 submitted once fewer than "
+                             f"{args.stream_threshold} LPs of earlier passes run" if not args.no_stream
+                             else "one engine per pass, passes one after another",
                    "engine": "batched (one workgroup per LP), FP64"},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 1) if lat else None,
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 1) if lat else None},
@@ -519,6 +642,13 @@ def main():
                      "avg_launch_ms": round(1e3 * adv_s / max(1, launches), 3),
                      "launches": launches},
     }
+    if not args.no_stream and not args.no_solve_phase:
+        # one pass alone on a fresh engine, first submit to last result
+        el, solved, _, _, latencies = batch_step(engine, insts, args.checks, args.handoff)
+        line["one_pass_alone"] = {"lps_per_s": round(solved / el, 2), "seconds": round(el, 3), "solved": solved,
+                                  "p50_ms": round(1e3 * float(np.percentile(latencies, 50)), 1),
+                                  "p99_ms": round(1e3 * float(np.percentile(latencies, 99)), 1)}
+        log(f"one pass alone: {el:.3f} s, {solved} solved")
     if not args.no_solve_phase:
         # the solve phase alone: every LP prepared and uploaded before the clock starts
         el, solved, _, _, _ = batch_step(engine, insts, args.checks, args.handoff, prepared=True)

@@ -56,7 +56,10 @@ int niryukti_batch_create(const niryukti_options *opts, int64_t arena_bytes, int
                           niryukti_batch **out);
 void niryukti_batch_destroy(niryukti_batch *b);
 /* Prepares and queues one LP (arrays are copied); optional warm start x0/y0
- * in original space (NULL for a cold start).  Returns its ticket. */
+ * in original space (NULL for a cold start).  Returns its ticket.  May be
+ * called from any thread, also while another thread is inside
+ * niryukti_batch_step / _run; every other call on one batch must come from
+ * one thread at a time. */
 int niryukti_batch_submit(niryukti_batch *b, const niryukti_lp *lp, const double *x0, const double *y0,
                           int64_t *ticket);
 /* One scheduling round: admit queued LPs into free slots, run

@@ -1426,8 +1426,10 @@ constexpr int kShapeCount = int(sizeof kShapes / sizeof kShapes[0]);
 // Autotuner of one crowded LDS class: the class's first scheduling rounds
 // try each shape once (every LP of the class in the launch, so the rounds
 // still do full work), then the class keeps the shape with the lowest
-// step-kernel time per attempt, measured by HIP events.  A round whose class
-// ran fewer than kMinTrialLps LPs does not count as a trial.
+// step-kernel time per algorithmic byte (HIP events over the attempts' bytes:
+// rounds of one class hold LPs of different sizes, so time per attempt would
+// compare unlike rounds).  A round whose class ran fewer than kMinTrialLps
+// LPs does not count as a trial.
 class ShapeTuner {
   public:
     static constexpr int kMinTrialLps = 64;
@@ -1437,16 +1439,16 @@ class ShapeTuner {
     }
     bool decided() const { return fixed_ || tried_ == kShapeCount; }
     int next() const { return decided() ? chosen_ : order_[tried_]; }
-    void record(int shape, double seconds, long long attempts, int lps) {
-        if (decided() || shape != order_[tried_] || lps < kMinTrialLps || attempts <= 0) return;
-        per_attempt_[shape] = seconds / double(attempts);
+    void record(int shape, double seconds, double bytes, int lps) {
+        if (decided() || shape != order_[tried_] || lps < kMinTrialLps || !(bytes > 0)) return;
+        per_byte_[shape] = seconds / bytes;
         if (++tried_ == kShapeCount) {
             chosen_ = 0;
             for (int k = 1; k < kShapeCount; ++k)
-                if (per_attempt_[k] < per_attempt_[chosen_]) chosen_ = k;
+                if (per_byte_[k] < per_byte_[chosen_]) chosen_ = k;
         }
     }
-    double per_attempt(int shape) const { return per_attempt_[shape]; }
+    double per_byte(int shape) const { return per_byte_[shape]; }
     // The measured-best shape is tried first, so a short batch already runs it.
     void start_with(int shape) {
         chosen_ = shape;
@@ -1460,7 +1462,7 @@ class ShapeTuner {
     bool fixed_ = false;
     int tried_ = 0, chosen_ = 0;
     int order_[kShapeCount] = {0, 1, 2, 3};
-    double per_attempt_[kShapeCount] = {};
+    double per_byte_[kShapeCount] = {};
 };
 
 class BatchEngine {
@@ -1491,8 +1493,8 @@ class BatchEngine {
                 for (int c = 0; c < kClasses - 1; ++c) tuner_[c].fix(c == 0 ? tuner_[0].next() : 3);
         // Class capacities (m + n) bounded by what one workgroup may allocate
         // next to the kernel's static LDS.
-        int device = 0;
-        HIP_OK(hipGetDevice(&device));
+        HIP_OK(hipGetDevice(&device_));
+        const int device = device_;
         hipDeviceProp_t prop;
         HIP_OK(hipGetDeviceProperties(&prop, device));
         const long long room = (long long)prop.sharedMemPerBlock - kStaticLds - 16;
@@ -1575,9 +1577,11 @@ class BatchEngine {
             if (cs) hipStreamSynchronize(cs);
     }
 
+    // Any thread, also while the scheduling thread is inside step(): the copy
+    // and the checks run on the caller's thread; the job reaches jobs_ when
+    // the scheduling thread next drains incoming_.
     int64_t submit(const niryukti_lp &lp, const double *x0, const double *y0) {
         auto job = std::make_unique<Job>();
-        job->ticket = int64_t(jobs_.size());
         job->start = Clock::now();
         nyk::Model &m0 = job->original;
         const int64_t nnz = lp.ptr[lp.rows];
@@ -1602,16 +1606,30 @@ class BatchEngine {
             if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
         for (double v : job->warm_y)
             if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
-        start_preparers();
-        jobs_.push_back(std::move(job));
-        Job *j = jobs_.back().get();
+        Job *j = job.get();
+        int64_t ticket = 0;
+        {
+            // tickets are handed out in the order jobs enter incoming_, so a
+            // drain appends them to jobs_ at their own index
+            std::lock_guard<std::mutex> lock(submit_mu_);
+            start_preparers();
+            ticket = j->ticket = next_ticket_++;
+            incoming_.push_back(std::move(job));
+        }
+        ++prep_outstanding_;
         {
             std::lock_guard<std::mutex> lock(prep_mu_);
             prep_todo_.push_back(j);
         }
-        ++prep_outstanding_;
         prep_cv_.notify_one();
-        return j->ticket;
+        return ticket;
+    }
+
+    // Scheduling thread: moves submitted jobs into jobs_ (indexed by ticket).
+    void drain_incoming() {
+        std::lock_guard<std::mutex> lock(submit_mu_);
+        for (auto &job : incoming_) jobs_.push_back(std::move(job));
+        incoming_.clear();
     }
 
     // ---- host prepare off the launch path ----
@@ -1623,8 +1641,7 @@ class BatchEngine {
     // frees some.  Results do not depend on when an LP is admitted.
     void start_preparers() {
         if (!preparers_.empty()) return;
-        int device = 0;
-        HIP_OK(hipGetDevice(&device));
+        const int device = device_;
         const int workers = std::max(1, nyk::host_threads());
         for (int w = 0; w < workers; ++w) preparers_.emplace_back([this, device] { prepare_loop(device); });
     }
@@ -1711,6 +1728,9 @@ class BatchEngine {
                 ready_cv_.wait(lock, [&] { return !prep_ready_.empty(); });
             ready.swap(prep_ready_);
         }
+        // every job taken from prep_ready_ entered incoming_ before its
+        // prepare started (submit_mu_, then prep_mu_), so it is drained here
+        drain_incoming();
         std::sort(ready.begin(), ready.end(), [](const Job *a, const Job *b) { return a->ticket < b->ticket; });
         for (Job *j : ready) {
             --prep_outstanding_;
@@ -1730,6 +1750,7 @@ class BatchEngine {
     }
 
     int64_t step(int checks) {
+        drain_incoming();
         admit();
         std::vector<int> by_class[kClasses];
         std::vector<int> active;
@@ -1780,6 +1801,13 @@ class BatchEngine {
             }
             int shape_of[kClasses - 1];
             for (int c = 0; c < kClasses - 1; ++c) shape_of[c] = tuner_[c].next();
+            // The tail launch continues past `checks` periods only when it
+            // has the round to itself: beside crowded launches (which take
+            // `checks` periods) a longer tail launch would hold the round,
+            // and with it every crowded LP, until its budget ran out.
+            bool crowded = false;
+            for (const Group &g : groups) crowded = crowded || g.kind != kClasses;
+            const long long tail_budget = crowded ? 0 : tail_budget_ticks_;
             for (const Group &g : groups) {
                 auto &ev = class_events_[g.kind];
                 while (ev.size() < size_t(2 * checks + 2)) {
@@ -1793,7 +1821,7 @@ class BatchEngine {
                 const int *slots = d_slots_ + size_t(g.kind) * max_slots_;
                 HIP_OK(hipEventRecord(ev[0], cs));
                 if (g.kind == kClasses) {
-                    launch_tail(g.cap, grid, cs, slots, bo, checks);
+                    launch_tail(g.cap, grid, cs, slots, bo, checks, tail_budget);
                     HIP_OK(hipEventRecord(ev[1], cs));
                 } else {
                     for (int k = 0; k < checks; ++k) {
@@ -1832,8 +1860,7 @@ class BatchEngine {
                 }
                 for (int s : g.slots) slot_group_[s] = g.kind;
             }
-            double group_seconds[kClasses + 1] = {};
-            long long group_attempts[kClasses + 1] = {};
+            double group_seconds[kClasses + 1] = {}, group_bytes[kClasses + 1] = {};
             int group_lps[kClasses + 1] = {};
             for (const Group &g : groups) group_lps[g.kind] = int(g.slots.size());
             for (const Group &g : groups) {
@@ -1847,7 +1874,7 @@ class BatchEngine {
             for (size_t k = 0; k < active.size(); ++k) {
                 const LpState &st = state_[active[k]];
                 const long long done = st.accepted + st.rejected - before[k];
-                group_attempts[slot_group_[active[k]]] += done;
+                group_bytes[slot_group_[active[k]]] += double(done) * slot_bytes_[active[k]];
                 attempts_ += done;
                 advance_bytes_ += double(done) * slot_bytes_[active[k]];
                 class_bytes_[slot_group_[active[k]]] += double(done) * slot_bytes_[active[k]];
@@ -1855,7 +1882,7 @@ class BatchEngine {
                     class_bytes_[kClasses] += double(st.checks - checks_before[k]) * slot_check_bytes_[active[k]];
             }
             for (int c = 0; c < kClasses - 1; ++c)
-                if (group_lps[c]) tuner_[c].record(shape_of[c], group_seconds[c], group_attempts[c], group_lps[c]);
+                if (group_lps[c]) tuner_[c].record(shape_of[c], group_seconds[c], group_bytes[c], group_lps[c]);
             device_seconds_ += std::chrono::duration<double>(Clock::now() - t0).count();
             ++launches_;
         }
@@ -1894,12 +1921,14 @@ class BatchEngine {
 
     // Frees the host copy of a finished LP (model, prepare maps, result).
     void release(int64_t ticket) {
+        drain_incoming();
         if (ticket < 0 || ticket >= int64_t(jobs_.size()) || !jobs_[ticket]) throw std::runtime_error("Unknown ticket");
         if (!jobs_[ticket]->finished) throw std::runtime_error("Ticket has not finished");
         jobs_[ticket].reset();
     }
 
     bool result(int64_t ticket, niryukti_result *out) {
+        drain_incoming();
         if (ticket < 0 || ticket >= int64_t(jobs_.size())) throw std::runtime_error("Unknown ticket");
         if (!jobs_[ticket]) throw std::runtime_error("Ticket was released");
         Job &j = *jobs_[ticket];
@@ -1945,7 +1974,8 @@ class BatchEngine {
         return true;
     }
 
-    void stats(niryukti_batch_stats *s) const {
+    void stats(niryukti_batch_stats *s) {
+        drain_incoming();
         s->submitted = int64_t(jobs_.size());
         s->queued = int64_t(queue_.size()) + prep_outstanding_;
         s->active = 0;
@@ -2154,10 +2184,11 @@ class BatchEngine {
     // The tail group: one fused launch per scheduling round, one row per
     // thread over 1,024 threads (2 or 4 rows per thread spill registers at
     // this size and were measured slower).  Each LP runs at least `checks`
-    // periods and then continues until kTailBudgetMs have passed.
-    void launch_tail(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo, int checks) {
+    // periods and then continues until `budget` ticks have passed.
+    void launch_tail(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo, int checks,
+                     long long budget) {
         batch_tail_kernel<kTailThreads><<<grid, kTailThreads, lds_bytes(cap), cs>>>(
-            d_desc_, d_state_, slots, bo, checks, checks * kTailMaxPeriodsPerCheck, tail_budget_ticks_);
+            d_desc_, d_state_, slots, bo, checks, checks * kTailMaxPeriodsPerCheck, budget);
     }
     // A crowded LDS class in the launch shape its tuner picked.
     void launch_lds(int cap, int shape, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
@@ -2841,7 +2872,12 @@ class BatchEngine {
     double class_seconds_[kClasses + 1] = {}, class_bytes_[kClasses + 1] = {};  // per class, then the tail group
     int64_t class_launches_[kClasses + 1] = {};
     int64_t attempts_ = 0;
-    std::vector<std::unique_ptr<Job>> jobs_;
+    int device_ = 0;
+    std::vector<std::unique_ptr<Job>> jobs_;  // scheduling thread only, indexed by ticket
+    // submitted jobs not yet in jobs_, and the next ticket (submit_mu_)
+    std::mutex submit_mu_;
+    std::vector<std::unique_ptr<Job>> incoming_;
+    int64_t next_ticket_ = 0;
     std::deque<int64_t> queue_, done_;
     // prepare workers: jobs to prepare, prepared jobs, and how many of the
     // submitted jobs the scheduling thread has not yet collected
@@ -2852,7 +2888,7 @@ class BatchEngine {
     std::deque<Job *> prep_todo_;
     std::vector<Job *> prep_ready_;
     std::atomic<bool> prep_stop_{false};
-    int64_t prep_outstanding_ = 0;
+    std::atomic<int64_t> prep_outstanding_{0};
     // handoff to the grid-wide engine
     int64_t handoff_ = 0;
     int handoff_workers_ = 4;

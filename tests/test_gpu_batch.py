@@ -218,3 +218,47 @@ def test_autotuner_keeps_results(engine, monkeypatch):
     for a, b in zip(tuned, fixed):
         assert a["status"] == b["status"] and a["iterations"] == b["iterations"]
         assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
+
+
+def test_submit_from_another_thread_beside_steps(engine):
+    """niryukti_batch_submit on a second thread while the scheduling thread
+    steps (the streamed-pass pattern): every LP finishes once, under the ticket
+    submit returned, bit-identical to the same LP solved alone."""
+    import threading
+
+    lps = [engine.Instance(2, 42, 1, i).lp for i in range(48)]
+    b = engine.Batch(arena_bytes=4 << 30, max_slots=64)
+    tickets, errors = [None] * len(lps), []
+
+    def submitter():
+        try:
+            for k, lp in enumerate(lps):
+                tickets[k] = b.submit(lp)
+                if k % 8 == 7:
+                    threading.Event().wait(0.005)  # arrive over several rounds
+        except BaseException as e:
+            errors.append(e)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    import time
+
+    out, t0 = {}, time.perf_counter()
+    while True:
+        running = b.step(2)
+        for t in b.finished(1024):
+            assert t not in out
+            out[t] = b.result(t)
+        if not th.is_alive() and running == 0 and len(out) == len(lps):
+            break
+        assert time.perf_counter() - t0 < 60
+    th.join()
+    b.close()
+    assert not errors
+    assert sorted(tickets) == list(range(len(lps)))
+    for k in (0, 17, 47):
+        alone, _ = solve_batch(engine, [lps[k]])
+        r = out[tickets[k]]
+        assert r["status"] == alone[0]["status"] == "OPTIMAL"
+        assert r["iterations"] == alone[0]["iterations"]
+        assert np.array_equal(r["x"], alone[0]["x"]) and np.array_equal(r["y"], alone[0]["y"])
