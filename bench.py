@@ -44,6 +44,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "LPs solved/sec to rel-KKT≤1e-6 + p50/p99 latency, served vs plain PDHG, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+# launch shapes (threads, rows per thread) of the crowded classes' step kernel,
+# in the order of the engine's per-shape statistics
+LDS_SHAPES = ((256, 1), (256, 2), (256, 4), (1024, 1))
 SEED = 42
 
 
@@ -439,21 +442,17 @@ def summarize_traffic(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy
 TRAFFIC_PROFILE = ROOT / "profiles" / "batch_advance_hbm_traffic_4096lp.txt"
 
 
-def committed_traffic(kernel, rank):
-    """HBM bytes per launch of `kernel` from the committed FETCH/WRITE summary;
-    `rank` picks among instantiations that share a symbol by LDS size."""
+def committed_traffic(kernel):
+    """HBM bytes per launch of `kernel` (a kernel symbol, whichever classes
+    launched it) from the committed FETCH/WRITE summary, or None."""
     if not TRAFFIC_PROFILE.exists():
         return None
-    entries = []
     for line in TRAFFIC_PROFILE.read_text().splitlines():
         if line.startswith("{"):
             e = json.loads(line)
             if e["kernel"].replace(" ", "") == kernel.replace(" ", ""):
-                entries.append(e)
-    entries.sort(key=lambda e: e["lds_bytes"])
-    if rank >= len(entries):
-        return None
-    return entries[rank]["hbm_bytes_per_dispatch"]
+                return e["hbm_bytes_per_dispatch"]
+    return None
 
 
 def _cpu_model():
@@ -575,31 +574,36 @@ def main():
             log(f"pass {k}: {solved}/{len(insts)} solved")
         log(f"{args.steps} passes streamed in {total_t:.3f} s, stats {after}")
         stats_all.append({key: [a - c for a, c in zip(after[key], before[key])]
-                          for key in ("class_bytes", "class_seconds", "class_launches")})
+                          for key in ("class_bytes", "class_seconds", "class_launches",
+                                      "shape_bytes", "shape_seconds", "shape_launches")})
         stats_all[0].update({key: after[key] for key in ("class_threads", "class_rows", "class_tuned", "handoffs")})
         last = passes[-1]
     value = total_solved / total_t
     # Dominant kernel: the launch class (one step-kernel instantiation) that
-    # moved the most algorithmic bytes; its rate = bytes / its event time.
-    groups = len(stats_all[0]["class_bytes"])
-    cls_b = [sum(st["class_bytes"][c] for st in stats_all) for c in range(groups)]
-    cls_s = [sum(st["class_seconds"][c] for st in stats_all) for c in range(groups)]
-    cls_n = [sum(st["class_launches"][c] for st in stats_all) for c in range(groups)]
-    dom = max(range(groups), key=lambda c: cls_b[c])
-    adv_b, adv_s, launches = cls_b[dom], cls_s[dom], cls_n[dom]
-    achieved = adv_b / adv_s / 1e9 if adv_s > 0 else 0.0
+    # moved the most algorithmic bytes over the timed region, whichever
+    # classes launched it (a kernel trace reports per symbol, too); its rate
+    # = bytes / its HIP-event time.
+    def total(key, k):
+        return sum(st[key][k] for st in stats_all)
+
     last_stats = stats_all[-1]
-    shapes = [f"batch_advance_lds_kernel<{last_stats['class_threads'][c]}, {last_stats['class_rows'][c]}>"
-              for c in range(3)]
-    symbol = shapes + ["batch_advance_kernel", "batch_tail_kernel<1024>"]
-    symbol = symbol[dom]
-    kernel_name = symbol + [" (class m+n <= 2048, LDS)", " (class m+n <= 4096, LDS)", " (class m+n <= 8000, LDS)",
-                            " (global class)", " (tail group: attempts and checks in one launch)"][dom]
-    # classes that share a kernel symbol differ in dynamic LDS (it grows with
-    # the class), so the profile's entries for that symbol, sorted by LDS,
-    # follow the class order
-    rank = sum(1 for c in range(min(dom, 3)) if shapes[c] == symbol)
-    traffic = committed_traffic(symbol, rank)
+    kernels = [(f"batch_advance_lds_kernel<{t}, {r}>", total("shape_bytes", k), total("shape_seconds", k),
+                total("shape_launches", k)) for k, (t, r) in enumerate(LDS_SHAPES)]
+    kernels += [("batch_advance_kernel", total("class_bytes", 3), total("class_seconds", 3), total("class_launches", 3)),
+                ("batch_tail_kernel<1024>", total("class_bytes", 4), total("class_seconds", 4),
+                 total("class_launches", 4))]
+    symbol, adv_b, adv_s, launches = max(kernels, key=lambda k: k[1])
+    achieved = adv_b / adv_s / 1e9 if adv_s > 0 else 0.0
+    users = [name for c, name in enumerate(("m+n <= 2048", "m+n <= 4096", "m+n <= 8000"))
+             if (last_stats["class_threads"][c], last_stats["class_rows"][c]) in
+             [(t, r) for (t, r) in LDS_SHAPES if symbol == f"batch_advance_lds_kernel<{t}, {r}>"]]
+    if symbol.startswith("batch_advance_lds_kernel"):
+        kernel_name = f"{symbol} (crowded LDS classes; tuned choice of class {', '.join(users) or 'none'})"
+    elif symbol == "batch_advance_kernel":
+        kernel_name = symbol + " (global class)"
+    else:
+        kernel_name = symbol + " (tail group: attempts and checks in one launch)"
+    traffic = committed_traffic(symbol)
     statuses = {}
     for r in last:
         statuses[r["status"]] = statuses.get(r["status"], 0) + 1

@@ -48,6 +48,12 @@ typedef struct {
      * workgroup, rows per thread): the autotuner's current choice, and 1 once
      * its trials are over.  Shapes never change results. */
     int32_t class_threads[3], class_rows[3], class_tuned[3], pad;
+    /* Per step-kernel instantiation of the crowded LDS classes (256 x 1,
+     * 256 x 2, 256 x 4, 1024 x 1 threads x rows), whichever class launched it:
+     * HIP-event time, algorithmic bytes and launches (what a kernel trace
+     * reports per kernel symbol). */
+    double shape_seconds[4], shape_bytes[4];
+    int64_t shape_launches[4];
 } niryukti_batch_stats;
 
 /* opts: solver options shared by every LP of the batch (NULL = defaults).

@@ -112,7 +112,8 @@ class _BatchStats(C.Structure):
         [("attempts", C.c_int64), ("class_seconds", C.c_double * 5), ("class_bytes", C.c_double * 5),
          ("class_launches", C.c_int64 * 5), ("handoffs", C.c_int64), ("handoffs_running", C.c_int64),
          ("class_threads", C.c_int32 * 3), ("class_rows", C.c_int32 * 3), ("class_tuned", C.c_int32 * 3),
-         ("pad", C.c_int32)]
+         ("pad", C.c_int32), ("shape_seconds", C.c_double * 4), ("shape_bytes", C.c_double * 4),
+         ("shape_launches", C.c_int64 * 4)]
 
 
 _lib = None
@@ -659,7 +660,7 @@ class Batch:
         out = {}
         for k, _ in _BatchStats._fields_:
             v = getattr(st, k)
-            out[k] = list(v) if k.startswith("class_") else v
+            out[k] = list(v) if k.startswith(("class_", "shape_")) else v
         return out
 
     def close(self):

@@ -1422,47 +1422,88 @@ struct Shape {
 };
 constexpr Shape kShapes[] = {{256, 1}, {256, 2}, {256, 4}, {1024, 1}};
 constexpr int kShapeCount = int(sizeof kShapes / sizeof kShapes[0]);
+static_assert(kShapeCount == 4, "niryukti_batch_stats holds four shape entries");
 
-// Autotuner of one crowded LDS class: the class's first scheduling rounds
-// try each shape once (every LP of the class in the launch, so the rounds
-// still do full work), then the class keeps the shape with the lowest
-// step-kernel time per algorithmic byte (HIP events over the attempts' bytes:
-// rounds of one class hold LPs of different sizes, so time per attempt would
-// compare unlike rounds).  A round whose class ran fewer than kMinTrialLps
-// LPs does not count as a trial.
+// Autotuner of the crowded LDS classes' launch shapes, one class at a time
+// (coordinate descent from the start shapes, which were measured best on
+// gfx950): each shape of the class under trial runs kRepeats scheduling
+// rounds, interleaved with the other shapes in mirrored order (ABCD DCBA
+// ABCD DCBA: a drift of the rounds' cost falls on every shape alike; the
+// rounds still do full work), while the other classes keep their current
+// shapes.  The class then keeps the shape with the highest mean round
+// throughput -- algorithmic bytes of every crowded launch of the round over
+// the round's device time -- when it beats the current shape by at least
+// kMargin (single rounds of one class differ by several percent with no
+// change of shape).  The score is the round's and not the class's own: the
+// classes run side by side, and a shape with more workgroups per CU takes
+// slots from its neighbours (scored alone, such a shape won its class's
+// trials and lost end to end).  Trials count only once the class under trial
+// has run at least kMinTrialLps LPs (two per CU of the 256) for kSettle
+// rounds: below that a launch lasts as long as its slowest LP, whatever the
+// shape, and the first rounds of a batch run while prepares still admit LPs.
 class ShapeTuner {
   public:
-    static constexpr int kMinTrialLps = 64;
-    void fix(int shape) {
-        fixed_ = true;
-        chosen_ = shape;
+    static constexpr int kTuned = 3;  // the crowded LDS classes
+    static constexpr int kMinTrialLps = 512;  // NIRYUKTI_BATCH_TUNE_MIN_LPS overrides it
+    static constexpr int kSettle = 2;
+    static constexpr int kRepeats = 4;
+    static constexpr int kTrials = kRepeats * kShapeCount;
+    static constexpr double kMargin = 0.08;
+    void start_with(int c, int shape) { current_[c] = shape; }
+    void set_min_lps(int lps) { min_lps_ = std::max(1, lps); }
+    void fix(int c, int shape) {
+        current_[c] = shape;
+        fixed_[c] = true;
     }
-    bool decided() const { return fixed_ || tried_ == kShapeCount; }
-    int next() const { return decided() ? chosen_ : order_[tried_]; }
-    void record(int shape, double seconds, double bytes, int lps) {
-        if (decided() || shape != order_[tried_] || lps < kMinTrialLps || !(bytes > 0)) return;
-        per_byte_[shape] = seconds / bytes;
-        if (++tried_ == kShapeCount) {
-            chosen_ = 0;
-            for (int k = 1; k < kShapeCount; ++k)
-                if (per_byte_[k] < per_byte_[chosen_]) chosen_ = k;
+    bool decided(int c) const { return fixed_[c] || done_[c]; }
+    int current(int c) const { return current_[c]; }
+    // The shape class c runs in the next round.
+    int next(int c) const { return c == trial_class() && settled_ >= kSettle ? trial_shape(tried_) : current_[c]; }
+    // One round: `bytes` of every crowded launch over `seconds` of device time.
+    void record(const int *shape_of, const int *lps, double bytes, double seconds) {
+        const int c = trial_class();
+        if (c < 0) return;
+        if (lps[c] < min_lps_) {
+            settled_ = 0;  // trials so far stand; they resume after kSettle full rounds
+            return;
         }
-    }
-    double per_byte(int shape) const { return per_byte_[shape]; }
-    // The measured-best shape is tried first, so a short batch already runs it.
-    void start_with(int shape) {
-        chosen_ = shape;
-        int k = 0;
-        order_[k++] = shape;
-        for (int s = 0; s < kShapeCount; ++s)
-            if (s != shape) order_[k++] = s;
+        if (settled_ < kSettle) {
+            ++settled_;
+            return;
+        }
+        if (shape_of[c] != trial_shape(tried_) || !(bytes > 0) || !(seconds > 0)) return;
+        rate_[shape_of[c]] += bytes / seconds;
+        if (++tried_ < kTrials) return;
+        int best = current_[c];
+        for (int k = 0; k < kShapeCount; ++k)
+            if (rate_[k] > rate_[best]) best = k;
+        if (rate_[best] >= (1 + kMargin) * rate_[current_[c]]) current_[c] = best;
+        done_[c] = true;
+        tried_ = 0;
+        for (double &r : rate_) r = 0;
     }
 
   private:
-    bool fixed_ = false;
-    int tried_ = 0, chosen_ = 0;
-    int order_[kShapeCount] = {0, 1, 2, 3};
-    double per_byte_[kShapeCount] = {};
+    int trial_class() const {
+        for (int c = 0; c < kTuned; ++c)
+            if (!fixed_[c] && !done_[c]) return c;
+        return -1;
+    }
+    // Trial t of the class under trial: its current shape first, then the
+    // others by index, mirrored every other sweep.
+    int trial_shape(int t) const {
+        const int c = trial_class();
+        int order[kShapeCount], k = 0;
+        order[k++] = current_[c];
+        for (int s = 0; s < kShapeCount; ++s)
+            if (s != current_[c]) order[k++] = s;
+        const int i = t % kShapeCount;
+        return (t / kShapeCount) % 2 ? order[kShapeCount - 1 - i] : order[i];
+    }
+    int current_[kTuned] = {};
+    bool fixed_[kTuned] = {}, done_[kTuned] = {};
+    int tried_ = 0, settled_ = 0, min_lps_ = kMinTrialLps;
+    double rate_[kShapeCount] = {};
 };
 
 class BatchEngine {
@@ -1482,15 +1523,32 @@ class BatchEngine {
         // Shapes measured best on gfx950 (tried first): 256 threads x 1 row
         // up to m + n = 2,048, 1,024 x 1 above.  NIRYUKTI_BATCH_ROWS_PER_THREAD
         // (1, 2 or 4) fixes the smallest class at 256 threads x that many rows;
-        // NIRYUKTI_BATCH_TUNE=0 keeps the measured shapes without trials.
-        for (int c = 0; c < kClasses - 1; ++c) tuner_[c].start_with(c == 0 ? 0 : 3);
+        // NIRYUKTI_BATCH_TUNE=0 keeps the measured shapes without trials;
+        // NIRYUKTI_BATCH_TUNE_MIN_LPS=n lets rounds with n LPs in the class
+        // count as trials (a small batch then reaches a decision);
+        // NIRYUKTI_BATCH_SHAPES=a,b,c fixes the three classes' shapes (indices
+        // into kShapes), e.g. so that profiler runs of one workload launch the
+        // same kernels.
+        static_assert(kClasses - 1 == ShapeTuner::kTuned, "one tuned shape per crowded LDS class");
+        for (int c = 0; c < kClasses - 1; ++c) tuner_.start_with(c, c == 0 ? 0 : 3);
         if (const char *env = std::getenv("NIRYUKTI_BATCH_ROWS_PER_THREAD")) {
             const int r = std::atoi(env);
-            tuner_[0].fix(r == 4 ? 2 : r == 2 ? 1 : 0);
+            tuner_.fix(0, r == 4 ? 2 : r == 2 ? 1 : 0);
         }
         if (const char *env = std::getenv("NIRYUKTI_BATCH_TUNE"))
             if (std::atoi(env) == 0)
-                for (int c = 0; c < kClasses - 1; ++c) tuner_[c].fix(c == 0 ? tuner_[0].next() : 3);
+                for (int c = 0; c < kClasses - 1; ++c) tuner_.fix(c, tuner_.current(c));
+        if (const char *env = std::getenv("NIRYUKTI_BATCH_TUNE_MIN_LPS")) tuner_.set_min_lps(std::atoi(env));
+        if (const char *env = std::getenv("NIRYUKTI_BATCH_SHAPES")) {
+            int shape[kClasses - 1];
+            if (std::sscanf(env, "%d,%d,%d", &shape[0], &shape[1], &shape[2]) != kClasses - 1)
+                throw std::runtime_error("NIRYUKTI_BATCH_SHAPES takes three shape indices, e.g. 0,3,3");
+            for (int c = 0; c < kClasses - 1; ++c) {
+                if (shape[c] < 0 || shape[c] >= kShapeCount)
+                    throw std::runtime_error("NIRYUKTI_BATCH_SHAPES: shape index out of range");
+                tuner_.fix(c, shape[c]);
+            }
+        }
         // Class capacities (m + n) bounded by what one workgroup may allocate
         // next to the kernel's static LDS.
         HIP_OK(hipGetDevice(&device_));
@@ -1800,7 +1858,7 @@ class BatchEngine {
                 checks_before[k] = state_[active[k]].checks;
             }
             int shape_of[kClasses - 1];
-            for (int c = 0; c < kClasses - 1; ++c) shape_of[c] = tuner_[c].next();
+            for (int c = 0; c < kClasses - 1; ++c) shape_of[c] = tuner_.next(c);
             // The tail launch continues past `checks` periods only when it
             // has the round to itself: beside crowded launches (which take
             // `checks` periods) a longer tail launch would hold the round,
@@ -1881,8 +1939,14 @@ class BatchEngine {
                 if (slot_group_[active[k]] == kClasses)  // the fused tail launch also did the checks
                     class_bytes_[kClasses] += double(st.checks - checks_before[k]) * slot_check_bytes_[active[k]];
             }
+            const double round_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
+            tuner_.record(shape_of, group_lps, group_bytes[0] + group_bytes[1] + group_bytes[2], round_seconds);
             for (int c = 0; c < kClasses - 1; ++c)
-                if (group_lps[c]) tuner_[c].record(shape_of[c], group_seconds[c], group_bytes[c], group_lps[c]);
+                if (group_lps[c]) {
+                    shape_seconds_[shape_of[c]] += group_seconds[c];
+                    shape_bytes_[shape_of[c]] += group_bytes[c];
+                    shape_launches_[shape_of[c]] += checks;
+                }
             device_seconds_ += std::chrono::duration<double>(Clock::now() - t0).count();
             ++launches_;
         }
@@ -2003,12 +2067,17 @@ class BatchEngine {
             s->class_launches[c] = class_launches_[c];
         }
         for (int c = 0; c < kClasses - 1; ++c) {
-            const Shape sh = kShapes[tuner_[c].next()];
+            const Shape sh = kShapes[tuner_.next(c)];
             s->class_threads[c] = sh.threads;
             s->class_rows[c] = sh.rows;
-            s->class_tuned[c] = tuner_[c].decided() ? 1 : 0;
+            s->class_tuned[c] = tuner_.decided(c) ? 1 : 0;
         }
         s->pad = 0;
+        for (int k = 0; k < kShapeCount; ++k) {
+            s->shape_seconds[k] = shape_seconds_[k];
+            s->shape_bytes[k] = shape_bytes_[k];
+            s->shape_launches[k] = shape_launches_[k];
+        }
     }
 
     // Waits for every submitted LP's prepare, then admits what the slots hold.
@@ -2863,7 +2932,7 @@ class BatchEngine {
     CopyPiece *d_pieces_ = nullptr;
     long long stage_capacity_ = 0, piece_capacity_ = 0;
     long long pow_count_ = 0;
-    ShapeTuner tuner_[kClasses - 1];
+    ShapeTuner tuner_;
     int class_cap_[kClasses - 1] = {};
     std::vector<hipEvent_t> class_events_[kClasses + 1];
     hipEvent_t uploaded_ = nullptr;
@@ -2871,6 +2940,8 @@ class BatchEngine {
     double advance_seconds_ = 0, check_seconds_ = 0, advance_bytes_ = 0;
     double class_seconds_[kClasses + 1] = {}, class_bytes_[kClasses + 1] = {};  // per class, then the tail group
     int64_t class_launches_[kClasses + 1] = {};
+    double shape_seconds_[kShapeCount] = {}, shape_bytes_[kShapeCount] = {};
+    int64_t shape_launches_[kShapeCount] = {};
     int64_t attempts_ = 0;
     int device_ = 0;
     std::vector<std::unique_ptr<Job>> jobs_;  // scheduling thread only, indexed by ticket

@@ -204,20 +204,39 @@ def test_power_iterations_in_batch(engine):
 
 
 def test_autotuner_keeps_results(engine, monkeypatch):
-    """The launch-shape autotuner tries every shape on the first rounds of a
-    crowded class and keeps the fastest; results are bit-identical to a run
-    with the tuner off (shapes never change the arithmetic)."""
-    insts = [engine.Instance(2, 5, 1, i) for i in range(700)]
+    """The launch-shape autotuner tries every shape four times on the first
+    full rounds of a crowded class and keeps the start shape unless another is
+    clearly faster; results are bit-identical to a run with the tuner off
+    (shapes never change the arithmetic)."""
+    insts = [engine.Instance(2, 5, 1, i) for i in range(2000)]
     lps = [i.lp for i in insts]
-    tuned, stats = solve_batch(engine, lps, checks=1)
+    # every LP resident from the first round, and trials counted from 128 LPs
+    # in the class (512 by default): the smallest class (over 256 LPs, so not
+    # in the tail group) reaches a decision within this batch, after 2
+    # settling and 16 trial rounds
+    assert sum(i.lp.rows + i.lp.cols <= 2048 for i in insts) > 1200
+    monkeypatch.setenv("NIRYUKTI_BATCH_TUNE_MIN_LPS", "128")
+    b = engine.Batch(arena_bytes=8 << 30, max_slots=4096)
+    tickets = [b.submit(lp) for lp in lps]
+    assert b.admit() == len(lps)
+    b.run(1)
+    tuned, stats = [b.result(t) for t in tickets], b.stats()
+    b.close()
+    monkeypatch.delenv("NIRYUKTI_BATCH_TUNE_MIN_LPS")
     print("tuned shapes", stats["class_threads"], stats["class_rows"], stats["class_tuned"], flush=True)
     assert stats["class_tuned"][0] == 1
     monkeypatch.setenv("NIRYUKTI_BATCH_TUNE", "0")
-    fixed, fstats = solve_batch(engine, lps, checks=1)
+    fixed, fstats = solve_batch(engine, lps, checks=1, arena=8 << 30)
     assert list(fstats["class_threads"]) == [256, 1024, 1024] and list(fstats["class_rows"]) == [1, 1, 1]
+    monkeypatch.delenv("NIRYUKTI_BATCH_TUNE")
+    monkeypatch.setenv("NIRYUKTI_BATCH_SHAPES", "2,1,0")
+    other, ostats = solve_batch(engine, lps[:600], checks=1)
+    assert list(ostats["class_threads"]) == [256, 256, 256] and list(ostats["class_rows"]) == [4, 2, 1]
     for a, b in zip(tuned, fixed):
         assert a["status"] == b["status"] and a["iterations"] == b["iterations"]
         assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
+    for a, b in zip(other, fixed):
+        assert a["iterations"] == b["iterations"] and np.array_equal(a["x"], b["x"])
 
 
 def test_submit_from_another_thread_beside_steps(engine):
