@@ -705,19 +705,25 @@ __device__ __forceinline__ void rows_dot(const int *k, const int *e, const int *
 
 // Dot products like rows_dot, read from the sliced layout: the rows of A (or
 // A^T) are cut into slices of 64 consecutive rows, one per wave, and a slice
-// stores its entries entry-major (entry u of its 64 rows are 64 consecutive
-// values, padded to the slice's longest row).  A wave's load of entry u is
-// then one contiguous 512-byte request instead of 64 scattered ones: the step
-// kernel is bound by the per-CU rate of memory requests, not by bytes.  Each
-// row is still summed in storage order, so the sums equal rows_dot's.
+// stores its entries in pairs, entry-major: entries 2t and 2t+1 of its 64
+// rows are 128 consecutive values (lane l's two side by side), padded to an
+// even width at least the slice's longest row.  A wave's load of an entry
+// pair is then one contiguous 1 KiB request (16 bytes a lane) and of its two
+// 16-bit indices one 256-byte request: the step kernel is bound by the CU's
+// rate of memory instructions, not by bytes, and pairs halve the matrix's
+// instructions.  Each row is still summed in storage order, so the sums equal
+// rows_dot's.
+typedef double double2v __attribute__((ext_vector_type(2)));
+typedef unsigned short ushort2v __attribute__((ext_vector_type(2)));
 template <int NR>
 __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const int *sb,
                                               const GLOBAL_AS unsigned short *si, const GLOBAL_AS double *sv, int slices, int b0,
                                               int nt, const double *vec, double *acc) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // Every lane of a slice owns `w` stored entries (padding is value 0,
-    // index 0), so loads below a slice's width need no per-lane test: only
-    // the sum skips a row's padding.  Widths are the same for the whole wave.
+    // index 0; w is even), so loads below a slice's width need no per-lane
+    // test: only the sum skips a row's padding.  Widths are the same for the
+    // whole wave.
     int base[NR], len[NR], w[NR], width = 0, common = 1 << 30;
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
@@ -727,13 +733,14 @@ __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const 
         len[q] = e[q] - k[q];
         if (s < slices) {
             const int b = sb[s];
-            base[q] = b + lane;
+            base[q] = b + 2 * lane;
             w[q] = (sb[s + 1] - b) / 64;
         }
         w[q] = __builtin_amdgcn_readfirstlane(w[q]);
         width = max(width, w[q]);
         common = min(common, w[q]);
     }
+    static_assert(kStep % 2 == 0, "entries are stored in pairs");
     // one chunk of kStep entries per row; `full` chunks lie within every
     // row's slice and need no width test
     auto chunk = [&](int t, bool full) {
@@ -741,16 +748,21 @@ __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const 
         int ix[NR][kStep];
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-            const GLOBAL_AS double *pv = sv + base[q] + t * 64;
-            const GLOBAL_AS unsigned short *pi = si + base[q] + t * 64;
+            // pair t/2 + h of this lane starts at base + (t/2 + h) * 128
+            const GLOBAL_AS double2v *pv = (const GLOBAL_AS double2v *)(sv + base[q] + t * 64);
+            const GLOBAL_AS ushort2v *pi = (const GLOBAL_AS ushort2v *)(si + base[q] + t * 64);
 #pragma unroll
-            for (int u = 0; u < kStep; ++u) {
-                if (full || t + u < w[q]) {
-                    v[q][u] = pv[u * 64];
-                    ix[q][u] = pi[u * 64];
+            for (int h = 0; h < kStep / 2; ++h) {
+                if (full || t + 2 * h < w[q]) {
+                    const double2v vv = pv[h * 64];
+                    const ushort2v ii = pi[h * 64];
+                    v[q][2 * h] = vv.x;
+                    v[q][2 * h + 1] = vv.y;
+                    ix[q][2 * h] = ii.x;
+                    ix[q][2 * h + 1] = ii.y;
                 } else {
-                    v[q][u] = 0.0;
-                    ix[q][u] = 0;
+                    v[q][2 * h] = v[q][2 * h + 1] = 0.0;
+                    ix[q][2 * h] = ix[q][2 * h + 1] = 0;
                 }
             }
         }
@@ -1383,6 +1395,9 @@ struct Sliced {
     std::vector<double> value;
 };
 
+// The sliced layout of rows_dot_sell: per slice of 64 rows an even width
+// (the longest row, rounded up), entry u of row r at
+// base + (u / 2) * 128 + (r % 64) * 2 + u % 2.
 Sliced slice_rows(const std::vector<int64_t> &ptr, const std::vector<int64_t> &index, const std::vector<double> &value) {
     const int64_t rows = int64_t(ptr.size()) - 1, slices = (rows + 63) / 64;
     Sliced z;
@@ -1390,13 +1405,13 @@ Sliced slice_rows(const std::vector<int64_t> &ptr, const std::vector<int64_t> &i
     for (int64_t sl = 0; sl < slices; ++sl) {
         int64_t width = 0;
         for (int64_t r = sl * 64; r < std::min(rows, sl * 64 + 64); ++r) width = std::max(width, ptr[r + 1] - ptr[r]);
-        z.base[sl + 1] = z.base[sl] + width * 64;
+        z.base[sl + 1] = z.base[sl] + (width + 1) / 2 * 2 * 64;
     }
     z.index.assign(z.base[slices], 0);
     z.value.assign(z.base[slices], 0.0);
     for (int64_t r = 0; r < rows; ++r)
         for (int64_t u = 0; u < ptr[r + 1] - ptr[r]; ++u) {
-            const int64_t at = z.base[r / 64] + u * 64 + r % 64;
+            const int64_t at = z.base[r / 64] + u / 2 * 128 + r % 64 * 2 + u % 2;
             z.index[at] = index[ptr[r] + u];
             z.value[at] = value[ptr[r] + u];
         }
