@@ -121,20 +121,23 @@ def batch_stream(b, insts, passes, checks, threshold):
             with lock:
                 state["done_submitting"] = True
 
-    out = {}
+    out, collected = {}, []
     switch = sys.getswitchinterval()
     sys.setswitchinterval(1e-4)  # the scheduling thread's bookkeeping should not wait on the submitter
     t0 = time.perf_counter()
     th = threading.Thread(target=submitter, daemon=True)
     th.start()
+    in_step = in_results = 0.0
     try:
         while True:
+            ts = time.perf_counter()
             running = b.step(checks)
-            fin = b.finished(1 << 16)
-            for t in fin:
-                r = b.result(t)
-                out[t] = {k: r[k] for k in ("status", "kkt_error", "objective", "seconds", "iterations", "message")}
-                b.release(t)
+            tr = time.perf_counter()
+            fin, rows = b.collect()  # results and release of every LP that finished, in one call
+            if len(fin):
+                collected.append((fin, rows))
+            in_step += tr - ts
+            in_results += time.perf_counter() - tr
             with lock:
                 state["finished"] += len(fin)
                 lock.notify_all()
@@ -143,15 +146,21 @@ def batch_stream(b, insts, passes, checks, threshold):
                 raise state["error"]
             if finished_all:
                 break
-            if running == 0 and not fin:
+            if running == 0 and not len(fin):
                 time.sleep(1e-4)  # the submitter is between passes
         elapsed = time.perf_counter() - t0
+        log(f"stream: {elapsed:.3f} s, inside step {in_step:.3f} s, reading results {in_results:.3f} s")
     finally:
         sys.setswitchinterval(switch)
         with lock:
             state["abort"] = True
             lock.notify_all()
         th.join()
+    for fin, rows in collected:
+        for t, r in zip(fin.tolist(), rows):
+            out[t] = {"status": r["status"].decode(), "kkt_error": float(r["kkt_error"]),
+                      "objective": float(r["objective"]), "seconds": float(r["seconds"]),
+                      "iterations": int(r["iterations"]), "message": r["message"].decode()}
     return elapsed, [[out[t] for t in row] for row in tickets]
 
 
@@ -573,6 +582,10 @@ def main():
             lat += latencies
             log(f"pass {k}: {solved}/{len(insts)} solved")
         log(f"{args.steps} passes streamed in {total_t:.3f} s, stats {after}")
+        d = {k: after[k] - before[k] for k in ("device_seconds", "check_seconds", "advance_seconds", "launches")}
+        log(f"timed region: {d['launches']} rounds, host waiting on the device {d['device_seconds']:.3f} s of "
+            f"{total_t:.3f} s; summed event time: advance {d['advance_seconds']:.3f} s, checks {d['check_seconds']:.3f} s; "
+            "per class " + ", ".join(f"{a - c:.3f}" for a, c in zip(after["class_seconds"], before["class_seconds"])))
         stats_all.append({key: [a - c for a, c in zip(after[key], before[key])]
                           for key in ("class_bytes", "class_seconds", "class_launches",
                                       "shape_bytes", "shape_seconds", "shape_launches")})

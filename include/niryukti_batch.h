@@ -99,6 +99,12 @@ int niryukti_batch_finished(niryukti_batch *b, int64_t *tickets, int64_t capacit
 /* Frees the host copy of a finished LP once its result has been read; the
  * ticket is invalid afterwards.  Long-running servers call it per request. */
 int niryukti_batch_release(niryukti_batch *b, int64_t ticket);
+/* niryukti_batch_finished + niryukti_batch_result (+ niryukti_batch_release
+ * when `release` is 1) for up to `capacity` finished tickets in one call:
+ * tickets[k] and results[k] (every field but x and y, which are not copied;
+ * their pointers are overwritten with NULL).  *count receives how many. */
+int niryukti_batch_collect(niryukti_batch *b, int64_t *tickets, niryukti_result *results, int64_t capacity,
+                           int release, int64_t *count);
 const char *niryukti_batch_last_error(void);
 
 #ifdef __cplusplus

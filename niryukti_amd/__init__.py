@@ -64,6 +64,26 @@ class _Result(C.Structure):
                 ("first_restart_iteration", C.c_int64), ("fp32_iterations", C.c_int64), ("x", _f64p), ("y", _f64p)]
 
 
+def _struct_dtype(cls) -> np.dtype:
+    """numpy view of a ctypes structure of scalars, char arrays and pointers."""
+    kinds = {C.c_double: np.float64, C.c_int: np.int32, C.c_int64: np.int64}
+    names, formats, offsets = [], [], []
+    for name, ctype in cls._fields_:
+        if ctype in kinds:
+            fmt = kinds[ctype]
+        elif issubclass(ctype, C.Array) and ctype._type_ is C.c_char:
+            fmt = f"S{ctype._length_}"
+        else:  # pointers
+            fmt = np.uint64
+        names.append(name)
+        formats.append(fmt)
+        offsets.append(getattr(cls, name).offset)
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets, "itemsize": C.sizeof(cls)})
+
+
+_RESULT_DTYPE = _struct_dtype(_Result)
+
+
 class _Accuracy(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("objective", "primal", "dual", "gap", "kkt", "complementarity",
                                           "lower_bound", "primal_absolute", "dual_absolute")] + [("finite", C.c_int)]
@@ -170,6 +190,8 @@ def load_library() -> C.CDLL:
     lib.niryukti_batch_admit.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     lib.niryukti_batch_result.argtypes = [C.c_void_p, C.c_int64, C.POINTER(_Result)]
     lib.niryukti_batch_stats_get.argtypes = [C.c_void_p, C.POINTER(_BatchStats)]
+    lib.niryukti_batch_collect.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(_Result), C.c_int64, C.c_int,
+                                           C.POINTER(C.c_int64)]
     lib.niryukti_batch_last_error.restype = C.c_char_p
     lib.niryukti_batch_finished.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]
     lib.niryukti_batch_release.argtypes = [C.c_void_p, C.c_int64]
@@ -630,7 +652,8 @@ class Batch:
         self._check(self._lib.niryukti_batch_run(self._h, int(checks_per_launch)))
 
     def result(self, ticket: int):
-        n, m = self._shapes[ticket]
+        # an unknown or released ticket gets the engine's own error
+        n, m = self._shapes.get(ticket, (0, 0))
         x, y = np.zeros(n), np.zeros(m)
         res = _Result()
         res.x, res.y = _ptr(x), _ptr(y)
@@ -648,6 +671,26 @@ class Batch:
         n = C.c_int64()
         self._check(self._lib.niryukti_batch_finished(self._h, buf, int(capacity), C.byref(n)))
         return list(buf[:n.value])
+
+    def collect(self, capacity: int = 8192, release: bool = True):
+        """Finished tickets and their results (every field but x and y) in one
+        call: (tickets, numpy structured array with the fields of
+        niryukti_result, status and message as bytes); with `release` each
+        LP's host copy is freed."""
+        if getattr(self, "_collect_cap", 0) < capacity:
+            self._collect_tickets = (C.c_int64 * capacity)()
+            self._collect_results = (_Result * capacity)()
+            self._collect_cap = capacity
+        n = C.c_int64()
+        self._check(self._lib.niryukti_batch_collect(self._h, self._collect_tickets, self._collect_results,
+                                                     int(capacity), 1 if release else 0, C.byref(n)))
+        k = n.value
+        tickets = np.frombuffer(self._collect_tickets, dtype=np.int64, count=k).copy()
+        rows = np.frombuffer(self._collect_results, dtype=_RESULT_DTYPE, count=k).copy()
+        if release:
+            for t in tickets.tolist():
+                self._shapes.pop(t, None)
+        return tickets, rows
 
     def release(self, ticket: int) -> None:
         """Frees the host copy of a finished LP; the ticket is invalid afterwards."""

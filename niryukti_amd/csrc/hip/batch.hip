@@ -120,6 +120,9 @@ struct LpDev {
     const int *asb, *tsb;
     const unsigned short *asi, *tsi;
     const double *asv, *tsv;
+    // (lb, ub) and (rl, ru) interleaved, so the step reads a bound pair with
+    // one 16-byte load (the LDS classes; null for the global class)
+    const double *lub, *rlu;
     int nab, ntb, nob, notb, nabw, nobw, notbw;
 };
 
@@ -704,26 +707,28 @@ __device__ __forceinline__ void rows_dot(const int *k, const int *e, const int *
 }
 
 // Dot products like rows_dot, read from the sliced layout: the rows of A (or
-// A^T) are cut into slices of 64 consecutive rows, one per wave, and a slice
-// stores its entries in pairs, entry-major: entries 2t and 2t+1 of its 64
-// rows are 128 consecutive values (lane l's two side by side), padded to an
-// even width at least the slice's longest row.  A wave's load of an entry
-// pair is then one contiguous 1 KiB request (16 bytes a lane) and of its two
-// 16-bit indices one 256-byte request: the step kernel is bound by the CU's
-// rate of memory instructions, not by bytes, and pairs halve the matrix's
-// instructions.  Each row is still summed in storage order, so the sums equal
-// rows_dot's.
+// A^T) are cut into slices of 64 consecutive rows, one per wave, padded to an
+// even width w at least the slice's longest row.  Values are stored in
+// pairs, entry-major (entries 2t and 2t+1 of the 64 rows are 128 consecutive
+// values, lane l's two side by side), indices in quads (entries 4t..4t+3 of
+// the 64 rows are 256 consecutive 16-bit indices, lane l's four side by
+// side; when w is not a multiple of 4 the last two entries form a pair
+// region of 128).  A wave's load of a value pair is one contiguous 1 KiB
+// request and of four indices one 512-byte request: the step kernel is bound
+// by the CU's rate of memory instructions, not by bytes, and the grouping
+// takes the matrix from two instructions per entry to three per four.  Each
+// row is still summed in storage order, so the sums equal rows_dot's.
 typedef double double2v __attribute__((ext_vector_type(2)));
 typedef unsigned short ushort2v __attribute__((ext_vector_type(2)));
+typedef unsigned short ushort4v __attribute__((ext_vector_type(4)));
 template <int NR>
 __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const int *sb,
                                               const GLOBAL_AS unsigned short *si, const GLOBAL_AS double *sv, int slices, int b0,
                                               int nt, const double *vec, double *acc) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // Every lane of a slice owns `w` stored entries (padding is value 0,
-    // index 0; w is even), so loads below a slice's width need no per-lane
-    // test: only the sum skips a row's padding.  Widths are the same for the
-    // whole wave.
+    // index 0), so loads below a slice's width need no per-lane test: only
+    // the sum skips a row's padding.  Widths are the same for the whole wave.
     int base[NR], len[NR], w[NR], width = 0, common = 1 << 30;
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
@@ -733,14 +738,15 @@ __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const 
         len[q] = e[q] - k[q];
         if (s < slices) {
             const int b = sb[s];
-            base[q] = b + 2 * lane;
+            base[q] = b;
             w[q] = (sb[s + 1] - b) / 64;
         }
+        base[q] = __builtin_amdgcn_readfirstlane(base[q]);
         w[q] = __builtin_amdgcn_readfirstlane(w[q]);
         width = max(width, w[q]);
         common = min(common, w[q]);
     }
-    static_assert(kStep % 2 == 0, "entries are stored in pairs");
+    static_assert(kStep % 4 == 0, "chunks hold whole index quads");
     // one chunk of kStep entries per row; `full` chunks lie within every
     // row's slice and need no width test
     auto chunk = [&](int t, bool full) {
@@ -748,22 +754,36 @@ __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const 
         int ix[NR][kStep];
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-            // pair t/2 + h of this lane starts at base + (t/2 + h) * 128
-            const GLOBAL_AS double2v *pv = (const GLOBAL_AS double2v *)(sv + base[q] + t * 64);
-            const GLOBAL_AS ushort2v *pi = (const GLOBAL_AS ushort2v *)(si + base[q] + t * 64);
+            // value pair t/2 + h of this lane at base + (t/2 + h) * 128 + 2 lane
+            const GLOBAL_AS double2v *pv = (const GLOBAL_AS double2v *)(sv + base[q] + t * 64 + 2 * lane);
 #pragma unroll
             for (int h = 0; h < kStep / 2; ++h) {
                 if (full || t + 2 * h < w[q]) {
                     const double2v vv = pv[h * 64];
-                    const ushort2v ii = pi[h * 64];
                     v[q][2 * h] = vv.x;
                     v[q][2 * h + 1] = vv.y;
-                    ix[q][2 * h] = ii.x;
-                    ix[q][2 * h + 1] = ii.y;
                 } else {
                     v[q][2 * h] = v[q][2 * h + 1] = 0.0;
-                    ix[q][2 * h] = ix[q][2 * h + 1] = 0;
                 }
+            }
+            // index quad t/4 + h at base + (t/4 + h) * 256 + 4 lane (a last
+            // pair region at + 2 lane)
+            const GLOBAL_AS unsigned short *pi = si + base[q] + t * 64;
+#pragma unroll
+            for (int h = 0; h < kStep / 4; ++h) {
+                const int u0 = t + 4 * h;
+                ushort4v ii = {0, 0, 0, 0};
+                if (full || u0 + 4 <= w[q]) {
+                    ii = *(const GLOBAL_AS ushort4v *)(pi + h * 256 + 4 * lane);
+                } else if (u0 + 2 <= w[q]) {
+                    const ushort2v p2 = *(const GLOBAL_AS ushort2v *)(pi + h * 256 + 2 * lane);
+                    ii.x = p2.x;
+                    ii.y = p2.y;
+                }
+                ix[q][4 * h] = ii.x;
+                ix[q][4 * h + 1] = ii.y;
+                ix[q][4 * h + 2] = ii.z;
+                ix[q][4 * h + 3] = ii.w;
             }
         }
 #pragma unroll
@@ -825,7 +845,8 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
     GLOBAL_AS double *xn = gout(c1 ? L.x[0] : L.x[1]), *xan = gout(c1 ? L.xa[0] : L.xa[1]),
                      *yn = gout(c1 ? L.y[0] : L.y[1]), *yan = gout(c1 ? L.ya[0] : L.ya[1]),
                      *axn = gout(c1 ? L.ax[0] : L.ax[1]);
-    const GLOBAL_AS double *c = gin(L.c), *lb = gin(L.lb), *ub = gin(L.ub), *rl = gin(L.rl), *ru = gin(L.ru);
+    const GLOBAL_AS double *c = gin(L.c);
+    const GLOBAL_AS double2v *lub = (const GLOBAL_AS double2v *)L.lub, *rlu = (const GLOBAL_AS double2v *)L.rlu;
     // the step rule's powers load with y, off the decision's serial path
     // (the barriers after the two passes order them before the decision)
     if (threadIdx.x == 64 % NT) powlds[0] = pow_m3(o, S.accepted + 2);
@@ -853,8 +874,9 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             g[q] = ok[q] ? c[j] : 0.0;
             xj_[q] = x[j];
             xa_[q] = xa[j];
-            lo_[q] = lb[j];
-            hi_[q] = ub[j];
+            const double2v bj = lub[j];
+            lo_[q] = bj.x;
+            hi_[q] = bj.y;
         }
         rows_dot_sell<NR>(k, e, tsblds, gin(L.tsi), gin(L.tsv), (L.n + 63) / 64, b0, NT, ylds, g);
 #pragma unroll
@@ -892,8 +914,9 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             delta[q] = 0.0;
             a_[q] = ax[i];
             ya_[q] = ya[i];
-            lo_[q] = rl[i];
-            hi_[q] = ru[i];
+            const double2v bi = rlu[i];
+            lo_[q] = bi.x;
+            hi_[q] = bi.y;
         }
         rows_dot_sell<NR>(k, e, asblds, gin(L.asi), gin(L.asv), (L.m + 63) / 64, b0, NT, dxlds, delta);
 #pragma unroll
@@ -1216,6 +1239,8 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
     L.tsv = uniform(L.tsv);
     L.asi = uniform(L.asi);
     L.asv = uniform(L.asv);
+    L.lub = uniform(L.lub);
+    L.rlu = uniform(L.rlu);
     L.n = __builtin_amdgcn_readfirstlane(L.n);
     L.m = __builtin_amdgcn_readfirstlane(L.m);
     double *ylds = dyn, *dxlds = dyn + L.m;
@@ -1395,9 +1420,11 @@ struct Sliced {
     std::vector<double> value;
 };
 
-// The sliced layout of rows_dot_sell: per slice of 64 rows an even width
-// (the longest row, rounded up), entry u of row r at
-// base + (u / 2) * 128 + (r % 64) * 2 + u % 2.
+// The sliced layout of rows_dot_sell: per slice of 64 rows an even width w
+// (the longest row, rounded up); the value of entry u of row r at
+// base + (u / 2) * 128 + (r % 64) * 2 + u % 2, its index at
+// base + (u / 4) * 256 + (r % 64) * 4 + u % 4, or, in the last pair of a
+// width that is not a multiple of 4, at base + (u / 4) * 256 + (r % 64) * 2 + u % 2.
 Sliced slice_rows(const std::vector<int64_t> &ptr, const std::vector<int64_t> &index, const std::vector<double> &value) {
     const int64_t rows = int64_t(ptr.size()) - 1, slices = (rows + 63) / 64;
     Sliced z;
@@ -1411,9 +1438,10 @@ Sliced slice_rows(const std::vector<int64_t> &ptr, const std::vector<int64_t> &i
     z.value.assign(z.base[slices], 0.0);
     for (int64_t r = 0; r < rows; ++r)
         for (int64_t u = 0; u < ptr[r + 1] - ptr[r]; ++u) {
-            const int64_t at = z.base[r / 64] + u / 2 * 128 + r % 64 * 2 + u % 2;
-            z.index[at] = index[ptr[r] + u];
-            z.value[at] = value[ptr[r] + u];
+            const int64_t b = z.base[r / 64], w = (z.base[r / 64 + 1] - b) / 64;
+            const int64_t quad = u / 4 * 4 + 4 <= w;
+            z.value[b + u / 2 * 128 + r % 64 * 2 + u % 2] = value[ptr[r] + u];
+            z.index[b + u / 4 * 256 + (quad ? r % 64 * 4 + u % 4 : r % 64 * 2 + u % 2)] = index[ptr[r] + u];
         }
     return z;
 }
@@ -1623,6 +1651,12 @@ class BatchEngine {
         }
         handoff_cv_.notify_all();
         for (auto &w : workers_) w.join();
+        {
+            std::lock_guard<std::mutex> lock(reap_mu_);
+            reap_stop_ = true;
+        }
+        reap_cv_.notify_all();
+        if (reaper_.joinable()) reaper_.join();
         {
             std::lock_guard<std::mutex> lock(prep_mu_);
             prep_stop_ = true;
@@ -1998,12 +2032,48 @@ class BatchEngine {
         return k;
     }
 
+    // Finished tickets with their results in one call (x and y are not
+    // copied), each released after its copy when `release_after`.
+    int64_t collect(int64_t *tickets, niryukti_result *out, int64_t capacity, bool release_after) {
+        drain_incoming();
+        int64_t k = 0;
+        while (k < capacity && !done_.empty()) {
+            const int64_t t = done_.front();
+            done_.pop_front();
+            std::memset(&out[k], 0, sizeof out[k]);  // x = y = NULL, strings padded with zeros
+            result(t, &out[k]);
+            tickets[k++] = t;
+            if (release_after) release(t);
+        }
+        return k;
+    }
+
     // Frees the host copy of a finished LP (model, prepare maps, result).
+    // The memory is returned on a reaper thread: freeing an LP's large
+    // vectors gives pages back to the system, which on the scheduling
+    // thread would hold up the next launch.
     void release(int64_t ticket) {
         drain_incoming();
         if (ticket < 0 || ticket >= int64_t(jobs_.size()) || !jobs_[ticket]) throw std::runtime_error("Unknown ticket");
         if (!jobs_[ticket]->finished) throw std::runtime_error("Ticket has not finished");
-        jobs_[ticket].reset();
+        {
+            std::lock_guard<std::mutex> lock(reap_mu_);
+            if (!reaper_.joinable()) reaper_ = std::thread([this] { reap_loop(); });
+            reap_.push_back(std::move(jobs_[ticket]));
+        }
+        reap_cv_.notify_one();
+    }
+    void reap_loop() {
+        std::unique_lock<std::mutex> lock(reap_mu_);
+        for (;;) {
+            reap_cv_.wait(lock, [&] { return reap_stop_ || !reap_.empty(); });
+            std::vector<std::unique_ptr<Job>> batch;
+            batch.swap(reap_);
+            lock.unlock();
+            batch.clear();
+            lock.lock();
+            if (reap_stop_ && reap_.empty()) return;
+        }
     }
 
     bool result(int64_t ticket, niryukti_result *out) {
@@ -2389,6 +2459,8 @@ class BatchEngine {
                o_asv = L.add<double>(sa.value.size());
         size_t o_tsb = L.add<int>(st.base.size()), o_tsi = L.add<unsigned short>(st.index.size()),
                o_tsv = L.add<double>(st.value.size());
+        const bool lds = j.cls < kClasses - 1;
+        const size_t o_lub = lds ? L.add<double>(2 * size_t(n)) : 0, o_rlu = lds ? L.add<double>(2 * size_t(mr)) : 0;
         char *const base = place(L.total);
         image.assign(L.total, 0);
         auto put_i = [&](size_t off, const std::vector<int64_t> &v) {
@@ -2517,6 +2589,21 @@ class BatchEngine {
         put_i(o_tsb, st.base);
         put_u16(o_tsi, st.index);
         put_d(o_tsv, st.value);
+        d.lub = d.rlu = nullptr;
+        if (lds) {
+            double *lub = reinterpret_cast<double *>(image.data() + o_lub);
+            double *rlu = reinterpret_cast<double *>(image.data() + o_rlu);
+            for (int jj = 0; jj < n; ++jj) {
+                lub[2 * jj] = m.lb[size_t(jj)];
+                lub[2 * jj + 1] = m.ub[size_t(jj)];
+            }
+            for (int i = 0; i < mr; ++i) {
+                rlu[2 * i] = m.rl[size_t(i)];
+                rlu[2 * i + 1] = m.ru[size_t(i)];
+            }
+            d.lub = (const double *)P(o_lub);
+            d.rlu = (const double *)P(o_rlu);
+        }
         d.asb = (const int *)P(o_asb);
         d.asi = (const unsigned short *)P(o_asi);
         d.asv = (const double *)P(o_asv);
@@ -2964,6 +3051,12 @@ class BatchEngine {
     std::mutex submit_mu_;
     std::vector<std::unique_ptr<Job>> incoming_;
     int64_t next_ticket_ = 0;
+    // released jobs, freed on the reaper thread (reap_mu_)
+    std::mutex reap_mu_;
+    std::condition_variable reap_cv_;
+    std::vector<std::unique_ptr<Job>> reap_;
+    bool reap_stop_ = false;
+    std::thread reaper_;
     std::deque<int64_t> queue_, done_;
     // prepare workers: jobs to prepare, prepared jobs, and how many of the
     // submitted jobs the scheduling thread has not yet collected
@@ -3094,6 +3187,14 @@ int niryukti_batch_finished(niryukti_batch *b, int64_t *tickets, int64_t capacit
     return guarded([&] {
         if (!b || !count || (capacity > 0 && !tickets)) throw std::runtime_error("Null argument");
         *count = b->engine.finished(tickets, capacity);
+    });
+}
+
+int niryukti_batch_collect(niryukti_batch *b, int64_t *tickets, niryukti_result *results, int64_t capacity,
+                           int release, int64_t *count) {
+    return guarded([&] {
+        if (!b || !count || (capacity > 0 && (!tickets || !results))) throw std::runtime_error("Null argument");
+        *count = b->engine.collect(tickets, results, capacity, release != 0);
     });
 }
 

@@ -281,3 +281,30 @@ def test_submit_from_another_thread_beside_steps(engine):
         assert r["status"] == alone[0]["status"] == "OPTIMAL"
         assert r["iterations"] == alone[0]["iterations"]
         assert np.array_equal(r["x"], alone[0]["x"]) and np.array_equal(r["y"], alone[0]["y"])
+
+
+def test_collect_returns_every_result_once(engine):
+    """niryukti_batch_collect: every finished ticket once, with the fields
+    niryukti_batch_result gives, and released when asked."""
+    lps = [engine.Instance(2, 42, 1, i).lp for i in range(24)]
+    ref, _ = solve_batch(engine, lps)
+    b = engine.Batch(arena_bytes=4 << 30, max_slots=64)
+    tickets = [b.submit(lp) for lp in lps]
+    seen = {}
+    while True:
+        running = b.step(2)
+        fin, rows = b.collect(capacity=5)  # small capacity: several calls per round
+        for t, r in zip(fin.tolist(), rows):
+            assert t not in seen
+            seen[t] = r
+        if running == 0 and len(seen) == len(lps):
+            break
+    assert sorted(seen) == tickets
+    for t, r in zip(tickets, ref):
+        row = seen[t]
+        assert row["status"].decode() == r["status"] == "OPTIMAL"
+        assert row["iterations"] == r["iterations"] and row["objective"] == r["objective"]
+        assert row["kkt_error"] == r["kkt_error"] and row["x"] == 0
+    with pytest.raises(RuntimeError):
+        b.result(tickets[0])  # released by collect
+    b.close()
