@@ -582,7 +582,10 @@ def main():
             lat += latencies
             log(f"pass {k}: {solved}/{len(insts)} solved")
         log(f"{args.steps} passes streamed in {total_t:.3f} s, stats {after}")
-        d = {k: after[k] - before[k] for k in ("device_seconds", "check_seconds", "advance_seconds", "launches")}
+        d = {k: after[k] - before[k] for k in ("device_seconds", "check_seconds", "advance_seconds", "launches",
+                                                "certify_wait_seconds", "settle_seconds", "admit_seconds")}
+        log(f"timed region host work in step: waiting for certification {d['certify_wait_seconds']:.3f} s, "
+            f"settling {d['settle_seconds']:.3f} s, admitting {d['admit_seconds']:.3f} s")
         log(f"timed region: {d['launches']} rounds, host waiting on the device {d['device_seconds']:.3f} s of "
             f"{total_t:.3f} s; summed event time: advance {d['advance_seconds']:.3f} s, checks {d['check_seconds']:.3f} s; "
             "per class " + ", ".join(f"{a - c:.3f}" for a, c in zip(after["class_seconds"], before["class_seconds"])))

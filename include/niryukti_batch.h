@@ -54,6 +54,10 @@ typedef struct {
      * reports per kernel symbol). */
     double shape_seconds[4], shape_bytes[4];
     int64_t shape_launches[4];
+    /* Host time of the scheduling thread outside the launches: waiting for
+     * the previous round's certification, downloading and sorting the
+     * stopped LPs of a round, admitting queued LPs into slots. */
+    double certify_wait_seconds, settle_seconds, admit_seconds;
 } niryukti_batch_stats;
 
 /* opts: solver options shared by every LP of the batch (NULL = defaults).

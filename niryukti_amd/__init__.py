@@ -133,7 +133,8 @@ class _BatchStats(C.Structure):
          ("class_launches", C.c_int64 * 5), ("handoffs", C.c_int64), ("handoffs_running", C.c_int64),
          ("class_threads", C.c_int32 * 3), ("class_rows", C.c_int32 * 3), ("class_tuned", C.c_int32 * 3),
          ("pad", C.c_int32), ("shape_seconds", C.c_double * 4), ("shape_bytes", C.c_double * 4),
-         ("shape_launches", C.c_int64 * 4)]
+         ("shape_launches", C.c_int64 * 4), ("certify_wait_seconds", C.c_double), ("settle_seconds", C.c_double),
+         ("admit_seconds", C.c_double)]
 
 
 _lib = None

@@ -1858,7 +1858,9 @@ class BatchEngine {
 
     int64_t step(int checks) {
         drain_incoming();
+        const auto ta = Clock::now();
         admit();
+        admit_seconds_ += std::chrono::duration<double>(Clock::now() - ta).count();
         std::vector<int> by_class[kClasses];
         std::vector<int> active;
         for (int s = 0; s < max_slots_; ++s)
@@ -2158,6 +2160,9 @@ class BatchEngine {
             s->class_tuned[c] = tuner_.decided(c) ? 1 : 0;
         }
         s->pad = 0;
+        s->certify_wait_seconds = certify_wait_seconds_;
+        s->settle_seconds = settle_seconds_;
+        s->admit_seconds = admit_seconds_;
         for (int k = 0; k < kShapeCount; ++k) {
             s->shape_seconds[k] = shape_seconds_[k];
             s->shape_bytes[k] = shape_bytes_[k];
@@ -2856,13 +2861,17 @@ class BatchEngine {
     void finish_settling() {
         if (!settling_) return;
         std::unique_ptr<Settling> p = std::move(settling_);
+        const auto t0 = Clock::now();
         p->task.wait();  // rethrows a certification failure
+        certify_wait_seconds_ += std::chrono::duration<double>(Clock::now() - t0).count();
         apply(*p);
     }
 
     void settle(const std::vector<int> &active) {
         finish_settling();
+        const auto t0 = Clock::now();
         std::unique_ptr<Settling> p = collect(active);
+        settle_seconds_ += std::chrono::duration<double>(Clock::now() - t0).count();
         if (!p) return;
         if (p->slots.size() <= size_t(kInlineCertify)) {
             // a few LPs: certify now (a millisecond or so), so their results
@@ -3080,7 +3089,7 @@ class BatchEngine {
     bool stopping_ = false;
     bool dirty_ = true;
     int64_t launches_ = 0, certifications_ = 0, farkas_checks_ = 0;
-    double device_seconds_ = 0;
+    double device_seconds_ = 0, certify_wait_seconds_ = 0, settle_seconds_ = 0, admit_seconds_ = 0;
 };
 
 template <class F>
