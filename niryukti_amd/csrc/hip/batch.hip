@@ -1180,7 +1180,10 @@ __device__ __noinline__ void check_period_call(const GLOBAL_AS LpDev *L, LDS_AS 
     check_period<NT, STAGE>(*(const LpDev *)L, *(LpState *)S, o, (double *)lds, (double *)stage);
 }
 
-__global__ __launch_bounds__(kThreads) void batch_check_kernel(const LpDev *lps, LpState *states, const int *slots,
+// Held to 4 waves per SIMD (128 VGPRs, some spills): at its natural 248 VGPRs a
+// check workgroup leaves no room on its CU for a 1,024-thread step workgroup
+// of another class until it ends (streamed bench +1.5 %, results unchanged).
+__global__ __launch_bounds__(kThreads, 4) void batch_check_kernel(const LpDev *lps, LpState *states, const int *slots,
                                                                BatchOpts o) {
     const int s = slots[blockIdx.x];
     __shared__ LpState S;
