@@ -4,8 +4,10 @@
 #pragma once
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstdint>
 #include <exception>
+#include <mutex>
 #include <stdexcept>
 #include <thread>
 #include <utility>
@@ -94,6 +96,88 @@ class Beside {
   private:
     std::thread thread_;
     std::exception_ptr error_;
+};
+
+// Items handed in from any thread and taken, in order, by one consumer (the
+// batched engine's submissions: any thread submits, the scheduling thread
+// files them by ticket).  push() gives each item the next sequence number
+// (0, 1, ...) under the same lock that appends it, so drain() returns the
+// items in sequence order and item n always lands at position n of a list
+// the consumer appends them to.
+template <class T>
+class Inbox {
+  public:
+    // `number(item, n)` runs under the lock with the item's sequence number.
+    template <class Number>
+    int64_t push(T item, Number &&number) {
+        std::lock_guard<std::mutex> lock(mu_);
+        const int64_t n = next_++;
+        number(item, n);
+        items_.push_back(std::move(item));
+        return n;
+    }
+    // Moves every item pushed so far to the end of `out`, in sequence order.
+    void drain(std::vector<T> &out) {
+        std::lock_guard<std::mutex> lock(mu_);
+        for (auto &item : items_) out.push_back(std::move(item));
+        items_.clear();
+    }
+    int64_t pushed() const {
+        std::lock_guard<std::mutex> lock(mu_);
+        return next_;
+    }
+
+  private:
+    mutable std::mutex mu_;
+    std::vector<T> items_;
+    int64_t next_ = 0;
+};
+
+// Destroys what it is given on a thread of its own: freeing large host
+// buffers returns pages to the system, which a latency-critical thread (the
+// batched engine's scheduling thread) should not wait for.  The thread starts
+// with the first item; the destructor destroys what is left and joins it.
+template <class T>
+class Reaper {
+  public:
+    Reaper() = default;
+    Reaper(const Reaper &) = delete;
+    Reaper &operator=(const Reaper &) = delete;
+    ~Reaper() {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (thread_.joinable()) thread_.join();
+    }
+    void give(T item) {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            if (!thread_.joinable()) thread_ = std::thread([this] { loop(); });
+            items_.push_back(std::move(item));
+        }
+        cv_.notify_one();
+    }
+
+  private:
+    void loop() {
+        std::unique_lock<std::mutex> lock(mu_);
+        for (;;) {
+            cv_.wait(lock, [&] { return stop_ || !items_.empty(); });
+            std::vector<T> batch;
+            batch.swap(items_);
+            lock.unlock();
+            batch.clear();  // the destructors run outside the lock
+            lock.lock();
+            if (stop_ && items_.empty()) return;
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::vector<T> items_;
+    bool stop_ = false;
+    std::thread thread_;
 };
 
 }  // namespace nyk

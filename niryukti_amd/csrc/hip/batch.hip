@@ -1655,12 +1655,6 @@ class BatchEngine {
         handoff_cv_.notify_all();
         for (auto &w : workers_) w.join();
         {
-            std::lock_guard<std::mutex> lock(reap_mu_);
-            reap_stop_ = true;
-        }
-        reap_cv_.notify_all();
-        if (reaper_.joinable()) reaper_.join();
-        {
             std::lock_guard<std::mutex> lock(prep_mu_);
             prep_stop_ = true;
             prep_todo_.clear();
@@ -1717,15 +1711,13 @@ class BatchEngine {
         for (double v : job->warm_y)
             if (!std::isfinite(v)) throw std::runtime_error("Nonfinite warm start");
         Job *j = job.get();
-        int64_t ticket = 0;
         {
-            // tickets are handed out in the order jobs enter incoming_, so a
-            // drain appends them to jobs_ at their own index
             std::lock_guard<std::mutex> lock(submit_mu_);
             start_preparers();
-            ticket = j->ticket = next_ticket_++;
-            incoming_.push_back(std::move(job));
         }
+        // the ticket is the job's sequence number in incoming_, so a drain
+        // appends each job to jobs_ at its own index
+        const int64_t ticket = incoming_.push(std::move(job), [](std::unique_ptr<Job> &q, int64_t n) { q->ticket = n; });
         ++prep_outstanding_;
         {
             std::lock_guard<std::mutex> lock(prep_mu_);
@@ -1736,11 +1728,7 @@ class BatchEngine {
     }
 
     // Scheduling thread: moves submitted jobs into jobs_ (indexed by ticket).
-    void drain_incoming() {
-        std::lock_guard<std::mutex> lock(submit_mu_);
-        for (auto &job : incoming_) jobs_.push_back(std::move(job));
-        incoming_.clear();
-    }
+    void drain_incoming() { incoming_.drain(jobs_); }
 
     // ---- host prepare off the launch path ----
     //
@@ -2061,24 +2049,7 @@ class BatchEngine {
         drain_incoming();
         if (ticket < 0 || ticket >= int64_t(jobs_.size()) || !jobs_[ticket]) throw std::runtime_error("Unknown ticket");
         if (!jobs_[ticket]->finished) throw std::runtime_error("Ticket has not finished");
-        {
-            std::lock_guard<std::mutex> lock(reap_mu_);
-            if (!reaper_.joinable()) reaper_ = std::thread([this] { reap_loop(); });
-            reap_.push_back(std::move(jobs_[ticket]));
-        }
-        reap_cv_.notify_one();
-    }
-    void reap_loop() {
-        std::unique_lock<std::mutex> lock(reap_mu_);
-        for (;;) {
-            reap_cv_.wait(lock, [&] { return reap_stop_ || !reap_.empty(); });
-            std::vector<std::unique_ptr<Job>> batch;
-            batch.swap(reap_);
-            lock.unlock();
-            batch.clear();
-            lock.lock();
-            if (reap_stop_ && reap_.empty()) return;
-        }
+        reaper_.give(std::move(jobs_[ticket]));
     }
 
     bool result(int64_t ticket, niryukti_result *out) {
@@ -3059,16 +3030,9 @@ class BatchEngine {
     int64_t attempts_ = 0;
     int device_ = 0;
     std::vector<std::unique_ptr<Job>> jobs_;  // scheduling thread only, indexed by ticket
-    // submitted jobs not yet in jobs_, and the next ticket (submit_mu_)
-    std::mutex submit_mu_;
-    std::vector<std::unique_ptr<Job>> incoming_;
-    int64_t next_ticket_ = 0;
-    // released jobs, freed on the reaper thread (reap_mu_)
-    std::mutex reap_mu_;
-    std::condition_variable reap_cv_;
-    std::vector<std::unique_ptr<Job>> reap_;
-    bool reap_stop_ = false;
-    std::thread reaper_;
+    std::mutex submit_mu_;                           // the prepare workers' start
+    nyk::Inbox<std::unique_ptr<Job>> incoming_;      // submitted jobs not yet in jobs_
+    nyk::Reaper<std::unique_ptr<Job>> reaper_;       // released jobs, freed on its thread
     std::deque<int64_t> queue_, done_;
     // prepare workers: jobs to prepare, prepared jobs, and how many of the
     // submitted jobs the scheduling thread has not yet collected

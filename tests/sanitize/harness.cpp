@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -264,6 +265,68 @@ void certification_beside() {
     expect(rethrown, "a failing task's error reaches the caller");
     for (auto *p : insts) niryukti_instance_destroy(p);
 }
+// The batched engine's submission inbox and job reaper under concurrent
+// producers: every item gets a distinct sequence number equal to its position
+// in the consumer's list, while the consumer drains beside the producers, and
+// every item given to the reaper is destroyed exactly once.
+void inbox_and_reaper() {
+    struct Item {
+        int64_t ticket = -1, producer = 0, seq = 0;
+        std::vector<double> payload;
+    };
+    nyk::Inbox<std::unique_ptr<Item>> inbox;
+    constexpr int kProducers = 4, kEach = 2000;
+    std::vector<std::thread> producers;
+    std::vector<std::vector<int64_t>> returned(kProducers);
+    for (int w = 0; w < kProducers; ++w)
+        producers.emplace_back([&, w] {
+            for (int k = 0; k < kEach; ++k) {
+                auto item = std::make_unique<Item>();
+                item->producer = w;
+                item->seq = k;
+                item->payload.assign(64, double(k));
+                returned[size_t(w)].push_back(
+                    inbox.push(std::move(item), [](std::unique_ptr<Item> &q, int64_t n) { q->ticket = n; }));
+            }
+        });
+    std::vector<std::unique_ptr<Item>> filed;
+    while (filed.size() < size_t(kProducers * kEach)) inbox.drain(filed);  // beside the producers
+    for (auto &t : producers) t.join();
+    inbox.drain(filed);
+    expect(filed.size() == size_t(kProducers * kEach) && inbox.pushed() == kProducers * kEach, "inbox keeps every item");
+    bool at_own_index = true;
+    std::vector<int64_t> last(kProducers, -1);
+    for (size_t k = 0; k < filed.size(); ++k) {
+        at_own_index = at_own_index && filed[k]->ticket == int64_t(k);
+        // one producer's items keep their order
+        at_own_index = at_own_index && filed[k]->seq > last[size_t(filed[k]->producer)];
+        last[size_t(filed[k]->producer)] = filed[k]->seq;
+    }
+    expect(at_own_index, "sequence numbers are positions");
+    for (int w = 0; w < kProducers; ++w)
+        for (int k = 0; k < kEach; ++k)
+            expect(filed[size_t(returned[size_t(w)][size_t(k)])]->seq == k, "push returns the item's number");
+
+    std::atomic<int64_t> destroyed{0};
+    struct Counted {
+        explicit Counted(std::atomic<int64_t> *c) : count(c), payload(512, 1.0) {}
+        Counted(const Counted &) = delete;
+        ~Counted() { count->fetch_add(1); }
+        std::atomic<int64_t> *count;
+        std::vector<double> payload;
+    };
+    {
+        nyk::Reaper<std::unique_ptr<Counted>> reaper;
+        std::vector<std::thread> givers;
+        for (int w = 0; w < 2; ++w)
+            givers.emplace_back([&] {
+                for (int k = 0; k < 5000; ++k)
+                    reaper.give(std::make_unique<Counted>(&destroyed));
+            });
+        for (auto &t : givers) t.join();
+    }  // the destructor destroys what is left and joins
+    expect(destroyed.load() == 10000, "the reaper destroys every item once");
+}
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -279,6 +342,7 @@ int main(int argc, char **argv) {
         checkpoints(netlib, tmp);
         serving();
         certification_beside();
+        inbox_and_reaper();
     } catch (const std::exception &e) {
         std::fprintf(stderr, "FAIL: exception %s\n", e.what());
         return 1;
