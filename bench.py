@@ -486,7 +486,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--checks", type=int, default=5, help="check periods per launch")
-    ap.add_argument("--stream-threshold", type=int, default=1024,
+    ap.add_argument("--stream-threshold", type=int, default=2048,
                     help="submit the next pass once fewer than this many LPs of earlier passes run")
     ap.add_argument("--no-stream", action="store_true",
                     help="passes one after another, each on a fresh engine (no overlap between passes)")

@@ -112,6 +112,7 @@ struct LpDev {
     double *x[2], *xa[2], *dx, *y[2], *ya[2], *ax[2];
     double *epx, *epy;
     double *ox[2], *oy[2], *act, *aty, *atyabs;
+    double *act1, *aty1, *atyabs1;  // the second candidate's products in a paired check
     double *bx, *by;
     const int *ablk, *tblk, *oblk, *otblk;  // row blocks of A, A^T, original A, original A^T
     const int *ablkw, *oblkw, *otblkw;      // the same for the tail kernel's wider check stage
@@ -385,6 +386,61 @@ __device__ void stream_rows(const int *ptr_, const int *idx_, const double *val_
     }
 }
 
+// stream_rows over two vectors at once (row blocks of at most kChunk
+// nonzeros, stage of 2 * kChunk doubles): each entry's value and index are
+// read once, and each vector's products are formed and summed exactly as
+// stream_rows forms them.  epi(r, sum0, abs_sum0, sum1, abs_sum1).
+template <bool ABS, int NT, int NTR, class Epi>
+__device__ void stream_rows2(const int *ptr_, const int *idx_, const double *val_, const int *blk_, int nblk,
+                             const double *vec0_, const double *vec1_, double *stage, double *red, Epi epi) {
+    const GLOBAL_AS int *ptr = gin(ptr_), *idx = gin(idx_), *blk = gin(blk_);
+    const GLOBAL_AS double *val = gin(val_), *vec0 = gin(vec0_), *vec1 = gin(vec1_);
+    double *stage1 = stage + kChunk;
+    for (int b = 0; b < nblk; ++b) {
+        const int r0 = blk[b], r1 = blk[b + 1];
+        const int k0 = ptr[r0], k1 = ptr[r1];
+        if (k1 - k0 <= kChunk || r1 - r0 > 1) {
+            for (int k = k0 + threadIdx.x; k < k1; k += NT) {
+                const double v = val[k];
+                const int c = idx[k];
+                stage[k - k0] = v * vec0[c];
+                stage1[k - k0] = v * vec1[c];
+            }
+            __syncthreads();
+            for (int r = r0 + threadIdx.x; r < r1; r += NT) {
+                double acc0 = 0, a0 = 0, acc1 = 0, a1 = 0;
+                const int e = ptr[r + 1] - k0;
+                for (int k = ptr[r] - k0; k < e; ++k) {
+                    acc0 += stage[k];
+                    acc1 += stage1[k];
+                    if (ABS) {
+                        a0 += fabs(stage[k]);
+                        a1 += fabs(stage1[k]);
+                    }
+                }
+                epi(r, acc0, a0, acc1, a1);
+            }
+            __syncthreads();
+        } else {
+            Reducer<4, 0, 0> rr{};
+            for (int k = first_index<NTR>(k0, k1); k < k1; k += NTR) {
+                const double v = val[k];
+                const int c = idx[k];
+                const double t0 = v * vec0[c], t1 = v * vec1[c];
+                rr.s[0] += t0;
+                rr.s[2] += t1;
+                if (ABS) {
+                    rr.s[1] += fabs(t0);
+                    rr.s[3] += fabs(t1);
+                }
+            }
+            block_reduce<NTR / 64>(rr, red);
+            if (threadIdx.x == 0) epi(r0, 0.0 + rr.s[0], rr.s[1], 0.0 + rr.s[2], rr.s[3]);
+            __syncthreads();
+        }
+    }
+}
+
 template <int NT = kThreads, int NTR = NT, int STAGE = kChunk>
 __device__ void spmv(const int *ptr, const int *idx, const double *val, const int *blk, int nblk, const double *x,
                      double *out, double *stage, double *red) {
@@ -443,37 +499,36 @@ __device__ void farkas_columns(const LpDev &L, Reducer<NS, NM, ND> &r) {
     }
 }
 
-// NT threads copy and multiply; the first NTR form every sum.  stage holds
-// STAGE doubles (kChunk, or kWideChunk with the wide row blocks).
-template <int NT, int NTR, int STAGE = kChunk>
-__device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double *ox, double *oy, double *lds,
-                        double *stage) {
-    constexpr bool wide = STAGE > kChunk;
-    // restore_x / restore_y (src/preprocess.cpp:121-132)
+// restore_x / restore_y (src/preprocess.cpp:121-132) of one candidate into
+// (ox, oy); the caller's barrier orders the writes before their readers.
+template <int NT>
+__device__ void restore_fill(const LpDev &L, double *ox, double *oy) {
     copy_vec<NT>(ox, L.fixed, L.n0);
     zero_vec<NT>(oy, L.m0);
-    __syncthreads();
+}
+template <int NT>
+__device__ void restore_scatter(const LpDev &L, const double *x, const double *y, double *ox, double *oy) {
     for (int j = threadIdx.x; j < L.n; j += NT) ox[gin(L.cols)[j]] = x[j] * gin(L.dc)[j];
     for (int i = threadIdx.x; i < L.m; i += NT) {
         const int t = y[i] >= 0 ? gin(L.upper_rows)[i] : gin(L.lower_rows)[i];
         oy[t] = 0.0 + y[i] * gin(L.dr)[i] * L.sobj;
     }
-    __syncthreads();
-    spmv<NT, NTR, STAGE>(L.op, L.oi, L.ov, wide ? L.oblkw : L.oblk, wide ? L.nobw : L.nob, ox, L.act, stage, lds);
-    stream_rows<true, NT, NTR, STAGE>(L.otp, L.oti, L.otv, wide ? L.otblkw : L.otblk, wide ? L.notbw : L.notb, oy,
-                                      stage, lds, [](int) { return 0.0; },
-                      [&](int j, double v, double a) {
-                          gout(L.aty)[j] = v;
-                          gout(L.atyabs)[j] = a;
-                      });
-    __syncthreads();
+}
+
+// Original-space accuracy of a restored candidate (ox, oy) whose products
+// are in act = A ox, aty = A^T oy and atyabs = |A^T| |oy| (summed per entry).
+// The first NTR threads form every sum.
+template <int NTR>
+__device__ Acc assess(const LpDev &L, const double *ox, const double *oy, const double *act_, const double *aty_,
+                      const double *atyabs_, double *lds) {
+    const GLOBAL_AS double *act = gin(act_), *aty = gin(aty_), *atyabs = gin(atyabs_);
     // s : objective, Farkas column bound sum, its |.| sum, Lipschitz slack, sum |y b|
     // mx: nonfinite, Farkas column -inf, primal, primal_abs, dual, dual_abs,
     //     y != 0 at an infinite row endpoint, infinite Lagrangian box argument
     // d : support, complement, box_min, sum y b over rows
     Reducer<5, 8, 4> r{};
     for (int i = first_index<NTR>(0, L.m0); i < L.m0; i += NTR) {
-        const double a = gin(L.act)[i], yi = oy[i], lo = gin(L.rl0)[i], hi = gin(L.ru0)[i];
+        const double a = act[i], yi = oy[i], lo = gin(L.rl0)[i], hi = gin(L.ru0)[i];
         if (!isfinite(a) || !isfinite(yi)) r.mx[0] = 1;
         const double v = maxd(maxd(0., lo - a), a - hi);
         if (isfinite(lo)) r.mx[2] = maxd(r.mx[2], maxd(0., lo - a) / (1 + fabs(lo)));
@@ -500,7 +555,7 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
         r.mx[2] = maxd(r.mx[2], v / (1 + fabs(xj)));
         r.mx[3] = maxd(r.mx[3], v);
         r.s[0] += c * xj;
-        const double linear = c + gin(L.aty)[j];
+        const double linear = c + aty[j];
         if (!isfinite(linear)) r.mx[0] = 1;
         const double d = clampd(linear, xj - u, xj - l);
         r.mx[5] = maxd(r.mx[5], fabs(d));
@@ -511,7 +566,7 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
         if (!isfinite(arg)) r.mx[7] = 1;  // no finite Lagrangian box bound
         else dd_add(r.d[2], linear * arg);
         // Farkas column bound for the candidate's dual as a ray
-        const double g = gin(L.aty)[j], ga = gin(L.atyabs)[j];
+        const double g = aty[j], ga = atyabs[j];
         const int len = gin(L.otp)[j + 1] - gin(L.otp)[j];
         const double e = double(len + 4) * 2 * kEps * ga;
         double ubd = maxd(phi(g - e, l, u), phi(g + e, l, u));
@@ -548,6 +603,61 @@ __device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double
     const double slack = double(L.n0 + L.m0 + 16) * 4 * kEps * (r.s[2] + r.s[4] + r.s[3]);
     a.farkas_possible = !(neg_inf || value + slack <= 0);
     return a;
+}
+
+
+// NT threads copy and multiply; the first NTR form every sum.  stage holds
+// STAGE doubles (kChunk, or kWideChunk with the wide row blocks).
+template <int NT, int NTR, int STAGE = kChunk>
+__device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double *ox, double *oy, double *lds,
+                        double *stage) {
+    constexpr bool wide = STAGE > kChunk;
+    restore_fill<NT>(L, ox, oy);
+    __syncthreads();
+    restore_scatter<NT>(L, x, y, ox, oy);
+    __syncthreads();
+    spmv<NT, NTR, STAGE>(L.op, L.oi, L.ov, wide ? L.oblkw : L.oblk, wide ? L.nobw : L.nob, ox, L.act, stage, lds);
+    stream_rows<true, NT, NTR, STAGE>(L.otp, L.oti, L.otv, wide ? L.otblkw : L.otblk, wide ? L.notbw : L.notb, oy,
+                                      stage, lds, [](int) { return 0.0; },
+                      [&](int j, double v, double a) {
+                          gout(L.aty)[j] = v;
+                          gout(L.atyabs)[j] = a;
+                      });
+    __syncthreads();
+    return assess<NTR>(L, ox, oy, L.act, L.aty, L.atyabs, lds);
+}
+
+// Both candidates of a check (current and average) at once: one pass over
+// the original A and one over its transpose read each entry once for the two
+// products, which are formed and summed exactly as evaluate forms them (same
+// products, same storage-order row sums, same tree for rows longer than
+// kChunk), so each Acc equals evaluate's bit for bit.  stage holds
+// 2 * kChunk doubles; the candidates' products go to (act, aty, atyabs) and
+// (act1, aty1, atyabs1).
+template <int NT, int NTR>
+__device__ void evaluate_pair(const LpDev &L, const double *x0, const double *y0, const double *x1, const double *y1,
+                              double *lds, double *stage, Acc &a0, Acc &a1) {
+    restore_fill<NT>(L, L.ox[0], L.oy[0]);
+    restore_fill<NT>(L, L.ox[1], L.oy[1]);
+    __syncthreads();
+    restore_scatter<NT>(L, x0, y0, L.ox[0], L.oy[0]);
+    restore_scatter<NT>(L, x1, y1, L.ox[1], L.oy[1]);
+    __syncthreads();
+    stream_rows2<false, NT, NTR>(L.op, L.oi, L.ov, L.oblk, L.nob, L.ox[0], L.ox[1], stage, lds,
+                                 [&](int i, double v0, double, double v1, double) {
+                                     gout(L.act)[i] = v0;
+                                     gout(L.act1)[i] = v1;
+                                 });
+    stream_rows2<true, NT, NTR>(L.otp, L.oti, L.otv, L.otblk, L.notb, L.oy[0], L.oy[1], stage, lds,
+                                [&](int j, double v0, double b0, double v1, double b1) {
+                                    gout(L.aty)[j] = v0;
+                                    gout(L.atyabs)[j] = b0;
+                                    gout(L.aty1)[j] = v1;
+                                    gout(L.atyabs1)[j] = b1;
+                                });
+    __syncthreads();
+    a0 = assess<NTR>(L, L.ox[0], L.oy[0], L.act, L.aty, L.atyabs, lds);
+    a1 = assess<NTR>(L, L.ox[1], L.oy[1], L.act1, L.aty1, L.atyabs1, lds);
 }
 
 // Accept/reject and adaptive step factor (src/cpu.cpp:151-172).  Thread 0.
@@ -1027,7 +1137,9 @@ __global__ void gather_pieces_kernel(const CopyPiece *pieces, double *out) {
 // every sum, so the check decides the same in any workgroup size.  lds holds
 // kWaves * 24 doubles, stage STAGE (with the matching row blocks, which sum
 // every row as the kChunk blocks do).
-template <int NT, int STAGE = kChunk>
+// PAIR: evaluate both candidates with evaluate_pair (stage of 2 * kChunk
+// doubles); the Accs, and so every decision, are the same either way.
+template <int NT, int STAGE = kChunk, bool PAIR = false>
 __device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, double *lds, double *stage) {
     constexpr int NTR = kThreads;
     constexpr bool wide = STAGE > kChunk;
@@ -1037,8 +1149,14 @@ __device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, dou
     if (S.phase == kCheck) {
         // candidates(): rebase the activity recurrence (src/cpu.cpp:229-230)
         spmv<NT, NTR, STAGE>(L.ap, L.ai, L.av, wide ? L.ablkw : L.ablk, wide ? L.nabw : L.nab, L.x[cur], L.ax[cur], stage, lds);
-        const Acc cur_acc = evaluate<NT, NTR, STAGE>(L, L.x[cur], L.y[cur], L.ox[0], L.oy[0], lds, stage);
-        const Acc avg_acc = evaluate<NT, NTR, STAGE>(L, L.xa[cur], L.ya[cur], L.ox[1], L.oy[1], lds, stage);
+        Acc cur_acc, avg_acc;
+        if constexpr (PAIR) {
+            static_assert(!wide, "the paired check uses kChunk row blocks");
+            evaluate_pair<NT, NTR>(L, L.x[cur], L.y[cur], L.xa[cur], L.ya[cur], lds, stage, cur_acc, avg_acc);
+        } else {
+            cur_acc = evaluate<NT, NTR, STAGE>(L, L.x[cur], L.y[cur], L.ox[0], L.oy[0], lds, stage);
+            avg_acc = evaluate<NT, NTR, STAGE>(L, L.xa[cur], L.ya[cur], L.ox[1], L.oy[1], lds, stage);
+        }
         avg = avg_acc.kkt < cur_acc.kkt ? 1 : 0;
         const Acc ca = avg ? avg_acc : cur_acc;
         const bool better = ca.finite && ca.kkt < S.best_kkt;
@@ -1188,11 +1306,11 @@ __global__ __launch_bounds__(kThreads, 4) void batch_check_kernel(const LpDev *l
     const int s = slots[blockIdx.x];
     __shared__ LpState S;
     __shared__ double lds[kWaves * 24];
-    __shared__ double stage[kChunk];
+    __shared__ double stage[2 * kChunk];
     if (threadIdx.x == 0) S = states[s];
     __syncthreads();
     if (S.status != kRunning || S.phase == kAdvance) return;
-    check_period<kThreads>(lps[s], S, o, lds, stage);
+    check_period<kThreads, kChunk, true>(lps[s], S, o, lds, stage);
     if (threadIdx.x == 0) states[s] = S;
 }
 
@@ -1487,14 +1605,18 @@ static_assert(kShapeCount == 4, "niryukti_batch_stats holds four shape entries")
 // has run at least kMinTrialLps LPs (two per CU of the 256) for kSettle
 // rounds: below that a launch lasts as long as its slowest LP, whatever the
 // shape, and the first rounds of a batch run while prepares still admit LPs.
+// The smallest class (m + n <= 2,048) tries the 256-thread shapes only: a
+// 1,024-thread workgroup has at most two rows or columns per thread there and
+// idles most of its waves in each pass (its trials were within the rounds'
+// noise and moved the class in some runs and not in others).
 class ShapeTuner {
   public:
     static constexpr int kTuned = 3;  // the crowded LDS classes
     static constexpr int kMinTrialLps = 512;  // NIRYUKTI_BATCH_TUNE_MIN_LPS overrides it
     static constexpr int kSettle = 2;
     static constexpr int kRepeats = 4;
-    static constexpr int kTrials = kRepeats * kShapeCount;
     static constexpr double kMargin = 0.08;
+    static bool allowed(int c, int shape) { return c > 0 || kShapes[shape].threads <= 256; }
     void start_with(int c, int shape) { current_[c] = shape; }
     void set_min_lps(int lps) { min_lps_ = std::max(1, lps); }
     void fix(int c, int shape) {
@@ -1519,10 +1641,10 @@ class ShapeTuner {
         }
         if (shape_of[c] != trial_shape(tried_) || !(bytes > 0) || !(seconds > 0)) return;
         rate_[shape_of[c]] += bytes / seconds;
-        if (++tried_ < kTrials) return;
+        if (++tried_ < kRepeats * candidates(c)) return;
         int best = current_[c];
         for (int k = 0; k < kShapeCount; ++k)
-            if (rate_[k] > rate_[best]) best = k;
+            if (allowed(c, k) && rate_[k] > rate_[best]) best = k;
         if (rate_[best] >= (1 + kMargin) * rate_[current_[c]]) current_[c] = best;
         done_[c] = true;
         tried_ = 0;
@@ -1535,16 +1657,21 @@ class ShapeTuner {
             if (!fixed_[c] && !done_[c]) return c;
         return -1;
     }
+    static int candidates(int c) {
+        int k = 0;
+        for (int s = 0; s < kShapeCount; ++s) k += allowed(c, s);
+        return k;
+    }
     // Trial t of the class under trial: its current shape first, then the
-    // others by index, mirrored every other sweep.
+    // other candidates by index, mirrored every other sweep.
     int trial_shape(int t) const {
         const int c = trial_class();
         int order[kShapeCount], k = 0;
         order[k++] = current_[c];
         for (int s = 0; s < kShapeCount; ++s)
-            if (s != current_[c]) order[k++] = s;
-        const int i = t % kShapeCount;
-        return (t / kShapeCount) % 2 ? order[kShapeCount - 1 - i] : order[i];
+            if (s != current_[c] && allowed(c, s)) order[k++] = s;
+        const int i = t % k;
+        return (t / k) % 2 ? order[k - 1 - i] : order[i];
     }
     int current_[kTuned] = {};
     bool fixed_[kTuned] = {}, done_[kTuned] = {};
@@ -1606,6 +1733,7 @@ class BatchEngine {
         HIP_OK(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device));
         if (wall_khz <= 0) throw std::runtime_error("The device reports no constant-rate clock");
         tail_budget_ticks_ = (long long)wall_khz * kTailBudgetMs;
+        compute_units_ = prop.multiProcessorCount;
         for (int c = 0; c < kClasses - 1; ++c) {
             int cap = kClassDoubles[c];
             while (cap > 0 && (long long)lds_bytes(cap) > room) --cap;
@@ -1886,6 +2014,7 @@ class BatchEngine {
                     groups.push_back(Group{c, c < kClasses - 1 ? class_cap_[c] : 0, by_class[c]});
                 }
             }
+            const size_t tail_lps = tail.slots.size();
             if (!tail.slots.empty()) groups.push_back(std::move(tail));
             for (const Group &g : groups)
                 HIP_OK(hipMemcpyAsync(d_slots_ + size_t(g.kind) * max_slots_, g.slots.data(), sizeof(int) * g.slots.size(),
@@ -1902,12 +2031,22 @@ class BatchEngine {
             int shape_of[kClasses - 1];
             for (int c = 0; c < kClasses - 1; ++c) shape_of[c] = tuner_.next(c);
             // The tail launch continues past `checks` periods only when it
-            // has the round to itself: beside crowded launches (which take
-            // `checks` periods) a longer tail launch would hold the round,
-            // and with it every crowded LP, until its budget ran out.
+            // has the round to itself and no LP waits for admission: beside
+            // crowded launches (which take `checks` periods) a longer tail
+            // launch would hold the round, and with it every crowded LP,
+            // until its budget ran out; with LPs prepared or in preparation
+            // it would hold their admission (between two streamed passes the
+            // tail group holds hundreds of LPs and its budget spans several
+            // waves of workgroups).
             bool crowded = false;
             for (const Group &g : groups) crowded = crowded || g.kind != kClasses;
-            const long long tail_budget = crowded ? 0 : tail_budget_ticks_;
+            // The budget also needs the whole tail group resident at once
+            // (one 1,024-thread workgroup per CU): a larger group runs in
+            // several waves of workgroups, each of which would spend the
+            // budget in turn while the LPs of the next wave wait.
+            const bool waiting = !queue_.empty() || prep_outstanding_ > 0;
+            const bool one_wave = tail_lps <= size_t(compute_units_);
+            const long long tail_budget = crowded || waiting || !one_wave ? 0 : tail_budget_ticks_;
             for (const Group &g : groups) {
                 auto &ev = class_events_[g.kind];
                 while (ev.size() < size_t(2 * checks + 2)) {
@@ -2420,6 +2559,7 @@ class BatchEngine {
             o_oy[h] = L.add<double>(mr0);
         }
         size_t o_act = L.add<double>(mr0), o_aty = L.add<double>(n0), o_atyabs = L.add<double>(n0);
+        size_t o_act1 = L.add<double>(mr0), o_aty1 = L.add<double>(n0), o_atyabs1 = L.add<double>(n0);
         size_t o_bx = L.add<double>(n0), o_by = L.add<double>(mr0);
         const auto ba = row_blocks(m.A.ptr), bt = row_blocks(t_ptr(m)), bo = row_blocks(m0.A.ptr), bot = row_blocks(t0.ptr);
         size_t o_ab = L.add<int>(ba.size()), o_tb = L.add<int>(bt.size()), o_ob = L.add<int>(bo.size()),
@@ -2539,6 +2679,9 @@ class BatchEngine {
         d.act = (double *)P(o_act);
         d.aty = (double *)P(o_aty);
         d.atyabs = (double *)P(o_atyabs);
+        d.act1 = (double *)P(o_act1);
+        d.aty1 = (double *)P(o_aty1);
+        d.atyabs1 = (double *)P(o_atyabs1);
         d.bx = (double *)P(o_bx);
         d.by = (double *)P(o_by);
         put_i(o_ab, ba);
@@ -3005,6 +3148,7 @@ class BatchEngine {
     std::vector<double> slot_bytes_, slot_check_bytes_;
     std::unique_ptr<Settling> settling_;  // the certification running beside the next round
     long long tail_budget_ticks_ = 0;
+    int compute_units_ = 0;
     hipStream_t class_stream_[kClasses + 1] = {};  // per class, then the tail group
     double *d_pow3_ = nullptr, *d_pow6_ = nullptr;
     struct Pending {
