@@ -72,16 +72,18 @@ void niryukti_batch_destroy(niryukti_batch *b);
  * one thread at a time. */
 int niryukti_batch_submit(niryukti_batch *b, const niryukti_lp *lp, const double *x0, const double *y0,
                           int64_t *ticket);
-/* One scheduling round: admit queued LPs into free slots, run
- * `checks_per_launch` check periods per resident LP (LPs of a class with at
- * most 256 LPs left run in one fused launch that continues past that number
- * until about 32 ms have passed, so LPs of different speed do not wait for
- * each other).  LPs that stopped in a round are certified before the call
- * returns when they are few (at most 32); a larger set is certified on a host
- * thread while the next round runs on the device and is retired (or resumes)
- * at the end of the next call.  Results do not depend on how periods fall into
- * rounds.  *running receives the number still queued, resident or being
- * certified. */
+/* One scheduling round: wait for the round launched by the previous call
+ * and read back its LPs' states, admit queued LPs into free slots, launch the
+ * next round -- `checks_per_launch` check periods per resident LP (LPs of a
+ * class with at most 256 LPs left run in one fused launch that, when it has
+ * the device to itself and no LP waits for admission, continues past that
+ * number until about 32 ms have passed, so LPs of different speed do not wait
+ * for each other) -- and return while it runs.  LPs that stopped in the
+ * finished round are certified before the call returns when they are few (at
+ * most 32); a larger set is certified on a host thread beside the new round
+ * and is retired (or resumes) at the end of the next call.  Results do not
+ * depend on how periods fall into rounds.  *running receives the number still
+ * queued, resident or being certified, plus one while a round is in flight. */
 int niryukti_batch_step(niryukti_batch *b, int checks_per_launch, int64_t *running);
 /* Admits queued LPs into free slots (uploading them to the arena) without
  * launching; returns the number admitted in *admitted. */

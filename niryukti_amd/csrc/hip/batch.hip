@@ -1598,7 +1598,9 @@ static_assert(kShapeCount == 4, "niryukti_batch_stats holds four shape entries")
 // throughput -- algorithmic bytes of every crowded launch of the round over
 // the round's device time -- when it beats the current shape by at least
 // kMargin (single rounds of one class differ by several percent with no
-// change of shape).  The score is the round's and not the class's own: the
+// change of shape, and in streamed runs the means of the trials still differ
+// by up to about 10 %: with an 8 % margin the smallest class moved to 256 x 2
+// in some runs, which measured 4-8 % slower end to end).  The score is the round's and not the class's own: the
 // classes run side by side, and a shape with more workgroups per CU takes
 // slots from its neighbours (scored alone, such a shape won its class's
 // trials and lost end to end).  Trials count only once the class under trial
@@ -1615,7 +1617,7 @@ class ShapeTuner {
     static constexpr int kMinTrialLps = 512;  // NIRYUKTI_BATCH_TUNE_MIN_LPS overrides it
     static constexpr int kSettle = 2;
     static constexpr int kRepeats = 4;
-    static constexpr double kMargin = 0.08;
+    static constexpr double kMargin = 0.15;
     static bool allowed(int c, int shape) { return c > 0 || kShapes[shape].threads <= 256; }
     void start_with(int c, int shape) { current_[c] = shape; }
     void set_min_lps(int lps) { min_lps_ = std::max(1, lps); }
@@ -1793,7 +1795,10 @@ class BatchEngine {
         }
         arena_cv_.notify_all();
         for (auto &w : preparers_) w.join();
+        // a round may still be in flight on the class streams
         if (stream_) hipStreamSynchronize(stream_);
+        for (auto &cs : class_stream_)
+            if (cs) hipStreamSynchronize(cs);
         hipFree(d_desc_);
         hipFree(d_state_);
         hipFree(d_slots_);
@@ -1805,8 +1810,6 @@ class BatchEngine {
         for (auto &list : class_events_)
             for (auto &e : list) hipEventDestroy(e);
         if (uploaded_) hipEventDestroy(uploaded_);
-        for (auto &cs : class_stream_)
-            if (cs) hipStreamSynchronize(cs);
     }
 
     // Any thread, also while the scheduling thread is inside step(): the copy
@@ -1975,169 +1978,235 @@ class BatchEngine {
         }
     }
 
+    // One call: finish the round launched by the previous call (wait for
+    // it, read back every LP's state, account its time and bytes, stop LPs
+    // past their time limit), admit queued LPs into free slots, launch the
+    // next round, and only then settle the finished round's stopped LPs
+    // (certification, retirement, resumption) while the new round runs.  The
+    // device so idles between two rounds only for the read-back, admission
+    // and launch, not for certification or the caller's own work between
+    // calls.  An LP that stops in one round is left out of the next (its
+    // stop is known before the next round is planned); its slot and arena
+    // space are freed after that round is launched, so a new LP takes them
+    // one round later.
     int64_t step(int checks) {
         drain_incoming();
+        std::vector<int> finished;
+        if (round_) finished = finish_round();
         const auto ta = Clock::now();
         admit();
         admit_seconds_ += std::chrono::duration<double>(Clock::now() - ta).count();
-        std::vector<int> by_class[kClasses];
-        std::vector<int> active;
-        for (int s = 0; s < max_slots_; ++s)
-            if (slot_job_[s] >= 0 && state_[s].status == kRunning) {
-                active.push_back(s);
-                by_class[slot_class_[s]].push_back(s);
-            }
-        if (!active.empty()) {
-            if (dirty_) {
-                HIP_OK(hipMemcpyAsync(d_desc_, desc_.data(), sizeof(LpDev) * max_slots_, hipMemcpyHostToDevice, stream_));
-                dirty_ = false;
-            }
-            HIP_OK(hipMemcpyAsync(d_state_, state_.data(), sizeof(LpState) * max_slots_, hipMemcpyHostToDevice, stream_));
-            // Launch groups: each crowded LDS class, the global class, and one
-            // tail group that gathers every LDS class with few LPs left, so a
-            // handful of slow LPs share one launch per check period instead
-            // of one per class (launches of different classes can land on the
-            // same hardware queue and then run one after another).
-            struct Group {
-                int kind;  // 0..kClasses-1: that class; kClasses: the tail group
-                int cap;
-                std::vector<int> slots;
-            };
-            std::vector<Group> groups;
-            Group tail{kClasses, 0, {}};
-            for (int c = 0; c < kClasses; ++c) {
-                if (by_class[c].empty()) continue;
-                if (c < kClasses - 1 && int(by_class[c].size()) <= kTailLps) {
-                    tail.cap = std::max(tail.cap, class_cap_[c]);
-                    tail.slots.insert(tail.slots.end(), by_class[c].begin(), by_class[c].end());
-                } else {
-                    groups.push_back(Group{c, c < kClasses - 1 ? class_cap_[c] : 0, by_class[c]});
-                }
-            }
-            const size_t tail_lps = tail.slots.size();
-            if (!tail.slots.empty()) groups.push_back(std::move(tail));
-            for (const Group &g : groups)
-                HIP_OK(hipMemcpyAsync(d_slots_ + size_t(g.kind) * max_slots_, g.slots.data(), sizeof(int) * g.slots.size(),
-                                      hipMemcpyHostToDevice, stream_));
-            HIP_OK(hipEventRecord(uploaded_, stream_));
-            BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0, handoff_,
-                         d_pow3_,     d_pow6_,          pow_count_};
-            auto t0 = Clock::now();
-            std::vector<long long> before(active.size()), checks_before(active.size());
-            for (size_t k = 0; k < active.size(); ++k) {
-                before[k] = state_[active[k]].accepted + state_[active[k]].rejected;
-                checks_before[k] = state_[active[k]].checks;
-            }
-            int shape_of[kClasses - 1];
-            for (int c = 0; c < kClasses - 1; ++c) shape_of[c] = tuner_.next(c);
-            // The tail launch continues past `checks` periods only when it
-            // has the round to itself and no LP waits for admission: beside
-            // crowded launches (which take `checks` periods) a longer tail
-            // launch would hold the round, and with it every crowded LP,
-            // until its budget ran out; with LPs prepared or in preparation
-            // it would hold their admission (between two streamed passes the
-            // tail group holds hundreds of LPs and its budget spans several
-            // waves of workgroups).
-            bool crowded = false;
-            for (const Group &g : groups) crowded = crowded || g.kind != kClasses;
-            // The budget also needs the whole tail group resident at once
-            // (one 1,024-thread workgroup per CU): a larger group runs in
-            // several waves of workgroups, each of which would spend the
-            // budget in turn while the LPs of the next wave wait.
-            const bool waiting = !queue_.empty() || prep_outstanding_ > 0;
-            const bool one_wave = tail_lps <= size_t(compute_units_);
-            const long long tail_budget = crowded || waiting || !one_wave ? 0 : tail_budget_ticks_;
-            for (const Group &g : groups) {
-                auto &ev = class_events_[g.kind];
-                while (ev.size() < size_t(2 * checks + 2)) {
-                    hipEvent_t e;
-                    HIP_OK(hipEventCreate(&e));
-                    ev.push_back(e);
-                }
-                hipStream_t cs = class_stream_[g.kind];
-                HIP_OK(hipStreamWaitEvent(cs, uploaded_, 0));
-                const int grid = int(g.slots.size());
-                const int *slots = d_slots_ + size_t(g.kind) * max_slots_;
-                HIP_OK(hipEventRecord(ev[0], cs));
-                if (g.kind == kClasses) {
-                    launch_tail(g.cap, grid, cs, slots, bo, checks, tail_budget);
-                    HIP_OK(hipEventRecord(ev[1], cs));
-                } else {
-                    for (int k = 0; k < checks; ++k) {
-                        if (g.kind < kClasses - 1)
-                            launch_lds(g.cap, shape_of[g.kind], grid, cs, slots, bo);
-                        else
-                            batch_advance_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
-                        HIP_OK(hipEventRecord(ev[2 * k + 1], cs));
-                        batch_check_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
-                        HIP_OK(hipEventRecord(ev[2 * k + 2], cs));
-                    }
-                }
-                HIP_OK(hipGetLastError());
-                HIP_OK(hipStreamWaitEvent(stream_, ev[g.kind == kClasses ? 1 : 2 * checks], 0));
-            }
-            HIP_OK(hipMemcpyAsync(state_.data(), d_state_, sizeof(LpState) * max_slots_, hipMemcpyDeviceToHost, stream_));
-            HIP_OK(hipStreamSynchronize(stream_));
-            for (const Group &g : groups) {
-                auto &ev = class_events_[g.kind];
-                if (g.kind == kClasses) {
-                    // the fused launch: attempts and checks together
-                    float a = 0;
-                    HIP_OK(hipEventElapsedTime(&a, ev[0], ev[1]));
-                    advance_seconds_ += a * 1e-3;
-                    class_seconds_[g.kind] += a * 1e-3;
-                    class_launches_[g.kind] += 1;
-                }
-                for (int k = 0; k < (g.kind == kClasses ? 0 : checks); ++k) {
-                    float a = 0, b = 0;
-                    HIP_OK(hipEventElapsedTime(&a, ev[2 * k], ev[2 * k + 1]));
-                    HIP_OK(hipEventElapsedTime(&b, ev[2 * k + 1], ev[2 * k + 2]));
-                    advance_seconds_ += a * 1e-3;
-                    check_seconds_ += b * 1e-3;
-                    class_seconds_[g.kind] += a * 1e-3;
-                    class_launches_[g.kind] += 1;
-                }
-                for (int s : g.slots) slot_group_[s] = g.kind;
-            }
-            double group_seconds[kClasses + 1] = {}, group_bytes[kClasses + 1] = {};
-            int group_lps[kClasses + 1] = {};
-            for (const Group &g : groups) group_lps[g.kind] = int(g.slots.size());
-            for (const Group &g : groups) {
-                auto &ev = class_events_[g.kind];
-                for (int k = 0; k < (g.kind == kClasses ? 0 : checks); ++k) {
-                    float a = 0;
-                    HIP_OK(hipEventElapsedTime(&a, ev[2 * k], ev[2 * k + 1]));
-                    group_seconds[g.kind] += a * 1e-3;
-                }
-            }
-            for (size_t k = 0; k < active.size(); ++k) {
-                const LpState &st = state_[active[k]];
-                const long long done = st.accepted + st.rejected - before[k];
-                group_bytes[slot_group_[active[k]]] += double(done) * slot_bytes_[active[k]];
-                attempts_ += done;
-                advance_bytes_ += double(done) * slot_bytes_[active[k]];
-                class_bytes_[slot_group_[active[k]]] += double(done) * slot_bytes_[active[k]];
-                if (slot_group_[active[k]] == kClasses)  // the fused tail launch also did the checks
-                    class_bytes_[kClasses] += double(st.checks - checks_before[k]) * slot_check_bytes_[active[k]];
-            }
-            const double round_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
-            tuner_.record(shape_of, group_lps, group_bytes[0] + group_bytes[1] + group_bytes[2], round_seconds);
-            for (int c = 0; c < kClasses - 1; ++c)
-                if (group_lps[c]) {
-                    shape_seconds_[shape_of[c]] += group_seconds[c];
-                    shape_bytes_[shape_of[c]] += group_bytes[c];
-                    shape_launches_[shape_of[c]] += checks;
-                }
-            device_seconds_ += std::chrono::duration<double>(Clock::now() - t0).count();
-            ++launches_;
-        }
-        settle(active);
-        drain_handoffs(active.empty());
+        launch_round(checks);
+        settle(finished);
+        drain_handoffs(!round_);
         int64_t resident = 0;
         for (int s = 0; s < max_slots_; ++s) resident += slot_job_[s] >= 0;
         // Nothing to launch yet: wait for a prepare worker instead of spinning.
-        if (!resident && queue_.empty() && prep_outstanding_ > 0) collect_prepared(true);
-        return int64_t(queue_.size()) + handoffs_outstanding_ + prep_outstanding_ + resident;
+        if (!round_ && !resident && queue_.empty() && prep_outstanding_ > 0) collect_prepared(true);
+        return int64_t(queue_.size()) + handoffs_outstanding_ + prep_outstanding_ + resident + (round_ ? 1 : 0);
+    }
+
+    // A launched scheduling round: its launch groups, the LPs it runs and
+    // their counters at launch (for the round's statistics).
+    struct Round {
+        struct Group {
+            int kind;  // 0..kClasses-1: that class; kClasses: the tail group
+            int cap;
+            std::vector<int> slots;
+        };
+        std::vector<Group> groups;
+        std::vector<int> active;
+        std::vector<long long> before, checks_before;
+        int shape_of[kClasses - 1] = {};
+        int checks = 0;
+    };
+
+    // Plans a round from the host's view of every LP and launches it without
+    // waiting (no round is in flight: the previous one was finished first, so
+    // the whole state array can be uploaded).
+    void launch_round(int checks) {
+        auto R = std::make_unique<Round>();
+        std::vector<int> by_class[kClasses];
+        for (int s = 0; s < max_slots_; ++s)
+            if (slot_job_[s] >= 0 && state_[s].status == kRunning) {
+                R->active.push_back(s);
+                by_class[slot_class_[s]].push_back(s);
+            }
+        if (R->active.empty()) return;
+        R->checks = checks;
+        if (dirty_) {
+            HIP_OK(hipMemcpyAsync(d_desc_, desc_.data(), sizeof(LpDev) * max_slots_, hipMemcpyHostToDevice, stream_));
+            dirty_ = false;
+        }
+        // from a copy: settle changes state_ (of LPs outside this round)
+        // while the upload may still be reading it
+        upload_ = state_;
+        HIP_OK(hipMemcpyAsync(d_state_, upload_.data(), sizeof(LpState) * max_slots_, hipMemcpyHostToDevice, stream_));
+        // Launch groups: each crowded LDS class, the global class, and one
+        // tail group that gathers every LDS class with few LPs left, so a
+        // handful of slow LPs share one launch per check period instead
+        // of one per class (launches of different classes can land on the
+        // same hardware queue and then run one after another).
+        auto &groups = R->groups;
+        Round::Group tail{kClasses, 0, {}};
+        for (int c = 0; c < kClasses; ++c) {
+            if (by_class[c].empty()) continue;
+            if (c < kClasses - 1 && int(by_class[c].size()) <= kTailLps) {
+                tail.cap = std::max(tail.cap, class_cap_[c]);
+                tail.slots.insert(tail.slots.end(), by_class[c].begin(), by_class[c].end());
+            } else {
+                groups.push_back(Round::Group{c, c < kClasses - 1 ? class_cap_[c] : 0, by_class[c]});
+            }
+        }
+        const size_t tail_lps = tail.slots.size();
+        if (!tail.slots.empty()) groups.push_back(std::move(tail));
+        for (const auto &g : groups)
+            HIP_OK(hipMemcpyAsync(d_slots_ + size_t(g.kind) * max_slots_, g.slots.data(), sizeof(int) * g.slots.size(),
+                                  hipMemcpyHostToDevice, stream_));
+        HIP_OK(hipEventRecord(uploaded_, stream_));
+        BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0, handoff_,
+                     d_pow3_,     d_pow6_,          pow_count_};
+        R->before.resize(R->active.size());
+        R->checks_before.resize(R->active.size());
+        for (size_t k = 0; k < R->active.size(); ++k) {
+            R->before[k] = state_[R->active[k]].accepted + state_[R->active[k]].rejected;
+            R->checks_before[k] = state_[R->active[k]].checks;
+        }
+        for (int c = 0; c < kClasses - 1; ++c) R->shape_of[c] = tuner_.next(c);
+        // The tail launch continues past `checks` periods only when it
+        // has the round to itself and no LP waits for admission: beside
+        // crowded launches (which take `checks` periods) a longer tail
+        // launch would hold the round, and with it every crowded LP,
+        // until its budget ran out; with LPs prepared or in preparation
+        // it would hold their admission (between two streamed passes the
+        // tail group holds hundreds of LPs and its budget spans several
+        // waves of workgroups).  The budget also needs the whole tail group
+        // resident at once (one 1,024-thread workgroup per CU): a larger
+        // group runs in several waves of workgroups, each of which would
+        // spend the budget in turn while the LPs of the next wave wait.
+        bool crowded = false;
+        for (const auto &g : groups) crowded = crowded || g.kind != kClasses;
+        const bool waiting = !queue_.empty() || prep_outstanding_ > 0;
+        const bool one_wave = tail_lps <= size_t(compute_units_);
+        const long long tail_budget = crowded || waiting || !one_wave ? 0 : tail_budget_ticks_;
+        for (const auto &g : groups) {
+            auto &ev = class_events_[g.kind];
+            while (ev.size() < size_t(2 * checks + 2)) {
+                hipEvent_t e;
+                HIP_OK(hipEventCreate(&e));
+                ev.push_back(e);
+            }
+            hipStream_t cs = class_stream_[g.kind];
+            HIP_OK(hipStreamWaitEvent(cs, uploaded_, 0));
+            const int grid = int(g.slots.size());
+            const int *slots = d_slots_ + size_t(g.kind) * max_slots_;
+            HIP_OK(hipEventRecord(ev[0], cs));
+            if (g.kind == kClasses) {
+                launch_tail(g.cap, grid, cs, slots, bo, checks, tail_budget);
+                HIP_OK(hipEventRecord(ev[1], cs));
+            } else {
+                for (int k = 0; k < checks; ++k) {
+                    if (g.kind < kClasses - 1)
+                        launch_lds(g.cap, R->shape_of[g.kind], grid, cs, slots, bo);
+                    else
+                        batch_advance_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
+                    HIP_OK(hipEventRecord(ev[2 * k + 1], cs));
+                    batch_check_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
+                    HIP_OK(hipEventRecord(ev[2 * k + 2], cs));
+                }
+            }
+            HIP_OK(hipGetLastError());
+        }
+        round_ = std::move(R);
+    }
+
+    // Waits for the round in flight, reads back the states of its LPs and
+    // accounts the round; stops running LPs past their time limit (they are
+    // left out of the next round and retired by settle).  Returns the
+    // round's LPs.  Only the round's own slots take the device's values: the
+    // host may have changed other slots meanwhile (an LP resumed after a
+    // failed certification, a newly admitted one), and the next launch
+    // uploads those changes.
+    std::vector<int> finish_round() {
+        std::unique_ptr<Round> R = std::move(round_);
+        // stream_ waits for the round only here, so that settle's downloads
+        // on stream_ run beside the round
+        for (const auto &g : R->groups)
+            HIP_OK(hipStreamWaitEvent(stream_, class_events_[g.kind][g.kind == kClasses ? 1 : 2 * R->checks], 0));
+        readback_.resize(size_t(max_slots_));
+        HIP_OK(hipMemcpyAsync(readback_.data(), d_state_, sizeof(LpState) * max_slots_, hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        for (int s : R->active) state_[s] = readback_[s];
+        const int checks = R->checks;
+        float span = 0;
+        const auto &groups = R->groups;
+        for (const auto &g : groups) {
+            auto &ev = class_events_[g.kind];
+            if (g.kind == kClasses) {
+                // the fused launch: attempts and checks together
+                float a = 0;
+                HIP_OK(hipEventElapsedTime(&a, ev[0], ev[1]));
+                advance_seconds_ += a * 1e-3;
+                class_seconds_[g.kind] += a * 1e-3;
+                class_launches_[g.kind] += 1;
+            }
+            for (int k = 0; k < (g.kind == kClasses ? 0 : checks); ++k) {
+                float a = 0, b = 0;
+                HIP_OK(hipEventElapsedTime(&a, ev[2 * k], ev[2 * k + 1]));
+                HIP_OK(hipEventElapsedTime(&b, ev[2 * k + 1], ev[2 * k + 2]));
+                advance_seconds_ += a * 1e-3;
+                check_seconds_ += b * 1e-3;
+                class_seconds_[g.kind] += a * 1e-3;
+                class_launches_[g.kind] += 1;
+            }
+            float t = 0;  // from the round's upload to the group's last event
+            HIP_OK(hipEventElapsedTime(&t, uploaded_, ev[g.kind == kClasses ? 1 : 2 * checks]));
+            span = std::max(span, t);
+            for (int s : g.slots) slot_group_[s] = g.kind;
+        }
+        double group_seconds[kClasses + 1] = {}, group_bytes[kClasses + 1] = {};
+        int group_lps[kClasses + 1] = {};
+        for (const auto &g : groups) group_lps[g.kind] = int(g.slots.size());
+        for (const auto &g : groups) {
+            auto &ev = class_events_[g.kind];
+            for (int k = 0; k < (g.kind == kClasses ? 0 : checks); ++k) {
+                float a = 0;
+                HIP_OK(hipEventElapsedTime(&a, ev[2 * k], ev[2 * k + 1]));
+                group_seconds[g.kind] += a * 1e-3;
+            }
+        }
+        const auto &active = R->active;
+        for (size_t k = 0; k < active.size(); ++k) {
+            const LpState &st = state_[active[k]];
+            const long long done = st.accepted + st.rejected - R->before[k];
+            group_bytes[slot_group_[active[k]]] += double(done) * slot_bytes_[active[k]];
+            attempts_ += done;
+            advance_bytes_ += double(done) * slot_bytes_[active[k]];
+            class_bytes_[slot_group_[active[k]]] += double(done) * slot_bytes_[active[k]];
+            if (slot_group_[active[k]] == kClasses)  // the fused tail launch also did the checks
+                class_bytes_[kClasses] += double(st.checks - R->checks_before[k]) * slot_check_bytes_[active[k]];
+        }
+        const double round_seconds = span * 1e-3;
+        tuner_.record(R->shape_of, group_lps, group_bytes[0] + group_bytes[1] + group_bytes[2], round_seconds);
+        for (int c = 0; c < kClasses - 1; ++c)
+            if (group_lps[c]) {
+                shape_seconds_[R->shape_of[c]] += group_seconds[c];
+                shape_bytes_[R->shape_of[c]] += group_bytes[c];
+                shape_launches_[R->shape_of[c]] += checks;
+            }
+        device_seconds_ += round_seconds;
+        ++launches_;
+        // LPs past their time limit stop here (settle retires them with
+        // TIME_LIMIT), before the next round is planned.
+        for (int s : active) {
+            Job &j = *jobs_[slot_job_[s]];
+            if (state_[s].status == kRunning &&
+                std::chrono::duration<double>(Clock::now() - j.start).count() >= opt_.time_limit) {
+                state_[s].status = kIterationLimit;
+                j.result.status = "TIME_LIMIT";
+            }
+        }
+        return active;
     }
 
     // Handoff of long-running LPs to the grid-wide single-LP engine: after
@@ -2802,21 +2871,13 @@ class BatchEngine {
         nyk::Beside task;  // the certification, when it runs beside the next round
     };
 
-    // Collects the LPs that stopped in the round just run (and running LPs
-    // past the time limit) and downloads what their certification needs.
+    // Collects the LPs that stopped in the round just finished (including
+    // those finish_round stopped at their time limit) and downloads what
+    // their certification needs.
     std::unique_ptr<Settling> collect(const std::vector<int> &active) {
         std::vector<int> stopped;
         for (int s : active)
             if (state_[s].status != kRunning) stopped.push_back(s);
-        for (int s : active) {
-            Job &j = *jobs_[slot_job_[s]];
-            if (state_[s].status == kRunning &&
-                std::chrono::duration<double>(Clock::now() - j.start).count() >= opt_.time_limit) {
-                state_[s].status = kIterationLimit;  // retired below with TIME_LIMIT
-                j.result.status = "TIME_LIMIT";
-                stopped.push_back(s);
-            }
-        }
         if (stopped.empty()) return nullptr;
         auto p = std::make_unique<Settling>();
         const size_t ns = stopped.size();
@@ -3165,6 +3226,9 @@ class BatchEngine {
     int class_cap_[kClasses - 1] = {};
     std::vector<hipEvent_t> class_events_[kClasses + 1];
     hipEvent_t uploaded_ = nullptr;
+    std::unique_ptr<Round> round_;  // launched, not yet finished
+    std::vector<LpState> readback_;  // the device's state array after a round
+    std::vector<LpState> upload_;    // the state array a launch uploads
     std::vector<int> slot_class_, slot_group_;
     double advance_seconds_ = 0, check_seconds_ = 0, advance_bytes_ = 0;
     double class_seconds_[kClasses + 1] = {}, class_bytes_[kClasses + 1] = {};  // per class, then the tail group
