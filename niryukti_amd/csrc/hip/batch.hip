@@ -1314,108 +1314,122 @@ __global__ __launch_bounds__(kThreads, 4) void batch_check_kernel(const LpDev *l
     if (threadIdx.x == 0) states[s] = S;
 }
 
-// The tail group in one launch: each workgroup takes its LP through whole
-// check periods, the attempts of batch_advance_lds_kernel followed by
-// check_period, until the LP stops running (parked for the host, finished,
-// handed off), `max_periods` periods are done, or at least `min_periods` are
-// done and `budget` ticks of the constant-rate clock have passed since the
-// workgroup started.  An LP's periods do not wait for the other LPs of the
-// launch, and its trajectory is the one the two-kernel cycle gives: the
-// attempts are the same code, and check_period forms its sums on the first
-// kThreads threads exactly as batch_check_kernel does.
+// The tail group in one launch, as a persistent grid over a device-side work
+// queue: the launch has at most one workgroup per CU, and each workgroup
+// takes the next LP of the group from the queue (an atomic counter over the
+// group's slot list), drives it through whole check periods -- the attempts
+// of batch_advance_lds_kernel followed by check_period -- until the LP stops
+// running (parked for the host, finished, handed off), `max_periods` periods
+// are done, or at least `min_periods` are done and `budget` ticks of the
+// constant-rate clock have passed since the workgroup started; then it takes
+// the next LP, until the queue is empty.  An LP's periods do not wait for the
+// other LPs of the launch, an LP that stops early frees its workgroup for the
+// next one, and a group larger than the grid shares one budget window instead
+// of spending it once per wave of workgroups.  Each LP's trajectory is the
+// one the two-kernel cycle gives: the attempts are the same code, and
+// check_period forms its sums on the first kThreads threads exactly as
+// batch_check_kernel does.  `queue` is zero at launch.
 template <int NT>
 __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpState *states, const int *slots,
-                                                        BatchOpts o, int min_periods, int max_periods,
-                                                        long long budget) {
+                                                        int count, unsigned *queue, BatchOpts o, int min_periods,
+                                                        int max_periods, long long budget) {
     extern __shared__ double dyn[];
     const long long start = wall_clock64();
-    const int s = slots[blockIdx.x];
     __shared__ LpState S;
     __shared__ double tiles[3 * kMaxTiles];
     __shared__ double powlds[2];
     __shared__ double lds[kWaves * 24];
     __shared__ double stage[kWideChunk];
-    __shared__ int flag, stop;
-    if (threadIdx.x == 0) S = states[s];
-    __syncthreads();
-    if (S.status != kRunning) return;
-    // The attempts' copy of the descriptor: its pointers and sizes moved to
-    // scalar registers, where they stay across the check calls (read through
-    // the descriptor after a call, they would be vector loads, and every
-    // address of the attempts would be formed in vector registers).
-    LpDev L = lps[s];
-    for (int h = 0; h < 2; ++h) {
-        L.x[h] = uniform(L.x[h]);
-        L.xa[h] = uniform(L.xa[h]);
-        L.y[h] = uniform(L.y[h]);
-        L.ya[h] = uniform(L.ya[h]);
-        L.ax[h] = uniform(L.ax[h]);
-    }
-    L.c = uniform(L.c);
-    L.lb = uniform(L.lb);
-    L.ub = uniform(L.ub);
-    L.rl = uniform(L.rl);
-    L.ru = uniform(L.ru);
-    L.tsi = uniform(L.tsi);
-    L.tsv = uniform(L.tsv);
-    L.asi = uniform(L.asi);
-    L.asv = uniform(L.asv);
-    L.lub = uniform(L.lub);
-    L.rlu = uniform(L.rlu);
-    L.n = __builtin_amdgcn_readfirstlane(L.n);
-    L.m = __builtin_amdgcn_readfirstlane(L.m);
-    double *ylds = dyn, *dxlds = dyn + L.m;
-    int *tplds = reinterpret_cast<int *>(dyn + L.m + L.n), *aplds = tplds + L.n + 1;
-    const int ns = (L.n + 63) / 64, ms = (L.m + 63) / 64;
-    int *tsblds = aplds + L.m + 1, *asblds = tsblds + ns + 1;
-    const GLOBAL_AS int *tp = gin(L.tp), *ap = gin(L.ap), *tsb = gin(L.tsb), *asb = gin(L.asb);
-    for (int j = threadIdx.x; j <= L.n; j += NT) tplds[j] = tp[j];
-    for (int i = threadIdx.x; i <= L.m; i += NT) aplds[i] = ap[i];
-    for (int j = threadIdx.x; j <= ns; j += NT) tsblds[j] = tsb[j];
-    for (int i = threadIdx.x; i <= ms; i += NT) asblds[i] = asb[i];
-    __syncthreads();
-    bool ystaged = false;
-    for (int period = 0;;) {
-        if (S.phase != kAdvance) {
-            check_period_call<NT, kWideChunk>((const GLOBAL_AS LpDev *)(lps + s), (LDS_AS LpState *)&S, o, (LDS_AS double *)lds,
-                                  (LDS_AS double *)stage);
-            ystaged = false;  // a restart may have replaced y
-            if (S.status != kRunning) break;
-        }
+    __shared__ int flag, stop, taken;
+    for (;;) {
+        __syncthreads();  // the previous LP's readers of S and `taken` are done
         if (threadIdx.x == 0)
-            stop = period >= max_periods || (period >= min_periods && wall_clock64() - start >= budget);
+            taken = int(__hip_atomic_fetch_add(queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         __syncthreads();
-        if (stop) break;
-        const long long remaining = S.iteration_limit - S.iterations;
-        if (remaining <= 0) {
-            __syncthreads();
-            if (threadIdx.x == 0) S.status = kIterationLimit;
-            break;
+        const int q = taken;
+        if (q >= count) break;
+        const int s = slots[q];
+        if (threadIdx.x == 0) S = states[s];
+        __syncthreads();
+        if (S.status != kRunning) continue;
+        // The attempts' copy of the descriptor: its pointers and sizes moved to
+        // scalar registers, where they stay across the check calls (read through
+        // the descriptor after a call, they would be vector loads, and every
+        // address of the attempts would be formed in vector registers).
+        LpDev L = lps[s];
+        for (int h = 0; h < 2; ++h) {
+            L.x[h] = uniform(L.x[h]);
+            L.xa[h] = uniform(L.xa[h]);
+            L.y[h] = uniform(L.y[h]);
+            L.ya[h] = uniform(L.ya[h]);
+            L.ax[h] = uniform(L.ax[h]);
         }
-        const long long count = remaining < o.check_every ? remaining : o.check_every;
-        const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
-        for (long long done = 0; done < count;) {
-            attempt_lds<NT, 1>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds, &flag,
-                               ystaged);
-            if (flag == 1) ++done;
-            else if (flag < 0) {
-                if (threadIdx.x == 0) S.status = kExhausted;
+        L.c = uniform(L.c);
+        L.lb = uniform(L.lb);
+        L.ub = uniform(L.ub);
+        L.rl = uniform(L.rl);
+        L.ru = uniform(L.ru);
+        L.tsi = uniform(L.tsi);
+        L.tsv = uniform(L.tsv);
+        L.asi = uniform(L.asi);
+        L.asv = uniform(L.asv);
+        L.lub = uniform(L.lub);
+        L.rlu = uniform(L.rlu);
+        L.n = __builtin_amdgcn_readfirstlane(L.n);
+        L.m = __builtin_amdgcn_readfirstlane(L.m);
+        double *ylds = dyn, *dxlds = dyn + L.m;
+        int *tplds = reinterpret_cast<int *>(dyn + L.m + L.n), *aplds = tplds + L.n + 1;
+        const int ns = (L.n + 63) / 64, ms = (L.m + 63) / 64;
+        int *tsblds = aplds + L.m + 1, *asblds = tsblds + ns + 1;
+        const GLOBAL_AS int *tp = gin(L.tp), *ap = gin(L.ap), *tsb = gin(L.tsb), *asb = gin(L.asb);
+        for (int j = threadIdx.x; j <= L.n; j += NT) tplds[j] = tp[j];
+        for (int i = threadIdx.x; i <= L.m; i += NT) aplds[i] = ap[i];
+        for (int j = threadIdx.x; j <= ns; j += NT) tsblds[j] = tsb[j];
+        for (int i = threadIdx.x; i <= ms; i += NT) asblds[i] = asb[i];
+        __syncthreads();
+        bool ystaged = false;
+        for (int period = 0;;) {
+            if (S.phase != kAdvance) {
+                check_period_call<NT, kWideChunk>((const GLOBAL_AS LpDev *)(lps + s), (LDS_AS LpState *)&S, o,
+                                                  (LDS_AS double *)lds, (LDS_AS double *)stage);
+                ystaged = false;  // a restart may have replaced y
+                if (S.status != kRunning) break;
+            }
+            if (threadIdx.x == 0)
+                stop = period >= max_periods || (period >= min_periods && wall_clock64() - start >= budget);
+            __syncthreads();
+            if (stop) break;
+            const long long remaining = S.iteration_limit - S.iterations;
+            if (remaining <= 0) {
+                __syncthreads();
+                if (threadIdx.x == 0) S.status = kIterationLimit;
                 break;
             }
+            const long long count_ = remaining < o.check_every ? remaining : o.check_every;
+            const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
+            for (long long done = 0; done < count_;) {
+                attempt_lds<NT, 1>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds,
+                                   &flag, ystaged);
+                if (flag == 1) ++done;
+                else if (flag < 0) {
+                    if (threadIdx.x == 0) S.status = kExhausted;
+                    break;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0 && S.status == kRunning) {
+                S.iterations += count_;
+                S.since_restart += count_;
+                S.checks += 1;
+                S.phase = kCheck;
+            }
+            __syncthreads();
+            if (S.status != kRunning) break;
+            ++period;
         }
         __syncthreads();
-        if (threadIdx.x == 0 && S.status == kRunning) {
-            S.iterations += count;
-            S.since_restart += count;
-            S.checks += 1;
-            S.phase = kCheck;
-        }
-        __syncthreads();
-        if (S.status != kRunning) break;
-        ++period;
+        if (threadIdx.x == 0) states[s] = S;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) states[s] = S;
 }
 
 // ------------------------------------------------------------------ host --
@@ -1744,6 +1758,7 @@ class BatchEngine {
         HIP_OK(hipMalloc(&d_desc_, sizeof(LpDev) * max_slots_));
         HIP_OK(hipMalloc(&d_state_, sizeof(LpState) * max_slots_));
         HIP_OK(hipMalloc(&d_slots_, sizeof(int) * max_slots_ * (kClasses + 1)));
+        HIP_OK(hipMalloc(&d_queue_, sizeof(unsigned)));
         // k^-0.3 and k^-0.6 by std::pow for every k the step rule can reach
         // (k = accepted + 2 <= iteration_limit + 1), up to 2^22 entries.
         pow_count_ = std::min<long long>(std::max<long long>(opt_.iteration_limit, 0) + 3, 1LL << 22);
@@ -1802,6 +1817,7 @@ class BatchEngine {
         hipFree(d_desc_);
         hipFree(d_state_);
         hipFree(d_slots_);
+        hipFree(d_queue_);
         hipFree(d_pow3_);
         hipFree(d_pow6_);
         if (d_stage_) hipFree(d_stage_);
@@ -2063,6 +2079,7 @@ class BatchEngine {
         for (const auto &g : groups)
             HIP_OK(hipMemcpyAsync(d_slots_ + size_t(g.kind) * max_slots_, g.slots.data(), sizeof(int) * g.slots.size(),
                                   hipMemcpyHostToDevice, stream_));
+        if (tail_lps) HIP_OK(hipMemcpyAsync(d_queue_, &zero_, sizeof(unsigned), hipMemcpyHostToDevice, stream_));
         HIP_OK(hipEventRecord(uploaded_, stream_));
         BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0, handoff_,
                      d_pow3_,     d_pow6_,          pow_count_};
@@ -2078,17 +2095,12 @@ class BatchEngine {
         // crowded launches (which take `checks` periods) a longer tail
         // launch would hold the round, and with it every crowded LP,
         // until its budget ran out; with LPs prepared or in preparation
-        // it would hold their admission (between two streamed passes the
-        // tail group holds hundreds of LPs and its budget spans several
-        // waves of workgroups).  The budget also needs the whole tail group
-        // resident at once (one 1,024-thread workgroup per CU): a larger
-        // group runs in several waves of workgroups, each of which would
-        // spend the budget in turn while the LPs of the next wave wait.
+        // it would hold their admission (between two streamed passes, while
+        // the next pass is prepared, the tail group holds hundreds of LPs).
         bool crowded = false;
         for (const auto &g : groups) crowded = crowded || g.kind != kClasses;
         const bool waiting = !queue_.empty() || prep_outstanding_ > 0;
-        const bool one_wave = tail_lps <= size_t(compute_units_);
-        const long long tail_budget = crowded || waiting || !one_wave ? 0 : tail_budget_ticks_;
+        const long long tail_budget = crowded || waiting ? 0 : tail_budget_ticks_;
         for (const auto &g : groups) {
             auto &ev = class_events_[g.kind];
             while (ev.size() < size_t(2 * checks + 2)) {
@@ -2526,10 +2538,14 @@ class BatchEngine {
     // thread over 1,024 threads (2 or 4 rows per thread spill registers at
     // this size and were measured slower).  Each LP runs at least `checks`
     // periods and then continues until `budget` ticks have passed.
-    void launch_tail(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo, int checks,
+    // The tail group's LPs come from the device-side queue at d_queue_ (zeroed
+    // with the round's uploads), so the grid needs no more workgroups than the
+    // device runs at once: one 1,024-thread workgroup per CU.
+    void launch_tail(int cap, int count, hipStream_t cs, const int *slots, const BatchOpts &bo, int checks,
                      long long budget) {
+        const int grid = std::max(1, std::min(count, compute_units_));
         batch_tail_kernel<kTailThreads><<<grid, kTailThreads, lds_bytes(cap), cs>>>(
-            d_desc_, d_state_, slots, bo, checks, checks * kTailMaxPeriodsPerCheck, budget);
+            d_desc_, d_state_, slots, count, d_queue_, bo, checks, checks * kTailMaxPeriodsPerCheck, budget);
     }
     // A crowded LDS class in the launch shape its tuner picked.
     void launch_lds(int cap, int shape, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
@@ -3203,6 +3219,8 @@ class BatchEngine {
     LpDev *d_desc_ = nullptr;
     LpState *d_state_ = nullptr;
     int *d_slots_ = nullptr;
+    unsigned *d_queue_ = nullptr;  // the tail launch's work-queue counter
+    const unsigned zero_ = 0;
     std::vector<LpDev> desc_;
     std::vector<LpState> state_;
     std::vector<int> slot_job_, free_slots_;
