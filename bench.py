@@ -512,6 +512,9 @@ def main():
     ap.add_argument("--matrix-precision", default="fp64", choices=("fp64", "mixed", "fp32"),
                     help="single-LP mode: FP64; mixed = FP32 matrix values (FP64 accumulation and iterates); "
                          "fp32 = FP32 SpMV arithmetic over FP64 iterates, then FP64 refinement")
+    ap.add_argument("--batch-precision", default="fp64", choices=("fp64", "mixed"),
+                    help="batched engine: FP64 matrix values, or FP32 values (FP64 arithmetic and iterates) "
+                         "until an LP stalls, then FP64")
     ap.add_argument("--qps", type=float, default=1.0, help="serving stream: Poisson arrival rate")
     ap.add_argument("--duration", type=float, default=30.0, help="serving stream: arrival window in seconds")
     ap.add_argument("--no-serve", action="store_true", help="leave the serving stream out of the default run")
@@ -571,7 +574,8 @@ def main():
     else:
         # one engine for every pass, warm-up passes included (its launch-shape
         # tuner settles there); the timed passes start once they have drained
-        b = engine.Batch(arena_bytes=96 << 30, max_slots=2 * len(insts), time_limit=1e9)
+        b = engine.Batch(arena_bytes=96 << 30, max_slots=2 * len(insts), time_limit=1e9,
+                         matrix_precision=args.batch_precision)
         b.set_handoff(args.handoff, 4)
         if args.warmup:
             el, passes = batch_stream(b, insts, args.warmup, args.checks, args.stream_threshold)

@@ -86,6 +86,16 @@ constexpr int kInlineCertify = 32;
 constexpr int kTailMaxPeriodsPerCheck = 1000;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxRejects = 40;
+// Mixed precision (FP32 matrix values, FP64 products, sums and iterates): an
+// LP moves to FP64 values when a candidate reaches kFp64Below, or after
+// kStallChecks checks without halving the candidate kkt -- the rule of the
+// single-LP engine's FP32 phase.  Measured on the 4,096-LP workload: FP32
+// values take more iterations to 1e-6 (mean 2,381 against 2,229 with this
+// rule; 2,882 when only a 20-check stall moves an LP) and the streamed batch
+// runs at 6,511 against 7,230 LPs/s in FP64; switching after 3-5 checks
+// without halving moves most LPs within a few checks (6,440-6,654).
+constexpr int kStallChecks = 20;
+constexpr double kFp64Below = 1e-5;
 
 enum Status : int {
     kRunning = 0,
@@ -121,6 +131,10 @@ struct LpDev {
     const int *asb, *tsb;
     const unsigned short *asi, *tsi;
     const double *asv, *tsv;
+    // the same values in FP32 at the positions of their indices (value quads
+    // beside index quads), for the FP32-value step of the mixed precision
+    // mode; null in FP64 mode
+    const float *asv32, *tsv32;
     // (lb, ub) and (rl, ru) interleaved, so the step reads a bound pair with
     // one 16-byte load (the LDS classes; null for the global class)
     const double *lub, *rlu;
@@ -134,11 +148,17 @@ struct LpState {
     long long first_restart;  // iteration of the first restart (0: none yet)
     int cur, status, phase, avg;
     int consecutive_rejects, farkas_parks;
+    // mixed precision: fp64 = 1 once the LP steps with FP64 values (always in
+    // FP64 mode); stall_kkt / stall_checks track checks since the best
+    // candidate kkt last halved
+    double stall_kkt;
+    long long fp32_iterations;
+    int fp64, stall_checks;
 };
 
 struct BatchOpts {
     double tol;
-    int check_every, adaptive, restart, pad;
+    int check_every, adaptive, restart, mixed;
     long long handoff;  // iterations after which an LP leaves the batch (0: never)
     // k^-0.3 and k^-0.6 for k < pow_count, computed on the host with std::pow
     // (the adaptive step rule's exponents, src/cpu.cpp:156-163)
@@ -829,11 +849,17 @@ __device__ __forceinline__ void rows_dot(const int *k, const int *e, const int *
 // takes the matrix from two instructions per entry to three per four.  Each
 // row is still summed in storage order, so the sums equal rows_dot's.
 typedef double double2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef float float4v __attribute__((ext_vector_type(4)));
 typedef unsigned short ushort2v __attribute__((ext_vector_type(2)));
 typedef unsigned short ushort4v __attribute__((ext_vector_type(4)));
-template <int NR>
+// VT = double: FP64 values in pairs.  VT = float (mixed precision): FP32
+// values stored like the indices, four entries of a lane per 16-byte load
+// (a last pair region for widths that are not a multiple of 4); products and
+// sums stay FP64.
+template <int NR, class VT = double>
 __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const int *sb,
-                                              const GLOBAL_AS unsigned short *si, const GLOBAL_AS double *sv, int slices, int b0,
+                                              const GLOBAL_AS unsigned short *si, const GLOBAL_AS VT *sv, int slices, int b0,
                                               int nt, const double *vec, double *acc) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // Every lane of a slice owns `w` stored entries (padding is value 0,
@@ -864,16 +890,37 @@ __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const 
         int ix[NR][kStep];
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-            // value pair t/2 + h of this lane at base + (t/2 + h) * 128 + 2 lane
-            const GLOBAL_AS double2v *pv = (const GLOBAL_AS double2v *)(sv + base[q] + t * 64 + 2 * lane);
+            if constexpr (sizeof(VT) == 8) {
+                // value pair t/2 + h of this lane at base + (t/2 + h) * 128 + 2 lane
+                const GLOBAL_AS double2v *pv = (const GLOBAL_AS double2v *)(sv + base[q] + t * 64 + 2 * lane);
 #pragma unroll
-            for (int h = 0; h < kStep / 2; ++h) {
-                if (full || t + 2 * h < w[q]) {
-                    const double2v vv = pv[h * 64];
-                    v[q][2 * h] = vv.x;
-                    v[q][2 * h + 1] = vv.y;
-                } else {
-                    v[q][2 * h] = v[q][2 * h + 1] = 0.0;
+                for (int h = 0; h < kStep / 2; ++h) {
+                    if (full || t + 2 * h < w[q]) {
+                        const double2v vv = pv[h * 64];
+                        v[q][2 * h] = vv.x;
+                        v[q][2 * h + 1] = vv.y;
+                    } else {
+                        v[q][2 * h] = v[q][2 * h + 1] = 0.0;
+                    }
+                }
+            } else {
+                // value quad t/4 + h at base + (t/4 + h) * 256 + 4 lane, as the indices
+                const GLOBAL_AS float *pf = sv + base[q] + t * 64;
+#pragma unroll
+                for (int h = 0; h < kStep / 4; ++h) {
+                    const int u0 = t + 4 * h;
+                    float4v f = {0.f, 0.f, 0.f, 0.f};
+                    if (full || u0 + 4 <= w[q]) {
+                        f = *(const GLOBAL_AS float4v *)(pf + h * 256 + 4 * lane);
+                    } else if (u0 + 2 <= w[q]) {
+                        const float2v f2 = *(const GLOBAL_AS float2v *)(pf + h * 256 + 2 * lane);
+                        f.x = f2.x;
+                        f.y = f2.y;
+                    }
+                    v[q][4 * h] = double(f.x);
+                    v[q][4 * h + 1] = double(f.y);
+                    v[q][4 * h + 2] = double(f.z);
+                    v[q][4 * h + 3] = double(f.w);
                 }
             }
             // index quad t/4 + h at base + (t/4 + h) * 256 + 4 lane (a last
@@ -939,7 +986,14 @@ __device__ __forceinline__ double tile_total(const double *tiles, int count) {
 // of A and A^T sit in LDS for the whole launch (so a row's entries are one
 // memory round trip away, not two), and each thread carries NR rows at a
 // time (rows are short in this class).
-template <int NT, int NR>
+// The sliced values a step of value type VT reads.
+template <class VT>
+__device__ __forceinline__ const GLOBAL_AS VT *sliced_values(const LpDev &L, bool transpose) {
+    if constexpr (sizeof(VT) == 8) return gin(transpose ? L.tsv : L.asv);
+    else return gin(transpose ? L.tsv32 : L.asv32);
+}
+
+template <int NT, int NR, class VT = double>
 __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, double tau0, double sigma0,
                             double *tiles, double *ylds, double *dxlds, const int *tplds, const int *aplds,
                             const int *tsblds, const int *asblds, double *powlds, int *accept_flag,
@@ -988,7 +1042,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             lo_[q] = bj.x;
             hi_[q] = bj.y;
         }
-        rows_dot_sell<NR>(k, e, tsblds, gin(L.tsi), gin(L.tsv), (L.n + 63) / 64, b0, NT, ylds, g);
+        rows_dot_sell<NR, VT>(k, e, tsblds, gin(L.tsi), sliced_values<VT>(L, true), (L.n + 63) / 64, b0, NT, ylds, g);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             double d2 = 0;
@@ -1028,7 +1082,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             lo_[q] = bi.x;
             hi_[q] = bi.y;
         }
-        rows_dot_sell<NR>(k, e, asblds, gin(L.asi), gin(L.asv), (L.m + 63) / 64, b0, NT, dxlds, delta);
+        rows_dot_sell<NR, VT>(k, e, asblds, gin(L.asi), sliced_values<VT>(L, false), (L.m + 63) / 64, b0, NT, dxlds, delta);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             double sq = 0, cr = 0;
@@ -1067,7 +1121,9 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
 }
 
 // Hot loop for the LDS-resident class; dynamic LDS holds y (m) and dx (n).
-template <int NT, int NR>
+// VT: the value type of the step (float: the FP32-value phase of the mixed
+// precision mode; LPs that moved to FP64 values run in the tail launch).
+template <int NT, int NR, class VT = double>
 __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps, LpState *states, const int *slots,
                                                                BatchOpts o) {
     extern __shared__ double dyn[];
@@ -1078,7 +1134,10 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     __shared__ int flag;
     if (threadIdx.x == 0) S = states[s];
     __syncthreads();
-    if (S.status != kRunning || S.phase != kAdvance) return;
+    // an LP whose check moved it to FP64 values in this round skips the rest
+    // of the round's FP32-value launches (the next round runs it in the tail
+    // launch), so its periods do not depend on the launch it is in
+    if (S.status != kRunning || S.phase != kAdvance || (sizeof(VT) == 4) == (S.fp64 != 0)) return;
     const LpDev &L = lps[s];
     const long long remaining = S.iteration_limit - S.iterations;
     if (remaining <= 0) {
@@ -1099,8 +1158,8 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     __syncthreads();
     bool ystaged = false;
     for (long long done = 0; done < count;) {
-        attempt_lds<NT, NR>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds, &flag,
-                            ystaged);
+        attempt_lds<NT, NR, VT>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds, &flag,
+                                ystaged);
         if (flag == 1) ++done;
         else if (flag < 0) {
             if (threadIdx.x == 0) S.status = kExhausted;
@@ -1113,6 +1172,7 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
             S.since_restart += count;
             S.checks += 1;
             S.phase = kCheck;
+            if (sizeof(VT) == 4) S.fp32_iterations += count;
         }
         states[s] = S;
     }
@@ -1172,6 +1232,19 @@ __device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, dou
             S.phase = kAdvance;
             if (!ca.finite) S.status = kNumerical;
             else if (ca.kkt <= o.tol) S.status = kParkOptimal;
+            // mixed precision: FP64 values from the check on whose
+            // candidate reaches kFp64Below or after which kStallChecks checks
+            // passed without halving the candidate kkt (the FP32 values'
+            // rounding bounds how close the iterates get)
+            if (o.mixed && !S.fp64) {
+                if (ca.kkt < .5 * S.stall_kkt) {
+                    S.stall_kkt = ca.kkt;
+                    S.stall_checks = 0;
+                } else if (++S.stall_checks >= kStallChecks) {
+                    S.fp64 = 1;
+                }
+                if (ca.kkt <= kFp64Below) S.fp64 = 1;
+            }
         }
         __syncthreads();
         if (S.status != kRunning) return;
@@ -1373,6 +1446,8 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
         L.tsv = uniform(L.tsv);
         L.asi = uniform(L.asi);
         L.asv = uniform(L.asv);
+        L.tsv32 = uniform(L.tsv32);
+        L.asv32 = uniform(L.asv32);
         L.lub = uniform(L.lub);
         L.rlu = uniform(L.rlu);
         L.n = __builtin_amdgcn_readfirstlane(L.n);
@@ -1407,9 +1482,14 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
             }
             const long long count_ = remaining < o.check_every ? remaining : o.check_every;
             const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
+            const bool fp32 = !S.fp64;
             for (long long done = 0; done < count_;) {
-                attempt_lds<NT, 1>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds,
-                                   &flag, ystaged);
+                if (fp32)
+                    attempt_lds<NT, 1, float>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds,
+                                              powlds, &flag, ystaged);
+                else
+                    attempt_lds<NT, 1, double>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds,
+                                               asblds, powlds, &flag, ystaged);
                 if (flag == 1) ++done;
                 else if (flag < 0) {
                     if (threadIdx.x == 0) S.status = kExhausted;
@@ -1422,6 +1502,7 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
                 S.since_restart += count_;
                 S.checks += 1;
                 S.phase = kCheck;
+                if (fp32) S.fp32_iterations += count_;
             }
             __syncthreads();
             if (S.status != kRunning) break;
@@ -1506,7 +1587,7 @@ struct Job {
     // Filled by a prepare worker before the job reaches the ready list.
     LpDev dev{};               // device pointers into the job's arena range
     int cls = 0;               // launch class
-    double attempt_b = 0, check_b = 0;
+    double attempt_b = 0, attempt_b32 = 0, check_b = 0;
     bool terminal = false;     // the host decided the result without iterating
     bool resident = false;     // the image is in the arena at `offset`
     std::string prepare_error; // non-empty: prepare or upload failed
@@ -1553,6 +1634,7 @@ std::vector<int64_t> t_ptr(const nyk::Model &m) {
 struct Sliced {
     std::vector<int64_t> base, index;
     std::vector<double> value;
+    std::vector<float> value32;  // with `fp32`: the values at their indices' positions
 };
 
 // The sliced layout of rows_dot_sell: per slice of 64 rows an even width w
@@ -1560,7 +1642,8 @@ struct Sliced {
 // base + (u / 2) * 128 + (r % 64) * 2 + u % 2, its index at
 // base + (u / 4) * 256 + (r % 64) * 4 + u % 4, or, in the last pair of a
 // width that is not a multiple of 4, at base + (u / 4) * 256 + (r % 64) * 2 + u % 2.
-Sliced slice_rows(const std::vector<int64_t> &ptr, const std::vector<int64_t> &index, const std::vector<double> &value) {
+Sliced slice_rows(const std::vector<int64_t> &ptr, const std::vector<int64_t> &index, const std::vector<double> &value,
+                  bool fp32 = false) {
     const int64_t rows = int64_t(ptr.size()) - 1, slices = (rows + 63) / 64;
     Sliced z;
     z.base.assign(slices + 1, 0);
@@ -1571,12 +1654,15 @@ Sliced slice_rows(const std::vector<int64_t> &ptr, const std::vector<int64_t> &i
     }
     z.index.assign(z.base[slices], 0);
     z.value.assign(z.base[slices], 0.0);
+    if (fp32) z.value32.assign(z.base[slices], 0.f);
     for (int64_t r = 0; r < rows; ++r)
         for (int64_t u = 0; u < ptr[r + 1] - ptr[r]; ++u) {
             const int64_t b = z.base[r / 64], w = (z.base[r / 64 + 1] - b) / 64;
             const int64_t quad = u / 4 * 4 + 4 <= w;
+            const int64_t at = b + u / 4 * 256 + (quad ? r % 64 * 4 + u % 4 : r % 64 * 2 + u % 2);
             z.value[b + u / 2 * 128 + r % 64 * 2 + u % 2] = value[ptr[r] + u];
-            z.index[b + u / 4 * 256 + (quad ? r % 64 * 4 + u % 4 : r % 64 * 2 + u % 2)] = index[ptr[r] + u];
+            z.index[at] = index[ptr[r] + u];
+            if (fp32) z.value32[at] = float(value[ptr[r] + u]);
         }
     return z;
 }
@@ -1699,6 +1785,7 @@ class BatchEngine {
   public:
     BatchEngine(const nyk::Options &o, int64_t arena_bytes, int64_t max_slots) : opt_(o) {
         if (o.method != "pdhg") throw std::runtime_error("The batched engine runs the pdhg method");
+        mixed_ = o.matrix_precision == "mixed";
         if (arena_bytes <= 0 || max_slots <= 0) throw std::runtime_error("Invalid arena size or slot count");
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count < 1) throw std::runtime_error("No HIP device is visible");
@@ -1780,6 +1867,7 @@ class BatchEngine {
         state_.resize(max_slots_);
         slot_job_.assign(max_slots_, -1);
         slot_bytes_.assign(max_slots_, 0);
+        slot_bytes32_.assign(max_slots_, 0);
         slot_check_bytes_.assign(max_slots_, 0);
         slot_class_.assign(max_slots_, kClasses - 1);
         slot_group_.assign(max_slots_, kClasses - 1);
@@ -1929,6 +2017,7 @@ class BatchEngine {
         if (j.terminal) return;
         j.cls = lds_class(j);
         j.attempt_b = attempt_bytes(j);
+        j.attempt_b32 = attempt_bytes(j, true);
         j.check_b = check_bytes(j);
         std::vector<char> image;
         const size_t bytes = build_image(j, image, j.dev, [&](size_t total) {
@@ -2033,6 +2122,7 @@ class BatchEngine {
         std::vector<Group> groups;
         std::vector<int> active;
         std::vector<long long> before, checks_before;
+        std::vector<char> fp32;  // the LP stepped with FP32 values at launch
         int shape_of[kClasses - 1] = {};
         int checks = 0;
     };
@@ -2042,11 +2132,14 @@ class BatchEngine {
     // the whole state array can be uploaded).
     void launch_round(int checks) {
         auto R = std::make_unique<Round>();
-        std::vector<int> by_class[kClasses];
+        std::vector<int> by_class[kClasses], refine;
         for (int s = 0; s < max_slots_; ++s)
             if (slot_job_[s] >= 0 && state_[s].status == kRunning) {
                 R->active.push_back(s);
-                by_class[slot_class_[s]].push_back(s);
+                // mixed precision: LDS-class LPs on FP64 values run in the
+                // tail launch (the crowded launches step with FP32 values)
+                if (mixed_ && state_[s].fp64 && slot_class_[s] < kClasses - 1) refine.push_back(s);
+                else by_class[slot_class_[s]].push_back(s);
             }
         if (R->active.empty()) return;
         R->checks = checks;
@@ -2064,7 +2157,8 @@ class BatchEngine {
         // of one per class (launches of different classes can land on the
         // same hardware queue and then run one after another).
         auto &groups = R->groups;
-        Round::Group tail{kClasses, 0, {}};
+        Round::Group tail{kClasses, 0, refine};
+        for (int s : refine) tail.cap = std::max(tail.cap, class_cap_[slot_class_[s]]);
         for (int c = 0; c < kClasses; ++c) {
             if (by_class[c].empty()) continue;
             if (c < kClasses - 1 && int(by_class[c].size()) <= kTailLps) {
@@ -2081,13 +2175,15 @@ class BatchEngine {
                                   hipMemcpyHostToDevice, stream_));
         if (tail_lps) HIP_OK(hipMemcpyAsync(d_queue_, &zero_, sizeof(unsigned), hipMemcpyHostToDevice, stream_));
         HIP_OK(hipEventRecord(uploaded_, stream_));
-        BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, 0, handoff_,
-                     d_pow3_,     d_pow6_,          pow_count_};
+        BatchOpts bo{opt_.tol, opt_.check_every, opt_.adaptive ? 1 : 0, opt_.restart ? 1 : 0, mixed_ ? 1 : 0,
+                     handoff_,    d_pow3_,         d_pow6_,                  pow_count_};
         R->before.resize(R->active.size());
         R->checks_before.resize(R->active.size());
+        R->fp32.resize(R->active.size());
         for (size_t k = 0; k < R->active.size(); ++k) {
             R->before[k] = state_[R->active[k]].accepted + state_[R->active[k]].rejected;
             R->checks_before[k] = state_[R->active[k]].checks;
+            R->fp32[k] = !state_[R->active[k]].fp64;
         }
         for (int c = 0; c < kClasses - 1; ++c) R->shape_of[c] = tuner_.next(c);
         // The tail launch continues past `checks` periods only when it
@@ -2191,10 +2287,11 @@ class BatchEngine {
         for (size_t k = 0; k < active.size(); ++k) {
             const LpState &st = state_[active[k]];
             const long long done = st.accepted + st.rejected - R->before[k];
-            group_bytes[slot_group_[active[k]]] += double(done) * slot_bytes_[active[k]];
+            const double bytes = R->fp32[k] ? slot_bytes32_[active[k]] : slot_bytes_[active[k]];
+            group_bytes[slot_group_[active[k]]] += double(done) * bytes;
             attempts_ += done;
-            advance_bytes_ += double(done) * slot_bytes_[active[k]];
-            class_bytes_[slot_group_[active[k]]] += double(done) * slot_bytes_[active[k]];
+            advance_bytes_ += double(done) * bytes;
+            class_bytes_[slot_group_[active[k]]] += double(done) * bytes;
             if (slot_group_[active[k]] == kClasses)  // the fused tail launch also did the checks
                 class_bytes_[kClasses] += double(st.checks - R->checks_before[k]) * slot_check_bytes_[active[k]];
         }
@@ -2313,7 +2410,7 @@ class BatchEngine {
         out->initial_weight = r.initial_weight;
         out->first_restart_weight = r.first_restart_weight;
         out->first_restart_iteration = r.first_restart_iteration;
-        out->fp32_iterations = 0;
+        out->fp32_iterations = r.fp32_iterations;
         if (out->x && r.x.size() == size_t(j.original.cols())) std::memcpy(out->x, r.x.data(), r.x.size() * 8);
         if (out->y && r.y.size() == size_t(j.original.rows())) std::memcpy(out->y, r.y.data(), r.y.size() * 8);
         return true;
@@ -2514,15 +2611,22 @@ class BatchEngine {
     // ub / write x, xa, dx; read y, ya, ax, rl, ru / write y, ya, ax).
     // (an entry is 8 value bytes and a 16-bit index in the sliced copies of
     // the LDS classes, a 32-bit index in the global class's CSR)
-    double attempt_bytes(const Job &j) const {
+    // Algorithmic bytes of one attempt: each entry of A and A^T (value and
+    // index: 8 + 2 in the sliced layouts, 4 + 2 with FP32 values, 8 + 4 in
+    // CSR), the offsets, one read of y and dx, and 64 bytes of vector
+    // streams per row and column.
+    double attempt_bytes(const Job &j, bool fp32 = false) const {
         const double n = double(j.prep.model.cols()), m = double(j.prep.model.rows()), nnz = double(j.prep.model.A.nnz());
-        const double entry = j.cls < kClasses - 1 ? 10 : 12;
+        const double entry = j.cls < kClasses - 1 ? (fp32 ? 6 : 10) : 12;
         return 2 * nnz * entry + (n + m + 2) * 4 + m * 8 + n * 8 + 64 * (n + m);
     }
 
     template <int NT, int NR>
     void launch_lds_t(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
-        batch_advance_lds_kernel<NT, NR><<<grid, NT, lds_bytes(cap), cs>>>(d_desc_, d_state_, slots, bo);
+        if (mixed_)
+            batch_advance_lds_kernel<NT, NR, float><<<grid, NT, lds_bytes(cap), cs>>>(d_desc_, d_state_, slots, bo);
+        else
+            batch_advance_lds_kernel<NT, NR, double><<<grid, NT, lds_bytes(cap), cs>>>(d_desc_, d_state_, slots, bo);
     }
     // Algorithmic bytes of one check period's check: the scaled SpMV that
     // rebases A x, then two original-space evaluations (restore, A x and A^T y
@@ -2592,6 +2696,7 @@ class BatchEngine {
             j.slot = s;
             slot_job_[s] = int(j.ticket);
             slot_bytes_[s] = j.attempt_b;
+            slot_bytes32_[s] = j.attempt_b32;
             slot_check_bytes_[s] = j.check_b;
             slot_class_[s] = j.cls;
             desc_[s] = j.dev;
@@ -2603,6 +2708,8 @@ class BatchEngine {
             st.best_kkt = j.result.accuracy.kkt;
             st.iteration_limit = opt_.iteration_limit;
             st.status = kRunning;
+            st.stall_kkt = INFINITY;
+            st.fp64 = mixed_ && j.cls < kClasses - 1 ? 0 : 1;  // the global class steps with FP64 values
             state_[s] = st;
             dirty_ = true;
         }
@@ -2655,14 +2762,17 @@ class BatchEngine {
         // the sliced copies serve only the LDS classes (the global class walks the CSR)
         Sliced sa, st;
         nyk::Csr t = m.A.transposed();
+        const bool fp32 = mixed_ && j.cls < kClasses - 1;
         if (j.cls < kClasses - 1) {
-            sa = slice_rows(m.A.ptr, m.A.index, m.A.value);
-            st = slice_rows(t.ptr, t.index, t.value);
+            sa = slice_rows(m.A.ptr, m.A.index, m.A.value, fp32);
+            st = slice_rows(t.ptr, t.index, t.value, fp32);
         }
         size_t o_asb = L.add<int>(sa.base.size()), o_asi = L.add<unsigned short>(sa.index.size()),
                o_asv = L.add<double>(sa.value.size());
         size_t o_tsb = L.add<int>(st.base.size()), o_tsi = L.add<unsigned short>(st.index.size()),
                o_tsv = L.add<double>(st.value.size());
+        const size_t o_asv32 = fp32 ? L.add<float>(sa.value32.size()) : 0,
+                     o_tsv32 = fp32 ? L.add<float>(st.value32.size()) : 0;
         const bool lds = j.cls < kClasses - 1;
         const size_t o_lub = lds ? L.add<double>(2 * size_t(n)) : 0, o_rlu = lds ? L.add<double>(2 * size_t(mr)) : 0;
         char *const base = place(L.total);
@@ -2796,6 +2906,13 @@ class BatchEngine {
         put_i(o_tsb, st.base);
         put_u16(o_tsi, st.index);
         put_d(o_tsv, st.value);
+        d.asv32 = d.tsv32 = nullptr;
+        if (fp32) {
+            if (!sa.value32.empty()) std::memcpy(image.data() + o_asv32, sa.value32.data(), sa.value32.size() * 4);
+            if (!st.value32.empty()) std::memcpy(image.data() + o_tsv32, st.value32.data(), st.value32.size() * 4);
+            d.asv32 = (const float *)P(o_asv32);
+            d.tsv32 = (const float *)P(o_tsv32);
+        }
         d.lub = d.rlu = nullptr;
         if (lds) {
             double *lub = reinterpret_cast<double *>(image.data() + o_lub);
@@ -3191,6 +3308,7 @@ class BatchEngine {
         r.host_candidate_checks = st.checks;
         r.first_restart_iteration = st.first_restart;
         r.first_restart_weight = st.first_weight;
+        r.fp32_iterations = st.fp32_iterations;
     }
 
     void retire(Job &j) {
@@ -3224,7 +3342,8 @@ class BatchEngine {
     std::vector<LpDev> desc_;
     std::vector<LpState> state_;
     std::vector<int> slot_job_, free_slots_;
-    std::vector<double> slot_bytes_, slot_check_bytes_;
+    std::vector<double> slot_bytes_, slot_bytes32_, slot_check_bytes_;
+    bool mixed_ = false;  // mixed precision: FP32 matrix values, then FP64
     std::unique_ptr<Settling> settling_;  // the certification running beside the next round
     long long tail_budget_ticks_ = 0;
     int compute_units_ = 0;
@@ -3333,8 +3452,8 @@ int niryukti_batch_create(const niryukti_options *opts, int64_t arena_bytes, int
     return guarded([&] {
         nyk::Options o = options_from(opts);
         nyk::validate_options(o);
-        if (o.primal_weight != "displacement" || o.matrix_precision != "fp64")
-            throw std::runtime_error("The batched engine supports the displacement weight and FP64 matrices");
+        if (o.primal_weight != "displacement" || (o.matrix_precision != "fp64" && o.matrix_precision != "mixed"))
+            throw std::runtime_error("The batched engine supports the displacement weight and FP64 or mixed matrices");
         if (o.polishing)
             throw std::runtime_error("Feasibility polishing runs in the single-LP engine, not in the batched engine");
         *out = new niryukti_batch(o, arena_bytes, max_slots);

@@ -308,3 +308,42 @@ def test_collect_returns_every_result_once(engine):
     with pytest.raises(RuntimeError):
         b.result(tickets[0])  # released by collect
     b.close()
+
+
+def test_mixed_precision_batch_certified_against_oracle(engine, oracle):
+    """Mixed precision in the batch (FP32 matrix values, FP64 products, sums
+    and iterates, FP64 values once a candidate reaches 1e-5 or the kkt stalls):
+    every LP is certified by the host's FP64 verifier at 1e-6 and its objective
+    lies within 1e-6 of the planted optimum and of the FP64 oracle's; the
+    objective error against the oracle is printed per LP.  Every LP ran on
+    FP32 values first, and most finished on FP64 values (an LP whose candidate
+    falls from above 1e-5 to 1e-6 in one check period needs none)."""
+    insts = [engine.Instance(2, 42, 1, i) for i in range(16)]
+    res, _ = solve_batch(engine, [i.lp for i in insts], matrix_precision="mixed")
+    assert sum(r["fp32_iterations"] < r["iterations"] for r in res) >= len(res) // 2
+    for k, (inst, r) in enumerate(zip(insts, res)):
+        assert r["status"] == "OPTIMAL" and r["kkt_error"] <= 1e-6, (r["status"], r["message"])
+        assert 0 < r["fp32_iterations"] <= r["iterations"], (r["fp32_iterations"], r["iterations"])
+        assert rel(r["objective"], inst.planted_objective) <= 1e-6
+        assert oracle.verify(inst.lp, r["x"], r["y"])["kkt"] <= 1e-6
+        if k < 4:
+            cpu = oracle.solve(inst.lp)
+            err = rel(r["objective"], cpu["objective"])
+            print(f"mixed LP {k}: fp32 iterations {r['fp32_iterations']} of {r['iterations']}, "
+                  f"objective error vs FP64 oracle {err:.2e}", flush=True)
+            assert err <= 1e-6
+
+
+def test_mixed_precision_batch_alone_equals_in_mix(engine):
+    """An LP's switch from FP32 to FP64 values depends only on its own checks,
+    and LPs on FP64 values run in the tail launch whatever their class: alone
+    (always the tail launch) and among 40 (crowded launches on FP32 values) an
+    LP follows the same trajectory bit for bit."""
+    insts = [engine.Instance(2, 7, 1, i) for i in range(40)]
+    mix, _ = solve_batch(engine, [i.lp for i in insts], matrix_precision="mixed")
+    for k in (0, 13, 39):
+        alone, _ = solve_batch(engine, [insts[k].lp], matrix_precision="mixed")
+        assert alone[0]["status"] == mix[k]["status"] == "OPTIMAL"
+        assert alone[0]["iterations"] == mix[k]["iterations"]
+        assert alone[0]["fp32_iterations"] == mix[k]["fp32_iterations"]
+        assert np.array_equal(alone[0]["x"], mix[k]["x"]) and np.array_equal(alone[0]["y"], mix[k]["y"])
