@@ -58,11 +58,11 @@ def make_instances(engine, count):
     return [engine.Instance(2, SEED, 1, i) for i in range(count)]
 
 
-def batch_step(engine, insts, checks, handoff=0, prepared=False):
+def batch_step(engine, insts, checks, handoff=0, prepared=False, precision="fp64"):
     """One step over the whole batch.  Timed from the first submit to the last
     certified result; with `prepared`, every LP is prepared and uploaded
     before the clock starts (the solve phase alone)."""
-    b = engine.Batch(arena_bytes=96 << 30, max_slots=len(insts), time_limit=1e9)
+    b = engine.Batch(arena_bytes=96 << 30, max_slots=len(insts), time_limit=1e9, matrix_precision=precision)
     b.set_handoff(handoff, 4)
     if prepared:
         tickets = [b.submit(i.lp) for i in insts]
@@ -561,10 +561,11 @@ def main():
     total_t, total_solved, lat, stats_all, last = 0.0, 0, [], [], None
     if args.no_stream:
         for w in range(args.warmup):
-            el, solved, _, _, _ = batch_step(engine, insts, args.checks, args.handoff)
+            el, solved, _, _, _ = batch_step(engine, insts, args.checks, args.handoff, precision=args.batch_precision)
             log(f"warmup {w}: {el:.3f} s, {solved} solved")
         for s in range(args.steps):
-            el, solved, results, stats, latencies = batch_step(engine, insts, args.checks, args.handoff)
+            el, solved, results, stats, latencies = batch_step(engine, insts, args.checks, args.handoff,
+                                                               precision=args.batch_precision)
             log(f"step {s}: {el:.3f} s, {solved}/{len(insts)} solved, stats {stats}")
             total_t += el
             total_solved += solved
@@ -612,7 +613,9 @@ def main():
         return sum(st[key][k] for st in stats_all)
 
     last_stats = stats_all[-1]
-    kernels = [(f"batch_advance_lds_kernel<{t}, {r}>", total("shape_bytes", k), total("shape_seconds", k),
+    # the crowded step kernel's value type: FP64, or FP32 values in mixed precision
+    vt = "float" if args.batch_precision == "mixed" else "double"
+    kernels = [(f"batch_advance_lds_kernel<{t}, {r}, {vt}>", total("shape_bytes", k), total("shape_seconds", k),
                 total("shape_launches", k)) for k, (t, r) in enumerate(LDS_SHAPES)]
     kernels += [("batch_advance_kernel", total("class_bytes", 3), total("class_seconds", 3), total("class_launches", 3)),
                 ("batch_tail_kernel<1024>", total("class_bytes", 4), total("class_seconds", 4),
@@ -621,7 +624,7 @@ def main():
     achieved = adv_b / adv_s / 1e9 if adv_s > 0 else 0.0
     users = [name for c, name in enumerate(("m+n <= 2048", "m+n <= 4096", "m+n <= 8000"))
              if (last_stats["class_threads"][c], last_stats["class_rows"][c]) in
-             [(t, r) for (t, r) in LDS_SHAPES if symbol == f"batch_advance_lds_kernel<{t}, {r}>"]]
+             [(t, r) for (t, r) in LDS_SHAPES if symbol == f"batch_advance_lds_kernel<{t}, {r}, {vt}>"]]
     if symbol.startswith("batch_advance_lds_kernel"):
         kernel_name = f"{symbol} (crowded LDS classes; tuned choice of class {', '.join(users) or 'none'})"
     elif symbol == "batch_advance_kernel":
@@ -644,7 +647,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.batch_precision == "fp64" else "f64 (f32 matrix values until 1e-5)",
         "data": "synthetic",
         "config": {"workload": f"batch of {args.count} independent LPs: m log-uniform [50,2000], n uniform "
                                f"[2m,3m], 3-8 nnz/col, N(0,1), box [0,U], U~U(1,50), planted optimum, seed {SEED}",
@@ -653,7 +656,8 @@ def main():
                    "passes": "one engine, pass k+1 submitted once fewer than "
                              f"{args.stream_threshold} LPs of earlier passes run" if not args.no_stream
                              else "one engine per pass, passes one after another",
-                   "engine": "batched (one workgroup per LP), FP64"},
+                   "engine": "batched (one workgroup per LP), " +
+                             ("FP64" if args.batch_precision == "fp64" else "FP32 matrix values, FP64 arithmetic")},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 1) if lat else None,
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 1) if lat else None},
         "statuses": statuses,
@@ -673,14 +677,14 @@ def main():
     }
     if not args.no_stream and not args.no_solve_phase:
         # one pass alone on a fresh engine, first submit to last result
-        el, solved, _, _, latencies = batch_step(engine, insts, args.checks, args.handoff)
+        el, solved, _, _, latencies = batch_step(engine, insts, args.checks, args.handoff, precision=args.batch_precision)
         line["one_pass_alone"] = {"lps_per_s": round(solved / el, 2), "seconds": round(el, 3), "solved": solved,
                                   "p50_ms": round(1e3 * float(np.percentile(latencies, 50)), 1),
                                   "p99_ms": round(1e3 * float(np.percentile(latencies, 99)), 1)}
         log(f"one pass alone: {el:.3f} s, {solved} solved")
     if not args.no_solve_phase:
         # the solve phase alone: every LP prepared and uploaded before the clock starts
-        el, solved, _, _, _ = batch_step(engine, insts, args.checks, args.handoff, prepared=True)
+        el, solved, _, _, _ = batch_step(engine, insts, args.checks, args.handoff, prepared=True, precision=args.batch_precision)
         line["solve_phase"] = {"lps_per_s": round(solved / el, 2), "seconds": round(el, 3), "solved": solved,
                                "note": "host prepare and upload before the timed region (not the headline)"}
         log(f"solve phase: {el:.3f} s, {solved} solved")

@@ -27,7 +27,7 @@ def dominant_kernel():
     summary's header states them."""
     text = (PROFILES / "batch_advance_trace_4096lp.txt").read_text()
     bench_ms = float(re.search(r"avg_launch_ms ([0-9.]+)", text).group(1))
-    kernel = re.search(r"^#\s+kernel (\S+(?:, \S+)?) \(", text, re.M).group(1)
+    kernel = re.search(r"^#\s+kernel (\S+(?:, \S+)*) \(", text, re.M).group(1)
     return text, kernel, bench_ms
 
 
