@@ -189,9 +189,13 @@ def test_xcd_chunked_products_track_oracle(engine, oracle, monkeypatch):
     direct.close()
 
 
-@pytest.mark.parametrize("case", [(0, 4, 0), (2, 1, 0), (2, 1, 1), (4, 200, 0), (1, 400, 0), (1, 20, 0)])
+@pytest.mark.parametrize("case", [(0, 4, 0), (2, 1, 0), (2, 1, 1), (4, 200, 0), (1, 400, 0), (1, 20, 0),
+                                  (1, 1, 0), (4, 1, 0)])
 def test_adaptive_chunks_track_oracle(engine, oracle, case):
-    """Three chunks of 10 adaptive steps on a generated (unscaled) LP."""
+    """Three chunks of 10 adaptive steps on a generated (unscaled) LP, up to
+    the full-size instances of configs 1 (1,000,000 x 2,000,000, 12.5M
+    nonzeros: hub rows as long-row segments, A dx in XCD column chunks) and 4
+    (1,282,000 x 2,560,000, 29.7M nonzeros: the linking rows as segments)."""
     config, scale, index = case
     inst = engine.Instance(config, 11, scale, index)
     lp = inst.lp
