@@ -958,15 +958,37 @@ __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const 
 }
 
 // Sums over rows are taken per tile of 64 consecutive rows (one wave, fixed
-// butterfly) and the tiles are then added in tile order, so an attempt's
-// energy and coupling do not depend on the workgroup size or rows per thread:
-// an LP can change launch shape (the tail mode below) and still follow
-// exactly the trajectory it follows alone.
+// butterfly) and the tiles are then added in tile order, so an attempt's energy
+// and coupling do not depend on the workgroup size or rows per thread: an LP
+// can change launch shape (the tail mode below) and still follow exactly the
+// trajectory it follows alone.
 constexpr int kMaxTiles = 128;  // 64-row tiles: m, n <= 8000 in the LDS classes
 
+// v rotated within each row of 16 lanes by the DPP control CTRL (row_ror);
+// both 32-bit halves move alike.
+template <int CTRL>
+__device__ __forceinline__ double dpp_rotate(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(unsigned(b)), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(unsigned(b >> 32)), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, (unsigned long long)unsigned(hi) << 32 | unsigned(lo));
+}
+
+// Sum of v over the 64 lanes of a wave, valid in every lane: the butterfly
+// v += v[lane ^ off] for off = 32, 16, 8, 4, 2, 1.  The two cross-row levels
+// are lane shuffles; the four within a row of 16 lanes are DPP row rotations
+// on the VALU (no LDS round trip): rotating by 8 pairs lane i with i ^ 8, and
+// once the values are symmetric under ^ 8 (after that level), rotating by 4
+// brings lane i the value of lane i ^ 4, and so on for 2 and 1, so every
+// lane adds exactly the partner value the butterfly adds and the sum is the
+// butterfly's, bit for bit.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    v += __shfl_xor(v, 32, 64);
+    v += __shfl_xor(v, 16, 64);
+    v += dpp_rotate<0x128>(v);  // row_ror:8
+    v += dpp_rotate<0x124>(v);  // row_ror:4
+    v += dpp_rotate<0x122>(v);  // row_ror:2
+    v += dpp_rotate<0x121>(v);  // row_ror:1
     return v;
 }
 
