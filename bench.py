@@ -518,6 +518,10 @@ def main():
     ap.add_argument("--qps", type=float, default=1.0, help="serving stream: Poisson arrival rate")
     ap.add_argument("--duration", type=float, default=30.0, help="serving stream: arrival window in seconds")
     ap.add_argument("--no-serve", action="store_true", help="leave the serving stream out of the default run")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="leave the capped config-1 single-LP comparison out of the default run")
+    ap.add_argument("--config1-iterations", type=int, default=3000,
+                    help="iteration cap of the config-1 single-LP comparison (GPU and CPU alike)")
     ap.add_argument("--no-solve-phase", action="store_true",
                     help="leave the solve-phase step (LPs prepared before the clock) out of the default run")
     args = ap.parse_args()
@@ -705,6 +709,19 @@ def main():
         # warm re-solves): an extra key beside the batch headline
         line["serving"] = serve_stream(engine, args.qps, args.duration)
         log(f"serving: {line['serving']}")
+    if not args.no_baselines and not args.no_config1:
+        # the single large LP (1,000,000 x 2,000,000, power-law rows) on the
+        # grid-wide single-LP engine beside the reference's CPU PDHG on all
+        # cores, both capped at the same iteration count: neither converges
+        # within the 100,000-iteration default, so the record is "not
+        # converged at the cap" with the KKT each reached, and the time per
+        # iteration (an extra key, not the headline)
+        c1 = single_lp(engine, 1, 1, args.config1_iterations, 1e-6, cpu_threads=cores)
+        c1["note"] = (f"capped at {args.config1_iterations} iterations: GPU {c1['status']} at kkt {c1['kkt']:.3g}, "
+                      f"CPU {c1.get('cpu_baseline', {}).get('status')} at kkt "
+                      f"{c1.get('cpu_baseline', {}).get('kkt', float('nan')):.3g}")
+        line["config1_single_lp"] = c1
+        log(f"config 1: {c1}")
     print(json.dumps(line), flush=True)
 
 
