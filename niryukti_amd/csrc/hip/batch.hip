@@ -1726,9 +1726,13 @@ static_assert(kShapeCount == 4, "niryukti_batch_stats holds four shape entries")
 // classes run side by side, and a shape with more workgroups per CU takes
 // slots from its neighbours (scored alone, such a shape won its class's
 // trials and lost end to end).  Trials count only once the class under trial
-// has run at least kMinTrialLps LPs (two per CU of the 256) for kSettle
-// rounds: below that a launch lasts as long as its slowest LP, whatever the
-// shape, and the first rounds of a batch run while prepares still admit LPs.
+// has run enough LPs to fill the device with its start shape -- two
+// 256-thread workgroups per CU of the 256 for the smallest class, one
+// 1,024-thread workgroup per CU for the larger two -- for kSettle rounds:
+// below that a launch lasts as long as its slowest LP, whatever the shape,
+// and the first rounds of a batch run while prepares still admit LPs.  (A
+// fixed 512 left the larger classes untuned in streamed runs, where fewer of
+// their LPs are resident at once.)
 // The smallest class (m + n <= 2,048) tries the 256-thread shapes only: a
 // 1,024-thread workgroup has at most two rows or columns per thread there and
 // idles most of its waves in each pass (its trials were within the rounds'
@@ -1736,13 +1740,15 @@ static_assert(kShapeCount == 4, "niryukti_batch_stats holds four shape entries")
 class ShapeTuner {
   public:
     static constexpr int kTuned = 3;  // the crowded LDS classes
-    static constexpr int kMinTrialLps = 512;  // NIRYUKTI_BATCH_TUNE_MIN_LPS overrides it
+    static constexpr int kMinTrialLps[kTuned] = {512, 256, 256};  // NIRYUKTI_BATCH_TUNE_MIN_LPS overrides them
     static constexpr int kSettle = 2;
     static constexpr int kRepeats = 4;
     static constexpr double kMargin = 0.15;
     static bool allowed(int c, int shape) { return c > 0 || kShapes[shape].threads <= 256; }
     void start_with(int c, int shape) { current_[c] = shape; }
-    void set_min_lps(int lps) { min_lps_ = std::max(1, lps); }
+    void set_min_lps(int lps) {
+        for (int &v : min_lps_) v = std::max(1, lps);
+    }
     void fix(int c, int shape) {
         current_[c] = shape;
         fixed_[c] = true;
@@ -1755,7 +1761,7 @@ class ShapeTuner {
     void record(const int *shape_of, const int *lps, double bytes, double seconds) {
         const int c = trial_class();
         if (c < 0) return;
-        if (lps[c] < min_lps_) {
+        if (lps[c] < min_lps_[c]) {
             settled_ = 0;  // trials so far stand; they resume after kSettle full rounds
             return;
         }
@@ -1799,7 +1805,7 @@ class ShapeTuner {
     }
     int current_[kTuned] = {};
     bool fixed_[kTuned] = {}, done_[kTuned] = {};
-    int tried_ = 0, settled_ = 0, min_lps_ = kMinTrialLps;
+    int tried_ = 0, settled_ = 0, min_lps_[kTuned] = {kMinTrialLps[0], kMinTrialLps[1], kMinTrialLps[2]};
     double rate_[kShapeCount] = {};
 };
 
