@@ -5,7 +5,8 @@
  * (python/niryukti/service.py:32-71).  This server queues instead and keeps
  * the GPU busy with continuous batching:
  *
- *  - LPs below `large_nonzeros` go to one batched engine (include/niryukti_batch.h):
+ *  - LPs below `large_nonzeros` and `batch_max_size` rows + columns go to one
+ *    batched engine (include/niryukti_batch.h):
  *    a scheduler thread submits them (their host prepare runs on the engine's
  *    prepare workers, off the launch path), steps the engine, and completes
  *    requests as their LPs are certified, while other LPs keep iterating;
@@ -35,6 +36,10 @@ typedef struct {
     int grid_workers;       /* concurrent grid-route solves, 1..16 */
     int checks_per_launch;  /* check periods per batched scheduling round */
     int64_t max_slots;      /* resident LPs of the batched engine */
+    int64_t batch_max_size; /* LPs with more rows + columns than this take the grid route: above the
+                             * batched engine's largest LDS class (8,000) an LP runs on one
+                             * workgroup walking its CSR from memory, 2-80x slower per iteration
+                             * than on the whole GPU */
 } niryukti_server_limits;
 
 typedef struct {

@@ -90,7 +90,8 @@ class Server {
   public:
     Server(const niryukti_options *opts, const niryukti_server_limits &l) : limits_(l) {
         if (!(l.batch_fraction > 0 && l.batch_fraction < 1) || l.hbm_budget <= 0 || l.grid_workers < 1 ||
-            l.grid_workers > 16 || l.checks_per_launch < 1 || l.max_slots < 1 || l.large_nonzeros < 0)
+            l.grid_workers > 16 || l.checks_per_launch < 1 || l.max_slots < 1 || l.large_nonzeros < 0 ||
+            l.batch_max_size < 0)
             throw std::runtime_error("Invalid server limits");
         niryukti_default_options(&defaults_);
         if (opts) defaults_ = *opts;
@@ -159,7 +160,8 @@ class Server {
             keep_options(*q, *opts);
             q->own_options = true;
         }
-        q->route = route >= 0 ? route : (nnz >= limits_.large_nonzeros || opts ? 1 : 0);
+        const bool large = nnz >= limits_.large_nonzeros || lp.rows + lp.cols > limits_.batch_max_size;
+        q->route = route >= 0 ? route : (large || opts ? 1 : 0);
         if (q->route == 0 && opts) throw std::runtime_error("Requests with options of their own take the grid route");
         q->grid_bytes = int64_t(2 * (48.0 * double(nnz) + 240.0 * double(lp.rows + lp.cols + 2)));
         q->x.assign(size_t(lp.cols), 0.0);
@@ -391,6 +393,7 @@ void niryukti_server_default_limits(niryukti_server_limits *l) {
     l->grid_workers = 2;
     l->checks_per_launch = 1;
     l->max_slots = 4096;
+    l->batch_max_size = 8000;
 }
 
 int niryukti_server_create(const niryukti_options *opts, const niryukti_server_limits *limits, niryukti_server **out) {

@@ -7,10 +7,10 @@ admitted as device memory frees up, by the native server of
 include/niryukti_serve.h (a C++ scheduler thread and grid workers; this module
 only binds it):
 
-* LPs below `large_nonzeros` go to one batched engine: each is a workgroup of
-  the batched kernel, prepared on the engine's host workers, admitted into the
-  HBM arena between launches, certified on the host and retired while the
-  others keep running.
+* LPs below `large_nonzeros` nonzeros and `batch_max_size` rows + columns go
+  to one batched engine: each is a workgroup of the batched kernel, prepared
+  on the engine's host workers, admitted into the HBM arena between launches,
+  certified on the host and retired while the others keep running.
 * Larger LPs run on the grid-wide single-LP engine, a few at a time on worker
   threads, each on its own HIP stream, admitted against the rest of the HBM
   budget.
@@ -73,7 +73,8 @@ def parse_request(request, tol: float = 1e-6, time_limit: float = 60.0, iteratio
 
 class _Limits(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("batch_fraction", C.c_double), ("large_nonzeros", C.c_int64),
-                ("grid_workers", C.c_int), ("checks_per_launch", C.c_int), ("max_slots", C.c_int64)]
+                ("grid_workers", C.c_int), ("checks_per_launch", C.c_int), ("max_slots", C.c_int64),
+                ("batch_max_size", C.c_int64)]
 
 
 class _Stats(C.Structure):
@@ -160,17 +161,18 @@ class Server:
 
     def __init__(self, hbm_budget: int = 64 << 30, batch_fraction: float = 0.75, large_nonzeros: int = 200_000,
                  grid_workers: int = 2, checks_per_launch: int = 1, max_slots: int = 4096, tol: float = 1e-6,
-                 time_limit: float = 60.0, iteration_limit: int = 100_000):
+                 time_limit: float = 60.0, iteration_limit: int = 100_000, batch_max_size: int = 8000):
         if not 0 < batch_fraction < 1 or hbm_budget <= 0 or grid_workers < 1:
             raise ValueError("invalid server limits")
         self._lib = _bind(load_library())
         self.tol, self.time_limit, self.iteration_limit = tol, time_limit, iteration_limit
         self.large_nonzeros = large_nonzeros
+        self.batch_max_size = batch_max_size
         self._keep = []
         opts = _options_struct(self._lib, dict(tol=tol, time_limit=time_limit, iteration_limit=iteration_limit),
                                self._keep)
         limits = _Limits(int(hbm_budget), float(batch_fraction), int(large_nonzeros), int(grid_workers),
-                         int(checks_per_launch), int(max_slots))
+                         int(checks_per_launch), int(max_slots), int(batch_max_size))
         h = C.c_void_p()
         if self._lib.niryukti_server_create(C.byref(opts), C.byref(limits), C.byref(h)) != 0:
             raise RuntimeError(self._lib.niryukti_server_last_error().decode())
@@ -203,8 +205,8 @@ class Server:
                                               C.byref(opts) if opts is not None else None, C.byref(rid))
         if rc != 0:
             raise ValueError(self._lib.niryukti_server_last_error().decode())
-        guess = "grid" if (route == "grid" or options or (route is None and lp.nnz >= self.large_nonzeros)) \
-            else "batch"
+        large = lp.nnz >= self.large_nonzeros or lp.rows + lp.cols > self.batch_max_size
+        guess = "grid" if (route == "grid" or options or (route is None and large)) else "batch"
         req = Request(self, rid.value, lp, x0, y0, guess)
         with self._open_lock:
             self._open[req.id] = req

@@ -85,3 +85,18 @@ def test_solve_request_shape(engine):
         assert server.solve_request({"model": model, "extra": 1})[0] == 400
         assert server.solve_request({"model": model, "options": {"threads": 0}})[0] == 400
         assert server.solve_request({"model": {"variables": "bad"}})[0] == 400
+
+
+def test_lps_beyond_the_lds_classes_take_the_grid_route(engine):
+    """An LP with more than 8,000 rows + columns would run in the batched
+    engine's global class (one workgroup walking its CSR from memory, 2-80x
+    slower per iteration than the grid engine), so the server routes it to the
+    grid engine by size even when its nonzeros are below large_nonzeros."""
+    small, big = engine.Instance(3, 42, 1, 3), engine.Instance(3, 42, 1, 17)
+    assert small.lp.rows + small.lp.cols <= 8000 < big.lp.rows + big.lp.cols
+    with engine.Server(hbm_budget=8 << 30, large_nonzeros=10 ** 9, grid_workers=1) as server:
+        rs, rb = server.submit(small.lp), server.submit(big.lp)
+        a, b = rs.wait(), rb.wait()
+    assert rs.route == "batch" and rb.route == "grid"
+    assert a["status"] == b["status"] == "OPTIMAL"
+    assert np.array_equal(b["x"], engine.solve_lp(big.lp)["x"])
