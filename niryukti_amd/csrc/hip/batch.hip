@@ -1,6 +1,9 @@
 // Batched and served restarted PDHG on MI355X (gfx950): one workgroup per LP.
 //
-// Every resident LP owns one 256-thread workgroup, which runs the reference's continuous driver for that LP entirely on the device
+// Every running LP is driven by one workgroup (256 or 1,024 threads by size
+// class; the last LPs of a class share a persistent tail grid that takes
+// them from a device-side queue), which runs the reference's continuous
+// driver for that LP entirely on the device
 // (src/solver.cpp:511-726): chunks of `check_every` accepted adaptive PDHG
 // steps (src/cpu.cpp:96-222), activity rebasing, original-space evaluation of
 // the current and averaged candidates (src/verify.cpp:103-240 formulae),
@@ -18,7 +21,9 @@
 // is bit-identical whether it runs alone or among thousands of others.
 //
 // The host side admits LPs into a fixed HBM arena as room frees up
-// (continuous batching between launches) and retires finished ones.
+// (continuous batching between launches) and retires finished ones; its
+// scheduling rounds are pipelined (a round is launched before the previous
+// one's stopped LPs are certified).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -2666,13 +2671,13 @@ class BatchEngine {
         return nnz * 12 + m * 12 + n * 8 + 2 * (24 * nnz0 + 68 * m0 + 100 * n0 + 28 * n + 32 * m);
     }
 
-    // The tail group: one fused launch per scheduling round, one row per
-    // thread over 1,024 threads (2 or 4 rows per thread spill registers at
-    // this size and were measured slower).  Each LP runs at least `checks`
-    // periods and then continues until `budget` ticks have passed.
-    // The tail group's LPs come from the device-side queue at d_queue_ (zeroed
-    // with the round's uploads), so the grid needs no more workgroups than the
-    // device runs at once: one 1,024-thread workgroup per CU.
+    // The tail group: one launch per scheduling round, one row per thread
+    // over 1,024 threads (2 or 4 rows per thread spill registers at this size
+    // and were measured slower).  Each LP runs at least `checks` periods and
+    // then continues until `budget` ticks have passed.  The LPs come from the
+    // device-side queue at d_queue_ (zeroed with the round's uploads), so the
+    // grid needs no more workgroups than the device runs at once: one
+    // 1,024-thread workgroup per CU.
     void launch_tail(int cap, int count, hipStream_t cs, const int *slots, const BatchOpts &bo, int checks,
                      long long budget) {
         const int grid = std::max(1, std::min(count, compute_units_));
