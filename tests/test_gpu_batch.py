@@ -347,3 +347,21 @@ def test_mixed_precision_batch_alone_equals_in_mix(engine):
         assert alone[0]["iterations"] == mix[k]["iterations"]
         assert alone[0]["fp32_iterations"] == mix[k]["fp32_iterations"]
         assert np.array_equal(alone[0]["x"], mix[k]["x"]) and np.array_equal(alone[0]["y"], mix[k]["y"])
+
+
+def test_time_limit_stops_only_the_slow_lp(engine):
+    """A time limit stops a running LP at the round that finds it past the
+    limit (TIME_LIMIT, with its best candidate), is left out of the next round
+    and frees its slot; the LPs around it finish OPTIMAL and equal to their
+    solves alone."""
+    slow = engine.Instance(2, 42, 1, 222)  # runs to the 100,000-iteration limit (~1 s alone)
+    quick = [engine.Instance(2, 42, 1, i) for i in (0, 1, 2)]
+    res, stats = solve_batch(engine, [slow.lp] + [q.lp for q in quick], time_limit=0.3)
+    assert stats["finished"] == 4 and stats["active"] == 0
+    assert res[0]["status"] == "TIME_LIMIT"
+    assert 0 < res[0]["iterations"] < 100_000
+    assert res[0]["x"] is not None and np.all(np.isfinite(res[0]["x"]))
+    alone = [solve_batch(engine, [q.lp])[0][0] for q in quick]
+    for r, a in zip(res[1:], alone):
+        assert r["status"] == a["status"] == "OPTIMAL"
+        assert r["iterations"] == a["iterations"] and np.array_equal(r["x"], a["x"])
