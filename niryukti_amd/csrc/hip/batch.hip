@@ -66,8 +66,7 @@ constexpr int kClasses = 4;
 constexpr int kClassDoubles[kClasses - 1] = {2048, 4096, 8000};
 static_assert(kClassDoubles[kClasses - 2] <= 65536, "sliced entry indices are 16-bit");
 // Upper bound of the LDS step kernels' static allocation (state, tile sums,
-// flags; the fused tail kernel adds the check's reduction slots and its
-// kWideChunk stage).
+// flags; the fused tail kernel adds the check's reduction slots).
 constexpr int kStaticLds = 57344;
 constexpr int64_t kShortRow = 128;
 // The two larger LDS classes (m + n above 2,048) run 1,024-thread workgroups
@@ -129,8 +128,9 @@ struct LpDev {
     double *ox[2], *oy[2], *act, *aty, *atyabs;
     double *act1, *aty1, *atyabs1;  // the second candidate's products in a paired check
     double *bx, *by;
-    const int *ablk, *tblk, *oblk, *otblk;  // row blocks of A, A^T, original A, original A^T
-    const int *ablkw, *oblkw, *otblkw;      // the same for the tail kernel's wider check stage
+    const int *ablk, *tblk;  // row blocks of A and A^T (the global class's stream_rows)
+    // rows longer than kChunk of A, original A, original A^T (the checks' trees)
+    const int *along, *olong, *otlong;
     // A and A^T in 64-row slices for the LDS step kernel (see rows_dot_sell)
     // (entry indices in 16 bits: an LDS-class LP has m + n <= 8,000)
     const int *asb, *tsb;
@@ -143,7 +143,7 @@ struct LpDev {
     // (lb, ub) and (rl, ru) interleaved, so the step reads a bound pair with
     // one 16-byte load (the LDS classes; null for the global class)
     const double *lub, *rlu;
-    int nab, ntb, nob, notb, nabw, nobw, notbw;
+    int nab, ntb, nalong, nolong, notlong;
 };
 
 struct LpState {
@@ -227,23 +227,49 @@ __device__ __forceinline__ double wave_tree(const double *lds, int W, int k) {
     }
 }
 
+// v from the lane CTRL names within the lane's row of 16 (a DPP row
+// rotation: 0x128 + k rotates right by 8 + k lanes).
+template <int CTRL>
+__device__ __forceinline__ double dpp_rotate(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(unsigned(b)), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(unsigned(b >> 32)), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, (unsigned long long)unsigned(hi) << 32 | unsigned(lo));
+}
+
+// One level of the wave butterfly for every value of r: each lane combines
+// its value with the one `partner` brings (sums, maxima, double-double sums).
+template <int NS, int NM, int ND, class Partner>
+__device__ __forceinline__ void butterfly_level(Reducer<NS, NM, ND> &r, Partner partner) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) r.s[k] += partner(r.s[k]);
+#pragma unroll
+    for (int k = 0; k < NM; ++k) r.mx[k] = maxd(r.mx[k], partner(r.mx[k]));
+#pragma unroll
+    for (int k = 0; k < ND; ++k) r.d[k] = dd_merge(r.d[k], DD{partner(r.d[k].hi), partner(r.d[k].lo)});
+}
+
 // Workgroup reduction over the first NW waves; the lanes of any further
 // waves take no part (a larger workgroup can run an NW-wave reduction and get
 // exactly the NW-wave result), but every thread reaches the barriers.
+// Within a wave the butterfly combines lane i with lane i ^ off for off = 32,
+// 16, 8, 4, 2, 1.  The two cross-row levels are lane shuffles; the four
+// within a row of 16 lanes are DPP row rotations: rotating by 8 pairs lane i
+// with i ^ 8, and once the values are symmetric under ^ 8 (after that level)
+// rotating by 4 brings lane i the value of lane i ^ 4, and so on for 2 and
+// 1.  Every combination here is commutative bit for bit (sums; maxima of
+// values that are never NaN or -0; double-double sums, whose two-sum error
+// term is exact), so every lane gets exactly the butterfly's value.
 template <int NW = kWaves, int NS, int NM, int ND>
 __device__ void block_reduce(Reducer<NS, NM, ND> &r, double *lds) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int k = 0; k < NS; ++k) r.s[k] += __shfl_xor(r.s[k], off, 64);
-#pragma unroll
-        for (int k = 0; k < NM; ++k) r.mx[k] = maxd(r.mx[k], __shfl_xor(r.mx[k], off, 64));
-#pragma unroll
-        for (int k = 0; k < ND; ++k) {
-            DD o{__shfl_xor(r.d[k].hi, off, 64), __shfl_xor(r.d[k].lo, off, 64)};
-            r.d[k] = dd_merge(r.d[k], o);
-        }
+    if (wave < NW) {
+        butterfly_level(r, [](double v) { return __shfl_xor(v, 32, 64); });
+        butterfly_level(r, [](double v) { return __shfl_xor(v, 16, 64); });
+        butterfly_level(r, [](double v) { return dpp_rotate<0x128>(v); });  // row_ror:8
+        butterfly_level(r, [](double v) { return dpp_rotate<0x124>(v); });  // row_ror:4
+        butterfly_level(r, [](double v) { return dpp_rotate<0x122>(v); });  // row_ror:2
+        butterfly_level(r, [](double v) { return dpp_rotate<0x121>(v); });  // row_ror:1
     }
     constexpr int W = NS + NM + 2 * ND;
     __syncthreads();
@@ -263,58 +289,6 @@ __device__ void block_reduce(Reducer<NS, NM, ND> &r, double *lds) {
     for (int k = 0; k < NS; ++k) r.s[k] = wave_tree<0, NW>(lds, W, k);
 #pragma unroll
     for (int k = 0; k < NM; ++k) r.mx[k] = wave_tree<0, NW, true>(lds, W, NS + k);
-#pragma unroll
-    for (int k = 0; k < ND; ++k) {
-        const int b = NS + NM + 2 * k;
-        DD acc{lds[b], lds[b + 1]};
-        for (int w = 1; w < NW; ++w) acc = dd_merge(acc, DD{lds[w * W + b], lds[w * W + b + 1]});
-        r.d[k] = acc;
-    }
-    __syncthreads();
-}
-
-// block_reduce for a workgroup of NW waves (sums are combined wave 0 first).
-template <int NW, int NS, int NM, int ND>
-__device__ void block_reduce_n(Reducer<NS, NM, ND> &r, double *lds) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int k = 0; k < NS; ++k) r.s[k] += __shfl_xor(r.s[k], off, 64);
-#pragma unroll
-        for (int k = 0; k < NM; ++k) r.mx[k] = maxd(r.mx[k], __shfl_xor(r.mx[k], off, 64));
-#pragma unroll
-        for (int k = 0; k < ND; ++k) {
-            DD o{__shfl_xor(r.d[k].hi, off, 64), __shfl_xor(r.d[k].lo, off, 64)};
-            r.d[k] = dd_merge(r.d[k], o);
-        }
-    }
-    constexpr int W = NS + NM + 2 * ND;
-    __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NS; ++k) lds[wave * W + k] = r.s[k];
-#pragma unroll
-        for (int k = 0; k < NM; ++k) lds[wave * W + NS + k] = r.mx[k];
-#pragma unroll
-        for (int k = 0; k < ND; ++k) {
-            lds[wave * W + NS + NM + 2 * k] = r.d[k].hi;
-            lds[wave * W + NS + NM + 2 * k + 1] = r.d[k].lo;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        double acc = lds[k];
-        for (int w = 1; w < NW; ++w) acc += lds[w * W + k];
-        r.s[k] = acc;
-    }
-#pragma unroll
-    for (int k = 0; k < NM; ++k) {
-        double acc = lds[NS + k];
-        for (int w = 1; w < NW; ++w) acc = maxd(acc, lds[w * W + NS + k]);
-        r.mx[k] = acc;
-    }
 #pragma unroll
     for (int k = 0; k < ND; ++k) {
         const int b = NS + NM + 2 * k;
@@ -358,12 +332,6 @@ __device__ __forceinline__ GLOBAL_AS T *gout(T *p) {
 
 constexpr int kChunk = 2048;
 
-// Stage of the check in the fused tail kernel: its row blocks hold up to
-// kWideChunk nonzeros of rows that are each at most kChunk long, so a check
-// takes a third of the dependent passes (each pass is a chain of memory round
-// trips and two barriers, which is what bounds a check on one workgroup).
-constexpr int kWideChunk = 6144;
-
 // Start of a thread's strided loop over [begin, end) in which only the first
 // NT threads of the workgroup take part (the others start at `end`).
 template <int NT>
@@ -400,6 +368,7 @@ __device__ void stream_rows(const int *ptr_, const int *idx_, const double *val_
         } else {
             Reducer<2, 0, 0> rr{};
             for (int k = first_index<NTR>(k0, k1); k < k1; k += NTR) {
+#pragma clang fp contract(off)
                 const double t = val[k] * vec[idx[k]];
                 rr.s[0] += t;
                 if (ABS) rr.s[1] += fabs(t);
@@ -411,67 +380,69 @@ __device__ void stream_rows(const int *ptr_, const int *idx_, const double *val_
     }
 }
 
-// stream_rows over two vectors at once (row blocks of at most kChunk
-// nonzeros, stage of 2 * kChunk doubles): each entry's value and index are
-// read once, and each vector's products are formed and summed exactly as
-// stream_rows forms them.  epi(r, sum0, abs_sum0, sum1, abs_sum1).
-template <bool ABS, int NT, int NTR, class Epi>
-__device__ void stream_rows2(const int *ptr_, const int *idx_, const double *val_, const int *blk_, int nblk,
-                             const double *vec0_, const double *vec1_, double *stage, double *red, Epi epi) {
-    const GLOBAL_AS int *ptr = gin(ptr_), *idx = gin(idx_), *blk = gin(blk_);
-    const GLOBAL_AS double *val = gin(val_), *vec0 = gin(vec0_), *vec1 = gin(vec1_);
-    double *stage1 = stage + kChunk;
-    for (int b = 0; b < nblk; ++b) {
-        const int r0 = blk[b], r1 = blk[b + 1];
-        const int k0 = ptr[r0], k1 = ptr[r1];
-        if (k1 - k0 <= kChunk || r1 - r0 > 1) {
-            for (int k = k0 + threadIdx.x; k < k1; k += NT) {
-                const double v = val[k];
-                const int c = idx[k];
-                stage[k - k0] = v * vec0[c];
-                stage1[k - k0] = v * vec1[c];
+// CSR row sums for the checks, one thread per row: each row of up to kChunk
+// nonzeros is summed by one thread straight from memory, left to right from
+// 0 over products rounded before they are added (the order of the
+// reference's sequential row sums, and the values stream_rows stages), so
+// the rows need no stage and no barrier between them.  The rows longer than
+// kChunk, listed by the host in `longs`, are then summed one after another
+// by the first NTR threads in stream_rows' tree.  NV (1 or 2) vectors share
+// each entry's value and index loads; epi(r, sums, abs_sums) takes arrays of
+// NV values.  Ends with a barrier.
+template <bool ABS, int NV, int NT, int NTR, class Epi>
+__device__ void rows_direct(const int *ptr_, const int *idx_, const double *val_, int nrows, const int *longs_,
+                            int nlong, const double *vec0_, const double *vec1_, double *red, Epi epi) {
+    const GLOBAL_AS int *ptr = gin(ptr_), *idx = gin(idx_);
+    const GLOBAL_AS double *val = gin(val_);
+    const GLOBAL_AS double *vec[2] = {gin(vec0_), gin(vec1_)};
+    for (int r = threadIdx.x; r < nrows; r += NT) {
+        const int k0 = ptr[r], k1 = ptr[r + 1];
+        if (k1 - k0 > kChunk) continue;
+        double acc[NV], ab[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] = ab[q] = 0;
+#pragma unroll 4
+        for (int k = k0; k < k1; ++k) {
+            const double v = val[k];
+            const int c = idx[k];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+#pragma clang fp contract(off)
+                const double t = v * vec[q][c];
+                acc[q] += t;
+                if (ABS) ab[q] += fabs(t);
             }
-            __syncthreads();
-            for (int r = r0 + threadIdx.x; r < r1; r += NT) {
-                double acc0 = 0, a0 = 0, acc1 = 0, a1 = 0;
-                const int e = ptr[r + 1] - k0;
-                for (int k = ptr[r] - k0; k < e; ++k) {
-                    acc0 += stage[k];
-                    acc1 += stage1[k];
-                    if (ABS) {
-                        a0 += fabs(stage[k]);
-                        a1 += fabs(stage1[k]);
-                    }
-                }
-                epi(r, acc0, a0, acc1, a1);
+        }
+        epi(r, acc, ab);
+    }
+    for (int q = 0; q < nlong; ++q) {
+        const int r = gin(longs_)[q], k0 = ptr[r], k1 = ptr[r + 1];
+        Reducer<2 * NV, 0, 0> rr{};
+        for (int k = first_index<NTR>(k0, k1); k < k1; k += NTR) {
+            const double v = val[k];
+            const int c = idx[k];
+#pragma unroll
+            for (int p = 0; p < NV; ++p) {
+#pragma clang fp contract(off)
+                const double t = v * vec[p][c];
+                rr.s[2 * p] += t;
+                if (ABS) rr.s[2 * p + 1] += fabs(t);
             }
-            __syncthreads();
-        } else {
-            Reducer<4, 0, 0> rr{};
-            for (int k = first_index<NTR>(k0, k1); k < k1; k += NTR) {
-                const double v = val[k];
-                const int c = idx[k];
-                const double t0 = v * vec0[c], t1 = v * vec1[c];
-                rr.s[0] += t0;
-                rr.s[2] += t1;
-                if (ABS) {
-                    rr.s[1] += fabs(t0);
-                    rr.s[3] += fabs(t1);
-                }
+        }
+        block_reduce<NTR / 64>(rr, red);
+        if (threadIdx.x == 0) {
+            double v[NV], a[NV];
+#pragma unroll
+            for (int p = 0; p < NV; ++p) {
+                v[p] = 0.0 + rr.s[2 * p];
+                a[p] = rr.s[2 * p + 1];
             }
-            block_reduce<NTR / 64>(rr, red);
-            if (threadIdx.x == 0) epi(r0, 0.0 + rr.s[0], rr.s[1], 0.0 + rr.s[2], rr.s[3]);
-            __syncthreads();
+            epi(r, v, a);
         }
     }
+    __syncthreads();
 }
 
-template <int NT = kThreads, int NTR = NT, int STAGE = kChunk>
-__device__ void spmv(const int *ptr, const int *idx, const double *val, const int *blk, int nblk, const double *x,
-                     double *out, double *stage, double *red) {
-    stream_rows<false, NT, NTR, STAGE>(ptr, idx, val, blk, nblk, x, stage, red, [](int) { return 0.0; },
-                       [&](int r, double v, double) { out[r] = v; });
-}
 template <int NT = kThreads>
 __device__ void copy_vec(double *dst, const double *src, int n) {
     GLOBAL_AS double *d = gout(dst);
@@ -631,56 +602,33 @@ __device__ Acc assess(const LpDev &L, const double *ox, const double *oy, const 
 }
 
 
-// NT threads copy and multiply; the first NTR form every sum.  stage holds
-// STAGE doubles (kChunk, or kWideChunk with the wide row blocks).
-template <int NT, int NTR, int STAGE = kChunk>
-__device__ Acc evaluate(const LpDev &L, const double *x, const double *y, double *ox, double *oy, double *lds,
-                        double *stage) {
-    constexpr bool wide = STAGE > kChunk;
-    restore_fill<NT>(L, ox, oy);
-    __syncthreads();
-    restore_scatter<NT>(L, x, y, ox, oy);
-    __syncthreads();
-    spmv<NT, NTR, STAGE>(L.op, L.oi, L.ov, wide ? L.oblkw : L.oblk, wide ? L.nobw : L.nob, ox, L.act, stage, lds);
-    stream_rows<true, NT, NTR, STAGE>(L.otp, L.oti, L.otv, wide ? L.otblkw : L.otblk, wide ? L.notbw : L.notb, oy,
-                                      stage, lds, [](int) { return 0.0; },
-                      [&](int j, double v, double a) {
-                          gout(L.aty)[j] = v;
-                          gout(L.atyabs)[j] = a;
-                      });
-    __syncthreads();
-    return assess<NTR>(L, ox, oy, L.act, L.aty, L.atyabs, lds);
-}
-
-// Both candidates of a check (current and average) at once: one pass over
-// the original A and one over its transpose read each entry once for the two
-// products, which are formed and summed exactly as evaluate forms them (same
-// products, same storage-order row sums, same tree for rows longer than
-// kChunk), so each Acc equals evaluate's bit for bit.  stage holds
-// 2 * kChunk doubles; the candidates' products go to (act, aty, atyabs) and
-// (act1, aty1, atyabs1).
+// Original-space accuracy of both candidates of a check (current and
+// average): restore each into (ox, oy), then one pass over the original A and
+// one over its transpose read each entry once for the two candidates'
+// products, which go to (act, aty, atyabs) and (act1, aty1, atyabs1).  Each
+// candidate's sums are formed as they would be alone.  NT threads copy and
+// multiply; the first NTR form every sum across rows and columns.
 template <int NT, int NTR>
 __device__ void evaluate_pair(const LpDev &L, const double *x0, const double *y0, const double *x1, const double *y1,
-                              double *lds, double *stage, Acc &a0, Acc &a1) {
+                              double *lds, Acc &a0, Acc &a1) {
     restore_fill<NT>(L, L.ox[0], L.oy[0]);
     restore_fill<NT>(L, L.ox[1], L.oy[1]);
     __syncthreads();
     restore_scatter<NT>(L, x0, y0, L.ox[0], L.oy[0]);
     restore_scatter<NT>(L, x1, y1, L.ox[1], L.oy[1]);
     __syncthreads();
-    stream_rows2<false, NT, NTR>(L.op, L.oi, L.ov, L.oblk, L.nob, L.ox[0], L.ox[1], stage, lds,
-                                 [&](int i, double v0, double, double v1, double) {
-                                     gout(L.act)[i] = v0;
-                                     gout(L.act1)[i] = v1;
-                                 });
-    stream_rows2<true, NT, NTR>(L.otp, L.oti, L.otv, L.otblk, L.notb, L.oy[0], L.oy[1], stage, lds,
-                                [&](int j, double v0, double b0, double v1, double b1) {
-                                    gout(L.aty)[j] = v0;
-                                    gout(L.atyabs)[j] = b0;
-                                    gout(L.aty1)[j] = v1;
-                                    gout(L.atyabs1)[j] = b1;
-                                });
-    __syncthreads();
+    rows_direct<false, 2, NT, NTR>(L.op, L.oi, L.ov, L.m0, L.olong, L.nolong, L.ox[0], L.ox[1], lds,
+                                   [&](int i, const double *v, const double *) {
+                                       gout(L.act)[i] = v[0];
+                                       gout(L.act1)[i] = v[1];
+                                   });
+    rows_direct<true, 2, NT, NTR>(L.otp, L.oti, L.otv, L.n0, L.otlong, L.notlong, L.oy[0], L.oy[1], lds,
+                                  [&](int j, const double *v, const double *a) {
+                                      gout(L.aty)[j] = v[0];
+                                      gout(L.atyabs)[j] = a[0];
+                                      gout(L.aty1)[j] = v[1];
+                                      gout(L.atyabs1)[j] = a[1];
+                                  });
     a0 = assess<NTR>(L, L.ox[0], L.oy[0], L.act, L.aty, L.atyabs, lds);
     a1 = assess<NTR>(L, L.ox[1], L.oy[1], L.act1, L.aty1, L.atyabs1, lds);
 }
@@ -971,22 +919,9 @@ constexpr int kMaxTiles = 128;  // 64-row tiles: m, n <= 8000 in the LDS classes
 
 // v rotated within each row of 16 lanes by the DPP control CTRL (row_ror);
 // both 32-bit halves move alike.
-template <int CTRL>
-__device__ __forceinline__ double dpp_rotate(double v) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, int(unsigned(b)), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, int(unsigned(b >> 32)), CTRL, 0xf, 0xf, false);
-    return __builtin_bit_cast(double, (unsigned long long)unsigned(hi) << 32 | unsigned(lo));
-}
-
 // Sum of v over the 64 lanes of a wave, valid in every lane: the butterfly
-// v += v[lane ^ off] for off = 32, 16, 8, 4, 2, 1.  The two cross-row levels
-// are lane shuffles; the four within a row of 16 lanes are DPP row rotations
-// on the VALU (no LDS round trip): rotating by 8 pairs lane i with i ^ 8, and
-// once the values are symmetric under ^ 8 (after that level), rotating by 4
-// brings lane i the value of lane i ^ 4, and so on for 2 and 1, so every
-// lane adds exactly the partner value the butterfly adds and the sum is the
-// butterfly's, bit for bit.
+// of block_reduce (lane shuffles across rows, DPP row rotations within
+// them), so the sum is the butterfly's bit for bit.
 __device__ __forceinline__ double wave_sum(double v) {
     v += __shfl_xor(v, 32, 64);
     v += __shfl_xor(v, 16, 64);
@@ -1221,29 +1156,27 @@ __global__ void gather_pieces_kernel(const CopyPiece *pieces, double *out) {
 // tracking, parking for host certification / Farkas checks, restart rule.
 // For an LP whose phase is kCheck or kResume; S is the workgroup's shared
 // copy of its state.  NT threads copy and multiply, the first kThreads form
-// every sum, so the check decides the same in any workgroup size.  lds holds
-// kWaves * 24 doubles, stage STAGE (with the matching row blocks, which sum
-// every row as the kChunk blocks do).
-// PAIR: evaluate both candidates with evaluate_pair (stage of 2 * kChunk
-// doubles); the Accs, and so every decision, are the same either way.
-template <int NT, int STAGE = kChunk, bool PAIR = false>
-__device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, double *lds, double *stage) {
+// every sum across rows and columns, so the check decides the same in any
+// workgroup size.  lds holds kWaves * 24 doubles.
+
+// ax = A x of the current iterate in the row order of the scaled A.
+template <int NT, int NTR>
+__device__ void rebase_activity(const LpDev &L, int cur, double *lds) {
+    rows_direct<false, 1, NT, NTR>(L.ap, L.ai, L.av, L.m, L.along, L.nalong, L.x[cur], nullptr, lds,
+                                   [&](int r, const double *v, const double *) { gout(L.ax[cur])[r] = v[0]; });
+}
+
+template <int NT>
+__device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, double *lds) {
     constexpr int NTR = kThreads;
-    constexpr bool wide = STAGE > kChunk;
     const int cur = S.cur;
     double cand_kkt;
     int avg;
     if (S.phase == kCheck) {
         // candidates(): rebase the activity recurrence (src/cpu.cpp:229-230)
-        spmv<NT, NTR, STAGE>(L.ap, L.ai, L.av, wide ? L.ablkw : L.ablk, wide ? L.nabw : L.nab, L.x[cur], L.ax[cur], stage, lds);
+        rebase_activity<NT, NTR>(L, cur, lds);
         Acc cur_acc, avg_acc;
-        if constexpr (PAIR) {
-            static_assert(!wide, "the paired check uses kChunk row blocks");
-            evaluate_pair<NT, NTR>(L, L.x[cur], L.y[cur], L.xa[cur], L.ya[cur], lds, stage, cur_acc, avg_acc);
-        } else {
-            cur_acc = evaluate<NT, NTR, STAGE>(L, L.x[cur], L.y[cur], L.ox[0], L.oy[0], lds, stage);
-            avg_acc = evaluate<NT, NTR, STAGE>(L, L.xa[cur], L.ya[cur], L.ox[1], L.oy[1], lds, stage);
-        }
+        evaluate_pair<NT, NTR>(L, L.x[cur], L.y[cur], L.xa[cur], L.ya[cur], lds, cur_acc, avg_acc);
         avg = avg_acc.kkt < cur_acc.kkt ? 1 : 0;
         const Acc ca = avg ? avg_acc : cur_acc;
         const bool better = ca.finite && ca.kkt < S.best_kkt;
@@ -1288,14 +1221,11 @@ __device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, dou
                     gout(L.oy[0])[t] = 0.0 + d * gin(L.dr)[i] * L.sobj;
                 }
                 __syncthreads();
-                stream_rows<true, NT, NTR, STAGE>(L.otp, L.oti, L.otv, wide ? L.otblkw : L.otblk,
-                                                  wide ? L.notbw : L.notb, L.oy[0], stage, lds,
-                                           [](int) { return 0.0; },
-                                  [&](int j, double v, double a) {
-                                      gout(L.aty)[j] = v;
-                                      gout(L.atyabs)[j] = a;
-                                  });
-                __syncthreads();
+                rows_direct<true, 1, NT, NTR>(L.otp, L.oti, L.otv, L.n0, L.otlong, L.notlong, L.oy[0], nullptr,
+                                              lds, [&](int j, const double *v, const double *a) {
+                                                  gout(L.aty)[j] = v[0];
+                                                  gout(L.atyabs)[j] = a[0];
+                                              });
                 Reducer<4, 2, 1> r{};
                 for (int i = first_index<NTR>(0, L.m0); i < L.m0; i += NTR) {
                     const double yi = L.oy[0][i];
@@ -1363,7 +1293,7 @@ __device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, dou
             copy_vec<NT>(L.y[cur], L.ya[cur], L.m);
             __syncthreads();
         }
-        spmv<NT, NTR, STAGE>(L.ap, L.ai, L.av, wide ? L.ablkw : L.ablk, wide ? L.nabw : L.nab, L.x[cur], L.ax[cur], stage, lds);
+        rebase_activity<NT, NTR>(L, cur, lds);
         copy_vec<NT>(L.epx, L.x[cur], L.n);
         copy_vec<NT>(L.epy, L.y[cur], L.m);
         zero_vec<NT>(L.xa[cur], L.n);
@@ -1392,10 +1322,10 @@ __device__ void check_period(const LpDev &L, LpState &S, const BatchOpts &o, dou
 // attempts.  The pointers carry their address spaces (LDS, global) into the
 // call, so its LDS and global accesses stay direct.
 #define LDS_AS __attribute__((address_space(3)))
-template <int NT, int STAGE>
+template <int NT>
 __device__ __noinline__ void check_period_call(const GLOBAL_AS LpDev *L, LDS_AS LpState *S, BatchOpts o,
-                                               LDS_AS double *lds, LDS_AS double *stage) {
-    check_period<NT, STAGE>(*(const LpDev *)L, *(LpState *)S, o, (double *)lds, (double *)stage);
+                                               LDS_AS double *lds) {
+    check_period<NT>(*(const LpDev *)L, *(LpState *)S, o, (double *)lds);
 }
 
 // Held to 4 waves per SIMD (128 VGPRs, some spills): at its natural 248 VGPRs a
@@ -1406,11 +1336,10 @@ __global__ __launch_bounds__(kThreads, 4) void batch_check_kernel(const LpDev *l
     const int s = slots[blockIdx.x];
     __shared__ LpState S;
     __shared__ double lds[kWaves * 24];
-    __shared__ double stage[2 * kChunk];
     if (threadIdx.x == 0) S = states[s];
     __syncthreads();
     if (S.status != kRunning || S.phase == kAdvance) return;
-    check_period<kThreads, kChunk, true>(lps[s], S, o, lds, stage);
+    check_period<kThreads>(lps[s], S, o, lds);
     if (threadIdx.x == 0) states[s] = S;
 }
 
@@ -1439,7 +1368,6 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
     __shared__ double tiles[3 * kMaxTiles];
     __shared__ double powlds[2];
     __shared__ double lds[kWaves * 24];
-    __shared__ double stage[kWideChunk];
     __shared__ int flag, stop, taken;
     for (;;) {
         __syncthreads();  // the previous LP's readers of S and `taken` are done
@@ -1492,8 +1420,8 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
         bool ystaged = false;
         for (int period = 0;;) {
             if (S.phase != kAdvance) {
-                check_period_call<NT, kWideChunk>((const GLOBAL_AS LpDev *)(lps + s), (LDS_AS LpState *)&S, o,
-                                                  (LDS_AS double *)lds, (LDS_AS double *)stage);
+                check_period_call<NT>((const GLOBAL_AS LpDev *)(lps + s), (LDS_AS LpState *)&S, o,
+                                      (LDS_AS double *)lds);
                 ystaged = false;  // a restart may have replaced y
                 if (S.status != kRunning) break;
             }
@@ -1631,10 +1559,11 @@ std::vector<hipStream_t> &stream_pool() {
     return pool;
 }
 
-// Splits CSR rows into consecutive blocks of at most `limit` nonzeros whose
-// rows are each at most kChunk long; a row longer than kChunk is a block of
-// its own.  Returns block start rows plus the final row count.
-std::vector<int64_t> row_blocks(const std::vector<int64_t> &ptr, int64_t limit = kChunk) {
+// Splits CSR rows into consecutive blocks of at most kChunk nonzeros; a row
+// longer than kChunk is a block of its own.  Returns block start rows plus
+// the final row count.
+std::vector<int64_t> row_blocks(const std::vector<int64_t> &ptr) {
+    constexpr int64_t limit = kChunk;
     const int64_t rows = int64_t(ptr.size()) - 1;
     std::vector<int64_t> b{0};
     for (int64_t r = 0; r < rows;) {
@@ -1646,6 +1575,15 @@ std::vector<int64_t> row_blocks(const std::vector<int64_t> &ptr, int64_t limit =
         b.push_back(r);
     }
     return b;
+}
+
+// The rows of a CSR matrix longer than kChunk, in order (the checks sum them
+// with the workgroup's tree).
+std::vector<int64_t> long_rows(const std::vector<int64_t> &ptr) {
+    std::vector<int64_t> r;
+    for (size_t i = 0; i + 1 < ptr.size(); ++i)
+        if (ptr[i + 1] - ptr[i] > kChunk) r.push_back(int64_t(i));
+    return r;
 }
 
 // Offsets of the transpose of a CSR matrix (column counts, prefix-summed).
@@ -2662,13 +2600,14 @@ class BatchEngine {
             batch_advance_lds_kernel<NT, NR, double><<<grid, NT, lds_bytes(cap), cs>>>(d_desc_, d_state_, slots, bo);
     }
     // Algorithmic bytes of one check period's check: the scaled SpMV that
-    // rebases A x, then two original-space evaluations (restore, A x and A^T y
-    // of the original model, and the row and column passes of the verifier).
+    // rebases A x, then both candidates' original-space evaluations (restore,
+    // one pass over the original A and one over its transpose for the two
+    // candidates' products, and the row and column passes of the verifier).
     static double check_bytes(const Job &j) {
         const nyk::Model &s = j.prep.model, &o = j.original;
         const double n = double(s.cols()), m = double(s.rows()), nnz = double(s.A.nnz());
         const double n0 = double(o.cols()), m0 = double(o.rows()), nnz0 = double(o.A.nnz());
-        return nnz * 12 + m * 12 + n * 8 + 2 * (24 * nnz0 + 68 * m0 + 100 * n0 + 28 * n + 32 * m);
+        return nnz * 12 + m * 12 + n * 8 + 24 * nnz0 + 2 * (68 * m0 + 100 * n0 + 28 * n + 32 * m);
     }
 
     // The tail group: one launch per scheduling round, one row per thread
@@ -2786,12 +2725,10 @@ class BatchEngine {
         size_t o_act = L.add<double>(mr0), o_aty = L.add<double>(n0), o_atyabs = L.add<double>(n0);
         size_t o_act1 = L.add<double>(mr0), o_aty1 = L.add<double>(n0), o_atyabs1 = L.add<double>(n0);
         size_t o_bx = L.add<double>(n0), o_by = L.add<double>(mr0);
-        const auto ba = row_blocks(m.A.ptr), bt = row_blocks(t_ptr(m)), bo = row_blocks(m0.A.ptr), bot = row_blocks(t0.ptr);
-        size_t o_ab = L.add<int>(ba.size()), o_tb = L.add<int>(bt.size()), o_ob = L.add<int>(bo.size()),
-               o_otb = L.add<int>(bot.size());
-        const auto baw = row_blocks(m.A.ptr, kWideChunk), bow = row_blocks(m0.A.ptr, kWideChunk),
-                   botw = row_blocks(t0.ptr, kWideChunk);
-        size_t o_abw = L.add<int>(baw.size()), o_obw = L.add<int>(bow.size()), o_otbw = L.add<int>(botw.size());
+        const auto ba = row_blocks(m.A.ptr), bt = row_blocks(t_ptr(m));
+        size_t o_ab = L.add<int>(ba.size()), o_tb = L.add<int>(bt.size());
+        const auto la = long_rows(m.A.ptr), lo = long_rows(m0.A.ptr), lot = long_rows(t0.ptr);
+        size_t o_la = L.add<int>(la.size()), o_lo = L.add<int>(lo.size()), o_lot = L.add<int>(lot.size());
         // the sliced copies serve only the LDS classes (the global class walks the CSR)
         Sliced sa, st;
         nyk::Csr t = m.A.transposed();
@@ -2914,21 +2851,17 @@ class BatchEngine {
         d.by = (double *)P(o_by);
         put_i(o_ab, ba);
         put_i(o_tb, bt);
-        put_i(o_ob, bo);
-        put_i(o_otb, bot);
         d.ablk = (const int *)P(o_ab);
         d.tblk = (const int *)P(o_tb);
-        d.oblk = (const int *)P(o_ob);
-        d.otblk = (const int *)P(o_otb);
-        put_i(o_abw, baw);
-        put_i(o_obw, bow);
-        put_i(o_otbw, botw);
-        d.ablkw = (const int *)P(o_abw);
-        d.oblkw = (const int *)P(o_obw);
-        d.otblkw = (const int *)P(o_otbw);
-        d.nabw = int(baw.size()) - 1;
-        d.nobw = int(bow.size()) - 1;
-        d.notbw = int(botw.size()) - 1;
+        put_i(o_la, la);
+        put_i(o_lo, lo);
+        put_i(o_lot, lot);
+        d.along = (const int *)P(o_la);
+        d.olong = (const int *)P(o_lo);
+        d.otlong = (const int *)P(o_lot);
+        d.nalong = int(la.size());
+        d.nolong = int(lo.size());
+        d.notlong = int(lot.size());
         auto put_u16 = [&](size_t off, const std::vector<int64_t> &v) {
             unsigned short *p = reinterpret_cast<unsigned short *>(image.data() + off);
             for (size_t k = 0; k < v.size(); ++k) p[k] = (unsigned short)v[k];
@@ -2969,8 +2902,6 @@ class BatchEngine {
         d.tsv = (const double *)P(o_tsv);
         d.nab = int(ba.size()) - 1;
         d.ntb = int(bt.size()) - 1;
-        d.nob = int(bo.size()) - 1;
-        d.notb = int(bot.size()) - 1;
         return L.total;
     }
 
