@@ -380,6 +380,9 @@ __device__ void stream_rows(const int *ptr_, const int *idx_, const double *val_
     }
 }
 
+// Entries of a row that rows_direct loads at once.
+constexpr int kRowGroup = 8;
+
 // CSR row sums for the checks, one thread per row: each row of up to kChunk
 // nonzeros is summed by one thread straight from memory, left to right from
 // 0 over products rounded before they are added (the order of the
@@ -395,25 +398,74 @@ __device__ void rows_direct(const int *ptr_, const int *idx_, const double *val_
     const GLOBAL_AS int *ptr = gin(ptr_), *idx = gin(idx_);
     const GLOBAL_AS double *val = gin(val_);
     const GLOBAL_AS double *vec[2] = {gin(vec0_), gin(vec1_)};
-    for (int r = threadIdx.x; r < nrows; r += NT) {
-        const int k0 = ptr[r], k1 = ptr[r + 1];
-        if (k1 - k0 > kChunk) continue;
-        double acc[NV], ab[NV];
+    if constexpr (NT >= 1024) {
+        // one or two rows per thread (a lone LP in the tail launch): the
+        // entries' loads of a few consecutive positions issued together
+        for (int r = threadIdx.x; r < nrows; r += NT) {
+            const int k0 = ptr[r], k1 = ptr[r + 1];
+            if (k1 - k0 > kChunk) continue;
+            double acc[NV], ab[NV];
 #pragma unroll
-        for (int q = 0; q < NV; ++q) acc[q] = ab[q] = 0;
+            for (int q = 0; q < NV; ++q) acc[q] = ab[q] = 0;
 #pragma unroll 4
-        for (int k = k0; k < k1; ++k) {
-            const double v = val[k];
-            const int c = idx[k];
+            for (int k = k0; k < k1; ++k) {
+                const double v = val[k];
+                const int c = idx[k];
 #pragma unroll
-            for (int q = 0; q < NV; ++q) {
+                for (int q = 0; q < NV; ++q) {
 #pragma clang fp contract(off)
-                const double t = v * vec[q][c];
-                acc[q] += t;
-                if (ABS) ab[q] += fabs(t);
+                    const double t = v * vec[q][c];
+                    acc[q] += t;
+                    if (ABS) ab[q] += fabs(t);
+                }
             }
+            epi(r, acc, ab);
         }
-        epi(r, acc, ab);
+    } else {
+        // many rows per thread (a crowded check): a thread's rows one after
+        // another, the next row's offsets loaded while the current row is
+        // summed; a row's entries in groups of kRowGroup whose loads are all
+        // issued before their products are added (the group's positions past
+        // the row end load the row's last entry and add nothing)
+        int r = threadIdx.x;
+        int k0 = r < nrows ? ptr[r] : 0, k1 = r < nrows ? ptr[r + 1] : 0;
+        for (; r < nrows; r += NT) {
+            const int rn = r + NT;
+            const int k0n = rn < nrows ? ptr[rn] : 0, k1n = rn < nrows ? ptr[rn + 1] : 0;
+            if (k1 - k0 <= kChunk) {
+                double acc[NV], ab[NV];
+#pragma unroll
+                for (int q = 0; q < NV; ++q) acc[q] = ab[q] = 0;
+                for (int k = k0; k < k1; k += kRowGroup) {
+                    double v[kRowGroup], g[NV][kRowGroup];
+                    int c[kRowGroup];
+#pragma unroll
+                    for (int e = 0; e < kRowGroup; ++e) {
+                        const int kk = k + e < k1 ? k + e : k1 - 1;
+                        v[e] = val[kk];
+                        c[e] = idx[kk];
+                    }
+#pragma unroll
+                    for (int e = 0; e < kRowGroup; ++e)
+#pragma unroll
+                        for (int q = 0; q < NV; ++q) g[q][e] = vec[q][c[e]];
+#pragma unroll
+                    for (int e = 0; e < kRowGroup; ++e)
+                        if (k + e < k1) {
+#pragma unroll
+                            for (int q = 0; q < NV; ++q) {
+#pragma clang fp contract(off)
+                                const double t = v[e] * g[q][e];
+                                acc[q] += t;
+                                if (ABS) ab[q] += fabs(t);
+                            }
+                        }
+                }
+                epi(r, acc, ab);
+            }
+            k0 = k0n;
+            k1 = k1n;
+        }
     }
     for (int q = 0; q < nlong; ++q) {
         const int r = gin(longs_)[q], k0 = ptr[r], k1 = ptr[r + 1];
@@ -495,36 +547,94 @@ __device__ void farkas_columns(const LpDev &L, Reducer<NS, NM, ND> &r) {
     }
 }
 
-// restore_x / restore_y (src/preprocess.cpp:121-132) of one candidate into
-// (ox, oy); the caller's barrier orders the writes before their readers.
+// restore_x / restore_y (src/preprocess.cpp:121-132) of both candidates of a
+// check, (x0, y0) into (ox[0], oy[0]) and (x1, y1) into (ox[1], oy[1]): the
+// fixed columns and zero rows first, then the scaled values scattered to
+// their original positions.  Each thread loads the inputs of kRestoreBatch
+// positions before it stores any (a store could alias a later load, so the
+// compiler keeps them in program order).  Ends with a barrier.
+constexpr int kRestoreBatch = 4;
 template <int NT>
-__device__ void restore_fill(const LpDev &L, double *ox, double *oy) {
-    copy_vec<NT>(ox, L.fixed, L.n0);
-    zero_vec<NT>(oy, L.m0);
-}
-template <int NT>
-__device__ void restore_scatter(const LpDev &L, const double *x, const double *y, double *ox, double *oy) {
-    for (int j = threadIdx.x; j < L.n; j += NT) ox[gin(L.cols)[j]] = x[j] * gin(L.dc)[j];
-    for (int i = threadIdx.x; i < L.m; i += NT) {
-        const int t = y[i] >= 0 ? gin(L.upper_rows)[i] : gin(L.lower_rows)[i];
-        oy[t] = 0.0 + y[i] * gin(L.dr)[i] * L.sobj;
+__device__ void restore_pair(const LpDev &L, const double *x0_, const double *y0_, const double *x1_,
+                             const double *y1_) {
+    const GLOBAL_AS double *x0 = gin(x0_), *y0 = gin(y0_), *x1 = gin(x1_), *y1 = gin(y1_);
+    GLOBAL_AS double *ox0 = gout(L.ox[0]), *ox1 = gout(L.ox[1]), *oy0 = gout(L.oy[0]), *oy1 = gout(L.oy[1]);
+    for (int j = threadIdx.x; j < L.n0; j += NT) {
+        const double f = gin(L.fixed)[j];
+        ox0[j] = f;
+        ox1[j] = f;
     }
+    for (int i = threadIdx.x; i < L.m0; i += NT) oy0[i] = oy1[i] = 0;
+    __syncthreads();
+    for (int j0 = threadIdx.x; j0 < L.n; j0 += kRestoreBatch * NT) {
+        int col[kRestoreBatch];
+        double a[kRestoreBatch], b[kRestoreBatch];
+#pragma unroll
+        for (int e = 0; e < kRestoreBatch; ++e) {
+            const int j = j0 + e * NT;
+            if (j < L.n) {
+                const double d = gin(L.dc)[j];
+                col[e] = gin(L.cols)[j];
+                a[e] = x0[j] * d;
+                b[e] = x1[j] * d;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < kRestoreBatch; ++e)
+            if (j0 + e * NT < L.n) {
+                ox0[col[e]] = a[e];
+                ox1[col[e]] = b[e];
+            }
+    }
+    for (int i0 = threadIdx.x; i0 < L.m; i0 += kRestoreBatch * NT) {
+        int t0[kRestoreBatch], t1[kRestoreBatch];
+        double a[kRestoreBatch], b[kRestoreBatch];
+#pragma unroll
+        for (int e = 0; e < kRestoreBatch; ++e) {
+            const int i = i0 + e * NT;
+            if (i < L.m) {
+                const double r = gin(L.dr)[i], u = y0[i], v = y1[i];
+                const int up = gin(L.upper_rows)[i], lo = gin(L.lower_rows)[i];
+                t0[e] = u >= 0 ? up : lo;
+                t1[e] = v >= 0 ? up : lo;
+                a[e] = 0.0 + u * r * L.sobj;
+                b[e] = 0.0 + v * r * L.sobj;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < kRestoreBatch; ++e)
+            if (i0 + e * NT < L.m) {
+                oy0[t0[e]] = a[e];
+                oy1[t1[e]] = b[e];
+            }
+    }
+    __syncthreads();
 }
 
 // Original-space accuracy of a restored candidate (ox, oy) whose products
 // are in act = A ox, aty = A^T oy and atyabs = |A^T| |oy| (summed per entry).
 // The first NTR threads form every sum.
 template <int NTR>
-__device__ Acc assess(const LpDev &L, const double *ox, const double *oy, const double *act_, const double *aty_,
+__device__ Acc assess(const LpDev &L, const double *ox_, const double *oy_, const double *act_, const double *aty_,
                       const double *atyabs_, double *lds) {
     const GLOBAL_AS double *act = gin(act_), *aty = gin(aty_), *atyabs = gin(atyabs_);
+    const GLOBAL_AS double *ox = gin(ox_), *oy = gin(oy_);
     // s : objective, Farkas column bound sum, its |.| sum, Lipschitz slack, sum |y b|
     // mx: nonfinite, Farkas column -inf, primal, primal_abs, dual, dual_abs,
     //     y != 0 at an infinite row endpoint, infinite Lagrangian box argument
     // d : support, complement, box_min, sum y b over rows
     Reducer<5, 8, 4> r{};
-    for (int i = first_index<NTR>(0, L.m0); i < L.m0; i += NTR) {
-        const double a = act[i], yi = oy[i], lo = gin(L.rl0)[i], hi = gin(L.ru0)[i];
+    // Each loop loads the next position's inputs before it works on the
+    // current one, so a thread always has a position's loads in flight.
+    int i = first_index<NTR>(0, L.m0);
+    double na = 0, ny = 0, nlo = 0, nhi = 0;
+    if (i < L.m0) na = act[i], ny = oy[i], nlo = gin(L.rl0)[i], nhi = gin(L.ru0)[i];
+    for (; i < L.m0; i += NTR) {
+        const double a = na, yi = ny, lo = nlo, hi = nhi;
+        if (i + NTR < L.m0) {
+            const int k = i + NTR;
+            na = act[k], ny = oy[k], nlo = gin(L.rl0)[k], nhi = gin(L.ru0)[k];
+        }
         if (!isfinite(a) || !isfinite(yi)) r.mx[0] = 1;
         const double v = maxd(maxd(0., lo - a), a - hi);
         if (isfinite(lo)) r.mx[2] = maxd(r.mx[2], maxd(0., lo - a) / (1 + fabs(lo)));
@@ -544,14 +654,24 @@ __device__ Acc assess(const LpDev &L, const double *ox, const double *oy, const 
             }
         }
     }
-    for (int j = first_index<NTR>(0, L.n0); j < L.n0; j += NTR) {
-        const double xj = ox[j], l = gin(L.lb0)[j], u = gin(L.ub0)[j], c = gin(L.c0)[j];
+    int j = first_index<NTR>(0, L.n0);
+    double nx = 0, nl = 0, nu = 0, nc = 0, ng = 0, nga = 0;
+    int nlen = 0;
+    auto load_column = [&](int k) {
+        nx = ox[k], nl = gin(L.lb0)[k], nu = gin(L.ub0)[k], nc = gin(L.c0)[k];
+        ng = aty[k], nga = atyabs[k], nlen = gin(L.otp)[k + 1] - gin(L.otp)[k];
+    };
+    if (j < L.n0) load_column(j);
+    for (; j < L.n0; j += NTR) {
+        const double xj = nx, l = nl, u = nu, c = nc, g = ng, ga = nga;
+        const int len = nlen;
+        if (j + NTR < L.n0) load_column(j + NTR);
         if (!isfinite(xj)) r.mx[0] = 1;
         const double v = maxd(maxd(0., l - xj), xj - u);
         r.mx[2] = maxd(r.mx[2], v / (1 + fabs(xj)));
         r.mx[3] = maxd(r.mx[3], v);
         r.s[0] += c * xj;
-        const double linear = c + aty[j];
+        const double linear = c + g;
         if (!isfinite(linear)) r.mx[0] = 1;
         const double d = clampd(linear, xj - u, xj - l);
         r.mx[5] = maxd(r.mx[5], fabs(d));
@@ -562,8 +682,6 @@ __device__ Acc assess(const LpDev &L, const double *ox, const double *oy, const 
         if (!isfinite(arg)) r.mx[7] = 1;  // no finite Lagrangian box bound
         else dd_add(r.d[2], linear * arg);
         // Farkas column bound for the candidate's dual as a ray
-        const double g = aty[j], ga = atyabs[j];
-        const int len = gin(L.otp)[j + 1] - gin(L.otp)[j];
         const double e = double(len + 4) * 2 * kEps * ga;
         double ubd = maxd(phi(g - e, l, u), phi(g + e, l, u));
         if (g - e <= 0 && g + e >= 0) ubd = maxd(ubd, 0.0);
@@ -611,12 +729,7 @@ __device__ Acc assess(const LpDev &L, const double *ox, const double *oy, const 
 template <int NT, int NTR>
 __device__ void evaluate_pair(const LpDev &L, const double *x0, const double *y0, const double *x1, const double *y1,
                               double *lds, Acc &a0, Acc &a1) {
-    restore_fill<NT>(L, L.ox[0], L.oy[0]);
-    restore_fill<NT>(L, L.ox[1], L.oy[1]);
-    __syncthreads();
-    restore_scatter<NT>(L, x0, y0, L.ox[0], L.oy[0]);
-    restore_scatter<NT>(L, x1, y1, L.ox[1], L.oy[1]);
-    __syncthreads();
+    restore_pair<NT>(L, x0, y0, x1, y1);
     rows_direct<false, 2, NT, NTR>(L.op, L.oi, L.ov, L.m0, L.olong, L.nolong, L.ox[0], L.ox[1], lds,
                                    [&](int i, const double *v, const double *) {
                                        gout(L.act)[i] = v[0];
