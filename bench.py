@@ -625,6 +625,7 @@ def main():
                 ("batch_tail_kernel<1024>", total("class_bytes", 4), total("class_seconds", 4),
                  total("class_launches", 4))]
     symbol, adv_b, adv_s, launches = max(kernels, key=lambda k: k[1])
+    device_b = sum(total("class_bytes", k) for k in range(len(stats_all[0]["class_bytes"])))
     achieved = adv_b / adv_s / 1e9 if adv_s > 0 else 0.0
     users = [name for c, name in enumerate(("m+n <= 2048", "m+n <= 4096", "m+n <= 8000"))
              if (last_stats["class_threads"][c], last_stats["class_rows"][c]) in
@@ -677,7 +678,12 @@ def main():
                      "kernel": kernel_name,
                      "algorithmic_bytes_per_launch": round(adv_b / max(1, launches), 1),
                      "avg_launch_ms": round(1e3 * adv_s / max(1, launches), 3),
-                     "launches": launches},
+                     "launches": launches,
+                     # the whole device: every step launch's algorithmic bytes over the timed
+                     # region (the classes' launches run side by side and share HBM)
+                     "device": {"achieved": round(device_b / total_t / 1e9, 1),
+                                "frac": round(device_b / total_t / 1e9 / HBM_PEAK_GBS, 4),
+                                "algorithmic_bytes": device_b, "seconds": round(total_t, 3)}},
     }
     if not args.no_stream and not args.no_solve_phase:
         # one pass alone on a fresh engine, first submit to last result

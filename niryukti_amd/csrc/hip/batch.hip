@@ -67,8 +67,9 @@ constexpr int kClassDoubles[kClasses - 1] = {2048, 4096, 8000};
 static_assert(kClassDoubles[kClasses - 2] <= 65536, "sliced entry indices are 16-bit");
 // Upper bound of the LDS step kernels' static allocation (state, tile sums,
 // flags; the fused tail kernel adds the check's reduction slots).
-constexpr int kStaticLds = 57344;
+constexpr int kStaticLds = 8192;
 constexpr int64_t kShortRow = 128;
+static_assert(kShortRow <= 255, "LDS-class row lengths are stored in bytes");
 // The two larger LDS classes (m + n above 2,048) run 1,024-thread workgroups
 // with one row per thread: their LDS allows one or two workgroups per CU, and
 // 16 waves per workgroup keep more loads in flight (measured: the largest
@@ -924,7 +925,7 @@ typedef unsigned short ushort4v __attribute__((ext_vector_type(4)));
 // (a last pair region for widths that are not a multiple of 4); products and
 // sums stay FP64.
 template <int NR, class VT = double>
-__device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const int *sb,
+__device__ __forceinline__ void rows_dot_sell(const int *rlen, const int *sb,
                                               const GLOBAL_AS unsigned short *si, const GLOBAL_AS VT *sv, int slices, int b0,
                                               int nt, const double *vec, double *acc) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -937,7 +938,7 @@ __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const 
         const int s = (b0 + q * nt) / 64 + wave;
         base[q] = 0;
         w[q] = 0;
-        len[q] = e[q] - k[q];
+        len[q] = rlen[q];
         if (s < slices) {
             const int b = sb[s];
             base[q] = b;
@@ -1030,8 +1031,6 @@ __device__ __forceinline__ void rows_dot_sell(const int *k, const int *e, const 
 // trajectory it follows alone.
 constexpr int kMaxTiles = 128;  // 64-row tiles: m, n <= 8000 in the LDS classes
 
-// v rotated within each row of 16 lanes by the DPP control CTRL (row_ror);
-// both 32-bit halves move alike.
 // Sum of v over the 64 lanes of a wave, valid in every lane: the butterfly
 // of block_reduce (lane shuffles across rows, DPP row rotations within
 // them), so the sum is the butterfly's bit for bit.
@@ -1054,13 +1053,6 @@ __device__ __forceinline__ double tile_total(const double *tiles, int count) {
     return wave_sum(acc);
 }
 
-// One PDHG attempt for an LP whose dual iterate and primal step fit in LDS:
-// y lives in LDS (staged from memory only when `ystaged` is false: at the
-// start, after a rejected attempt, after a check; an accepted attempt leaves
-// its new y there), dx never leaves LDS, the CSR offsets
-// of A and A^T sit in LDS for the whole launch (so a row's entries are one
-// memory round trip away, not two), and each thread carries NR rows at a
-// time (rows are short in this class).
 // The sliced values a step of value type VT reads.
 template <class VT>
 __device__ __forceinline__ const GLOBAL_AS VT *sliced_values(const LpDev &L, bool transpose) {
@@ -1068,10 +1060,42 @@ __device__ __forceinline__ const GLOBAL_AS VT *sliced_values(const LpDev &L, boo
     else return gin(transpose ? L.tsv32 : L.asv32);
 }
 
+// An LP's dynamic LDS in the step kernels: y (m doubles) and dx (n doubles),
+// the slice bases of A^T and A (ceil(n/64) + 1 and ceil(m/64) + 1 ints), and
+// the row lengths of A^T and A (n and m bytes: an LDS-class row holds at most
+// kShortRow entries).
+__device__ __forceinline__ void lds_layout(double *dyn, const LpDev &L, double *&ylds, double *&dxlds, int *&tsblds,
+                                           int *&asblds, unsigned char *&tlen, unsigned char *&alen) {
+    const int ns = (L.n + 63) / 64, ms = (L.m + 63) / 64;
+    ylds = dyn;
+    dxlds = dyn + L.m;
+    tsblds = reinterpret_cast<int *>(dyn + L.m + L.n);
+    asblds = tsblds + ns + 1;
+    tlen = reinterpret_cast<unsigned char *>(asblds + ms + 1);
+    alen = tlen + L.n;
+}
+template <int NT>
+__device__ void stage_layout(const LpDev &L, int *tsblds, int *asblds, unsigned char *tlen, unsigned char *alen) {
+    const int ns = (L.n + 63) / 64, ms = (L.m + 63) / 64;
+    const GLOBAL_AS int *tp = gin(L.tp), *ap = gin(L.ap), *tsb = gin(L.tsb), *asb = gin(L.asb);
+    for (int j = threadIdx.x; j < L.n; j += NT) tlen[j] = (unsigned char)(tp[j + 1] - tp[j]);
+    for (int i = threadIdx.x; i < L.m; i += NT) alen[i] = (unsigned char)(ap[i + 1] - ap[i]);
+    for (int j = threadIdx.x; j <= ns; j += NT) tsblds[j] = tsb[j];
+    for (int i = threadIdx.x; i <= ms; i += NT) asblds[i] = asb[i];
+}
+
+// One PDHG attempt for an LP whose dual iterate and primal step fit in LDS:
+// y lives in LDS (staged from memory only when `ystaged` is false: at the
+// start, after a rejected attempt, after a check; an accepted attempt leaves
+// its new y there), dx never leaves LDS, the row lengths and slice bases of
+// A and A^T sit in LDS for the whole launch (so a row's entries are one
+// memory round trip away, not two), and each thread carries NR rows at a
+// time (rows are short in this class).
 template <int NT, int NR, class VT = double>
 __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, double tau0, double sigma0,
-                            double *tiles, double *ylds, double *dxlds, const int *tplds, const int *aplds,
-                            const int *tsblds, const int *asblds, double *powlds, int *accept_flag,
+                            double *tiles, double *ylds, double *dxlds, const unsigned char *tlen,
+                            const unsigned char *alen, const int *tsblds, const int *asblds, double *powlds,
+                            int *accept_flag,
                             bool &ystaged) {
     const int cur = S.cur;
     const double f = S.factor, tau = tau0 * f, sigma = sigma0 * f, w = f / (S.mass + f);
@@ -1100,7 +1124,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
     // Passes are uniform over the workgroup (the tile sums need whole waves).
     for (int b0 = 0; b0 < L.n; b0 += NR * NT) {
         const int base = b0 + int(threadIdx.x);
-        int r[NR], k[NR], e[NR];
+        int r[NR], len[NR];
         bool ok[NR];
         double g[NR], xj_[NR], xa_[NR], lo_[NR], hi_[NR];
 #pragma unroll
@@ -1108,8 +1132,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             r[q] = base + q * NT;
             ok[q] = r[q] < L.n;
             const int j = ok[q] ? r[q] : 0;
-            k[q] = ok[q] ? tplds[j] : 0;
-            e[q] = ok[q] ? tplds[j + 1] : 0;
+            len[q] = ok[q] ? tlen[j] : 0;
             g[q] = ok[q] ? c[j] : 0.0;
             xj_[q] = x[j];
             xa_[q] = xa[j];
@@ -1117,7 +1140,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             lo_[q] = bj.x;
             hi_[q] = bj.y;
         }
-        rows_dot_sell<NR, VT>(k, e, tsblds, gin(L.tsi), sliced_values<VT>(L, true), (L.n + 63) / 64, b0, NT, ylds, g);
+        rows_dot_sell<NR, VT>(len, tsblds, gin(L.tsi), sliced_values<VT>(L, true), (L.n + 63) / 64, b0, NT, ylds, g);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             double d2 = 0;
@@ -1140,7 +1163,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
     // delta = A dx and the extrapolated dual step (src/cpu.cpp:136-150)
     for (int b0 = 0; b0 < L.m; b0 += NR * NT) {
         const int base = b0 + int(threadIdx.x);
-        int r[NR], k[NR], e[NR];
+        int r[NR], len[NR];
         bool ok[NR];
         double delta[NR], a_[NR], ya_[NR], lo_[NR], hi_[NR];
 #pragma unroll
@@ -1148,8 +1171,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             r[q] = base + q * NT;
             ok[q] = r[q] < L.m;
             const int i = ok[q] ? r[q] : 0;
-            k[q] = ok[q] ? aplds[i] : 0;
-            e[q] = ok[q] ? aplds[i + 1] : 0;
+            len[q] = ok[q] ? alen[i] : 0;
             delta[q] = 0.0;
             a_[q] = ax[i];
             ya_[q] = ya[i];
@@ -1157,7 +1179,7 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
             lo_[q] = bi.x;
             hi_[q] = bi.y;
         }
-        rows_dot_sell<NR, VT>(k, e, asblds, gin(L.asi), sliced_values<VT>(L, false), (L.m + 63) / 64, b0, NT, dxlds, delta);
+        rows_dot_sell<NR, VT>(len, asblds, gin(L.asi), sliced_values<VT>(L, false), (L.m + 63) / 64, b0, NT, dxlds, delta);
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             double sq = 0, cr = 0;
@@ -1221,19 +1243,15 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     }
     const long long count = remaining < o.check_every ? remaining : o.check_every;
     const double tau0 = S.step / S.weight, sigma0 = S.step * S.weight;
-    double *ylds = dyn, *dxlds = dyn + L.m;
-    int *tplds = reinterpret_cast<int *>(dyn + L.m + L.n), *aplds = tplds + L.n + 1;
-    const int ns = (L.n + 63) / 64, ms = (L.m + 63) / 64;
-    int *tsblds = aplds + L.m + 1, *asblds = tsblds + ns + 1;
-    const GLOBAL_AS int *tp = gin(L.tp), *ap = gin(L.ap), *tsb = gin(L.tsb), *asb = gin(L.asb);
-    for (int j = threadIdx.x; j <= L.n; j += NT) tplds[j] = tp[j];
-    for (int i = threadIdx.x; i <= L.m; i += NT) aplds[i] = ap[i];
-    for (int j = threadIdx.x; j <= ns; j += NT) tsblds[j] = tsb[j];
-    for (int i = threadIdx.x; i <= ms; i += NT) asblds[i] = asb[i];
+    double *ylds, *dxlds;
+    int *tsblds, *asblds;
+    unsigned char *tlen, *alen;
+    lds_layout(dyn, L, ylds, dxlds, tsblds, asblds, tlen, alen);
+    stage_layout<NT>(L, tsblds, asblds, tlen, alen);
     __syncthreads();
     bool ystaged = false;
     for (long long done = 0; done < count;) {
-        attempt_lds<NT, NR, VT>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds, powlds, &flag,
+        attempt_lds<NT, NR, VT>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tlen, alen, tsblds, asblds, powlds, &flag,
                                 ystaged);
         if (flag == 1) ++done;
         else if (flag < 0) {
@@ -1520,15 +1538,11 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
         L.rlu = uniform(L.rlu);
         L.n = __builtin_amdgcn_readfirstlane(L.n);
         L.m = __builtin_amdgcn_readfirstlane(L.m);
-        double *ylds = dyn, *dxlds = dyn + L.m;
-        int *tplds = reinterpret_cast<int *>(dyn + L.m + L.n), *aplds = tplds + L.n + 1;
-        const int ns = (L.n + 63) / 64, ms = (L.m + 63) / 64;
-        int *tsblds = aplds + L.m + 1, *asblds = tsblds + ns + 1;
-        const GLOBAL_AS int *tp = gin(L.tp), *ap = gin(L.ap), *tsb = gin(L.tsb), *asb = gin(L.asb);
-        for (int j = threadIdx.x; j <= L.n; j += NT) tplds[j] = tp[j];
-        for (int i = threadIdx.x; i <= L.m; i += NT) aplds[i] = ap[i];
-        for (int j = threadIdx.x; j <= ns; j += NT) tsblds[j] = tsb[j];
-        for (int i = threadIdx.x; i <= ms; i += NT) asblds[i] = asb[i];
+        double *ylds, *dxlds;
+        int *tsblds, *asblds;
+        unsigned char *tlen, *alen;
+        lds_layout(dyn, L, ylds, dxlds, tsblds, asblds, tlen, alen);
+        stage_layout<NT>(L, tsblds, asblds, tlen, alen);
         __syncthreads();
         bool ystaged = false;
         for (int period = 0;;) {
@@ -1553,10 +1567,10 @@ __global__ __launch_bounds__(NT) void batch_tail_kernel(const LpDev *lps, LpStat
             const bool fp32 = !S.fp64;
             for (long long done = 0; done < count_;) {
                 if (fp32)
-                    attempt_lds<NT, 1, float>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds, asblds,
+                    attempt_lds<NT, 1, float>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tlen, alen, tsblds, asblds,
                                               powlds, &flag, ystaged);
                 else
-                    attempt_lds<NT, 1, double>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tplds, aplds, tsblds,
+                    attempt_lds<NT, 1, double>(L, S, o, tau0, sigma0, tiles, ylds, dxlds, tlen, alen, tsblds,
                                                asblds, powlds, &flag, ystaged);
                 if (flag == 1) ++done;
                 else if (flag < 0) {
@@ -2746,10 +2760,10 @@ class BatchEngine {
         }
     }
 
-    // Dynamic LDS of an LP with m + n <= cap: y and dx (doubles), the two
-    // offset arrays (n + 1 and m + 1 ints) and the two slice-base arrays
-    // (ceil(n/64) + 1 and ceil(m/64) + 1 ints).
-    static size_t lds_bytes(int cap) { return size_t(cap) * 12 + size_t(cap / 64 + 3) * 8 + 16; }
+    // Dynamic LDS of an LP with m + n <= cap (lds_layout): y and dx
+    // (doubles), the two slice-base arrays (ceil(n/64) + 1 and ceil(m/64) + 1
+    // ints) and the two row-length arrays (n and m bytes).
+    static size_t lds_bytes(int cap) { return size_t(cap) * 9 + size_t(cap / 64 + 6) * 4 + 16; }
 
     int lds_class(const Job &j) const {
         const nyk::Model &m = j.prep.model;
