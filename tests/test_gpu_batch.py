@@ -365,3 +365,52 @@ def test_time_limit_stops_only_the_slow_lp(engine):
     for r, a in zip(res[1:], alone):
         assert r["status"] == a["status"] == "OPTIMAL"
         assert r["iterations"] == a["iterations"] and np.array_equal(r["x"], a["x"])
+
+
+def _long_row_lp(engine, m, n, per_row, seed, dense_col=False, dense_row=False):
+    """A sparse LP with per_row random entries a row, optionally one fixed
+    column in every row or one row over every column; a ranged row around the
+    activity of a random point in the box keeps it feasible."""
+    rng = np.random.default_rng(seed)
+    rows = [np.sort(rng.choice(n, size=per_row, replace=False)) for _ in range(m)]
+    if dense_row:
+        rows[0] = np.arange(n)
+    if dense_col:
+        rows = [np.append(r, n) for r in rows]
+    cols = n + (1 if dense_col else 0)
+    ptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])])
+    index = np.concatenate(rows)
+    value = rng.normal(size=len(index))
+    lb, ub = np.zeros(cols), rng.uniform(1, 10, size=cols)
+    if dense_col:
+        lb[n] = ub[n] = 1.0
+    point = rng.uniform(0, 1, size=cols) * ub
+    point[lb == ub] = lb[lb == ub]
+    act = np.array([value[ptr[i]:ptr[i + 1]] @ point[index[ptr[i]:ptr[i + 1]]] for i in range(m)])
+    rl, ru = act - rng.uniform(0.5, 2, size=m), act + rng.uniform(0.5, 2, size=m)
+    return engine.LP(ptr, index, value, rng.normal(size=cols), lb, ub, rl, ru)
+
+
+def test_long_rows_in_checks(engine, oracle):
+    """Rows longer than 2,048 entries are summed by the checks' workgroup tree
+    (the others one thread per row).  A fixed column in all 2,100 rows lies
+    in the original A^T only (presolve removes it, so the LP runs in an LDS
+    class, here in the tail launch); a row over all 2,200 columns stays in the
+    scaled A too (the LP runs in the global class, checked by the crowded check
+    kernel).  Both are certified, reach the oracle's objective, and equal their
+    solves alone bit for bit."""
+    lps = [_long_row_lp(engine, 2100, 2000, 2, 1, dense_col=True),
+           _long_row_lp(engine, 600, 2200, 3, 5, dense_row=True)]
+    assert max(np.diff(lps[0].ptr)) <= 3 and lps[0].rows > 2048
+    assert max(np.diff(lps[1].ptr)) == 2200
+    mix, stats = solve_batch(engine, lps)
+    assert stats["class_launches"][4] > 0 and stats["class_launches"][3] > 0
+    for lp, r in zip(lps, mix):
+        assert r["status"] == "OPTIMAL" and r["kkt_error"] <= 1e-6, (r["status"], r["message"])
+        assert oracle.verify(lp, r["x"], r["y"])["kkt"] <= 1e-6
+        cpu = oracle.solve(lp)
+        assert cpu["status"] == "OPTIMAL"
+        assert rel(r["objective"], cpu["objective"]) <= 1e-6
+        alone, _ = solve_batch(engine, [lp])
+        assert alone[0]["iterations"] == r["iterations"]
+        assert np.array_equal(alone[0]["x"], r["x"]) and np.array_equal(alone[0]["y"], r["y"])
