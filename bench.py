@@ -345,8 +345,9 @@ def single_lp(engine, config, scale, iterations, tol, seed=SEED, cpu_threads=16,
     attempts = r["iterations"] + r["rejected_steps"]
     s_p = 4 if nnz < 2 ** 31 else 8
     s_v = 4 if matrix_precision in ("mixed", "fp32") else 8
+    s_g = 4 if matrix_precision == "fp32" else 8  # the gathered y and dx (FP32 shadows in the FP32 phase)
     # algorithmic bytes of one attempt (s_v-byte values, int32 indices), original sizes
-    b_attempt = 2 * nnz * (s_v + 4) + (n + m + 2) * s_p + m * 8 + n * 8 + 64 * (n + m)
+    b_attempt = 2 * nnz * (s_v + 4) + (n + m + 2) * s_p + m * s_g + n * s_g + 64 * (n + m)
     achieved = b_attempt * attempts / max(r["iteration_seconds"], 1e-12) / 1e9
     out = {"mode": "single", "config": config, "scale": scale, "index": index, "rows": m, "cols": n, "nonzeros": nnz,
            "matrix_precision": matrix_precision,
@@ -728,6 +729,21 @@ def main():
                       f"{c1.get('cpu_baseline', {}).get('kkt', float('nan')):.3g}")
         line["config1_single_lp"] = c1
         log(f"config 1: {c1}")
+        # the same LP and cap in the FP32-bulk form (FP32 values, gathered
+        # vectors and lane sums over FP64 iterates; FP64 checks, and FP64
+        # refinement once a candidate reaches the switch point, which a capped
+        # run of this LP does not), timed against the same CPU run
+        c1f = single_lp(engine, 1, 1, args.config1_iterations, 1e-6, cpu_threads=0, matrix_precision="fp32")
+        cpu = c1.get("cpu_baseline")
+        if cpu:
+            c1f["cpu_baseline"] = dict(cpu, note="the run of config1_single_lp")
+            c1f["speedup_per_iteration"] = round(cpu["ms_per_iteration"] /
+                                                 max(1e3 * c1f["iteration_seconds"] / max(c1f["iterations"], 1), 1e-9), 1)
+            c1f["speedup_wall"] = round(cpu["seconds"] / max(c1f["seconds"], 1e-9), 1)
+        c1f["note"] = (f"capped at {args.config1_iterations} iterations: GPU {c1f['status']} at kkt {c1f['kkt']:.3g}, "
+                       f"{c1f['fp32_iterations']} iterations in the FP32 phase")
+        line["config1_single_lp_fp32"] = c1f
+        log(f"config 1, FP32 bulk: {c1f}")
     print(json.dumps(line), flush=True)
 
 
