@@ -400,8 +400,8 @@ __device__ void rows_direct(const int *ptr_, const int *idx_, const double *val_
     const GLOBAL_AS double *val = gin(val_);
     const GLOBAL_AS double *vec[2] = {gin(vec0_), gin(vec1_)};
     if constexpr (NT >= 1024) {
-        // one or two rows per thread (a lone LP in the tail launch): the
-        // entries' loads of a few consecutive positions issued together
+        // few rows per thread (the tail launch's 1,024 threads): the loads
+        // of a few consecutive entries issued together
         for (int r = threadIdx.x; r < nrows; r += NT) {
             const int k0 = ptr[r], k1 = ptr[r + 1];
             if (k1 - k0 > kChunk) continue;
