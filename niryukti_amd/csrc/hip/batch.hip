@@ -97,8 +97,10 @@ constexpr int kMaxRejects = 40;
 // single-LP engine's FP32 phase.  Measured on the 4,096-LP workload: FP32
 // values take more iterations to 1e-6 (mean 2,381 against 2,229 with this
 // rule; 2,882 when only a 20-check stall moves an LP) and the streamed batch
-// runs at 6,511 against 7,230 LPs/s in FP64; switching after 3-5 checks
-// without halving moves most LPs within a few checks (6,440-6,654).
+// runs at 6,842-7,049 against 7,478 LPs/s in FP64 (the FP32-value step moves
+// its fewer bytes at a lower rate); switching after 3-5 checks without
+// halving moves most LPs within a few checks (6,440-6,654 when the LPs on
+// FP64 values ran in the tail launch).
 constexpr int kStallChecks = 20;
 constexpr double kFp64Below = 1e-5;
 
@@ -1219,7 +1221,8 @@ __device__ void attempt_lds(const LpDev &L, LpState &S, const BatchOpts &o, doub
 
 // Hot loop for the LDS-resident class; dynamic LDS holds y (m) and dx (n).
 // VT: the value type of the step (float: the FP32-value phase of the mixed
-// precision mode; LPs that moved to FP64 values run in the tail launch).
+// precision mode; a class's LPs that moved to FP64 values step in the same
+// period's VT = double launch).
 template <int NT, int NR, class VT = double>
 __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps, LpState *states, const int *slots,
                                                                BatchOpts o) {
@@ -1232,8 +1235,8 @@ __global__ __launch_bounds__(NT) void batch_advance_lds_kernel(const LpDev *lps,
     if (threadIdx.x == 0) S = states[s];
     __syncthreads();
     // an LP whose check moved it to FP64 values in this round skips the rest
-    // of the round's FP32-value launches (the next round runs it in the tail
-    // launch), so its periods do not depend on the launch it is in
+    // of the round's FP32-value launches (the next round lists it for the
+    // FP64-value launches), so its periods do not depend on the launch it is in
     if (S.status != kRunning || S.phase != kAdvance || (sizeof(VT) == 4) == (S.fp64 != 0)) return;
     const LpDev &L = lps[s];
     const long long remaining = S.iteration_limit - S.iterations;
@@ -2216,6 +2219,7 @@ class BatchEngine {
             int kind;  // 0..kClasses-1: that class; kClasses: the tail group
             int cap;
             std::vector<int> slots;
+            int fp32 = 0;  // mixed precision: the first `fp32` LPs step with FP32 values
         };
         std::vector<Group> groups;
         std::vector<int> active;
@@ -2230,15 +2234,22 @@ class BatchEngine {
     // the whole state array can be uploaded).
     void launch_round(int checks) {
         auto R = std::make_unique<Round>();
-        std::vector<int> by_class[kClasses], refine;
+        // mixed precision: a crowded LDS class lists its LPs on FP32 values
+        // first and those on FP64 values after them (by_fp64); each check
+        // period launches the step kernel in FP32 values over the first part,
+        // in FP64 values over the second, and one check over both
+        std::vector<int> by_class[kClasses], by_fp64[kClasses];
         for (int s = 0; s < max_slots_; ++s)
             if (slot_job_[s] >= 0 && state_[s].status == kRunning) {
                 R->active.push_back(s);
-                // mixed precision: LDS-class LPs on FP64 values run in the
-                // tail launch (the crowded launches step with FP32 values)
-                if (mixed_ && state_[s].fp64 && slot_class_[s] < kClasses - 1) refine.push_back(s);
+                if (mixed_ && state_[s].fp64 && slot_class_[s] < kClasses - 1) by_fp64[slot_class_[s]].push_back(s);
                 else by_class[slot_class_[s]].push_back(s);
             }
+        int split[kClasses];
+        for (int c = 0; c < kClasses; ++c) {
+            split[c] = int(by_class[c].size());
+            by_class[c].insert(by_class[c].end(), by_fp64[c].begin(), by_fp64[c].end());
+        }
         if (R->active.empty()) return;
         R->checks = checks;
         if (dirty_) {
@@ -2255,15 +2266,14 @@ class BatchEngine {
         // of one per class (launches of different classes can land on the
         // same hardware queue and then run one after another).
         auto &groups = R->groups;
-        Round::Group tail{kClasses, 0, refine};
-        for (int s : refine) tail.cap = std::max(tail.cap, class_cap_[slot_class_[s]]);
+        Round::Group tail{kClasses, 0, {}};
         for (int c = 0; c < kClasses; ++c) {
             if (by_class[c].empty()) continue;
             if (c < kClasses - 1 && int(by_class[c].size()) <= kTailLps) {
                 tail.cap = std::max(tail.cap, class_cap_[c]);
                 tail.slots.insert(tail.slots.end(), by_class[c].begin(), by_class[c].end());
             } else {
-                groups.push_back(Round::Group{c, c < kClasses - 1 ? class_cap_[c] : 0, by_class[c]});
+                groups.push_back(Round::Group{c, c < kClasses - 1 ? class_cap_[c] : 0, by_class[c], split[c]});
             }
         }
         const size_t tail_lps = tail.slots.size();
@@ -2312,8 +2322,12 @@ class BatchEngine {
                 HIP_OK(hipEventRecord(ev[1], cs));
             } else {
                 for (int k = 0; k < checks; ++k) {
-                    if (g.kind < kClasses - 1)
-                        launch_lds(g.cap, R->shape_of[g.kind], grid, cs, slots, bo);
+                    if (g.kind < kClasses - 1 && mixed_) {
+                        if (g.fp32) launch_lds(g.cap, R->shape_of[g.kind], g.fp32, cs, slots, bo, true);
+                        if (grid > g.fp32)
+                            launch_lds(g.cap, R->shape_of[g.kind], grid - g.fp32, cs, slots + g.fp32, bo, false);
+                    } else if (g.kind < kClasses - 1)
+                        launch_lds(g.cap, R->shape_of[g.kind], grid, cs, slots, bo, false);
                     else
                         batch_advance_kernel<<<grid, kThreads, 0, cs>>>(d_desc_, d_state_, slots, bo);
                     HIP_OK(hipEventRecord(ev[2 * k + 1], cs));
@@ -2720,8 +2734,8 @@ class BatchEngine {
     }
 
     template <int NT, int NR>
-    void launch_lds_t(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
-        if (mixed_)
+    void launch_lds_t(int cap, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo, bool fp32) {
+        if (fp32)
             batch_advance_lds_kernel<NT, NR, float><<<grid, NT, lds_bytes(cap), cs>>>(d_desc_, d_state_, slots, bo);
         else
             batch_advance_lds_kernel<NT, NR, double><<<grid, NT, lds_bytes(cap), cs>>>(d_desc_, d_state_, slots, bo);
@@ -2751,12 +2765,12 @@ class BatchEngine {
             d_desc_, d_state_, slots, count, d_queue_, bo, checks, checks * kTailMaxPeriodsPerCheck, budget);
     }
     // A crowded LDS class in the launch shape its tuner picked.
-    void launch_lds(int cap, int shape, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo) {
+    void launch_lds(int cap, int shape, int grid, hipStream_t cs, const int *slots, const BatchOpts &bo, bool fp32) {
         switch (shape) {
-        case 3: launch_lds_t<kWideThreads, 1>(cap, grid, cs, slots, bo); break;
-        case 2: launch_lds_t<kThreads, 4>(cap, grid, cs, slots, bo); break;
-        case 1: launch_lds_t<kThreads, 2>(cap, grid, cs, slots, bo); break;
-        default: launch_lds_t<kThreads, 1>(cap, grid, cs, slots, bo);
+        case 3: launch_lds_t<kWideThreads, 1>(cap, grid, cs, slots, bo, fp32); break;
+        case 2: launch_lds_t<kThreads, 4>(cap, grid, cs, slots, bo, fp32); break;
+        case 1: launch_lds_t<kThreads, 2>(cap, grid, cs, slots, bo, fp32); break;
+        default: launch_lds_t<kThreads, 1>(cap, grid, cs, slots, bo, fp32);
         }
     }
 
