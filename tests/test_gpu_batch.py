@@ -335,13 +335,22 @@ def test_mixed_precision_batch_certified_against_oracle(engine, oracle):
 
 
 def test_mixed_precision_batch_alone_equals_in_mix(engine):
-    """An LP's switch from FP32 to FP64 values depends only on its own checks,
-    and LPs on FP64 values run in the tail launch whatever their class: alone
-    (always the tail launch) and among 40 (crowded launches on FP32 values) an
-    LP follows the same trajectory bit for bit."""
-    insts = [engine.Instance(2, 7, 1, i) for i in range(40)]
-    mix, _ = solve_batch(engine, [i.lp for i in insts], matrix_precision="mixed")
-    for k in (0, 13, 39):
+    """An LP's switch from FP32 to FP64 values depends only on its own checks.
+    Among 600 LPs the smallest class runs crowded (more than 256 LPs), each
+    period launching the step in FP32 values over its LPs still on them and
+    in FP64 values over those that switched; alone an LP runs in the tail
+    launch, whose workgroups pick the value type per LP.  Either way an LP
+    follows the same trajectory bit for bit; the LPs compared are small ones
+    that switched earliest in the mix (while their class was crowded) and
+    three others."""
+    insts = [engine.Instance(2, 7, 1, i) for i in range(600)]
+    mix, stats = solve_batch(engine, [i.lp for i in insts], matrix_precision="mixed")
+    assert stats["class_launches"][0] > 0
+    small = [k for k, i in enumerate(insts) if i.lp.rows + i.lp.cols <= 1500 and mix[k]["status"] == "OPTIMAL"
+             and mix[k]["fp32_iterations"] < mix[k]["iterations"]]
+    assert len(small) > 256
+    early = sorted(small, key=lambda k: mix[k]["fp32_iterations"])[:3]
+    for k in early + [0, 13, 39]:
         alone, _ = solve_batch(engine, [insts[k].lp], matrix_precision="mixed")
         assert alone[0]["status"] == mix[k]["status"] == "OPTIMAL"
         assert alone[0]["iterations"] == mix[k]["iterations"]
