@@ -633,6 +633,9 @@ def main():
              [(t, r) for (t, r) in LDS_SHAPES if symbol == f"batch_advance_lds_kernel<{t}, {r}, {vt}>"]]
     if symbol.startswith("batch_advance_lds_kernel"):
         kernel_name = f"{symbol} (crowded LDS classes; tuned choice of class {', '.join(users) or 'none'})"
+        if vt == "float":
+            # the shape's FP64-value launches (LPs that switched) are counted with it
+            kernel_name += " with the same shape's FP64-value launches"
     elif symbol == "batch_advance_kernel":
         kernel_name = symbol + " (global class)"
     else:
