@@ -230,8 +230,8 @@ __device__ __forceinline__ double wave_tree(const double *lds, int W, int k) {
     }
 }
 
-// v from the lane CTRL names within the lane's row of 16 (a DPP row
-// rotation: 0x128 + k rotates right by 8 + k lanes).
+// v from the lane CTRL names within the lane's row of 16 lanes (a DPP row
+// rotation: control 0x120 + k rotates the row right by k lanes).
 template <int CTRL>
 __device__ __forceinline__ double dpp_rotate(double v) {
     const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
