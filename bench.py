@@ -434,6 +434,24 @@ def _pmc_means(directory, counter):
     return {k: (len(v), sum(v.values()) / len(v)) for k, v in per.items()}
 
 
+def summarize_counters(directory):
+    """Per kernel: dispatches and the mean of every counter of a rocprofv3
+    --pmc run per dispatch (one JSON line per kernel, the busiest first)."""
+    sums, dispatches = {}, {}
+    for r in _csv_rows(directory, "counter_collection.csv"):
+        k = _short(_col(r, "Kernel_Name"))
+        c = _col(r, "Counter_Name")
+        sums.setdefault(k, {}).setdefault(c, 0.0)
+        sums[k][c] += float(_col(r, "Counter_Value"))
+        dispatches.setdefault(k, set()).add(_col(r, "Dispatch_Id"))
+    lines = []
+    for k in sorted(sums, key=lambda k: -sums[k].get("SQ_BUSY_CU_CYCLES", 0.0)):
+        n = len(dispatches[k])
+        lines.append(json.dumps({"kernel": k, "dispatches": n,
+                                 **{c: round(v / n) for c, v in sorted(sums[k].items())}}))
+    return "\n".join(lines)
+
+
 def summarize_traffic(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes):
     """HBM bytes per dispatch from FETCH_SIZE and WRITE_SIZE runs (KiB per
     dispatch), each scaled by what that counter reads for a plain copy of
@@ -497,6 +515,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (HBM counter calibration)")
     ap.add_argument("--summarize-trace", metavar="DIR", help="summarise a rocprofv3 --kernel-trace --stats CSV run")
+    ap.add_argument("--summarize-counters", metavar="DIR", help="summarise a rocprofv3 --pmc CSV run per kernel")
     ap.add_argument("--summarize-traffic", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="summarise FETCH_SIZE / WRITE_SIZE CSV runs of the bench and of --copy-probe")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
@@ -531,6 +550,9 @@ def main():
         return
     if args.summarize_traffic:
         print(summarize_traffic(*args.summarize_traffic, args.copy_bytes))
+        return
+    if args.summarize_counters:
+        print(summarize_counters(args.summarize_counters))
         return
     if args.copy_probe:
         import torch
